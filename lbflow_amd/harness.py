@@ -1,0 +1,226 @@
+"""Case construction, input files and VTI reading for LBFlow-compatible runs.
+
+Mirrors the reference harness (scripts/benchmarks.py:26-149 ``spec``/``Study.run``,
+scripts/vti.py:5-38 ``read_vti``) so that the same study cases can be written
+as ``input.prm`` + ``geometry.raw`` for the drop-in ``LBFlow`` executable, or
+handed to the C ABI as a plain lattice description.
+"""
+import math
+import struct
+import xml.etree.ElementTree as ET
+from pathlib import Path
+
+import numpy as np
+
+LABEL_TYPES = ('VD', 'GV', 'PB', 'VB', 'AB', 'BB', 'HF', 'SD')
+
+
+def spec(name, nx=8, ny=8, nz=4, dimensions=3, model='color', eq='MRT',
+         tau=(1., 1.), steps=1000, write=250, label=None, indices=None, **kw):
+    """A study case, same fields and defaults as scripts/benchmarks.py:26-34."""
+    result = dict(name=name, size=(nx, ny, nz), dimensions=dimensions,
+                  model=model, eq=eq, tau=tau, steps=steps, write=write,
+                  restart=steps, density=(1., 1.) if model == 'color' else (1.,),
+                  alpha=4 / 9 if dimensions == 2 else 1 / 3, sigma=.005, beta=.7,
+                  label=label or (lambda x, y, z: 0),
+                  indices=indices or ['0 "VD" "N/A" "N/A" "0"'])
+    result.update(kw)
+    return result
+
+
+def force_cases():
+    """Uniform periodic body force (scripts/benchmarks.py:176-188)."""
+    cases = []
+    for model in ('scsp', 'color'):
+        for ndim in (2, 3):
+            for eq in ('SRT', 'MRT'):
+                for tau in (.6, 1.4):
+                    name = f'force-{model}-d{ndim}-{eq}-tau{tau}'
+                    cases.append(spec(name, ny=6, nz=1 if ndim == 2 else 4,
+                                      dimensions=ndim, model=model, eq=eq,
+                                      tau=(.8, tau) if model == 'color' else (tau,), steps=100, write=50,
+                                      label=lambda x, y, z: 75,
+                                      indices=[f'75 "GV" "0.000001" "1" "{1 if model == "color" else 0}"']))
+    return cases
+
+
+def channel_cases():
+    """Poiseuille channels (scripts/benchmarks.py:191-208)."""
+    cases = []
+    for model, ndim in (('scsp', 3), ('color', 2)):
+        for eq in ('SRT', 'MRT'):
+            for tau in (.6, 1.4):
+                for h in (8, 16, 32):
+                    nu = (tau - .5) / 3
+                    n = math.ceil(18 * h * h / (nu * math.pi ** 2))
+                    steps = ((n + 499) // 500) * 500
+                    force = 1e-6 * (8 / h) ** 3
+                    cases.append(spec(
+                        f'channel-{model}-d{ndim}-{eq}-tau{tau}-H{h}',
+                        nx=4, ny=h + 2, nz=1 if ndim == 2 else 4, dimensions=ndim,
+                        model=model, eq=eq, tau=(.8, tau) if model == 'color' else (tau,),
+                        steps=steps, write=steps // 2, force=force, height=h,
+                        label=lambda x, y, z, h=h: 255 if y in (0, h + 1) else 75,
+                        indices=[f'75 "GV" "{force:.17g}" "1" "{1 if model == "color" else 0}"',
+                                 '255 "SD" "90" "N/A" "N/A"']))
+    return cases
+
+
+def layered_cases(interpolation='arithmetic'):
+    """Two-phase layered channels (scripts/layered_flow.py:10-23)."""
+    runs = []
+    for tau0, tau1 in ((.6, 1.4), (1.4, .6)):
+        for h in (16, 32, 64):
+            force = 1e-6 * (16 / h) ** 3
+            steps = math.ceil(20 * h * h / ((min(tau0, tau1) - .5) / 3 * math.pi ** 2))
+            steps = ((steps + 999) // 1000) * 1000
+            runs.append(spec(
+                f'layered-tau{tau0}-{tau1}-H{h}' + ('' if interpolation == 'arithmetic' else '-harmonic'),
+                nx=4, ny=h + 2, nz=1, dimensions=2, tau=(tau0, tau1), steps=steps, write=steps // 2,
+                height=h, force=force, viscosity_interpolation=interpolation,
+                label=lambda x, y, z, h=h: 255 if y in (0, h + 1) else 75 if y <= h // 2 else 76,
+                indices=[f'75 "GV" "{force:.17g}" "1" "0"', f'76 "GV" "{force:.17g}" "1" "1"',
+                         '255 "SD" "90" "N/A" "N/A"']))
+    return runs
+
+
+def bundle_cases(scales=(1, 2, 3), dimensions=(2, 3)):
+    """Two isolated slit pores (scripts/porous_media.py:67-100)."""
+    cases = []
+    for ndim in dimensions:
+        for scale in scales:
+            h, ny = 8 * scale, 36 * scale
+            widths = (h, 2 * h)
+            force = 1e-6 / scale ** 3
+            for taus in ((1.,), (.6, 1.4), (1.4, .6)):
+                model = 'scsp' if len(taus) == 1 else 'color'
+                viscosities = [(tau - .5) / 3 for tau in taus]
+                decay = max(widths[i] ** 2 / (viscosities[i % len(taus)] * math.pi ** 2) for i in (0, 1))
+                steps = math.ceil(18 * decay / 1000) * 1000
+
+                def label(x, y, z, h=h, scale=scale, model=model):
+                    if 4 * scale <= y < 4 * scale + h:
+                        return 75
+                    if 8 * scale + h <= y < 8 * scale + 3 * h:
+                        return 75 if model == 'scsp' else 76
+                    return 255
+
+                cases.append(spec(
+                    f'bundle-d{ndim}-s{scale}-{model}-tau' + '-'.join(map(str, taus)),
+                    nx=4 * scale, ny=ny, nz=1 if ndim == 2 else 4 * scale,
+                    dimensions=ndim, model=model, tau=taus, steps=steps, write=steps // 2,
+                    force=force, scale=scale, widths=widths, label=label,
+                    indices=[f'75 "GV" "{force:.17g}" "1" "0"'] +
+                            ([f'76 "GV" "{force:.17g}" "1" "1"'] if model == 'color' else []) +
+                            ['255 "SD" "90" "N/A" "N/A"']))
+    return cases
+
+
+def geometry(s):
+    """uint8 labels, x fastest then y then z (scripts/benchmarks.py:70-71)."""
+    if s.get('geometry') is not None:
+        return np.ascontiguousarray(s['geometry'], dtype=np.uint8).ravel()
+    nx, ny, nz = s['size']
+    return np.frombuffer(bytes(s['label'](x, y, z) for z in range(nz)
+                               for y in range(ny) for x in range(nx)), dtype=np.uint8)
+
+
+def _quoted(line):
+    out, cur, inside = [], '', False
+    for ch in line:
+        if ch == '"':
+            if inside:
+                out.append(cur)
+                cur = ''
+            inside = not inside
+        elif inside:
+            cur += ch
+    return out
+
+
+def parse_index(line):
+    """One label line ``<u8> "TYPE" "value" "dir" "phase"`` (fileprm.cpp:563-575, :633-670)."""
+    key = int(line.split()[0])
+    fields = _quoted(line)
+    typ = fields[0]
+
+    def num(text, conv):
+        try:
+            return conv(text)
+        except ValueError:
+            return conv(0)
+
+    value = num(fields[1], float) if typ in ('GV', 'PB', 'VB', 'SD') else 0.0
+    direction = num(fields[2], int) if typ in ('GV', 'PB', 'VB') else 0
+    phase = num(fields[3], int) if typ in ('VD', 'GV', 'PB', 'VB', 'HF') else 0
+    return key, (typ, value, direction, phase)
+
+
+def lattice_case(s):
+    """Plain lattice description of a study case (sizes in reference order)."""
+    nx, ny, nz = s['size']
+    ndim = s['dimensions']
+    return dict(D=ndim, Q=9 if ndim == 2 else 19,
+                size=(ny, nx) if ndim == 2 else (nz, ny, nx),
+                model=s['model'], eq=s['eq'], tau=tuple(s['tau']), density=tuple(s['density']),
+                alpha=s['alpha'], sigma=s['sigma'], beta=s['beta'],
+                harmonic=s.get('viscosity_interpolation', 'arithmetic') == 'harmonic',
+                labels=dict(parse_index(line) for line in s['indices']))
+
+
+def prm_text(s, folder, start=0):
+    """input.prm text (scripts/benchmarks.py:72-104)."""
+    folder = Path(folder)
+    nx, ny, nz = s['size']
+    dims = (ny, nx) if s['dimensions'] == 2 else (nz, ny, nx)
+    params = dict(gridFile=f'"{folder / "geometry.raw"}"', dimension=' '.join(map(str, dims)),
+                  directions=9 if s['dimensions'] == 2 else 19, fileType='"uint8"',
+                  latticeLen=1, startTime=start, endTime=s['steps'],
+                  writeInterval=s['write'], reportInterval=s.get('report_interval', s['write']),
+                  restartInterval=s['restart'], stopError=s.get('stop_error', 1e-8),
+                  outPath=f'"{folder / "field"}"', logPath=f'"{folder}"',
+                  reportDir=s.get('report_dir', 0), reportSlice=f'1 {nx - 2}',
+                  fluidModel=f'"{s["model"]}"', equilModel=f'"{s["eq"]}"',
+                  rexlationTime=' '.join(format(t, '.17g') for t in s['tau']),
+                  latticeSpeed=1, alpha0=format(s['alpha'], '.17g'),
+                  density=' '.join(map(str, s['density'])), contactAngle=90,
+                  surfaceTension=s['sigma'], interfaceThickness=s['beta'],
+                  wallGradient=f'"{s.get("wall_gradient", "legacy")}"')
+    params['viscosityInterpolation'] = f'"{s.get("viscosity_interpolation", "arithmetic")}"'
+    params.update(s.get('overrides', {}))
+    text = '\n'.join(f'{key} {val};' for key, val in params.items())
+    return text + '\n' + '\n'.join(s['indices']) + '\n'
+
+
+def write_case(s, folder, start=0):
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    (folder / 'geometry.raw').write_bytes(geometry(s).tobytes())
+    (folder / 'input.prm').write_text(prm_text(s, folder, start))
+    return folder / 'input.prm'
+
+
+def read_vti(path):
+    """Parse an appended-raw VTI (scripts/vti.py:5-38); returns ((nx,ny,nz), {name: array})."""
+    data = Path(path).read_bytes()
+    pos = data.index(b'<AppendedData')
+    doc = ET.fromstring(data[:pos] + b'</VTKFile>')
+    begin = data.index(b'>_', pos) + 2
+    im = doc.find('ImageData')
+    extent = list(map(int, im.attrib['WholeExtent'].split()))
+    nx, ny, nz = [extent[i + 1] - extent[i] + 1 for i in (0, 2, 4)]
+    fields = {}
+    for piece in im.findall('Piece'):
+        e = list(map(int, piece.attrib['Extent'].split()))
+        if (e[1] - e[0] + 1, e[3] - e[2] + 1, e[5] - e[4] + 1) != (nx, ny, nz):
+            raise ValueError('multi-piece VTI files are not read here')
+        for a in piece.find('PointData').findall('DataArray'):
+            name = a.attrib['Name']
+            n = int(a.attrib.get('NumberOfComponents', '1'))
+            off = begin + int(a.attrib['offset'])
+            length = struct.unpack_from('<Q', data, off)[0]
+            if length != nx * ny * nz * n * 8:
+                raise ValueError(f'{name}: block {length} != {nx * ny * nz * n * 8}')
+            values = np.frombuffer(data, dtype='<f8', count=length // 8, offset=off + 8)
+            fields[name] = values.reshape(nz, ny, nx, n).copy()
+    return (nx, ny, nz), fields
