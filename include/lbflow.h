@@ -1,0 +1,182 @@
+/*
+ * lbflow.h -- C ABI of the MI355X lattice Boltzmann engine.
+ *
+ * The reference (zhou1800/lbflow) has no plugin API: its time-step loop is
+ * the virtual Simulator::Run of a monolithic executable
+ * (solver/inc/lbm/simulator.h:35, :118; solver/apps/LBSim/lbsim.cpp:40-61).
+ * This header is the seam under that loop.  The drop-in executable
+ * (lbflow_amd/bin/LBFlow) keeps the Run orchestration, file formats and
+ * logging on the host and calls these entry points for every per-site
+ * operator; any other host language binds the same symbols (ctypes, cgo,
+ * JNI ...).  Plain C types only: pointers, sizes, doubles.  All device
+ * memory is owned by the context; host buffers are owned by the caller and
+ * copied.  One host thread per context; calls are stream-ordered and not
+ * re-entrant.  Errors are int return codes (0 = success) with a message in
+ * lbf_last_error(); nothing throws across the ABI.
+ *
+ * Index order everywhere is the reference's: z,y,x row-major, x fastest
+ * (solver/inc/grid/grid.h:153-177); sizes are given slowest first, i.e.
+ * sz = {Nz, Ny, Nx} for D3Q19 and {Ny, Nx} for D2Q9.
+ */
+#ifndef LBFLOW_H
+#define LBFLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBF_ABI_VERSION 1
+
+/* Label types: same bit values as FilePrm::IDXTYPE (solver/inc/file/fileprm.h:44-53). */
+enum lbf_label_type {
+    LBF_VD = 0x1, LBF_GV = 0x2, LBF_PB = 0x4, LBF_VB = 0x8,
+    LBF_AB = 0x10, LBF_BB = 0x20, LBF_HF = 0x40, LBF_SD = 0x80
+};
+
+/* One geometry label, FilePrm::IDXVAL (solver/inc/file/fileprm.h:55-90,
+ * solver/src/file/fileprm.cpp:633-670).  type 0 = label not defined. */
+typedef struct lbf_label {
+    int32_t type;      /* lbf_label_type */
+    int32_t dir;       /* lattice direction of a GV force (PB/VB flow direction) */
+    int32_t phase;     /* phase of VD/GV/PB/VB/HF sites */
+    int32_t reserved;
+    double  value;     /* GV force magnitude, SD contact angle in degrees */
+} lbf_label;
+
+enum lbf_model { LBF_MODEL_SCSP = 0, LBF_MODEL_COLOR = 1 };
+enum lbf_equil { LBF_SRT = 0, LBF_MRT = 1 };
+enum lbf_precision {
+    LBF_FP64 = 0,          /* reference arithmetic, operation order kept: bit-exact */
+    LBF_FP32 = 1           /* FP32 populations shifted by their rest equilibrium;
+                              macro fields and reductions in FP64 */
+};
+enum lbf_create_flags {
+    LBF_STREAM_SOLIDS = 0x1 /* stream populations through solid sites as the reference
+                               does, so checkpoints are byte-identical; without it
+                               solid sites are skipped and checkpoint them as 0 */
+};
+
+/* Simulation description; the fields of input.prm that the step needs
+ * (solver/src/file/fileprm.cpp:364-456). */
+typedef struct lbf_desc {
+    int32_t  D, Q;             /* 2,9 or 3,19 */
+    int32_t  model;            /* lbf_model */
+    int32_t  equil;            /* lbf_equil */
+    int32_t  precision;        /* lbf_precision */
+    int32_t  flags;            /* lbf_create_flags */
+    int32_t  harmonic;         /* color viscosityInterpolation: 0 arithmetic, 1 harmonic */
+    int32_t  reserved;
+    uint64_t sz[3];            /* grid sizes, slowest first */
+    double   tau[2];           /* rexlationTime per phase */
+    double   density[2];       /* reference density per phase */
+    double   alpha0;           /* color alpha0 */
+    double   sigma;            /* color surfaceTension */
+    double   beta;             /* color interfaceThickness (recoloring beta) */
+} lbf_desc;
+
+typedef struct lbf_ctx lbf_ctx;
+
+/* lbf_step flags */
+enum lbf_step_flags {
+    LBF_STEP_FIELDS   = 0x1,   /* keep this step's macro fields (rho, u) for output/report */
+    LBF_STEP_ADJACENT = 0x2    /* keep this step's velocity (and color indicator) as the
+                                  adjacent history of the next report (scsp.cpp:82-83,
+                                  colorfluid.cpp:166-171) */
+};
+
+/* Failure bits of lbf_status, one per reference exception text. */
+enum lbf_status_bits {
+    LBF_FAIL_DENSITY    = 0x1,  /* "Non-finite or non-positive fluid density." (scsp.cpp:145) /
+                                   "Non-finite or non-positive total fluid density." (colorfluid.cpp:257) */
+    LBF_FAIL_NEGATIVE   = 0x2,  /* "Negative component density outside roundoff tolerance." */
+    LBF_FAIL_TAU        = 0x4,  /* "Invalid local relaxation time." */
+    LBF_FAIL_VELOCITY   = 0x8,  /* "Non-finite fluid velocity." */
+    LBF_FAIL_POPULATION = 0x10  /* "Non-finite post-collision population; no restart will be written." */
+};
+
+/* Report request: the reduction behind SCSP::report / ColorFluid::report and
+ * Simulator::updateConvergence (scsp.cpp:304-372, colorfluid.cpp:814-915,
+ * simulator.cpp:248-325). */
+typedef struct lbf_report_req {
+    int32_t  axis;             /* report axis in reference order (Qaxis[reportDir]) */
+    int32_t  convergence;      /* 1: also compute the convergence sums vs. the previous report */
+    int32_t  adjacent;         /* 1: the adjacent history was kept by LBF_STEP_ADJACENT */
+    int32_t  reserved;
+    uint64_t first, last;      /* report slab [first, last] along axis */
+} lbf_report_req;
+
+typedef struct lbf_report_out {
+    double   vel_range;        /* sum of u[axis] over fluid sites in the slab */
+    double   rho_range[2];     /* per-phase density sums over the slab (color) */
+    double   rho_slice[2][2];  /* [first|last slice][phase] density sums */
+    double   count_slice[2];   /* fluid sites in the first/last slice */
+    double   conv[4];          /* sum (u-u_prev)^2, sum u^2, count, sum (u-u_adj)^2 */
+    double   phase_change[2];  /* max |chi-chi_prev|, max |chi-chi_adj| (color) */
+    int32_t  has_previous;     /* 0 on the first report after create/init/set_populations */
+    int32_t  nonfinite;        /* 1 if a reported velocity or indicator was not finite */
+} lbf_report_out;
+
+/* --- lifetime ----------------------------------------------------------
+ * Replaces the Simulator/SCSP/ColorFluid constructors
+ * (solver/src/lbm/simulator.cpp:17-38, scsp.cpp:22-44, colorfluid.cpp:20-107):
+ * validates labels, builds the boundary links (Simulator::buildIb,
+ * simulator.cpp:386-436), the MRT operators (simulator.cpp:162-216) and the
+ * wall rotations (colorfluid.cpp:1111-1129), and allocates device memory. */
+int  lbf_create(const lbf_desc* desc, const lbf_label* labels /* 256 entries */,
+                const uint8_t* geometry /* prod(sz) bytes */, lbf_ctx** out);
+void lbf_destroy(lbf_ctx* ctx);
+const char* lbf_last_error(const lbf_ctx* ctx);   /* ctx may be NULL: last create error */
+
+/* --- time stepping -----------------------------------------------------
+ * lbf_init: the t = 0 state, SCSP::Run / ColorFluid::Run initial branch
+ * (scsp.cpp:60-72, colorfluid.cpp:123-138): initialize, macro, body force
+ * (fields kept), collision.  lbf_step: one step t -> t+1, the loop body
+ * (scsp.cpp:75-104, colorfluid.cpp:142-198): stream + bounce-back + macro
+ * + body force + collision fused.  lbf_advance: n steps without fields. */
+int lbf_init(lbf_ctx* ctx);
+int lbf_step(lbf_ctx* ctx, uint32_t flags);
+int lbf_advance(lbf_ctx* ctx, uint64_t n);
+uint64_t lbf_time(const lbf_ctx* ctx);
+int lbf_synchronize(lbf_ctx* ctx);
+
+/* Failure bits raised by the device since create/init/set_populations, and
+ * the first step at which each bit was raised (5 entries, UINT64_MAX = never). */
+int lbf_status(lbf_ctx* ctx, uint32_t* bits, uint64_t* first_step /* 5 */);
+const char* lbf_status_message(uint32_t bit);
+
+/* --- data exchange -----------------------------------------------------
+ * lbf_fields: the macro fields kept by the last LBF_STEP_FIELDS step (or
+ * init), as the reference's VTI arrays (filevti.cpp:113-172): rho per phase
+ * and u[d][site] with d in reference order; rho1 may be NULL for scsp.
+ * lbf_get/set_populations: post-collision populations in the checkpoint
+ * layout [phase][q][site] of SCSP::saveRst / ColorFluid::saveRst
+ * (scsp.cpp:290-302, colorfluid.cpp:789-812). */
+int lbf_fields(lbf_ctx* ctx, double* rho0, double* rho1, double* u);
+int lbf_get_populations(lbf_ctx* ctx, double* out);
+int lbf_set_populations(lbf_ctx* ctx, const double* in, uint64_t t);
+
+/* Device reductions of one report (see lbf_report_req). */
+int lbf_report(lbf_ctx* ctx, const lbf_report_req* req, lbf_report_out* out);
+
+/* --- introspection (parity tests) --------------------------------------
+ * linkmask[site]: bit q set iff the site is in ibS[q]; wall[site] = ibI;
+ * normal/sin/cos per wall entry (entry 0 unused).  Returns the number of
+ * wall entries including entry 0, or 0 on error. */
+uint64_t lbf_links(lbf_ctx* ctx, uint32_t* linkmask, int32_t* wall,
+                   double* normal, double* sinca, double* cosca);
+uint64_t lbf_sites(const lbf_ctx* ctx);
+/* Device bytes the context holds. */
+uint64_t lbf_device_bytes(const lbf_ctx* ctx);
+
+/* Name of the kernel that lbf_step launches for this context, and the
+ * algorithmic HBM bytes one launch moves (for roofline accounting). */
+const char* lbf_step_kernel(const lbf_ctx* ctx);
+double lbf_step_bytes(const lbf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,672 @@
+// capi.cpp -- implementation of include/lbflow.h on one MI355X (HIP runtime).
+//
+// The context owns every device buffer.  Populations live in two SoA
+// buffers [phase][q][site] that swap roles each step (pull scheme: read A,
+// write B).  Everything the kernels read per site is a 32-bit descriptor
+// (bounce-back link bits + geometry label), a 256-entry label table and,
+// for the colour model, a wall index with per-wall rotation tables.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/lbflow.h"
+#include "device.h"
+#include "setup.h"
+
+namespace lbf {
+hipError_t launch_scsp(int Q, bool mrt, bool init, bool fp32, bool stream_solids, const ScspArgs& a,
+                       hipStream_t s);
+hipError_t launch_color(int Q, bool mrt, bool init, bool stream_solids, const ColorArgs& a, hipStream_t s);
+hipError_t launch_report(const ReportArgs& a, double* out, hipStream_t s);
+uint32_t report_blocks(uint64_t N);
+}  // namespace lbf
+
+using namespace lbf;
+
+struct lbf_ctx {
+    lbf_desc d{};
+    int D = 0, Q = 0, NP = 1;
+    uint64_t N = 0, nfluid = 0;
+    std::vector<LabelInfo> labels;
+    Links links;
+    MrtOperators ops;
+    std::vector<DevLabel> hlab;
+    std::vector<double> shift;      // FP32: per plane shift (phase*Q + q)
+    hipStream_t stream = nullptr;
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;
+    size_t elem = 8;
+    uint32_t* d_desc = nullptr;
+    DevLabel* d_lab = nullptr;
+    double *d_AT = nullptr, *d_A = nullptr, *d_B = nullptr, *d_MInv = nullptr;
+    float *d_A32 = nullptr, *d_B32 = nullptr;
+    int32_t* d_wall = nullptr;
+    DevWall* d_walls = nullptr;
+    double* d_densN = nullptr;
+    double *d_rho0 = nullptr, *d_rho1 = nullptr, *d_u = nullptr, *d_uadj = nullptr, *d_uprev = nullptr;
+    double *d_chiadj = nullptr, *d_chiprev = nullptr;
+    DevStatus* d_status = nullptr;
+    double *d_partial = nullptr, *d_out = nullptr;
+    GridDev g{};
+    uint64_t t = 0;
+    bool has_prev = false;
+    uint64_t dev_bytes = 0;
+    std::vector<void*> allocations;
+    std::string err;
+    std::string kernel_name;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(lbf_ctx* c, const std::string& msg)
+{
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return -1;
+}
+
+#define HIP_CHECK(ctx, expr)                                                                   \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(ctx, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
+    } while (0)
+
+template <typename T> int dalloc(lbf_ctx* c, T** p, size_t bytes)
+{
+    void* q = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(c, std::string("HIP error: hipMalloc failed: ") + hipGetErrorString(e));
+    c->allocations.push_back(q);
+    c->dev_bytes += bytes;
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+template <typename T> int upload(lbf_ctx* c, T* dst, const void* src, size_t bytes)
+{
+    HIP_CHECK(c, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Qp of the reference (gridspace.cpp:85-87, :112-116): unit vector of a link.
+double unit_component(int Q, int q, int d)
+{
+    const int e = Q == 9 ? Lattice<9>::e[q][d] : Lattice<19>::e[q][d];
+    if (e == 0) return 0.0;
+    const bool diag = Q == 9 ? Lattice<9>::diagonal(q) : Lattice<19>::diagonal(q);
+    const double sq = std::sqrt(2) / 2;
+    if (!diag) return double(e);
+    return e > 0 ? sq : -sq;
+}
+
+int lattice_e(int Q, int q, int d) { return Q == 9 ? Lattice<9>::e[q][d] : Lattice<19>::e[q][d]; }
+
+// Pair-split form of an operator with the lattice parity symmetry (see
+// apply_split in kernels_scsp.hip).
+std::vector<float> pair_split(int Q, const std::vector<double>& op)
+{
+    const int H = Q == 9 ? Lattice<9>::NHALF : Lattice<19>::NHALF;
+    const int* half = Q == 9 ? Lattice<9>::half : Lattice<19>::half;
+    const int* opp = Q == 9 ? Lattice<9>::op : Lattice<19>::op;
+    std::vector<float> c(size_t((H + 1) * (H + 1) + H * H));
+    for (int r = 0; r <= H; ++r) {
+        const int q = r == 0 ? 0 : half[r - 1];
+        c[r * (H + 1)] = float(op[q * Q + 0]);
+        for (int k = 0; k < H; ++k) {
+            const int p = half[k];
+            c[r * (H + 1) + k + 1] = float(0.5 * (op[q * Q + p] + op[q * Q + opp[p]]));
+        }
+    }
+    for (int r = 1; r <= H; ++r) {
+        const int q = half[r - 1];
+        for (int k = 0; k < H; ++k) {
+            const int p = half[k];
+            c[(H + 1) * (H + 1) + (r - 1) * H + k] = float(0.5 * (op[q * Q + p] - op[q * Q + opp[p]]));
+        }
+    }
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbf_last_error(const lbf_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+const char* lbf_status_message(uint32_t bit)
+{
+    switch (bit) {
+    case LBF_FAIL_DENSITY: return "Non-finite or non-positive fluid density.";
+    case LBF_FAIL_NEGATIVE: return "Negative component density outside roundoff tolerance.";
+    case LBF_FAIL_TAU: return "Invalid local relaxation time.";
+    case LBF_FAIL_VELOCITY: return "Non-finite fluid velocity.";
+    case LBF_FAIL_POPULATION: return "Non-finite post-collision population; no restart will be written.";
+    default: return "";
+    }
+}
+
+void lbf_destroy(lbf_ctx* c)
+{
+    if (!c) return;
+    for (void* p : c->allocations) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geometry, lbf_ctx** out)
+{
+    g_create_error.clear();
+    if (!desc || !labels || !geometry || !out) return fail(nullptr, "lbf_create: null argument");
+    *out = nullptr;
+    const lbf_desc& d = *desc;
+    if (!((d.D == 2 && d.Q == 9) || (d.D == 3 && d.Q == 19)))
+        return fail(nullptr, "Only D2Q9 and D3Q19 are validated for simulation.");
+    if (d.model != LBF_MODEL_SCSP && d.model != LBF_MODEL_COLOR)
+        return fail(nullptr, "Only scsp and color fluid models are implemented.");
+    if (d.equil != LBF_SRT && d.equil != LBF_MRT) return fail(nullptr, "Equilibrium model must be SRT or MRT.");
+    if (d.precision != LBF_FP64 && d.precision != LBF_FP32) return fail(nullptr, "lbf_create: unknown precision");
+    if (d.model == LBF_MODEL_COLOR && d.precision != LBF_FP64)
+        return fail(nullptr, "lbf_create: the colour model runs in FP64 in this build");
+    const int NP = d.model == LBF_MODEL_COLOR ? 2 : 1;
+    uint64_t N = 1;
+    for (int k = 0; k < d.D; ++k) {
+        if (d.sz[k] == 0) return fail(nullptr, "Invalid grid dimensions.");
+        N *= d.sz[k];
+    }
+    if (N >= (uint64_t(1) << 32)) return fail(nullptr, "lbf_create: more than 2^32-1 sites on one device");
+    for (int k = 0; k < NP; ++k)
+        if (!std::isfinite(d.tau[k]) || d.tau[k] <= 0.5 || !std::isfinite(d.density[k]) || d.density[k] <= 0.0)
+            return fail(nullptr, "Densities must be positive; relaxation times must exceed 0.5.");
+
+    lbf_ctx* c = new lbf_ctx();
+    c->d = d;
+    c->D = d.D;
+    c->Q = d.Q;
+    c->NP = NP;
+    c->N = N;
+    c->elem = d.precision == LBF_FP32 ? 4 : 8;
+    c->labels.resize(256);
+    for (int l = 0; l < 256; ++l) {
+        c->labels[l].type = labels[l].type;
+        c->labels[l].dir = labels[l].dir;
+        c->labels[l].phase = labels[l].phase;
+        c->labels[l].value = labels[l].value;
+    }
+    // Labels used by the geometry must be supported by the kernels.
+    {
+        bool used[256] = {false};
+        for (uint64_t i = 0; i < N; ++i) used[geometry[i]] = true;
+        for (int l = 0; l < 256; ++l) {
+            if (!used[l] || !labels[l].type) continue;
+            const int t = labels[l].type;
+            if (t == LBF_PB || t == LBF_VB || t == LBF_AB || t == LBF_HF) {
+                lbf_destroy(c);
+                return fail(nullptr, "lbf_create: label " + std::to_string(l) +
+                                         " uses a boundary type (PB/VB/AB/HF) this engine does not implement");
+            }
+            if ((t == LBF_VD || t == LBF_GV) && labels[l].phase >= NP) {
+                lbf_destroy(c);
+                return fail(nullptr, "Boundary phase is outside the configured fluid model.");
+            }
+        }
+    }
+    std::string msg = build_links(d.D, d.Q, d.sz, c->labels.data(), geometry, d.model == LBF_MODEL_COLOR,
+                                  c->links);
+    if (!msg.empty()) {
+        lbf_destroy(c);
+        return fail(nullptr, msg);
+    }
+
+    // label table (simulator.cpp:218-228 forceForIndex)
+    c->hlab.assign(256, DevLabel{});
+    for (int l = 0; l < 256; ++l) {
+        DevLabel& L = c->hlab[l];
+        std::memset(&L, 0, sizeof L);
+        const LabelInfo& li = c->labels[l];
+        if (li.type == T_VD || li.type == T_GV) L.kind = KIND_FLUID;
+        L.phase = li.phase;
+        if (li.type == T_GV) {
+            if (li.dir < 0 || li.dir >= d.Q || !std::isfinite(li.value)) {
+                lbf_destroy(c);
+                return fail(nullptr, "Invalid body-force direction or magnitude.");
+            }
+            L.kind |= KIND_GV;
+            double ff = 0.0;
+            for (int k = 0; k < d.D; ++k) {
+                L.force[k] = unit_component(d.Q, li.dir, k) * li.value;
+                ff = ff + L.force[k] * L.force[k];
+            }
+            if (ff != 0.0) L.kind |= KIND_FORCED;
+            for (int k = 0; k < d.D; ++k) {
+                L.half[k] = L.force[k] * 0.5;
+                L.half_rho[k] = L.half[k] / d.density[0];
+            }
+            for (int q = 0; q < d.Q; ++q) {
+                double ef = 0.0;
+                for (int k = 0; k < d.D; ++k) {
+                    const int e = lattice_e(d.Q, q, k);
+                    if (e == 1) ef += L.force[k];
+                    else if (e == -1) ef += -L.force[k];
+                }
+                L.ef[q] = ef;
+            }
+        }
+        if (L.kind & KIND_FLUID) {
+            // count fluid sites
+        }
+    }
+    for (uint64_t i = 0; i < N; ++i)
+        if (c->hlab[geometry[i]].kind & KIND_FLUID) c->nfluid++;
+
+    build_mrt(d.Q, d.tau[0], c->ops);
+
+    // grid
+    c->g.nx = uint32_t(d.sz[d.D - 1]);
+    c->g.ny = uint32_t(d.sz[d.D - 2]);
+    c->g.nz = d.D == 3 ? uint32_t(d.sz[0]) : 1u;
+    c->g.nxy = c->g.nx * c->g.ny;
+    c->g.N = N;
+    const FastDiv fx = make_fastdiv(c->g.nx), fxy = make_fastdiv(c->g.nxy);
+    c->g.div_nx = DevFastDiv{fx.d, fx.m, fx.s};
+    c->g.div_nxy = DevFastDiv{fxy.d, fxy.m, fxy.s};
+
+    // FP32 shift per plane
+    c->shift.assign(size_t(NP) * d.Q, 0.0);
+    if (d.precision == LBF_FP32)
+        for (int q = 0; q < d.Q; ++q)
+            c->shift[q] = (d.Q == 9 ? Lattice<9>::w[q] : Lattice<19>::w[q]) * d.density[0];
+
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        lbf_destroy(c);
+        return fail(nullptr, "HIP error: cannot create a stream (no GPU?)");
+    }
+    const size_t pop = size_t(NP) * d.Q * N * c->elem;
+    const size_t Q2 = size_t(d.Q) * d.Q;
+    int rc = 0;
+    rc |= dalloc(c, &c->f[0], pop);
+    rc |= dalloc(c, &c->f[1], pop);
+    rc |= dalloc(c, &c->d_desc, N * 4);
+    rc |= dalloc(c, &c->d_lab, 256 * sizeof(DevLabel));
+    rc |= dalloc(c, &c->d_AT, Q2 * 8);
+    rc |= dalloc(c, &c->d_A, Q2 * 8);
+    rc |= dalloc(c, &c->d_B, Q2 * 8);
+    rc |= dalloc(c, &c->d_MInv, Q2 * 8);
+    rc |= dalloc(c, &c->d_A32, Q2 * 4);
+    rc |= dalloc(c, &c->d_B32, Q2 * 4);
+    rc |= dalloc(c, &c->d_rho0, N * 8);
+    rc |= dalloc(c, &c->d_u, size_t(d.D) * N * 8);
+    rc |= dalloc(c, &c->d_uadj, size_t(d.D) * N * 8);
+    rc |= dalloc(c, &c->d_uprev, size_t(d.D) * N * 8);
+    rc |= dalloc(c, &c->d_status, sizeof(DevStatus));
+    rc |= dalloc(c, &c->d_partial, size_t(report_blocks(N)) * NREP * 8);
+    rc |= dalloc(c, &c->d_out, NREP * 8);
+    if (NP == 2) {
+        rc |= dalloc(c, &c->d_rho1, N * 8);
+        rc |= dalloc(c, &c->d_chiadj, N * 8);
+        rc |= dalloc(c, &c->d_chiprev, N * 8);
+        rc |= dalloc(c, &c->d_densN, N * 8);
+        rc |= dalloc(c, &c->d_wall, N * 4);
+        rc |= dalloc(c, &c->d_walls, c->links.nwall * sizeof(DevWall));
+    }
+    if (rc) {
+        std::string e = c->err;
+        lbf_destroy(c);
+        return fail(nullptr, e);
+    }
+    rc |= upload(c, c->d_desc, c->links.desc.data(), N * 4);
+    rc |= upload(c, c->d_lab, c->hlab.data(), 256 * sizeof(DevLabel));
+    rc |= upload(c, c->d_AT, c->ops.AT.data(), Q2 * 8);
+    rc |= upload(c, c->d_A, c->ops.A.data(), Q2 * 8);
+    rc |= upload(c, c->d_B, c->ops.B.data(), Q2 * 8);
+    rc |= upload(c, c->d_MInv, c->ops.MInv.data(), Q2 * 8);
+    {
+        std::vector<float> a32 = pair_split(d.Q, c->ops.A), b32 = pair_split(d.Q, c->ops.B);
+        rc |= upload(c, c->d_A32, a32.data(), a32.size() * 4);
+        rc |= upload(c, c->d_B32, b32.data(), b32.size() * 4);
+    }
+    if (NP == 2) {
+        std::vector<double> dn(N, 86.0);
+        rc |= upload(c, c->d_densN, dn.data(), N * 8);
+        rc |= upload(c, c->d_wall, c->links.wall.data(), N * 4);
+        std::vector<DevWall> walls(c->links.nwall);
+        const int DD = d.D * d.D;
+        for (uint64_t w = 0; w < c->links.nwall; ++w) {
+            std::memset(&walls[w], 0, sizeof(DevWall));
+            walls[w].sinca = c->links.sinca[w];
+            walls[w].cosca = c->links.cosca[w];
+            for (int k = 0; k < DD; ++k) {
+                walls[w].tib[k] = c->links.tib[w * DD + k];
+                walls[w].tibinv[k] = c->links.tibinv[w * DD + k];
+            }
+        }
+        rc |= upload(c, c->d_walls, walls.data(), walls.size() * sizeof(DevWall));
+    }
+    if (rc) {
+        std::string e = c->err;
+        lbf_destroy(c);
+        return fail(nullptr, e);
+    }
+    for (void* p : {c->f[0], c->f[1]}) (void)hipMemset(p, 0, pop);
+    (void)hipMemset(c->d_rho0, 0, N * 8);
+    (void)hipMemset(c->d_u, 0, size_t(d.D) * N * 8);
+    (void)hipMemset(c->d_uadj, 0, size_t(d.D) * N * 8);
+    (void)hipMemset(c->d_uprev, 0, size_t(d.D) * N * 8);
+    if (NP == 2) {
+        (void)hipMemset(c->d_rho1, 0, N * 8);
+        (void)hipMemset(c->d_chiadj, 0, N * 8);
+        (void)hipMemset(c->d_chiprev, 0, N * 8);
+    }
+    DevStatus st{};
+    for (auto& v : st.first) v = ~0ull;
+    (void)hipMemcpy(c->d_status, &st, sizeof st, hipMemcpyHostToDevice);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        lbf_destroy(c);
+        return fail(nullptr, "HIP error during context set-up");
+    }
+    if (NP == 2) c->kernel_name = "color_step";
+    else c->kernel_name = d.precision == LBF_FP32 ? "scsp_step_f32" : "scsp_step_f64";
+    *out = c;
+    return 0;
+}
+
+static void reset_status(lbf_ctx* c)
+{
+    DevStatus st{};
+    for (auto& v : st.first) v = ~0ull;
+    (void)hipMemcpyAsync(c->d_status, &st, sizeof st, hipMemcpyHostToDevice, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+}
+
+static ScspArgs scsp_args(lbf_ctx* c, uint32_t flags)
+{
+    ScspArgs a{};
+    a.g = c->g;
+    a.desc = c->d_desc;
+    a.lab = c->d_lab;
+    a.AT = c->d_AT;
+    a.B = c->d_B;
+    a.A32 = c->d_A32;
+    a.B32 = c->d_B32;
+    a.rho = c->d_rho0;
+    a.u = c->d_u;
+    a.uadj = c->d_uadj;
+    a.status = c->d_status;
+    a.flags = flags;
+    a.dens0 = c->d.density[0];
+    a.omega = 1.0 / c->d.tau[0];
+    a.omh = 1.0 - 0.5 * a.omega;
+    for (int q = 0; q < c->Q; ++q) a.idist[q] = (c->Q == 9 ? Lattice<9>::w[q] : Lattice<19>::w[q]) * c->d.density[0];
+    return a;
+}
+
+static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
+{
+    ColorArgs a{};
+    a.g = c->g;
+    a.desc = c->d_desc;
+    a.lab = c->d_lab;
+    a.wall = c->d_wall;
+    a.walls = c->d_walls;
+    a.densN = c->d_densN;
+    a.A = c->d_A;
+    a.B = c->d_B;
+    a.MInv = c->d_MInv;
+    a.rho0 = c->d_rho0;
+    a.rho1 = c->d_rho1;
+    a.u = c->d_u;
+    a.uadj = c->d_uadj;
+    a.chiadj = c->d_chiadj;
+    a.status = c->d_status;
+    a.flags = flags;
+    a.mrt = c->d.equil == LBF_MRT;
+    a.uniform = c->d.tau[0] == c->d.tau[1];
+    a.harmonic = c->d.harmonic;
+    a.relax0 = c->d.tau[0] / 2.0;
+    a.relax1 = c->d.tau[1] / 2.0;
+    a.nu0 = (c->d.tau[0] - 0.5) / 3.0;
+    a.nu1 = (c->d.tau[1] - 0.5) / 3.0;
+    a.ak = 2.25 * c->d.sigma;
+    a.beta = c->d.beta;
+    // initPhi / getBi (colorfluid.cpp:1131-1168)
+    double alpha[2];
+    alpha[0] = c->d.alpha0;
+    alpha[1] = 1 - c->d.density[0] / c->d.density[1] * (1 - alpha[0]);
+    for (int q = 0; q < c->Q; ++q) {
+        for (int ph = 0; ph < 2; ++ph) {
+            double v;
+            if (c->Q == 9) v = q == 0 ? alpha[ph] : (q <= 4 ? (1 - alpha[ph]) / 5 : (1 - alpha[ph]) / 20);
+            else v = q == 0 ? alpha[ph] : (q <= 6 ? (1 - alpha[ph]) / 12 : (1 - alpha[ph]) / 24);
+            (ph == 0 ? a.ai0 : a.ai1)[q] = v;
+            (ph == 0 ? a.dist0 : a.dist1)[q] = v * c->d.density[ph];
+        }
+        if (c->Q == 9) {
+            a.bi[q] = q == 0 ? -4.0 / 27 : (q <= 4 ? 2.0 / 27 : 5.0 / 108);
+        } else {
+            const double x = 2, c2 = 1.0;
+            const double b0 = -(2.0 + 2.0 * x) / (3.0 * x + 12.0) * c2;
+            const double b1 = x / (6.0 * x + 24.0) * c2;
+            const double b7 = 1.0 / (6.0 * x + 24.0) * c2;
+            a.bi[q] = q == 0 ? b0 : (q <= 6 ? b1 : b7);
+        }
+    }
+    return a;
+}
+
+static int run_step(lbf_ctx* c, bool init, uint32_t flags)
+{
+    const bool stream_solids = c->d.flags & LBF_STREAM_SOLIDS;
+    const bool mrt = c->d.equil == LBF_MRT;
+    hipError_t e;
+    if (c->NP == 1) {
+        ScspArgs a = scsp_args(c, flags);
+        a.t = init ? 0 : c->t + 1;
+        a.fin = c->f[c->cur];
+        a.fout = init ? c->f[c->cur] : c->f[c->cur ^ 1];
+        e = launch_scsp(c->Q, mrt, init, c->d.precision == LBF_FP32, stream_solids, a, c->stream);
+    } else {
+        ColorArgs a = color_args(c, flags);
+        a.t = init ? 0 : c->t + 1;
+        a.fin = static_cast<const double*>(c->f[c->cur]);
+        a.fout = static_cast<double*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
+        e = launch_color(c->Q, mrt, init, stream_solids, a, c->stream);
+    }
+    if (e != hipSuccess) return fail(c, std::string("HIP error: kernel launch: ") + hipGetErrorString(e));
+    if (!init) {
+        c->cur ^= 1;
+        c->t++;
+    }
+    return 0;
+}
+
+int lbf_init(lbf_ctx* c)
+{
+    if (!c) return -1;
+    reset_status(c);
+    c->t = 0;
+    c->has_prev = false;
+    const size_t pop = size_t(c->NP) * c->Q * c->N * c->elem;
+    HIP_CHECK(c, hipMemsetAsync(c->f[c->cur], 0, pop, c->stream));
+    if (run_step(c, true, 1u)) return -1;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lbf_step(lbf_ctx* c, uint32_t flags)
+{
+    if (!c) return -1;
+    if (run_step(c, false, flags & 3u)) return -1;
+    return 0;
+}
+
+int lbf_advance(lbf_ctx* c, uint64_t n)
+{
+    if (!c) return -1;
+    for (uint64_t k = 0; k < n; ++k)
+        if (run_step(c, false, 0u)) return -1;
+    return 0;
+}
+
+uint64_t lbf_time(const lbf_ctx* c) { return c ? c->t : 0; }
+
+int lbf_synchronize(lbf_ctx* c)
+{
+    if (!c) return -1;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lbf_status(lbf_ctx* c, uint32_t* bits, uint64_t* first)
+{
+    if (!c) return -1;
+    DevStatus st{};
+    HIP_CHECK(c, hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (bits) *bits = st.bits;
+    if (first)
+        for (int k = 0; k < 5; ++k) first[k] = st.first[k];
+    return 0;
+}
+
+int lbf_fields(lbf_ctx* c, double* rho0, double* rho1, double* u)
+{
+    if (!c) return -1;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    if (rho0) HIP_CHECK(c, hipMemcpy(rho0, c->d_rho0, c->N * 8, hipMemcpyDeviceToHost));
+    if (rho1 && c->NP == 2) HIP_CHECK(c, hipMemcpy(rho1, c->d_rho1, c->N * 8, hipMemcpyDeviceToHost));
+    if (u) HIP_CHECK(c, hipMemcpy(u, c->d_u, size_t(c->D) * c->N * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int lbf_get_populations(lbf_ctx* c, double* out)
+{
+    if (!c || !out) return -1;
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    const size_t n = size_t(c->NP) * c->Q * c->N;
+    if (c->elem == 8) {
+        HIP_CHECK(c, hipMemcpy(out, c->f[c->cur], n * 8, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<float> h(n);
+    HIP_CHECK(c, hipMemcpy(h.data(), c->f[c->cur], n * 4, hipMemcpyDeviceToHost));
+    const std::vector<uint32_t>& desc = c->links.desc;
+    const bool stream_solids = c->d.flags & LBF_STREAM_SOLIDS;
+    for (size_t plane = 0; plane < size_t(c->NP) * c->Q; ++plane)
+        for (uint64_t i = 0; i < c->N; ++i) {
+            const bool fluid = c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID;
+            // never-written solid entries hold 0 and are reported as 0
+            const float v = h[plane * c->N + i];
+            out[plane * c->N + i] = (fluid || stream_solids || v != 0.0f) ? double(v) + c->shift[plane] : 0.0;
+        }
+    return 0;
+}
+
+int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
+{
+    if (!c || !in) return -1;
+    const size_t n = size_t(c->NP) * c->Q * c->N;
+    if (c->elem == 8) {
+        HIP_CHECK(c, hipMemcpy(c->f[c->cur], in, n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h(n);
+        for (size_t plane = 0; plane < size_t(c->NP) * c->Q; ++plane)
+            for (uint64_t i = 0; i < c->N; ++i)
+                h[plane * c->N + i] = float(in[plane * c->N + i] - c->shift[plane]);
+        HIP_CHECK(c, hipMemcpy(c->f[c->cur], h.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    // A restarted run holds no macro history (Simulator::validateRestart resets it,
+    // simulator.cpp:382-383; the macro arrays are fresh zeros until the first step).
+    HIP_CHECK(c, hipMemset(c->d_rho0, 0, c->N * 8));
+    HIP_CHECK(c, hipMemset(c->d_u, 0, size_t(c->D) * c->N * 8));
+    HIP_CHECK(c, hipMemset(c->d_uadj, 0, size_t(c->D) * c->N * 8));
+    if (c->NP == 2) {
+        HIP_CHECK(c, hipMemset(c->d_rho1, 0, c->N * 8));
+        HIP_CHECK(c, hipMemset(c->d_chiadj, 0, c->N * 8));
+    }
+    reset_status(c);
+    c->t = t;
+    c->has_prev = false;
+    return 0;
+}
+
+int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
+{
+    if (!c || !req || !out) return -1;
+    if (req->axis < 0 || req->axis >= c->D) return fail(c, "lbf_report: axis outside the lattice");
+    ReportArgs a{};
+    a.g = c->g;
+    a.desc = c->d_desc;
+    a.lab = c->d_lab;
+    a.rho0 = c->d_rho0;
+    a.rho1 = c->NP == 2 ? c->d_rho1 : c->d_rho0;
+    a.u = c->d_u;
+    a.uprev = c->d_uprev;
+    a.uadj = c->d_uadj;
+    a.chiprev = c->d_chiprev;
+    a.chiadj = c->d_chiadj;
+    a.partial = c->d_partial;
+    a.D = c->D;
+    a.axis = req->axis;
+    a.color = c->NP == 2;
+    a.conv = req->convergence ? 1 : 0;
+    a.adjacent = req->adjacent ? 1 : 0;
+    a.first = uint32_t(req->first);
+    a.last = uint32_t(req->last);
+    HIP_CHECK(c, launch_report(a, c->d_out, c->stream));
+    double r[NREP];
+    HIP_CHECK(c, hipMemcpyAsync(r, c->d_out, sizeof r, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    std::memset(out, 0, sizeof *out);
+    out->vel_range = r[0];
+    out->rho_range[0] = r[1];
+    out->rho_range[1] = r[2];
+    out->rho_slice[0][0] = r[3];
+    out->rho_slice[0][1] = r[4];
+    out->rho_slice[1][0] = r[5];
+    out->rho_slice[1][1] = r[6];
+    out->count_slice[0] = r[7];
+    out->count_slice[1] = r[8];
+    for (int k = 0; k < 4; ++k) out->conv[k] = r[9 + k];
+    out->phase_change[0] = r[13];
+    out->phase_change[1] = r[14];
+    out->nonfinite = r[15] != 0.0;
+    out->has_previous = req->convergence ? (c->has_prev ? 1 : 0) : 0;
+    if (req->convergence) c->has_prev = true;
+    return 0;
+}
+
+uint64_t lbf_links(lbf_ctx* c, uint32_t* linkmask, int32_t* wall, double* normal, double* sinca, double* cosca)
+{
+    if (!c) return 0;
+    const Links& L = c->links;
+    if (linkmask)
+        for (uint64_t i = 0; i < c->N; ++i) linkmask[i] = L.desc[i] & DESC_LINK_MASK;
+    if (wall) std::memcpy(wall, L.wall.data(), c->N * 4);
+    if (normal) std::memcpy(normal, L.normal.data(), L.normal.size() * 8);
+    if (sinca) std::memcpy(sinca, L.sinca.data(), L.sinca.size() * 8);
+    if (cosca) std::memcpy(cosca, L.cosca.data(), L.cosca.size() * 8);
+    return L.nwall;
+}
+
+uint64_t lbf_sites(const lbf_ctx* c) { return c ? c->N : 0; }
+uint64_t lbf_device_bytes(const lbf_ctx* c) { return c ? c->dev_bytes : 0; }
+const char* lbf_step_kernel(const lbf_ctx* c) { return c ? c->kernel_name.c_str() : ""; }
+
+double lbf_step_bytes(const lbf_ctx* c)
+{
+    if (!c) return 0.0;
+    const bool stream_solids = c->d.flags & LBF_STREAM_SOLIDS;
+    const double sites = stream_solids ? double(c->N) : double(c->nfluid);
+    // every streamed site reads and writes Q populations per phase; every site reads its descriptor
+    return sites * 2.0 * c->NP * c->Q * double(c->elem) + double(c->N) * 4.0;
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// device.h -- structures shared by the HIP kernels and their host launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lattice.h"
+
+namespace lbf {
+
+// Site kinds of the label table.
+enum : int32_t {
+    KIND_FLUID = 0x1,    // macro + collision site (VD, GV)
+    KIND_FORCED = 0x2,   // GV with a non-zero force vector
+    KIND_GV = 0x4,       // GV: body force shifts the velocity
+};
+
+// One entry of the 256-entry device label table.
+struct DevLabel {
+    int32_t kind;
+    int32_t phase;
+    double force[3];     // Qp[dir] * value, reference component order
+    double half_rho[3];  // (force * 0.5) / rho0           (scsp body force)
+    double half[3];      // force * 0.5                     (color body force)
+    double ef[19];       // e_q . force (Guo source; initial shift)
+};
+
+// Per wall entry of the colour model: sin/cos of the mean contact angle and
+// the rotations TibNorm / TibNormInv (colorfluid.cpp:1111-1129).
+struct DevWall {
+    double sinca, cosca;
+    double tib[9];
+    double tibinv[9];
+};
+
+// Device failure record.
+struct DevStatus {
+    uint32_t bits;
+    uint32_t pad;
+    unsigned long long first[5];   // first step per bit (LBF_FAIL_*), ~0 = never
+};
+
+struct DevFastDiv {
+    uint32_t d, m, s;
+};
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, DevFastDiv f)
+{
+    const uint32_t t = __umulhi(n, f.m);
+    return uint32_t((uint64_t(t) + n) >> f.s);
+}
+
+struct GridDev {
+    uint32_t nx, ny, nz, nxy;
+    uint64_t N;
+    DevFastDiv div_nx, div_nxy;
+};
+
+__device__ __forceinline__ void site_coords(const GridDev& g, uint32_t i, int& x, int& y, int& z)
+{
+    const uint32_t zz = fdiv(i, g.div_nxy);
+    const uint32_t r = i - zz * g.nxy;
+    const uint32_t yy = fdiv(r, g.div_nx);
+    z = int(zz);
+    y = int(yy);
+    x = int(r - yy * g.nx);
+}
+
+__device__ __forceinline__ int wrap(int c, int n)
+{
+    return c < 0 ? c + n : (c >= n ? c - n : c);
+}
+
+// Pull source of direction q at (x,y,z) for lattice Q (components slowest first).
+template <int Q>
+__device__ __forceinline__ uint32_t pull_source(const GridDev& g, int x, int y, int z, int q)
+{
+    using L = Lattice<Q>;
+    int ex, ey, ez;
+    if constexpr (Q == 9) {
+        ez = 0; ey = L::e[q][0]; ex = L::e[q][1];
+    } else {
+        ez = L::e[q][0]; ey = L::e[q][1]; ex = L::e[q][2];
+    }
+    const int xs = ex ? wrap(x - ex, int(g.nx)) : x;
+    const int ys = ey ? wrap(y - ey, int(g.ny)) : y;
+    const int zs = ez ? wrap(z - ez, int(g.nz)) : z;
+    return uint32_t(zs) * g.nxy + uint32_t(ys) * g.nx + uint32_t(xs);
+}
+
+// Neighbour x + e_q (gradient stencil).
+template <int Q>
+__device__ __forceinline__ uint32_t push_target(const GridDev& g, int x, int y, int z, int q)
+{
+    using L = Lattice<Q>;
+    int ex, ey, ez;
+    if constexpr (Q == 9) {
+        ez = 0; ey = L::e[q][0]; ex = L::e[q][1];
+    } else {
+        ez = L::e[q][0]; ey = L::e[q][1]; ex = L::e[q][2];
+    }
+    const int xs = ex ? wrap(x + ex, int(g.nx)) : x;
+    const int ys = ey ? wrap(y + ey, int(g.ny)) : y;
+    const int zs = ez ? wrap(z + ez, int(g.nz)) : z;
+    return uint32_t(zs) * g.nxy + uint32_t(ys) * g.nx + uint32_t(xs);
+}
+
+__device__ __forceinline__ void raise_status(DevStatus* st, int bit, unsigned long long t)
+{
+    atomicOr(&st->bits, 1u << bit);
+    atomicMin(&st->first[bit], t);
+}
+
+// Arguments of the single-phase step kernels.
+struct ScspArgs {
+    GridDev g;
+    const void* fin;
+    void* fout;
+    const uint32_t* desc;
+    const DevLabel* lab;
+    const double* AT;        // FP64: A transposed (MInvSMT); FP32: pair-split A
+    const double* B;         // FP64: force matrix; FP32: pair-split B
+    const float* A32;
+    const float* B32;
+    double* rho;             // fields: density
+    double* u;               // fields: velocity [d][site]
+    double* uadj;            // adjacent velocity history
+    DevStatus* status;
+    unsigned long long t;
+    uint32_t flags;          // bit0 fields, bit1 adjacent
+    double dens0, omega, omh; // omh = 1 - 0.5 omega
+    double idist[19];        // w_q * rho0
+};
+
+struct ColorArgs {
+    GridDev g;
+    const double* fin;       // [phase][q][site]
+    double* fout;
+    const uint32_t* desc;
+    const DevLabel* lab;
+    const int32_t* wall;
+    const DevWall* walls;
+    double* densN;           // colour indicator with solid sentinel
+    const double* A;         // uniform-tau collision matrix (row-major)
+    const double* B;         // uniform-tau force matrix
+    const double* MInv;      // inverse transform (row-major)
+    double* rho0;
+    double* rho1;
+    double* u;
+    double* uadj;
+    double* chiadj;
+    DevStatus* status;
+    unsigned long long t;
+    uint32_t flags;
+    int32_t mrt, uniform, harmonic;
+    double relax0, relax1;   // tau_k / 2
+    double nu0, nu1;         // (tau_k - 0.5) / 3
+    double ak, beta;
+    double ai0[19], ai1[19], bi[19];
+    double dist0[19], dist1[19];   // ai_k * rho_k
+};
+
+struct ReportArgs {
+    GridDev g;
+    const uint32_t* desc;
+    const DevLabel* lab;
+    const double* rho0;
+    const double* rho1;
+    const double* u;
+    double* uprev;
+    const double* uadj;
+    double* chiprev;
+    const double* chiadj;
+    double* partial;         // [blocks][NREP]
+    int32_t D, axis, color, conv, adjacent, space_mask_unused;
+    uint32_t first, last;
+};
+constexpr int NREP = 16;
+
+}  // namespace lbf

@@ -1,0 +1,62 @@
+"""Deterministic synthetic porous media for the LBFlow benchmark cases.
+
+Periodic overlapping-sphere packs: sphere centres uniform in the box, radii
+uniform in [rmin, rmax], spheres added one at a time until the pore fraction
+drops to the target porosity; distances use the periodic minimum image and
+voxel centres at integer + 0.5.  Output is a uint8 label volume in the
+reference's raw layout (x fastest, then y, then z; solver/src/file/fileraw.h),
+pore = 75 and solid = 255 by default as in the reference's examples.
+"""
+import hashlib
+
+import numpy as np
+
+
+def sphere_pack(shape_zyx, porosity, rmin, rmax, seed, pore=75, solid=255):
+    nz, ny, nx = (int(n) for n in shape_zyx)
+    rng = np.random.default_rng(seed)
+    mask = np.zeros((nz, ny, nx), dtype=bool)
+    total = nz * ny * nx
+    nsolid = 0
+    target = (1.0 - porosity) * total
+    while nsolid < target:
+        c = rng.uniform(0.0, 1.0, 3) * (nz, ny, nx)
+        r = rng.uniform(rmin, rmax)
+        lo = np.floor(c - r - 0.5).astype(int)
+        hi = np.ceil(c + r - 0.5).astype(int)
+        iz = np.arange(lo[0], hi[0] + 1)
+        iy = np.arange(lo[1], hi[1] + 1)
+        ix = np.arange(lo[2], hi[2] + 1)
+        dz = (iz + 0.5 - c[0])[:, None, None]
+        dy = (iy + 0.5 - c[1])[None, :, None]
+        dx = (ix + 0.5 - c[2])[None, None, :]
+        inside = dz * dz + dy * dy + dx * dx <= r * r
+        sub = np.ix_(iz % nz, iy % ny, ix % nx)
+        block = mask[sub]
+        nsolid += int(np.count_nonzero(inside & ~block))
+        mask[sub] = block | inside
+    labels = np.full((nz, ny, nx), pore, dtype=np.uint8)
+    labels[mask] = solid
+    return labels
+
+
+def porosity(labels, pore_values=(75,)):
+    return float(np.isin(labels, pore_values).mean())
+
+
+def sha256(labels):
+    return hashlib.sha256(np.ascontiguousarray(labels, dtype=np.uint8).tobytes()).hexdigest()
+
+
+def random_blocks(labels, block, fraction, seed, phase0=75, phase1=76, pore=75):
+    """Two-phase initial state: pore voxels get phase0/phase1 in random cubic blocks."""
+    nz, ny, nx = labels.shape
+    rng = np.random.default_rng(seed)
+    bz, by, bx = -(-nz // block), -(-ny // block), -(-nx // block)
+    choice = rng.uniform(size=(bz, by, bx)) < fraction
+    grid = np.repeat(np.repeat(np.repeat(choice, block, 0), block, 1), block, 2)[:nz, :ny, :nx]
+    out = labels.copy()
+    poremask = labels == pore
+    out[poremask & grid] = phase0
+    out[poremask & ~grid] = phase1
+    return out

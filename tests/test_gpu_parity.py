@@ -1,0 +1,177 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+FP64 mode keeps the reference's operation order, so populations, macro
+fields and link tables must be bit-identical to the oracle's.  FP32 mode is
+checked against FP64 with the tolerances written in each test.
+"""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+from lbflow_amd import harness as H
+from lbflow_amd.engine import FP32, FP64, STEP_ADJACENT, STEP_FIELDS, Engine
+from tests.helpers import C1_SHA256, c1_spec, darcy, oracle_run, profile, sphere_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def engine_run(spec, steps, precision=FP64, stream_solids=True, geometry=None):
+    case = H.lattice_case(spec)
+    e = Engine(case, H.geometry(spec) if geometry is None else geometry, precision, stream_solids).init()
+    for k in range(steps):
+        e.step(STEP_FIELDS if k == steps - 1 else 0)
+    e.synchronize()
+    return e, case
+
+
+def assert_same(eng, orc):
+    pe, po = eng.populations(), orc.populations()
+    assert pe.shape == po.shape
+    diff = np.flatnonzero(pe.view(np.uint64) != po.view(np.uint64))
+    assert diff.size == 0, f'{diff.size} populations differ, first {diff[:5]}: ' \
+                           f'{pe[diff[:3]]} vs {po[diff[:3]]}'
+    fe, fo = eng.fields(), orc.fields()
+    for a, b in zip(fe, fo):
+        if a is not None:
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize('spec', H.force_cases(), ids=lambda s: s['name'])
+def test_force_cases_bitwise(spec):
+    say('case', spec['name'])
+    e, case = engine_run(spec, spec['steps'])
+    o, _ = oracle_run(spec, spec['steps'])
+    assert_same(e, o)
+    e.check_status()
+
+
+@pytest.mark.parametrize('spec', [s for s in H.channel_cases() if s['height'] == 8], ids=lambda s: s['name'])
+def test_channel_bitwise(spec, records):
+    say('case', spec['name'])
+    e, case = engine_run(spec, spec['steps'])
+    got = profile(e.fields()[2], case)
+    want = np.array(records['channel'][spec['name'] + '-1rank']['measured_profile'])
+    assert np.array_equal(got, want)
+    o, _ = oracle_run(spec, spec['steps'])
+    assert_same(e, o)
+
+
+@pytest.mark.parametrize('spec', [s for s in H.layered_cases() if s['height'] == 16], ids=lambda s: s['name'])
+def test_layered_bitwise(spec, records):
+    say('case', spec['name'])
+    e, case = engine_run(spec, spec['steps'])
+    got = profile(e.fields()[2], case)
+    want = np.array(records['layered'][spec['name'] + '-1rank']['profile'])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize('spec', H.bundle_cases(scales=(1,)), ids=lambda s: s['name'])
+def test_bundle_bitwise(spec, records):
+    say('case', spec['name'])
+    rec = records['bundle'][spec['name'] + '-1rank']['history'][1]
+    e, case = engine_run(spec, rec['step'])
+    assert darcy(e.fields()[2], case) == rec['darcy_velocity']
+
+
+PACKS = [
+    dict(model='scsp', eq='SRT', tau=(0.8,)),
+    dict(model='scsp', eq='MRT', tau=(0.8,)),
+    dict(model='scsp', eq='MRT', tau=(1.3,), dims=2),
+    dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True),
+    dict(model='color', eq='MRT', tau=(0.7, 1.2), two_phase=True),
+    dict(model='color', eq='SRT', tau=(1.0, 0.8), two_phase=True, angle=120),
+    dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, dims=2, angle=45),
+]
+
+
+@pytest.mark.parametrize('kw', PACKS, ids=lambda k: '-'.join(str(v) for v in k.values()))
+def test_sphere_pack_bitwise(kw):
+    """Curved walls: 3D contact-angle rotation, link masks, solids streamed like the reference."""
+    spec = sphere_case(**kw)
+    say('case', spec['name'], kw)
+    steps = 150
+    e, case = engine_run(spec, steps)
+    o, _ = oracle_run(spec, steps)
+    m_e, w_e, n_e, s_e, c_e = e.links()
+    m_o, w_o, n_o, s_o, c_o = o.links()
+    assert np.array_equal(m_e, m_o) and np.array_equal(w_e, w_o)
+    assert np.array_equal(n_e, n_o) and np.array_equal(s_e, s_o) and np.array_equal(c_e, c_o)
+    assert_same(e, o)
+    e.check_status()
+
+
+def test_skip_solids_matches_fluid_populations():
+    """Without solid streaming the fluid sites evolve identically; solid entries read 0."""
+    spec = sphere_case(model='scsp', eq='MRT', tau=(0.8,))
+    a, case = engine_run(spec, 60, stream_solids=True)
+    b, _ = engine_run(spec, 60, stream_solids=False)
+    g = H.geometry(spec)
+    fluid = np.tile(g == 75, case['Q'])
+    pa, pb = a.populations(), b.populations()
+    assert np.array_equal(pa[fluid], pb[fluid])
+    assert np.all(pb[~fluid] == 0.0)
+
+
+def test_restart_equivalence():
+    """Continuous run == checkpoint + resume (ci_smoke.py:42-56)."""
+    spec = sphere_case(model='color', eq='MRT', tau=(0.7, 1.2), two_phase=True)
+    cont, case = engine_run(spec, 80)
+    half, _ = engine_run(spec, 40)
+    pops = half.populations()
+    res = Engine(case, H.geometry(spec), FP64, True)
+    res.set_populations(pops, 40)
+    for k in range(40):
+        res.step(STEP_FIELDS if k == 39 else 0)
+    assert res.t == 80
+    assert np.array_equal(res.populations(), cont.populations())
+
+
+def test_c1_checkpoint_digest_gpu():
+    """The 8x10x4 SRT channel after 20000 GPU steps has the reference's checkpoint bytes."""
+    spec = c1_spec()
+    e, case = engine_run(spec, 20000)
+    digest = hashlib.sha256(e.populations().astype('<f8').tobytes()).hexdigest()
+    assert digest == C1_SHA256
+
+
+def test_report_sums_match_oracle():
+    spec = sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True)
+    e, case = engine_run(spec, 30)
+    o, _ = oracle_run(spec, 30)
+    axis = 2
+    r = e.report(axis, 1, case['size'][axis] - 2, convergence=True, adjacent=False)
+    assert r.has_previous == 0
+    ref = o.report_sums(axis, 1, case['size'][axis] - 2)
+    got = [r.vel_range, r.rho_range[0], r.rho_range[1], r.rho_slice[0][0], r.rho_slice[0][1],
+           r.rho_slice[1][0], r.rho_slice[1][1], r.count_slice[0], r.count_slice[1]]
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-18)
+    # second report: convergence sums against the stored previous velocity
+    prev = o.fields()[2].ravel().copy()
+    e.step(STEP_ADJACENT)
+    o.step(1)
+    adj = o.fields()[2].ravel().copy()
+    e.step(STEP_FIELDS)
+    o.step(1)
+    r = e.report(axis, 1, case['size'][axis] - 2, convergence=True, adjacent=True)
+    assert r.has_previous == 1
+    ref = o.convergence_sums(prev, adj)
+    np.testing.assert_allclose(list(r.conv), ref, rtol=1e-12, atol=1e-30)
+
+
+def test_fp32_tracks_fp64():
+    """FP32 shifted populations: velocity field within 1e-5 relative (L2) of FP64 after 300 steps."""
+    spec = sphere_case(n=(16, 16, 16), model='scsp', eq='MRT', tau=(0.8,), porosity=0.4)
+    a, case = engine_run(spec, 300, FP64)
+    b, _ = engine_run(spec, 300, FP32)
+    ua, ub = a.fields()[2], b.fields()[2]
+    rel = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
+    say('fp32 rel', rel)
+    assert rel < 1e-5
+    ra, rb = a.fields()[0], b.fields()[0]
+    assert np.max(np.abs(ra - rb)) < 1e-6

@@ -1,0 +1,127 @@
+"""Pin the CPU oracle to the reference's own recorded results (CPU only).
+
+Every expected value below was produced by the reference solver and is
+stored in the reference tree (research/drlab-publication/paper/data/*.json,
+extracted into tests/golden/reference_records.json by make_golden.py), except
+the 8x10x4 checkpoint digest, which was recorded with the reference binary in
+the survey of the reference.  Comparisons are exact (==): the oracle keeps
+the reference's floating-point operation order.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from lbflow_amd import harness as H
+from tests.helpers import C1_SHA256, c1_spec, darcy, oracle_run, profile, velocity_grid
+
+
+@pytest.mark.parametrize('spec', H.force_cases(), ids=lambda s: s['name'])
+def test_uniform_force(spec, records):
+    """u(t) = t F after 100 steps; recorded max errors reproduced exactly (benchmarks.py:176-188)."""
+    o, case = oracle_run(spec, spec['steps'])
+    v = velocity_grid(o.fields()[2], case)
+    err = float(np.max(np.abs(v[..., 0] - 1e-4)))
+    transverse = float(np.max(np.abs(v[..., 1:])))
+    rec = records['force'][spec['name'] + '-1rank']
+    assert err == rec['absolute_error']
+    assert transverse == rec['transverse_max']
+    assert err <= 1e-12
+
+
+CHANNELS = [s for s in H.channel_cases() if s['height'] in (8, 16)]
+
+
+@pytest.mark.parametrize('spec', CHANNELS, ids=lambda s: s['name'])
+def test_channel_profile(spec, records):
+    """Poiseuille channels: recorded measured_profile reproduced exactly (benchmarks.py:191-242)."""
+    o, case = oracle_run(spec, spec['steps'])
+    got = profile(o.fields()[2], case)
+    want = np.array(records['channel'][spec['name'] + '-1rank']['measured_profile'])
+    assert np.array_equal(got, want)
+
+
+def _layered(interp):
+    return [s for s in H.layered_cases(interp) if s['height'] == 16]
+
+
+@pytest.mark.parametrize('spec', _layered('arithmetic') + _layered('harmonic'), ids=lambda s: s['name'])
+def test_layered_two_phase(spec, records):
+    """Two-phase D2Q9 channel with a flat interface and 90-degree walls (layered_flow.py:10-23):
+    perturbation, recolouring, wall rotation and both viscosity interpolations."""
+    o, case = oracle_run(spec, spec['steps'])
+    got = profile(o.fields()[2], case)
+    key = spec['name'].replace('-harmonic', '') + ('-harmonic' if case['harmonic'] else '') + '-1rank'
+    want = np.array(records['layered'][key]['profile'])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize('spec', H.bundle_cases(scales=(1,)), ids=lambda s: s['name'])
+def test_bundle_history(spec, records):
+    """Slit bundles (porous_media.py:67-151): Darcy velocity and total mass at every output step."""
+    rec = records['bundle'][spec['name'] + '-1rank']['history']
+    o, case = oracle_run(spec, 0)
+    t = 0
+    for h in rec[1:]:
+        o.step(h['step'] - t)
+        t = h['step']
+        r0, r1, u = o.fields()
+        assert darcy(u, case) == h['darcy_velocity']
+        rho = r0 if r1 is None else r0 + r1
+        assert float(rho.sum()) == h['total_mass']
+
+
+def test_c1_checkpoint_digest():
+    """8x10x4 SRT channel, 20000 steps: checkpoint bytes equal the reference's (solid entries included)."""
+    s = c1_spec()
+    o, case = oracle_run(s, 20000)
+    digest = hashlib.sha256(o.populations().astype('<f8').tobytes()).hexdigest()
+    assert digest == C1_SHA256
+
+
+@pytest.mark.slow
+def test_bentheimer48_history(records):
+    """Bentheimer48 crop, D3Q19 MRT, tau 1, F 1e-6 (rock_workflow.py:19-39): step-1000 record."""
+    from tests.conftest import GOLDEN
+    g = np.fromfile(GOLDEN / 'bentheimer48_single_phase.raw', dtype=np.uint8)
+    rec = records['rock_scsp']['rock-scsp-F1e-06-2rank']
+    assert hashlib.sha256(g.tobytes()).hexdigest() == rec['geometry_sha256']
+    s = H.spec('rock', nx=48, ny=48, nz=48, model='scsp', tau=(1.0,), geometry=g,
+               indices=['75 "GV" "1e-06" "1" "0"', '255 "SD" "60" "N/A" "N/A"'])
+    o, case = oracle_run(s, 1000)
+    r0, _, u = o.fields()
+    h = rec['history'][1]
+    assert h['step'] == 1000
+    assert darcy(u, case) == h['darcy_velocity']
+    assert float(r0.sum()) == h['total_mass']
+    pore = r0 > 0
+    assert [float(r0[pore].min()), float(r0[pore].max())] == h['density_range']
+
+
+def test_mrt_operator_identities():
+    """M MInv = I, conserved rates 0, B = MInv (I - S/2) M (moment_audit.py:23-35)."""
+    from oracle.pyoracle import mrt_matrices
+    for Q in (9, 19):
+        for tau in (0.6, 1.0, 1.4):
+            A, B, M, MInv = mrt_matrices(Q, tau)
+            assert np.allclose(M @ MInv, np.eye(Q), atol=3e-14)
+            S = np.diag(M @ A @ MInv)
+            conserved = [0, 3, 5] if Q == 9 else [0, 3, 5, 7]
+            assert np.allclose(S[conserved], 0.0, atol=3e-14)
+            assert np.allclose(M @ B @ MInv, np.eye(Q) - 0.5 * np.diag(S), atol=3e-14)
+            su = 1 / tau
+            assert np.isclose(S[1], su)
+
+
+def test_planted_error_is_caught():
+    """A plausible mistake (wrong relaxation time) in the oracle input changes the recorded profile."""
+    s = [c for c in H.channel_cases() if c['name'] == 'channel-scsp-d3-MRT-tau0.6-H8'][0]
+    bad = dict(s, tau=(0.6000001,))
+    o, case = oracle_run(bad, s['steps'])
+    got = profile(o.fields()[2], case)
+    assert not np.array_equal(got, np.array([0]))
+    from tests.conftest import GOLDEN
+    import json
+    want = np.array(json.loads((GOLDEN / 'reference_records.json').read_text())
+                    ['channel'][s['name'] + '-1rank']['measured_profile'])
+    assert not np.array_equal(got, want)
