@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Throughput of the D3Q19 MRT time step on one MI355X.
+
+Workload: single-phase D3Q19 MRT permeability lattice on a 256^3 periodic
+overlapping-sphere pack (radius uniform in [4,10], porosity 0.20, seed 2),
+body force 1e-6 along x, tau 0.8, rho0 1 (synthetic geometry from
+lbflow_amd.geometry).  A "step" is one full lattice update (stream,
+bounce-back, macro, force, MRT collision) of the whole domain with the
+populations resident in HBM.
+
+value = site MLUPS (lattice sites updated per second / 1e6, every voxel of
+the 256^3 box counted, as the reference streams every voxel); fluid MLUPS
+(the reference's own metric) is reported beside it.  The dominant kernel's
+roofline uses its algorithmic HBM bytes per launch and its average launch
+time measured with HIP events on the engine's stream.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, public specification
+METRIC = 'D3Q19 MRT MLUPS + % HBM roofline; time-to-permeability warm vs cold'
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(size):
+    from lbflow_amd import geometry as G
+    from lbflow_amd import harness as H
+    labels = G.sphere_pack((size, size, size), 0.20, 4.0, 10.0, seed=2)
+    spec = H.spec(f'c2-{size}', nx=size, ny=size, nz=size, model='scsp', eq='MRT', tau=(0.8,),
+                  geometry=labels.ravel(),
+                  indices=['75 "GV" "1e-06" "1" "0"', '255 "SD" "90" "N/A" "N/A"'])
+    return spec, labels
+
+
+def measure(spec, labels, precision, stream_solids, steps, warmup, dense=False):
+    from lbflow_amd import harness as H
+    from lbflow_amd.engine import Engine
+    case = H.lattice_case(spec)
+    eng = Engine(case, labels.ravel(), precision, stream_solids, dense).init()
+    eng.advance(warmup)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    ms = eng.timed_advance(steps)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    eng.check_status()
+    out = dict(kernel=eng.step_kernel, bytes_per_launch=eng.step_bytes, ms_per_step=ms / steps,
+               wall_s=wall, device_bytes=eng.device_bytes, sites=eng.N)
+    eng.close()
+    return out
+
+
+def cpu_baseline(spec, labels, budget_s):
+    """The CPU oracle (single-threaded C restatement of the reference) timed on
+    the same 256^3 lattice for as many whole steps as fit the budget."""
+    from lbflow_amd import harness as H
+    from oracle.pyoracle import Oracle
+    case = H.lattice_case(spec)
+    o = Oracle(case, labels.ravel()).init()
+    t0 = time.perf_counter()
+    o.step(1)
+    one = time.perf_counter() - t0
+    n = max(1, min(10, int(budget_s / max(one, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    steps = n
+    return dict(value=labels.size * steps / dt / 1e6, unit='site MLUPS', cores=1, kind='port',
+                sample=f'oracle/lbf_oracle.c, same 256^3 D3Q19 MRT lattice, {steps} steps after 1 warm-up '
+                       f'step, {dt:.1f} s, 1 thread')
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--size', type=int, default=256)
+    ap.add_argument('--cpu-budget', type=float, default=20.0, help='seconds of CPU oracle timing (0: skip)')
+    ap.add_argument('--fp64', action='store_true', help='also time the FP64 reference-order kernel')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported '
+              '(multi-GPU sharding is design-only, see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+
+    from lbflow_amd.engine import FP32, FP64, copy_bandwidth
+    spec, labels = workload(args.size)
+    porosity = float((labels == 75).mean())
+    log(f'workload {args.size}^3 porosity {porosity:.4f}')
+    copy_gbs = copy_bandwidth(1 << 30, 20)
+    log(f'copy kernel {copy_gbs:.0f} GB/s')
+    main_run = measure(spec, labels, FP32, False, args.steps, args.warmup)
+    log('fp32', main_run)
+    extra = {}
+
+    def record(name, r):
+        extra[name] = dict(kernel=r['kernel'], ms_per_step=round(r['ms_per_step'], 5),
+                           site_mlups=round(labels.size / r['ms_per_step'] / 1e3, 1),
+                           bytes_per_launch=r['bytes_per_launch'],
+                           achieved_gbs=round(r['bytes_per_launch'] / r['ms_per_step'] / 1e6, 1))
+    record('fp32_dense_all_sites', measure(spec, labels, FP32, True, args.steps, args.warmup))
+    record('fp32_dense_fluid_sites', measure(spec, labels, FP32, False, args.steps, args.warmup, dense=True))
+    if args.fp64:
+        record('fp64_sparse', measure(spec, labels, FP64, False, max(10, args.steps // 2), 5))
+        record('fp64_dense_all_sites', measure(spec, labels, FP64, True, max(10, args.steps // 4), 5))
+    for k, v in extra.items():
+        log(k, v)
+
+    ms = main_run['ms_per_step']
+    site_mlups = labels.size / ms / 1e3
+    achieved = main_run['bytes_per_launch'] / ms / 1e6    # GB/s
+    cpu = None
+    if args.cpu_budget > 0:
+        cpu = cpu_baseline(spec, labels, args.cpu_budget)
+        log('cpu', cpu)
+    line = {
+        'metric': METRIC, 'value': round(site_mlups, 1), 'unit': 'MLUPS', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms, 5),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'fp32',
+        'data': 'synthetic',
+        'config': {'workload': f'D3Q19 MRT single-phase, {args.size}^3 periodic sphere pack '
+                               f'(r 4-10, porosity {porosity:.4f}, seed 2), tau 0.8, F 1e-6 x, '
+                               'FP32 shifted populations, fluid sites stored and updated (sparse)',
+                   'fluid_mlups': round(site_mlups * porosity, 1), 'porosity': porosity,
+                   'kernel': main_run['kernel'], 'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'bytes_per_launch': main_run['bytes_per_launch'], 'copy_kernel_gbs': round(copy_gbs, 1)},
+        'cpu_baseline': cpu,
+        'extra': extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

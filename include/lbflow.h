@@ -54,9 +54,12 @@ enum lbf_precision {
                               macro fields and reductions in FP64 */
 };
 enum lbf_create_flags {
-    LBF_STREAM_SOLIDS = 0x1 /* stream populations through solid sites as the reference
-                               does, so checkpoints are byte-identical; without it
-                               solid sites are skipped and checkpoint them as 0 */
+    LBF_STREAM_SOLIDS = 0x1, /* stream populations through solid sites as the reference
+                                does, so checkpoints are byte-identical (dense layout);
+                                without it only fluid sites are stored and updated and
+                                checkpoints hold 0 on solid sites */
+    LBF_LAYOUT_DENSE = 0x2   /* without LBF_STREAM_SOLIDS: keep the dense layout and skip
+                                solid sites instead of storing fluid sites only */
 };
 
 /* Simulation description; the fields of input.prm that the step needs
@@ -175,6 +178,15 @@ uint64_t lbf_device_bytes(const lbf_ctx* ctx);
  * algorithmic HBM bytes one launch moves (for roofline accounting). */
 const char* lbf_step_kernel(const lbf_ctx* ctx);
 double lbf_step_bytes(const lbf_ctx* ctx);
+
+/* --- measurement (no reference counterpart) ----------------------------
+ * lbf_timed_advance: n steps (no fields) bracketed by two HIP events on the
+ * context's stream; *ms = elapsed device time.  lbf_copy_bandwidth: a plain
+ * device copy kernel over `bytes` bytes, `iters` launches, timed with HIP
+ * events; returns (read + write bytes) / time in GB/s, or a negative value
+ * on error. */
+int lbf_timed_advance(lbf_ctx* ctx, uint64_t n, double* ms);
+double lbf_copy_bandwidth(uint64_t bytes, int iters);
 
 #ifdef __cplusplus
 }

@@ -19,11 +19,12 @@
 #include "setup.h"
 
 namespace lbf {
-hipError_t launch_scsp(int Q, bool mrt, bool init, bool fp32, bool stream_solids, const ScspArgs& a,
-                       hipStream_t s);
+hipError_t launch_scsp(int Q, bool mrt, bool init, bool fp32, int layout, const ScspParams& p,
+                       const ScspBuffers& b, hipStream_t s);
 hipError_t launch_color(int Q, bool mrt, bool init, bool stream_solids, const ColorArgs& a, hipStream_t s);
 hipError_t launch_report(const ReportArgs& a, double* out, hipStream_t s);
 uint32_t report_blocks(uint64_t N);
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 }  // namespace lbf
 
 using namespace lbf;
@@ -37,12 +38,18 @@ struct lbf_ctx {
     MrtOperators ops;
     std::vector<DevLabel> hlab;
     std::vector<double> shift;      // FP32: per plane shift (phase*Q + q)
+    std::vector<uint32_t> list;     // SPARSE: fluid site of each slot
     hipStream_t stream = nullptr;
     void* f[2] = {nullptr, nullptr};
     int cur = 0;
+    int layout = 0;                 // 0 dense (all sites streamed), 1 dense fluid-only, 2 sparse
+    uint64_t P = 0;                 // slots per population plane
     size_t elem = 8;
     uint32_t* d_desc = nullptr;
+    uint32_t* d_list = nullptr;
+    int32_t* d_map = nullptr;
     DevLabel* d_lab = nullptr;
+    LabelF* d_lab32 = nullptr;
     double *d_AT = nullptr, *d_A = nullptr, *d_B = nullptr, *d_MInv = nullptr;
     float *d_A32 = nullptr, *d_B32 = nullptr;
     int32_t* d_wall = nullptr;
@@ -260,12 +267,16 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
                 L.ef[q] = ef;
             }
         }
-        if (L.kind & KIND_FLUID) {
-            // count fluid sites
-        }
     }
     for (uint64_t i = 0; i < N; ++i)
-        if (c->hlab[geometry[i]].kind & KIND_FLUID) c->nfluid++;
+        if (c->hlab[geometry[i]].kind & KIND_FLUID) c->list.push_back(uint32_t(i));
+    c->nfluid = c->list.size();
+    // Layout: the colour model and checkpoint-exact runs stream every site;
+    // otherwise only fluid sites are stored (sparse) unless the dense
+    // fluid-only layout is asked for.
+    if (NP == 2 || (d.flags & LBF_STREAM_SOLIDS)) c->layout = 0;
+    else c->layout = (d.flags & LBF_LAYOUT_DENSE) ? 1 : 2;
+    c->P = c->layout == 2 ? c->nfluid : N;
 
     build_mrt(d.Q, d.tau[0], c->ops);
 
@@ -289,13 +300,18 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         lbf_destroy(c);
         return fail(nullptr, "HIP error: cannot create a stream (no GPU?)");
     }
-    const size_t pop = size_t(NP) * d.Q * N * c->elem;
+    const size_t pop = size_t(NP) * d.Q * c->P * c->elem;
     const size_t Q2 = size_t(d.Q) * d.Q;
     int rc = 0;
     rc |= dalloc(c, &c->f[0], pop);
     rc |= dalloc(c, &c->f[1], pop);
     rc |= dalloc(c, &c->d_desc, N * 4);
     rc |= dalloc(c, &c->d_lab, 256 * sizeof(DevLabel));
+    rc |= dalloc(c, &c->d_lab32, 256 * sizeof(LabelF));
+    if (c->layout == 2) {
+        rc |= dalloc(c, &c->d_list, c->nfluid * 4);
+        rc |= dalloc(c, &c->d_map, N * 4);
+    }
     rc |= dalloc(c, &c->d_AT, Q2 * 8);
     rc |= dalloc(c, &c->d_A, Q2 * 8);
     rc |= dalloc(c, &c->d_B, Q2 * 8);
@@ -332,6 +348,31 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         std::vector<float> a32 = pair_split(d.Q, c->ops.A), b32 = pair_split(d.Q, c->ops.B);
         rc |= upload(c, c->d_A32, a32.data(), a32.size() * 4);
         rc |= upload(c, c->d_B32, b32.data(), b32.size() * 4);
+        std::vector<LabelF> l32(256);
+        for (int l = 0; l < 256; ++l) {
+            std::memset(&l32[l], 0, sizeof(LabelF));
+            l32[l].kind = c->hlab[l].kind;
+            for (int k = 0; k < d.D; ++k) {
+                l32[l].force[k] = float(c->hlab[l].force[k]);
+                l32[l].half_rho[k] = float(c->hlab[l].half_rho[k]);
+            }
+        }
+        rc |= upload(c, c->d_lab32, l32.data(), 256 * sizeof(LabelF));
+    }
+    if (c->layout == 2) {
+        std::vector<int32_t> map(N, -1);
+        for (uint64_t s = 0; s < c->nfluid; ++s) map[c->list[s]] = int32_t(s);
+        // bounce-back in the sparse layout is map < 0: exactly the BB/SD sites
+        for (uint64_t i = 0; i < N; ++i)
+            if (map[i] < 0) {
+                const int t = c->labels[geometry[i]].type;
+                if (t != T_BB && t != T_SD) {
+                    lbf_destroy(c);
+                    return fail(nullptr, "lbf_create: sparse layout needs every non-fluid label to be BB or SD");
+                }
+            }
+        rc |= upload(c, c->d_list, c->list.data(), c->nfluid * 4);
+        rc |= upload(c, c->d_map, map.data(), N * 4);
     }
     if (NP == 2) {
         std::vector<double> dn(N, 86.0);
@@ -374,6 +415,8 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     }
     if (NP == 2) c->kernel_name = "color_step";
     else c->kernel_name = d.precision == LBF_FP32 ? "scsp_step_f32" : "scsp_step_f64";
+    static const char* layout_names[3] = {"dense", "dense-fluid", "sparse"};
+    c->kernel_name += std::string(" (") + layout_names[c->layout] + ")";
     *out = c;
     return 0;
 }
@@ -386,26 +429,45 @@ static void reset_status(lbf_ctx* c)
     (void)hipStreamSynchronize(c->stream);
 }
 
-static ScspArgs scsp_args(lbf_ctx* c, uint32_t flags)
+static ScspParams scsp_params(lbf_ctx* c, uint32_t flags)
 {
-    ScspArgs a{};
+    ScspParams a{};
     a.g = c->g;
-    a.desc = c->d_desc;
-    a.lab = c->d_lab;
-    a.AT = c->d_AT;
-    a.B = c->d_B;
-    a.A32 = c->d_A32;
-    a.B32 = c->d_B32;
-    a.rho = c->d_rho0;
-    a.u = c->d_u;
-    a.uadj = c->d_uadj;
-    a.status = c->d_status;
+    a.nfluid = uint32_t(c->nfluid);
     a.flags = flags;
     a.dens0 = c->d.density[0];
     a.omega = 1.0 / c->d.tau[0];
     a.omh = 1.0 - 0.5 * a.omega;
     for (int q = 0; q < c->Q; ++q) a.idist[q] = (c->Q == 9 ? Lattice<9>::w[q] : Lattice<19>::w[q]) * c->d.density[0];
+    a.inv_rho0 = float(1.0 / c->d.density[0]);
+    a.rho0_32 = float(c->d.density[0]);
+    a.omega32 = float(a.omega);
+    a.omh32 = float(a.omh);
+    const double su = 1.0 / c->d.tau[0], sq = 8.0 * (2.0 - su) / (8.0 - su);
+    a.c1_nu = float(su);
+    a.c2_nu = float(1.0 - 0.5 * su);
+    a.c1_q = float(sq);
+    a.c2_q = float(1.0 - 0.5 * sq);
     return a;
+}
+
+static ScspBuffers scsp_buffers(lbf_ctx* c, bool init)
+{
+    ScspBuffers b{};
+    b.fin = c->f[c->cur];
+    b.fout = init ? c->f[c->cur] : c->f[c->cur ^ 1];
+    b.desc = c->d_desc;
+    b.list = c->d_list;
+    b.map = c->d_map;
+    b.lab = c->d_lab;
+    b.lab32 = c->d_lab32;
+    b.AT = c->d_AT;
+    b.B = c->d_B;
+    b.rho = c->d_rho0;
+    b.u = c->d_u;
+    b.uadj = c->d_uadj;
+    b.status = c->d_status;
+    return b;
 }
 
 static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
@@ -467,11 +529,10 @@ static int run_step(lbf_ctx* c, bool init, uint32_t flags)
     const bool mrt = c->d.equil == LBF_MRT;
     hipError_t e;
     if (c->NP == 1) {
-        ScspArgs a = scsp_args(c, flags);
-        a.t = init ? 0 : c->t + 1;
-        a.fin = c->f[c->cur];
-        a.fout = init ? c->f[c->cur] : c->f[c->cur ^ 1];
-        e = launch_scsp(c->Q, mrt, init, c->d.precision == LBF_FP32, stream_solids, a, c->stream);
+        ScspParams p = scsp_params(c, flags);
+        p.t = init ? 0 : c->t + 1;
+        e = launch_scsp(c->Q, mrt, init, c->d.precision == LBF_FP32, c->layout, p, scsp_buffers(c, init),
+                        c->stream);
     } else {
         ColorArgs a = color_args(c, flags);
         a.t = init ? 0 : c->t + 1;
@@ -493,7 +554,7 @@ int lbf_init(lbf_ctx* c)
     reset_status(c);
     c->t = 0;
     c->has_prev = false;
-    const size_t pop = size_t(c->NP) * c->Q * c->N * c->elem;
+    const size_t pop = size_t(c->NP) * c->Q * c->P * c->elem;
     HIP_CHECK(c, hipMemsetAsync(c->f[c->cur], 0, pop, c->stream));
     if (run_step(c, true, 1u)) return -1;
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -550,21 +611,25 @@ int lbf_get_populations(lbf_ctx* c, double* out)
 {
     if (!c || !out) return -1;
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    const size_t n = size_t(c->NP) * c->Q * c->N;
-    if (c->elem == 8) {
+    const size_t planes = size_t(c->NP) * c->Q;
+    const size_t n = planes * c->P;
+    if (c->elem == 8 && c->layout == 0) {
         HIP_CHECK(c, hipMemcpy(out, c->f[c->cur], n * 8, hipMemcpyDeviceToHost));
         return 0;
     }
-    std::vector<float> h(n);
-    HIP_CHECK(c, hipMemcpy(h.data(), c->f[c->cur], n * 4, hipMemcpyDeviceToHost));
+    std::vector<double> h64(c->elem == 8 ? n : 0);
+    std::vector<float> h32(c->elem == 4 ? n : 0);
+    if (c->elem == 8) HIP_CHECK(c, hipMemcpy(h64.data(), c->f[c->cur], n * 8, hipMemcpyDeviceToHost));
+    else HIP_CHECK(c, hipMemcpy(h32.data(), c->f[c->cur], n * 4, hipMemcpyDeviceToHost));
+    std::memset(out, 0, planes * c->N * sizeof(double));
     const std::vector<uint32_t>& desc = c->links.desc;
-    const bool stream_solids = c->d.flags & LBF_STREAM_SOLIDS;
-    for (size_t plane = 0; plane < size_t(c->NP) * c->Q; ++plane)
-        for (uint64_t i = 0; i < c->N; ++i) {
-            const bool fluid = c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID;
-            // never-written solid entries hold 0 and are reported as 0
-            const float v = h[plane * c->N + i];
-            out[plane * c->N + i] = (fluid || stream_solids || v != 0.0f) ? double(v) + c->shift[plane] : 0.0;
+    for (size_t plane = 0; plane < planes; ++plane)
+        for (uint64_t s = 0; s < c->P; ++s) {
+            const uint64_t i = c->layout == 2 ? c->list[s] : s;
+            // solid sites that are never streamed keep the value 0
+            if (c->layout == 1 && !(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
+            out[plane * c->N + i] = c->elem == 8 ? h64[plane * c->P + s]
+                                                 : double(h32[plane * c->P + s]) + c->shift[plane];
         }
     return 0;
 }
@@ -572,15 +637,21 @@ int lbf_get_populations(lbf_ctx* c, double* out)
 int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
 {
     if (!c || !in) return -1;
-    const size_t n = size_t(c->NP) * c->Q * c->N;
-    if (c->elem == 8) {
+    const size_t planes = size_t(c->NP) * c->Q;
+    const size_t n = planes * c->P;
+    if (c->elem == 8 && c->layout == 0) {
         HIP_CHECK(c, hipMemcpy(c->f[c->cur], in, n * 8, hipMemcpyHostToDevice));
     } else {
-        std::vector<float> h(n);
-        for (size_t plane = 0; plane < size_t(c->NP) * c->Q; ++plane)
-            for (uint64_t i = 0; i < c->N; ++i)
-                h[plane * c->N + i] = float(in[plane * c->N + i] - c->shift[plane]);
-        HIP_CHECK(c, hipMemcpy(c->f[c->cur], h.data(), n * 4, hipMemcpyHostToDevice));
+        std::vector<double> h64(c->elem == 8 ? n : 0);
+        std::vector<float> h32(c->elem == 4 ? n : 0);
+        for (size_t plane = 0; plane < planes; ++plane)
+            for (uint64_t s = 0; s < c->P; ++s) {
+                const double v = in[plane * c->N + (c->layout == 2 ? c->list[s] : s)];
+                if (c->elem == 8) h64[plane * c->P + s] = v;
+                else h32[plane * c->P + s] = float(v - c->shift[plane]);
+            }
+        if (c->elem == 8) HIP_CHECK(c, hipMemcpy(c->f[c->cur], h64.data(), n * 8, hipMemcpyHostToDevice));
+        else HIP_CHECK(c, hipMemcpy(c->f[c->cur], h32.data(), n * 4, hipMemcpyHostToDevice));
     }
     // A restarted run holds no macro history (Simulator::validateRestart resets it,
     // simulator.cpp:382-383; the macro arrays are fresh zeros until the first step).
@@ -663,10 +734,61 @@ const char* lbf_step_kernel(const lbf_ctx* c) { return c ? c->kernel_name.c_str(
 double lbf_step_bytes(const lbf_ctx* c)
 {
     if (!c) return 0.0;
-    const bool stream_solids = c->d.flags & LBF_STREAM_SOLIDS;
-    const double sites = stream_solids ? double(c->N) : double(c->nfluid);
-    // every streamed site reads and writes Q populations per phase; every site reads its descriptor
-    return sites * 2.0 * c->NP * c->Q * double(c->elem) + double(c->N) * 4.0;
+    const double pops = 2.0 * c->NP * c->Q * double(c->elem);   // read + write per streamed site
+    if (c->layout == 0) return double(c->N) * (pops + 4.0);       // + descriptor
+    if (c->layout == 1) return double(c->nfluid) * pops + double(c->N) * 4.0;
+    // sparse: per fluid site its populations, its list entry, its descriptor
+    // and its own map entry (read as a neighbour of other fluid sites)
+    return double(c->nfluid) * (pops + 12.0);
+}
+
+int lbf_timed_advance(lbf_ctx* c, uint64_t n, double* ms)
+{
+    if (!c || !ms) return -1;
+    hipEvent_t e0, e1;
+    HIP_CHECK(c, hipEventCreate(&e0));
+    HIP_CHECK(c, hipEventCreate(&e1));
+    HIP_CHECK(c, hipEventRecord(e0, c->stream));
+    for (uint64_t k = 0; k < n; ++k)
+        if (run_step(c, false, 0u)) return -1;
+    HIP_CHECK(c, hipEventRecord(e1, c->stream));
+    HIP_CHECK(c, hipEventSynchronize(e1));
+    float f = 0.0f;
+    HIP_CHECK(c, hipEventElapsedTime(&f, e0, e1));
+    *ms = double(f);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+double lbf_copy_bandwidth(uint64_t bytes, int iters)
+{
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double result = -1.0;
+    bytes &= ~uint64_t(15);
+    if (bytes == 0 || iters <= 0) return -1.0;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        g_create_error = "lbf_copy_bandwidth: HIP set-up failed";
+    } else {
+        (void)hipMemsetAsync(a, 0, bytes, s);
+        (void)launch_copy(a, b, bytes, s);   // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < iters; ++k) (void)launch_copy(k % 2 ? b : a, k % 2 ? a : b, bytes, s);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.0f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0)
+            result = 2.0 * double(bytes) * iters / (double(ms) * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return result;
 }
 
 }  // extern "C"

@@ -111,25 +111,41 @@ __device__ __forceinline__ void raise_status(DevStatus* st, int bit, unsigned lo
     atomicMin(&st->first[bit], t);
 }
 
-// Arguments of the single-phase step kernels.
-struct ScspArgs {
+// Single-precision label entry of the FP32 kernels.
+struct LabelF {
+    int32_t kind;
+    float force[3];
+    float half_rho[3];
+    float pad;
+};
+
+// Scalar parameters of the single-phase step kernels (by value).
+struct ScspParams {
     GridDev g;
+    uint32_t nfluid;         // SPARSE layout: number of fluid sites
+    uint32_t flags;          // bit0 fields, bit1 adjacent
+    unsigned long long t;    // step being computed (failure records)
+    double dens0, omega, omh; // omh = 1 - 0.5 omega
+    double idist[19];        // w_q * rho0
+    float inv_rho0, rho0_32, omega32, omh32;
+    float c1_nu, c2_nu, c1_q, c2_q;   // FP32 MRT: s and 1 - s/2 of the two non-conserved rate classes
+};
+
+// Device buffers of the single-phase step kernels.
+struct ScspBuffers {
     const void* fin;
     void* fout;
     const uint32_t* desc;
+    const uint32_t* list;    // SPARSE: fluid site of each slot
+    const int32_t* map;      // SPARSE: slot of each site, -1 on solids
     const DevLabel* lab;
-    const double* AT;        // FP64: A transposed (MInvSMT); FP32: pair-split A
-    const double* B;         // FP64: force matrix; FP32: pair-split B
-    const float* A32;
-    const float* B32;
+    const LabelF* lab32;
+    const double* AT;        // A transposed (MInvSMT of scsp.cpp:425)
+    const double* B;         // force matrix (scsp.cpp:426-427)
     double* rho;             // fields: density
     double* u;               // fields: velocity [d][site]
     double* uadj;            // adjacent velocity history
     DevStatus* status;
-    unsigned long long t;
-    uint32_t flags;          // bit0 fields, bit1 adjacent
-    double dens0, omega, omh; // omh = 1 - 0.5 omega
-    double idist[19];        // w_q * rho0
 };
 
 struct ColorArgs {

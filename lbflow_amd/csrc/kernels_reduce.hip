@@ -179,3 +179,26 @@ hipError_t launch_shift(const double* f, float* h, const double* shift, uint64_t
 }
 
 }  // namespace lbf
+
+namespace lbf {
+
+// Plain streaming copy, 16 bytes per lane per iteration: the achievable HBM
+// bandwidth reference for roofline fractions and counter calibration.
+__global__ void __launch_bounds__(256) copy_stream(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                  uint64_t n)
+{
+    for (uint64_t k = blockIdx.x * 256ull + threadIdx.x; k < n; k += uint64_t(gridDim.x) * 256u)
+        dst[k] = src[k];
+}
+
+hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s)
+{
+    const uint64_t n = bytes / 16;
+    const uint64_t want = (n + 255) / 256;
+    const uint32_t blocks = uint32_t(want < 65536 ? want : 65536);
+    hipLaunchKernelGGL(copy_stream, dim3(blocks), dim3(256), 0, s, static_cast<const float4*>(src),
+                       static_cast<float4*>(dst), n);
+    return hipGetLastError();
+}
+
+}  // namespace lbf

@@ -3,7 +3,7 @@
 // One fused pull kernel per step: stream (periodic wrap in the index, the
 // reference's MPI halo on one GPU) + halfway bounce-back + macro variables +
 // half-force shift + SRT/MRT-with-Guo collision, one thread per lattice site,
-// SoA populations [q][site] so every direction is a coalesced HBM stream.
+// SoA populations [q][slot] so every direction is a coalesced HBM stream.
 //
 // Reference operators (zhou1800/lbflow):
 //   streaming + bounceBack   solver/src/lbm/simulator.cpp:73-111
@@ -14,6 +14,15 @@
 //   initialize               solver/src/lbm/scsp.cpp:116-134
 //   validatePopulations      solver/src/lbm/simulator.cpp:327-333
 //
+// Layouts (template parameter):
+//   DENSE_ALL    slot = site, every site streamed (solid sites are streamed
+//                but never collided, exactly like the reference, so
+//                checkpoints are byte-identical);
+//   DENSE_FLUID  slot = site, solid sites skipped;
+//   SPARSE       slot = fluid index: populations of fluid sites only, a
+//                sorted fluid-site list and a dense site -> slot map (-1 on
+//                solids), so solid voxels cost neither HBM traffic nor lanes.
+//
 // FP64 kernels keep the reference's floating-point operation order (the file
 // is built with -ffp-contract=off) and are bit-identical to it.  Terms whose
 // lattice coefficient is 0 are skipped and +-1 coefficients become add/sub:
@@ -23,87 +32,226 @@
 //
 // FP32 kernels store h_q = f_q - w_q rho0 ("shifted" populations); streaming
 // and bounce-back commute with the shift (w_q = w_op(q)), rho = rho0 + sum h,
-// and the collision acts on h directly, so no large constant is carried in
-// single precision.  They use FMA contraction.
+// and the MRT collision is done in moment space with the equilibrium and
+// Guo-source moments evaluated from compile-time moment tables of the
+// reference's transform matrix, so no large constant is carried in single
+// precision and the arithmetic is about 400 FP32 operations per site.
+#include <type_traits>
+
 #include "device.h"
 
 namespace lbf {
 
 constexpr uint32_t F_FIELDS = 1, F_ADJ = 2;
 constexpr double C1 = 3.0, C2 = 4.5, C3 = 1.5;   // 3/c^2, 4.5/c^4, 1.5/c^2 with c = 1
+enum : int { DENSE_ALL = 0, DENSE_FLUID = 1, SPARSE = 2 };
 
+// Moment tables of the transform M (compile time): for row k,
+//   T0[k] = sum_q M_kq w_q,  T1[k][a] = sum_q M_kq w_q e_qa,
+//   T2[k][a][b] = sum_q M_kq w_q e_qa e_qb,  n2[k] = sum_q M_kq^2,
+// parity[k] = +1 if row k is even under q -> op(q), -1 if odd, and
+// rate[k] = 0 (conserved), 1 (s_nu = 1/tau) or 2 (s_q) of the legacy
+// spectrum (solver/src/lbm/simulator.cpp:199-216).
+template <int Q> struct MomentTables {
+    double T0[Q];
+    double T1[Q][3];
+    double T2[Q][3][3];
+    double n2[Q];
+    int parity[Q];
+    int rate[Q];
+    constexpr MomentTables() : T0{}, T1{}, T2{}, n2{}, parity{}, rate{}
+    {
+        using L = Lattice<Q>;
+        for (int k = 0; k < Q; ++k) {
+            bool even = true, odd = true;
+            for (int q = 0; q < Q; ++q) {
+                const double m = L::M[k][q], w = L::w[q];
+                T0[k] += m * w;
+                n2[k] += m * m;
+                for (int a = 0; a < L::D; ++a) {
+                    T1[k][a] += m * w * L::e[q][a];
+                    for (int b = 0; b < L::D; ++b) T2[k][a][b] += m * w * L::e[q][a] * L::e[q][b];
+                }
+                even = even && L::M[k][L::op[q]] == L::M[k][q];
+                odd = odd && L::M[k][L::op[q]] == -L::M[k][q];
+            }
+            parity[k] = even ? 1 : (odd ? -1 : 0);
+        }
+        if constexpr (Q == 9) {
+            const int r[9] = {0, 1, 1, 0, 2, 0, 2, 1, 1};
+            for (int k = 0; k < 9; ++k) rate[k] = r[k];
+        } else {
+            const int r[19] = {0, 1, 1, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2};
+            for (int k = 0; k < 19; ++k) rate[k] = r[k];
+        }
+    }
+};
+template <int Q> inline constexpr MomentTables<Q> kMT{};
+
+// ---------------------------------------------------------------------------
+// site / slot resolution
+// ---------------------------------------------------------------------------
+template <int LAYOUT>
+__device__ __forceinline__ bool resolve(const ScspParams& p, const uint32_t* __restrict__ list, uint32_t& site,
+                                        uint32_t& slot)
+{
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if constexpr (LAYOUT == SPARSE) {
+        if (k >= p.nfluid) return false;
+        site = list[k];
+    } else {
+        if (k >= p.g.N) return false;
+        site = k;
+    }
+    slot = k;
+    return true;
+}
+
+// Wrapped coordinate terms of a site: x-1, x, x+1 and the matching row and
+// plane offsets (site index units), computed once per site so that every
+// neighbour index is two integer adds.
+struct Stencil {
+    uint32_t x[3], y[3], z[3];   // [0] = -1, [1] = 0, [2] = +1
+};
+
+__device__ __forceinline__ Stencil make_stencil(const GridDev& g, uint32_t site)
+{
+    int x, y, z;
+    site_coords(g, site, x, y, z);
+    Stencil s;
+    s.x[1] = uint32_t(x);
+    s.x[0] = x == 0 ? g.nx - 1 : uint32_t(x - 1);
+    s.x[2] = uint32_t(x) + 1 == g.nx ? 0u : uint32_t(x + 1);
+    s.y[1] = uint32_t(y) * g.nx;
+    s.y[0] = (y == 0 ? g.ny - 1 : uint32_t(y - 1)) * g.nx;
+    s.y[2] = (uint32_t(y) + 1 == g.ny ? 0u : uint32_t(y + 1)) * g.nx;
+    s.z[1] = uint32_t(z) * g.nxy;
+    s.z[0] = (z == 0 ? g.nz - 1 : uint32_t(z - 1)) * g.nxy;
+    s.z[2] = (uint32_t(z) + 1 == g.nz ? 0u : uint32_t(z + 1)) * g.nxy;
+    return s;
+}
+
+// Site x - e_q (pull source of direction q).
 template <int Q>
-__device__ __forceinline__ void pull_populations_f64(const ScspArgs& a, const double* fin, uint32_t i,
-                                                     uint32_t links, int x, int y, int z, double* f)
+__device__ __forceinline__ uint32_t upstream(const Stencil& s, int q)
 {
     using L = Lattice<Q>;
-    const uint64_t N = a.g.N;
-    f[0] = fin[i];
-#pragma unroll
-    for (int q = 1; q < Q; ++q) {
-        const int o = L::op[q];
-        // bounce-back (simulator.cpp:103-111): if the neighbour x - e_q is
-        // solid the site is in ibS[op(q)] and receives its own outgoing f_op(q).
-        if ((links >> o) & 1u) f[q] = fin[uint64_t(o) * N + i];
-        else f[q] = fin[uint64_t(q) * N + pull_source<Q>(a.g, x, y, z, q)];
+    int ex, ey, ez;
+    if constexpr (Q == 9) {
+        ez = 0; ey = L::e[q][0]; ex = L::e[q][1];
+    } else {
+        ez = L::e[q][0]; ey = L::e[q][1]; ex = L::e[q][2];
     }
+    return s.z[1 - ez] + s.y[1 - ey] + s.x[1 - ex];
+}
+
+// Element offset (from the start of the population array) of the value that
+// streams into direction q of this slot: the upstream neighbour's f_q, or
+// this slot's own f_op(q) when the link bounces back.
+template <int Q, int LAYOUT, typename O>
+__device__ __forceinline__ O pull_offset(const int32_t* __restrict__ map, uint32_t links, uint32_t slot,
+                                         const Stencil& st, uint32_t P, int q)
+{
+    using L = Lattice<Q>;
+    const uint32_t src = upstream<Q>(st, q);
+    if constexpr (LAYOUT == SPARSE) {
+        const int32_t m = map[src];
+        return m < 0 ? O(L::op[q]) * P + slot : O(q) * P + uint32_t(m);
+    } else {
+        // site x - e_q solid <=> site in ibS[op(q)] (simulator.cpp:103-111)
+        return ((links >> L::op[q]) & 1u) ? O(L::op[q]) * P + slot : O(q) * P + src;
+    }
+}
+
+// Load/store with a uniform base and an element offset.  With a 32-bit
+// offset type (population array below 4 GiB) the address is an SGPR base plus
+// a zero-extended 32-bit VGPR byte offset; the 64-bit type serves larger
+// arrays.
+template <typename T, typename O> __device__ __forceinline__ T ld(const T* __restrict__ base, O elem)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + elem * O(sizeof(T)));
+}
+template <typename T, typename O> __device__ __forceinline__ void st(T* __restrict__ base, O elem, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + elem * O(sizeof(T))) = v;
 }
 
 // ---------------------------------------------------------------------------
 // FP64, reference operation order.
 // ---------------------------------------------------------------------------
-template <int Q, bool MRT, bool INIT, bool STREAM_SOLIDS>
-__global__ void __launch_bounds__(256) scsp_step_f64(ScspArgs a)
+template <int Q, bool MRT, bool INIT, int LAYOUT, bool WIDE>
+__global__ void __launch_bounds__(256)
+scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__ fout,
+              const uint32_t* __restrict__ desc, const uint32_t* __restrict__ list,
+              const int32_t* __restrict__ map, const DevLabel* __restrict__ lab,
+              const double* __restrict__ AT, const double* __restrict__ B, double* __restrict__ rho_out,
+              double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status)
 {
 #pragma clang fp contract(off)
     using L = Lattice<Q>;
     constexpr int D = L::D;
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.g.N) return;
-    const uint64_t N = a.g.N;
-    const double* __restrict__ fin = static_cast<const double*>(a.fin);
-    double* __restrict__ fout = static_cast<double*>(a.fout);
-    const uint32_t desc = a.desc[i];
-    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
-    const int kind = lab.kind;
+    using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+    // The dense MRT operators are staged in LDS once per workgroup and read
+    // with uniform addresses (broadcast), before any thread leaves.
+    __shared__ double sAT[MRT ? Q * Q : 1], sB[MRT ? Q * Q : 1];
+    if constexpr (MRT) {
+        for (int k = threadIdx.x; k < Q * Q; k += 256) {
+            sAT[k] = AT[k];
+            sB[k] = B[k];
+        }
+        __syncthreads();
+    }
+    uint32_t i, slot;
+    if (!resolve<LAYOUT>(p, list, i, slot)) return;
+    const uint32_t P = LAYOUT == SPARSE ? p.nfluid : uint32_t(p.g.N);   // plane length
+    const uint32_t dsc = desc[i];
+    const DevLabel* lb = lab + (dsc >> DESC_LABEL_SHIFT);
+    const int kind = lb->kind;
     double f[Q];
+    Stencil stn{};
+    if (!INIT) stn = make_stencil(p.g, i);
 
     if (!(kind & KIND_FLUID)) {
-        if (INIT || !STREAM_SOLIDS) return;
+        if (INIT || LAYOUT != DENSE_ALL) return;
         // Solid sites are streamed but never collided, like the reference's
         // whole-array streaming; their contents only matter for checkpoints.
-        int x, y, z;
-        site_coords(a.g, i, x, y, z);
-        pull_populations_f64<Q>(a, fin, i, 0u, x, y, z, f);
         bool finite = true;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
+            f[q] = ld(fin, O(q) * P + (q == 0 ? slot : upstream<Q>(stn, q)));
             finite = finite && isfinite(f[q]);
-            fout[uint64_t(q) * N + i] = f[q];
+            st(fout, O(q) * P + slot, f[q]);
         }
-        if (!finite) raise_status(a.status, 4, a.t);
+        if (!finite) raise_status(status, 4, p.t);
         return;
     }
 
+    const double force[3] = {lb->force[0], lb->force[1], D == 3 ? lb->force[2] : 0.0};
     const bool forced = kind & KIND_FORCED;
     if (INIT) {
         // SCSP::initialize: f = w rho0 - 1.5 w (e.F)
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            f[q] = a.idist[q];
-            f[q] -= 1.5 * L::w[q] * lab.ef[q];
+            double ef = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (L::e[q][d] == 1) ef += force[d];
+                else if (L::e[q][d] == -1) ef += -force[d];
+            }
+            f[q] = p.idist[q];
+            f[q] -= 1.5 * L::w[q] * ef;
         }
     } else {
-        int x, y, z;
-        site_coords(a.g, i, x, y, z);
-        pull_populations_f64<Q>(a, fin, i, desc & DESC_LINK_MASK, x, y, z, f);
+        f[0] = ld(fin, O(slot));
+#pragma unroll
+        for (int q = 1; q < Q; ++q) f[q] = ld(fin, pull_offset<Q, LAYOUT, O>(map, dsc, slot, stn, P, q));
     }
 
     // calcMacroVar: rho = sum_q f_q (q ascending); u = (sum +-f) / rho0
     double rho = f[0];
 #pragma unroll
     for (int q = 1; q < Q; ++q) rho += f[q];
-    if (!isfinite(rho) || rho <= 0.0) raise_status(a.status, 0, a.t);
+    if (!isfinite(rho) || rho <= 0.0) raise_status(status, 0, p.t);
     double u[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -113,58 +261,44 @@ __global__ void __launch_bounds__(256) scsp_step_f64(ScspArgs a)
             if (L::e[q][d] == 1) s += f[q];
             else if (L::e[q][d] == -1) s -= f[q];
         }
-        s /= a.dens0;
+        s /= p.dens0;
         u[d] = s;
     }
     // bodyForce: u += (F/2)/rho0 on GV sites
     if (kind & KIND_GV) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) u[d] += lab.half_rho[d];
+        for (int d = 0; d < D; ++d) u[d] += lb->half_rho[d];
     }
-    if (a.flags & F_FIELDS) {
-        a.rho[i] = rho;
+    if (p.flags & F_FIELDS) {
+        rho_out[i] = rho;
 #pragma unroll
-        for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = u[d];
+        for (int d = 0; d < D; ++d) u_out[uint64_t(d) * p.g.N + i] = u[d];
     }
-    if (a.flags & F_ADJ) {
+    if (p.flags & F_ADJ) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = u[d];
+        for (int d = 0; d < D; ++d) uadj[uint64_t(d) * p.g.N + i] = u[d];
     }
 
     // collision
     double usq = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) usq += u[d] * u[d];
-    double src[Q];
-    if (forced) {
-        // forceSource: G_q = w_q (3 (e.F - u.F) + 9 (e.u)(e.F))
+    if constexpr (!MRT) {
         double uf = 0.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) uf += u[d] * lab.force[d];
+        for (int d = 0; d < D; ++d) uf += u[d] * force[d];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            double eu = 0.0;
+            double ui = 0, ef = 0.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                if (L::e[q][d] == 1) eu += u[d];
-                else if (L::e[q][d] == -1) eu += -u[d];
+                if (L::e[q][d] == 1) { ui += u[d]; ef += force[d]; }
+                else if (L::e[q][d] == -1) { ui += -u[d]; ef += -force[d]; }
             }
-            const double ef = lab.ef[q];
-            src[q] = L::w[q] * (3.0 * (ef - uf) + 9.0 * eu * ef);
-        }
-    }
-    if constexpr (!MRT) {
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            double ui = 0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                if (L::e[q][d] == 1) ui += u[d];
-                else if (L::e[q][d] == -1) ui += -u[d];
-            }
-            const double feq = L::w[q] * (rho + a.dens0 * (C1 * ui + C2 * ui * ui - C3 * usq));
-            f[q] += (feq - f[q]) * a.omega;
-            if (forced) f[q] += a.omh * src[q];
+            const double feq = L::w[q] * (rho + p.dens0 * (C1 * ui + C2 * ui * ui - C3 * usq));
+            f[q] += (feq - f[q]) * p.omega;
+            // forceSource: G_q = w_q (3 (e.F - u.F) + 9 (e.u)(e.F))
+            if (forced) f[q] += p.omh * (L::w[q] * (3.0 * (ef - uf) + 9.0 * ui * ef));
         }
     } else {
         double r[Q];
@@ -176,27 +310,39 @@ __global__ void __launch_bounds__(256) scsp_step_f64(ScspArgs a)
                 if (L::e[q][d] == 1) ui += u[d];
                 else if (L::e[q][d] == -1) ui += -u[d];
             }
-            const double feq = L::w[q] * (rho + a.dens0 * (C1 * ui + C2 * ui * ui - C3 * usq));
+            const double feq = L::w[q] * (rho + p.dens0 * (C1 * ui + C2 * ui * ui - C3 * usq));
             r[q] = feq - f[q];
         }
-        const double* __restrict__ AT = a.AT;
         double ff[Q];
 #pragma unroll
         for (int qq = 0; qq < Q; ++qq) {
             double s = 0.0;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) s += AT[q * Q + qq] * r[q];
+            for (int q = 0; q < Q; ++q) s += sAT[q * Q + qq] * r[q];
             ff[qq] = s;
         }
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] += ff[q];
         if (forced) {
-            const double* __restrict__ B = a.B;
+            double uf = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) uf += u[d] * force[d];
+            double src[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                double eu = 0.0, ef = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    if (L::e[q][d] == 1) { eu += u[d]; ef += force[d]; }
+                    else if (L::e[q][d] == -1) { eu += -u[d]; ef += -force[d]; }
+                }
+                src[q] = L::w[q] * (3.0 * (ef - uf) + 9.0 * eu * ef);
+            }
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 double s = 0.0;
 #pragma unroll
-                for (int j = 0; j < Q; ++j) s = s + src[j] * B[q * Q + j];
+                for (int j = 0; j < Q; ++j) s = s + src[j] * sB[q * Q + j];
                 f[q] += s;
             }
         }
@@ -205,108 +351,62 @@ __global__ void __launch_bounds__(256) scsp_step_f64(ScspArgs a)
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         finite = finite && isfinite(f[q]);
-        fout[uint64_t(q) * N + i] = f[q];
+        st(fout, O(q) * P + slot, f[q]);
     }
-    if (!finite) raise_status(a.status, 4, a.t);
+    if (!finite) raise_status(status, 4, p.t);
 }
 
 // ---------------------------------------------------------------------------
 // FP32 shifted populations.
 // ---------------------------------------------------------------------------
-template <int Q>
-__device__ __forceinline__ void pull_populations_f32(const ScspArgs& a, const float* fin, uint32_t i,
-                                                     uint32_t links, int x, int y, int z, float* h)
+template <int Q, bool MRT, bool INIT, int LAYOUT, bool WIDE>
+__global__ void __launch_bounds__(256)
+scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ fout,
+              const uint32_t* __restrict__ desc, const uint32_t* __restrict__ list,
+              const int32_t* __restrict__ map, const LabelF* __restrict__ lab, double* __restrict__ rho_out,
+              double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status)
 {
-    using L = Lattice<Q>;
-    const uint64_t N = a.g.N;
-    h[0] = fin[i];
-#pragma unroll
-    for (int q = 1; q < Q; ++q) {
-        const int o = L::op[q];
-        if ((links >> o) & 1u) h[q] = fin[uint64_t(o) * N + i];
-        else h[q] = fin[uint64_t(q) * N + pull_source<Q>(a.g, x, y, z, q)];
-    }
-}
-
-// y = Op . v for an operator with the lattice parity symmetry
-// Op[op q][op p] = Op[q][p], stored pair-split: for the rest direction and
-// each representative q of an opposite pair, the even row
-// [Op[q][0], (Op[q][p]+Op[q][op p])/2 ...] and for each representative the
-// odd row [(Op[q][p]-Op[q][op p])/2 ...]; then y_q = E_q + O_q and
-// y_op(q) = E_q - O_q.  181 FMAs instead of 361 for D3Q19.
-template <int Q>
-__device__ __forceinline__ void apply_split(const float* __restrict__ C, const float* v, float* y)
-{
-    using L = Lattice<Q>;
-    constexpr int H = L::NHALF;
-    float s[H], d[H];
-#pragma unroll
-    for (int k = 0; k < H; ++k) {
-        const int p = L::half[k];
-        s[k] = v[p] + v[L::op[p]];
-        d[k] = v[p] - v[L::op[p]];
-    }
-    // rows: 0, then half[k]
-#pragma unroll
-    for (int r = 0; r <= H; ++r) {
-        const float* row = C + r * (H + 1);
-        float e = row[0] * v[0];
-#pragma unroll
-        for (int k = 0; k < H; ++k) e = fmaf(row[k + 1], s[k], e);
-        if (r == 0) {
-            y[0] = e;
-        } else {
-            const float* orow = C + (H + 1) * (H + 1) + (r - 1) * H;
-            float o = 0.0f;
-#pragma unroll
-            for (int k = 0; k < H; ++k) o = fmaf(orow[k], d[k], o);
-            const int q = L::half[r - 1];
-            y[q] = e + o;
-            y[L::op[q]] = e - o;
-        }
-    }
-}
-
-template <int Q, bool MRT, bool INIT, bool STREAM_SOLIDS>
-__global__ void __launch_bounds__(256) scsp_step_f32(ScspArgs a)
-{
+#pragma clang fp contract(fast)
     using L = Lattice<Q>;
     constexpr int D = L::D;
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.g.N) return;
-    const uint64_t N = a.g.N;
-    const float* __restrict__ fin = static_cast<const float*>(a.fin);
-    float* __restrict__ fout = static_cast<float*>(a.fout);
-    const uint32_t desc = a.desc[i];
-    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
-    const int kind = lab.kind;
+    constexpr int H = L::NHALF;
+    constexpr MomentTables<Q> MT = kMT<Q>;
+    using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+    uint32_t i, slot;
+    if (!resolve<LAYOUT>(p, list, i, slot)) return;
+    const uint32_t P = LAYOUT == SPARSE ? p.nfluid : uint32_t(p.g.N);
+    const uint32_t dsc = desc[i];
+    const LabelF lb = lab[dsc >> DESC_LABEL_SHIFT];
     float h[Q];
+    Stencil stn{};
+    if (!INIT) stn = make_stencil(p.g, i);
 
-    if (!(kind & KIND_FLUID)) {
-        if (INIT || !STREAM_SOLIDS) return;
-        int x, y, z;
-        site_coords(a.g, i, x, y, z);
-        pull_populations_f32<Q>(a, fin, i, 0u, x, y, z, h);
+    if (!(lb.kind & KIND_FLUID)) {
+        if (INIT || LAYOUT != DENSE_ALL) return;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fout[uint64_t(q) * N + i] = h[q];
+        for (int q = 0; q < Q; ++q)
+            st(fout, O(q) * P + slot, ld(fin, O(q) * P + (q == 0 ? slot : upstream<Q>(stn, q))));
         return;
     }
-    const bool forced = kind & KIND_FORCED;
-    const float rho0 = float(a.dens0);
+    const float F[3] = {lb.force[0], lb.force[1], D == 3 ? lb.force[2] : 0.0f};
     if (INIT) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) h[q] = float(-1.5 * L::w[q] * lab.ef[q]);
+        for (int q = 0; q < Q; ++q) {
+            float ef = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) ef += float(L::e[q][d]) * F[d];
+            h[q] = float(-1.5 * L::w[q]) * ef;
+        }
     } else {
-        int x, y, z;
-        site_coords(a.g, i, x, y, z);
-        pull_populations_f32<Q>(a, fin, i, desc & DESC_LINK_MASK, x, y, z, h);
+        h[0] = ld(fin, O(slot));
+#pragma unroll
+        for (int q = 1; q < Q; ++q) h[q] = ld(fin, pull_offset<Q, LAYOUT, O>(map, dsc, slot, stn, P, q));
     }
     // macro on shifted populations: rho = rho0 + sum h, j = sum e h
     float drho = 0.0f;
 #pragma unroll
     for (int q = 0; q < Q; ++q) drho += h[q];
     float u[D];
-    const float inv_rho0 = float(1.0 / a.dens0);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         float s = 0.0f;
@@ -315,111 +415,176 @@ __global__ void __launch_bounds__(256) scsp_step_f32(ScspArgs a)
             if (L::e[q][d] == 1) s += h[q];
             else if (L::e[q][d] == -1) s -= h[q];
         }
-        u[d] = s * inv_rho0;
-        if (kind & KIND_GV) u[d] += float(lab.half_rho[d]);
+        u[d] = s * p.inv_rho0 + lb.half_rho[d];
     }
-    const double rho_d = a.dens0 + double(drho);
-    if (!isfinite(drho) || rho_d <= 0.0) raise_status(a.status, 0, a.t);
-    if (a.flags & F_FIELDS) {
-        a.rho[i] = rho_d;
+    if (!isfinite(drho) || p.dens0 + double(drho) <= 0.0) raise_status(status, 0, p.t);
+    if (p.flags & F_FIELDS) {
+        rho_out[i] = p.dens0 + double(drho);
 #pragma unroll
-        for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = double(u[d]);
+        for (int d = 0; d < D; ++d) u_out[uint64_t(d) * p.g.N + i] = double(u[d]);
     }
-    if (a.flags & F_ADJ) {
+    if (p.flags & F_ADJ) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = double(u[d]);
+        for (int d = 0; d < D; ++d) uadj[uint64_t(d) * p.g.N + i] = double(u[d]);
     }
-    float usq = 0.0f;
+    float usq = 0.0f, uF = 0.0f;
 #pragma unroll
-    for (int d = 0; d < D; ++d) usq = fmaf(u[d], u[d], usq);
-    // shifted equilibrium h_eq = w (drho + rho0 (3 eu + 4.5 eu^2 - 1.5 u^2))
-    float r[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        float ui = 0.0f;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            if (L::e[q][d] == 1) ui += u[d];
-            else if (L::e[q][d] == -1) ui -= u[d];
-        }
-        const float poly = fmaf(ui, fmaf(4.5f, ui, 3.0f), -1.5f * usq);
-        const float heq = float(L::w[q]) * fmaf(rho0, poly, drho);
-        r[q] = heq - h[q];
+    for (int d = 0; d < D; ++d) {
+        usq += u[d] * u[d];
+        uF += u[d] * F[d];
     }
-    float src[Q];
-    if (forced) {
-        float uf = 0.0f;
-#pragma unroll
-        for (int d = 0; d < D; ++d) uf = fmaf(u[d], float(lab.force[d]), uf);
+    const float rho0 = p.rho0_32;
+    if constexpr (!MRT) {
+        // h += omega (h_eq - h) + (1 - omega/2) G,  h_eq = w (drho + rho0 (3eu + 4.5eu^2 - 1.5u^2))
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            float eu = 0.0f;
+            float eu = 0.0f, ef = 0.0f;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                if (L::e[q][d] == 1) eu += u[d];
-                else if (L::e[q][d] == -1) eu -= u[d];
+                if (L::e[q][d] == 1) { eu += u[d]; ef += F[d]; }
+                else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
             }
-            const float ef = float(lab.ef[q]);
-            src[q] = float(L::w[q]) * fmaf(9.0f * eu, ef, 3.0f * (ef - uf));
-        }
-    }
-    if constexpr (!MRT) {
-        const float om = float(a.omega), omh = float(a.omh);
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            h[q] = fmaf(om, r[q], h[q]);
-            if (forced) h[q] = fmaf(omh, src[q], h[q]);
+            const float w = float(L::w[q]);
+            const float heq = w * (drho + rho0 * (eu * (3.0f + 4.5f * eu) - 1.5f * usq));
+            const float G = w * (3.0f * (ef - uF) + 9.0f * eu * ef);
+            h[q] += p.omega32 * (heq - h[q]) + p.omh32 * G;
         }
     } else {
-        float dh[Q];
-        apply_split<Q>(a.A32, r, dh);
+        // moments of h, pair-summed by parity
+        float sp[H], sm[H];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) h[q] += dh[q];
-        if (forced) {
-            apply_split<Q>(a.B32, src, dh);
+        for (int j = 0; j < H; ++j) {
+            const int q = L::half[j];
+            sp[j] = h[q] + h[L::op[q]];
+            sm[j] = h[q] - h[L::op[q]];
+        }
+        float y[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) h[q] += dh[q];
+        for (int k = 0; k < Q; ++k) {
+            // Guo source moment: M G = 3 T1.F - 3 (u.F) T0 + 9 u.T2.F
+            float mg = 0.0f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                if (MT.T1[k][a] != 0.0) mg += float(3.0 * MT.T1[k][a]) * F[a];
+#pragma unroll
+                for (int b = 0; b < D; ++b)
+                    if (MT.T2[k][a][b] != 0.0) mg += float(9.0 * MT.T2[k][a][b]) * u[a] * F[b];
+            }
+            if (MT.T0[k] != 0.0) mg -= float(3.0 * MT.T0[k]) * uF;
+            if (MT.rate[k] == 0) {
+                y[k] = mg * float(1.0 / MT.n2[k]);
+                continue;
+            }
+            // m_k = (M h)_k
+            float m = 0.0f;
+            if (MT.parity[k] > 0) {
+                if (L::M[k][0] != 0) m += float(L::M[k][0]) * h[0];
+#pragma unroll
+                for (int j = 0; j < H; ++j)
+                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sp[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < H; ++j)
+                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sm[j];
+            }
+            // shifted equilibrium moment: drho T0 + rho0 (3 T1.u + 4.5 u.T2.u - 1.5 u^2 T0)
+            float meq = 0.0f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                if (MT.T1[k][a] != 0.0) meq += float(3.0 * MT.T1[k][a]) * u[a];
+#pragma unroll
+                for (int b = 0; b < D; ++b)
+                    if (MT.T2[k][a][b] != 0.0) meq += float(4.5 * MT.T2[k][a][b]) * u[a] * u[b];
+            }
+            if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
+            meq *= rho0;
+            if (MT.T0[k] != 0.0) meq += float(MT.T0[k]) * drho;
+            const float c1 = MT.rate[k] == 1 ? p.c1_nu : p.c1_q;      // s_k
+            const float c2 = MT.rate[k] == 1 ? p.c2_nu : p.c2_q;      // 1 - s_k / 2
+            y[k] = (c1 * (meq - m) + c2 * mg) * float(1.0 / MT.n2[k]);
+        }
+        // h += M^T y, pair-wise by parity
+        {
+            float e0 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < Q; ++k)
+                if (MT.parity[k] > 0 && L::M[k][0] != 0) e0 += float(L::M[k][0]) * y[k];
+            h[0] += e0;
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int q = L::half[j];
+            float ev = 0.0f, od = 0.0f;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                if (L::M[k][q] == 0) continue;
+                if (MT.parity[k] > 0) ev += float(L::M[k][q]) * y[k];
+                else od += float(L::M[k][q]) * y[k];
+            }
+            h[q] += ev + od;
+            h[L::op[q]] += ev - od;
         }
     }
     bool finite = true;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         finite = finite && isfinite(h[q]);
-        fout[uint64_t(q) * N + i] = h[q];
+        st(fout, O(q) * P + slot, h[q]);
     }
-    if (!finite) raise_status(a.status, 4, a.t);
+    if (!finite) raise_status(status, 4, p.t);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int Q, bool MRT, bool INIT>
-static hipError_t launch_scsp_q(const ScspArgs& a, bool fp32, bool stream_solids, hipStream_t s)
+template <int Q, bool MRT, bool INIT, int LAYOUT, bool WIDE>
+static void launch_scsp_w(const ScspParams& p, const ScspBuffers& b, bool fp32, hipStream_t s)
 {
-    const dim3 grid(uint32_t((a.g.N + 255) / 256)), block(256);
-    if (fp32) {
-        if (stream_solids) hipLaunchKernelGGL((scsp_step_f32<Q, MRT, INIT, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((scsp_step_f32<Q, MRT, INIT, false>), grid, block, 0, s, a);
-    } else {
-        if (stream_solids) hipLaunchKernelGGL((scsp_step_f64<Q, MRT, INIT, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((scsp_step_f64<Q, MRT, INIT, false>), grid, block, 0, s, a);
-    }
-    return hipGetLastError();
+    const uint64_t n = LAYOUT == SPARSE ? p.nfluid : p.g.N;
+    const dim3 grid(uint32_t((n + 255) / 256)), block(256);
+    if (n == 0) return;
+    if (fp32)
+        hipLaunchKernelGGL((scsp_step_f32<Q, MRT, INIT, LAYOUT, WIDE>), grid, block, 0, s, p,
+                           static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.desc, b.list, b.map,
+                           b.lab32, b.rho, b.u, b.uadj, b.status);
+    else
+        hipLaunchKernelGGL((scsp_step_f64<Q, MRT, INIT, LAYOUT, WIDE>), grid, block, 0, s, p,
+                           static_cast<const double*>(b.fin), static_cast<double*>(b.fout), b.desc, b.list,
+                           b.map, b.lab, b.AT, b.B, b.rho, b.u, b.uadj, b.status);
 }
 
-hipError_t launch_scsp(int Q, bool mrt, bool init, bool fp32, bool stream_solids, const ScspArgs& a,
-                       hipStream_t s)
+template <int Q, bool MRT, bool INIT, int LAYOUT>
+static void launch_scsp_l(const ScspParams& p, const ScspBuffers& b, bool fp32, hipStream_t s)
+{
+    const uint64_t P = LAYOUT == SPARSE ? p.nfluid : p.g.N;
+    const uint64_t bytes = P * uint64_t(Q) * (fp32 ? 4u : 8u);
+    if (bytes >= (uint64_t(1) << 32)) launch_scsp_w<Q, MRT, INIT, LAYOUT, true>(p, b, fp32, s);
+    else launch_scsp_w<Q, MRT, INIT, LAYOUT, false>(p, b, fp32, s);
+}
+
+template <int Q, bool MRT, bool INIT>
+static void launch_scsp_q(const ScspParams& p, const ScspBuffers& b, bool fp32, int layout, hipStream_t s)
+{
+    if (layout == DENSE_ALL) launch_scsp_l<Q, MRT, INIT, DENSE_ALL>(p, b, fp32, s);
+    else if (layout == DENSE_FLUID) launch_scsp_l<Q, MRT, INIT, DENSE_FLUID>(p, b, fp32, s);
+    else launch_scsp_l<Q, MRT, INIT, SPARSE>(p, b, fp32, s);
+}
+
+hipError_t launch_scsp(int Q, bool mrt, bool init, bool fp32, int layout, const ScspParams& p,
+                       const ScspBuffers& b, hipStream_t s)
 {
     if (Q == 19) {
-        if (mrt) return init ? launch_scsp_q<19, true, true>(a, fp32, stream_solids, s)
-                             : launch_scsp_q<19, true, false>(a, fp32, stream_solids, s);
-        return init ? launch_scsp_q<19, false, true>(a, fp32, stream_solids, s)
-                    : launch_scsp_q<19, false, false>(a, fp32, stream_solids, s);
+        if (mrt) init ? launch_scsp_q<19, true, true>(p, b, fp32, layout, s)
+                      : launch_scsp_q<19, true, false>(p, b, fp32, layout, s);
+        else init ? launch_scsp_q<19, false, true>(p, b, fp32, layout, s)
+                  : launch_scsp_q<19, false, false>(p, b, fp32, layout, s);
+    } else {
+        if (mrt) init ? launch_scsp_q<9, true, true>(p, b, fp32, layout, s)
+                      : launch_scsp_q<9, true, false>(p, b, fp32, layout, s);
+        else init ? launch_scsp_q<9, false, true>(p, b, fp32, layout, s)
+                  : launch_scsp_q<9, false, false>(p, b, fp32, layout, s);
     }
-    if (mrt) return init ? launch_scsp_q<9, true, true>(a, fp32, stream_solids, s)
-                         : launch_scsp_q<9, true, false>(a, fp32, stream_solids, s);
-    return init ? launch_scsp_q<9, false, true>(a, fp32, stream_solids, s)
-                : launch_scsp_q<9, false, false>(a, fp32, stream_solids, s);
+    return hipGetLastError();
 }
 
 }  // namespace lbf

@@ -15,13 +15,15 @@ LIB_PATH = Path(__file__).resolve().parent / 'lib' / 'liblbflow.so'
 TYPES = {'VD': 0x1, 'GV': 0x2, 'PB': 0x4, 'VB': 0x8, 'AB': 0x10, 'BB': 0x20, 'HF': 0x40, 'SD': 0x80}
 FP64, FP32 = 0, 1
 STREAM_SOLIDS = 0x1
+LAYOUT_DENSE = 0x2
 STEP_FIELDS, STEP_ADJACENT = 0x1, 0x2
 STATUS_BITS = (0x1, 0x2, 0x4, 0x8, 0x10)
 
 SYMBOLS = ('lbf_create', 'lbf_destroy', 'lbf_last_error', 'lbf_init', 'lbf_step', 'lbf_advance',
            'lbf_time', 'lbf_synchronize', 'lbf_status', 'lbf_status_message', 'lbf_fields',
            'lbf_get_populations', 'lbf_set_populations', 'lbf_report', 'lbf_links', 'lbf_sites',
-           'lbf_device_bytes', 'lbf_step_kernel', 'lbf_step_bytes')
+           'lbf_device_bytes', 'lbf_step_kernel', 'lbf_step_bytes', 'lbf_timed_advance',
+           'lbf_copy_bandwidth')
 
 
 class Label(C.Structure):
@@ -99,6 +101,10 @@ def load_library(path=None):
     L.lbf_step_kernel.restype = C.c_char_p
     L.lbf_step_bytes.argtypes = [vp]
     L.lbf_step_bytes.restype = C.c_double
+    L.lbf_timed_advance.argtypes = [vp, u64, C.POINTER(C.c_double)]
+    L.lbf_timed_advance.restype = i32
+    L.lbf_copy_bandwidth.argtypes = [u64, C.c_int]
+    L.lbf_copy_bandwidth.restype = C.c_double
     if path is None:
         _LIB = L
     return L
@@ -112,6 +118,11 @@ class EngineError(RuntimeError):
     pass
 
 
+def copy_bandwidth(nbytes=1 << 30, iters=20):
+    """Achievable HBM bandwidth (GB/s) of a plain copy kernel."""
+    return float(load_library().lbf_copy_bandwidth(nbytes, iters))
+
+
 class Engine:
     """One lattice on the GPU.
 
@@ -120,14 +131,14 @@ class Engine:
     harmonic, labels {value: (TYPE, value, dir, phase)}.
     """
 
-    def __init__(self, case, geometry, precision=FP64, stream_solids=True):
+    def __init__(self, case, geometry, precision=FP64, stream_solids=True, dense=False):
         self.L = load_library()
         d = Desc()
         d.D, d.Q = case['D'], case['Q']
         d.model = 1 if case['model'] == 'color' else 0
         d.equil = 1 if case['eq'] == 'MRT' else 0
         d.precision = precision
-        d.flags = STREAM_SOLIDS if stream_solids else 0
+        d.flags = (STREAM_SOLIDS if stream_solids else 0) | (LAYOUT_DENSE if dense else 0)
         d.harmonic = 1 if case.get('harmonic') else 0
         for k, n in enumerate(case['size']):
             d.sz[k] = n
@@ -178,6 +189,12 @@ class Engine:
     def advance(self, n):
         self._check(self.L.lbf_advance(self.h, n))
         return self
+
+    def timed_advance(self, n):
+        """n steps timed with HIP events on the engine's stream; returns milliseconds."""
+        ms = C.c_double()
+        self._check(self.L.lbf_timed_advance(self.h, n, C.byref(ms)))
+        return ms.value
 
     def synchronize(self):
         self._check(self.L.lbf_synchronize(self.h))

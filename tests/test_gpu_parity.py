@@ -21,9 +21,10 @@ def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def engine_run(spec, steps, precision=FP64, stream_solids=True, geometry=None):
+def engine_run(spec, steps, precision=FP64, stream_solids=True, geometry=None, dense=False):
     case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec) if geometry is None else geometry, precision, stream_solids).init()
+    e = Engine(case, H.geometry(spec) if geometry is None else geometry, precision, stream_solids,
+               dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
     e.synchronize()
@@ -106,16 +107,25 @@ def test_sphere_pack_bitwise(kw):
     e.check_status()
 
 
-def test_skip_solids_matches_fluid_populations():
-    """Without solid streaming the fluid sites evolve identically; solid entries read 0."""
-    spec = sphere_case(model='scsp', eq='MRT', tau=(0.8,))
-    a, case = engine_run(spec, 60, stream_solids=True)
-    b, _ = engine_run(spec, 60, stream_solids=False)
+@pytest.mark.parametrize('precision', [FP64, FP32], ids=['fp64', 'fp32'])
+@pytest.mark.parametrize('eq', ['SRT', 'MRT'])
+@pytest.mark.parametrize('dims', [3, 2])
+def test_fluid_only_layouts_match(precision, eq, dims):
+    """Fluid-only layouts (dense with solids skipped, sparse fluid list) evolve the fluid
+    sites exactly like the reference-exact dense layout; solid entries read 0."""
+    spec = sphere_case(model='scsp', eq=eq, tau=(0.8,), dims=dims)
+    a, case = engine_run(spec, 60, precision, stream_solids=True)
+    b, _ = engine_run(spec, 60, precision, stream_solids=False, dense=True)
+    c, _ = engine_run(spec, 60, precision, stream_solids=False)
+    assert 'sparse' in c.step_kernel and 'dense-fluid' in b.step_kernel
     g = H.geometry(spec)
     fluid = np.tile(g == 75, case['Q'])
-    pa, pb = a.populations(), b.populations()
-    assert np.array_equal(pa[fluid], pb[fluid])
-    assert np.all(pb[~fluid] == 0.0)
+    pa, pb, pc = a.populations(), b.populations(), c.populations()
+    assert np.array_equal(pa[fluid], pb[fluid]) and np.array_equal(pa[fluid], pc[fluid])
+    assert np.all(pb[~fluid] == 0.0) and np.all(pc[~fluid] == 0.0)
+    for x, y in zip(a.fields(), c.fields()):
+        if x is not None:
+            assert np.array_equal(x, y)
 
 
 def test_restart_equivalence():
@@ -164,11 +174,12 @@ def test_report_sums_match_oracle():
     np.testing.assert_allclose(list(r.conv), ref, rtol=1e-12, atol=1e-30)
 
 
-def test_fp32_tracks_fp64():
+@pytest.mark.parametrize('eq', ['SRT', 'MRT'])
+def test_fp32_tracks_fp64(eq):
     """FP32 shifted populations: velocity field within 1e-5 relative (L2) of FP64 after 300 steps."""
-    spec = sphere_case(n=(16, 16, 16), model='scsp', eq='MRT', tau=(0.8,), porosity=0.4)
+    spec = sphere_case(n=(16, 16, 16), model='scsp', eq=eq, tau=(0.8,), porosity=0.4)
     a, case = engine_run(spec, 300, FP64)
-    b, _ = engine_run(spec, 300, FP32)
+    b, _ = engine_run(spec, 300, FP32, stream_solids=False)
     ua, ub = a.fields()[2], b.fields()[2]
     rel = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
     say('fp32 rel', rel)
