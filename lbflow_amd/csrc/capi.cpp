@@ -556,7 +556,7 @@ int lbf_init(lbf_ctx* c)
     c->has_prev = false;
     const size_t pop = size_t(c->NP) * c->Q * c->P * c->elem;
     HIP_CHECK(c, hipMemsetAsync(c->f[c->cur], 0, pop, c->stream));
-    if (run_step(c, true, 1u)) return -1;
+    if (run_step(c, true, 3u)) return -1;   // fields + adjacent history of a report at t = 1
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

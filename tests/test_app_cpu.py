@@ -1,0 +1,105 @@
+"""The drop-in LBFlow executable's input contract, on the CPU.
+
+Every case here fails during set-up, before any GPU call, with the reference's
+message (solver/src/file/fileprm.cpp:136-295, solver/apps/LBSim/lbsim.cpp:66-70,
+scripts/ci_smoke.py:58-63, scripts/benchmarks.py:257-260).
+"""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from lbflow_amd import BIN_DIR
+from lbflow_amd import harness as H
+
+LBFLOW = BIN_DIR / 'LBFlow'
+
+
+def run_case(tmp_path, spec, start=0):
+    prm = H.write_case(spec, tmp_path, start)
+    p = subprocess.run([str(LBFLOW), str(prm)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    return p.returncode, p.stdout + p.stderr
+
+
+def base():
+    return H.force_cases()[-1]
+
+
+@pytest.mark.parametrize('overrides,message', [
+    ({'writeInterval': 0}, 'Invalid time range or zero output/report/restart interval.'),
+    ({'rexlationTime': '0.8 0.5'}, 'Densities must be positive; relaxation times must exceed 0.5.'),
+    ({'rexlationTime': '.5 .5'}, 'Densities must be positive; relaxation times must exceed 0.5.'),
+    ({'alpha0': 1.1}, 'Invalid phase weights, surface tension, or recoloring beta.'),
+    ({'latticeSpeed': 2}, 'This research implementation requires latticeSpeed=1.'),
+    ({'directions': 15}, 'Only D2Q9 and D3Q19 are validated for simulation.'),
+    ({'fileType': '"uint16"'}, 'Simulation geometry must use uint8 labels.'),
+    ({'stopError': 0}, 'Invalid convergence tolerance or lattice length.'),
+    ({'equilModel': '"TRT"'}, 'Equilibrium model must be SRT or MRT.'),
+    ({'wallGradient': '"bogus"'}, 'wallGradient must be legacy, renormalized, tanh, tangent, halfway or ghost.'),
+    ({'viscosityInterpolation': '"geometric"'}, 'viscosityInterpolation must be arithmetic or harmonic.'),
+    ({'reportDir': 25}, 'Report direction is outside the lattice.'),
+    ({'reportDir': 1, 'reportSlice': '1'}, 'At least two report slices are required.'),
+    ({'reportDir': 1, 'reportSlice': '1 40'}, 'Report slice is outside the grid.'),
+    ({'reportDir': 3, 'reportSlice': '1 6'}, 'Report slice is outside the selected axis.'),
+    ({'fluidModel': '"shanchen"'}, 'Only scsp and color fluid models are implemented.'),
+    ({'fluidModel': '"lattice-gas"'}, 'Main(): Unsupport fluid model'),
+    ({'wallGradient': '"renormalized"'}, 'wallGradient "renormalized" is not implemented by this engine'),
+])
+def test_invalid_inputs_fail_with_reference_message(tmp_path, overrides, message):
+    spec = dict(base(), overrides=overrides)
+    code, out = run_case(tmp_path, spec)
+    assert code != 0
+    assert '0 Failed to excute: ' in out and message in out
+    assert 'Simulation finished!' not in out
+
+
+def test_undefined_label(tmp_path):
+    spec = dict(base(), label=lambda x, y, z: 75 if x else 9)
+    code, out = run_case(tmp_path, spec)
+    assert code != 0 and '0 Failed to excute: map::at' in out
+
+
+def test_missing_input_file(tmp_path):
+    p = subprocess.run([str(LBFLOW), str(tmp_path / 'absent.prm')], capture_output=True, text=True)
+    assert p.returncode != 0 and 'FilePrm::Load() - Input file does not exist.' in p.stdout
+
+
+def test_missing_argument():
+    p = subprocess.run([str(LBFLOW)], capture_output=True, text=True)
+    assert p.returncode == 1 and 'you must specify an input filename.' in p.stderr
+
+
+def test_short_geometry(tmp_path):
+    spec = base()
+    prm = H.write_case(spec, tmp_path)
+    raw = tmp_path / 'geometry.raw'
+    raw.write_bytes(raw.read_bytes()[:-1])
+    p = subprocess.run([str(LBFLOW), str(prm)], cwd=tmp_path, capture_output=True, text=True)
+    assert p.returncode != 0 and 'FileRaw::Load() - Failed to read from raw file' in p.stdout
+
+
+def test_missing_key(tmp_path):
+    spec = base()
+    prm = H.write_case(spec, tmp_path)
+    text = '\n'.join(l for l in prm.read_text().splitlines() if not l.startswith('endTime'))
+    prm.write_text(text + '\n')
+    p = subprocess.run([str(LBFLOW), str(prm)], cwd=tmp_path, capture_output=True, text=True)
+    assert p.returncode != 0 and 'the value may not be defined in parameter file.' in p.stdout
+
+
+def test_unsupported_boundary_type(tmp_path):
+    spec = dict(base(), indices=['75 "PB" "1.0" "1" "1"'])
+    code, out = run_case(tmp_path, spec)
+    assert code != 0 and 'PB/VB/AB/HF' in out
+
+
+def test_library_exports_every_declared_symbol():
+    """Every function declared in include/lbflow.h is exported by liblbflow.so."""
+    import re
+    from lbflow_amd.engine import load_library
+    header = (Path(__file__).resolve().parents[1] / 'include' / 'lbflow.h').read_text()
+    names = set(re.findall(r'^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(lbf_[a-z_0-9]+)\s*\(', header, re.M))
+    assert len(names) >= 20
+    L = load_library()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing

@@ -1,0 +1,187 @@
+"""The drop-in LBFlow executable end to end on the GPU.
+
+Checks the process contract of the reference (solver/apps/LBSim/lbsim.cpp,
+scsp.cpp:46-114, colorfluid.cpp:109-212): outputs, checkpoints, restart,
+failure contracts, report lines and the convergence decision, against the
+reference's recorded digest and values and against the CPU oracle.
+"""
+import hashlib
+import json
+import math
+import re
+import struct
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from lbflow_amd import BIN_DIR
+from lbflow_amd import harness as H
+from tests.helpers import C1_SHA256, c1_spec, oracle_run, velocity_grid
+
+pytestmark = pytest.mark.gpu
+LBFLOW = BIN_DIR / 'LBFlow'
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run(spec, folder, start=0, checkpoint=None, extra=()):
+    prm = H.write_case(spec, folder, start)
+    if checkpoint:
+        for suffix in ('.rst', '.rst.meta'):
+            Path(folder / f'field.{start:010d}{suffix}').write_bytes(Path(str(checkpoint) + suffix).read_bytes())
+    say('LBFlow', spec['name'], folder)
+    p = subprocess.run([str(LBFLOW), str(prm), *extra], cwd=folder, capture_output=True, text=True, timeout=300)
+    timing = [json.loads(l[len('LBFlowTiming '):]) for l in p.stdout.splitlines() if l.startswith('LBFlowTiming ')]
+    return p.returncode, p.stdout, timing[0] if timing else None
+
+
+def test_c1_process_and_checkpoint_digest(tmp_path):
+    spec = c1_spec()
+    code, out, timing = run(spec, tmp_path)
+    assert code == 0, out[-2000:]
+    for marker in ('Model constructing...Done!', 'Simulation started!', 'Initializing...Done!',
+                   'Step 20000 - Time elapsed: ', 'Simulation finished!', 'Total Time elapsed: '):
+        assert marker in out
+    assert timing['completed_steps'] == 20000 and timing['ranks'] == 1
+    assert timing['fluid_nodes'] == 256 and timing['total_voxels'] == 320
+    rst = tmp_path / 'field.0000020000.rst'
+    assert hashlib.sha256(rst.read_bytes()).hexdigest() == C1_SHA256
+    meta = (tmp_path / 'field.0000020000.rst.meta').read_text()
+    assert meta == ('DRLAB_OUTGOING_V1\n20000 1 3 19\n4 10 8 \nNEXT_STREAM 20001\n'
+                    'INITIAL_COLLISION_INCLUDED 20001\nCONVERGENCE_HISTORY_RESET_ON_LOAD\n')
+    assert len(sorted(tmp_path.glob('field.*.vti'))) == 20000 // 250 + 1
+    # VTI fields equal the oracle's macro fields
+    (nx, ny, nz), f = H.read_vti(tmp_path / 'field.0000020000.vti')
+    o, case = oracle_run(spec, 20000)
+    r0, _, u = o.fields()
+    assert np.array_equal(f['density'][..., 0].ravel(), r0)
+    assert np.array_equal(f['velocity'], velocity_grid(u, case))
+    assert (tmp_path / 'field.0000020000.vti').stat().st_size == 10770
+
+
+@pytest.mark.parametrize('spec', [s for s in H.force_cases() if s['dimensions'] == 3], ids=lambda s: s['name'])
+def test_force_cases_through_process(spec, tmp_path, records):
+    code, out, timing = run(spec, tmp_path)
+    assert code == 0, out[-2000:]
+    assert timing['completed_steps'] == spec['steps']
+    _, f = H.read_vti(tmp_path / f'field.{spec["steps"]:010d}.vti')
+    v = f['v' if spec['model'] == 'color' else 'velocity']
+    rec = records['force'][spec['name'] + '-1rank']
+    assert float(np.max(np.abs(v[..., 0] - 1e-4))) == rec['absolute_error']
+
+
+def drop_spec():
+    """12^3 two-phase drop (scripts/ci_smoke.py:42-56)."""
+    return H.spec('drop-continuous', nx=12, ny=12, nz=12, steps=40, write=20, restart=20,
+                  label=lambda x, y, z: 100 if (x - 5.5) ** 2 + (y - 5.5) ** 2 + (z - 5.5) ** 2 <= 9 else 0,
+                  indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"'])
+
+
+def test_restart_is_bitwise_and_failure_contracts(tmp_path):
+    spec = drop_spec()
+    cont = tmp_path / 'continuous'
+    code, out, timing = run(spec, cont)
+    assert code == 0, out[-2000:]
+    checkpoint = cont / 'field.0000000020'
+    resumed = tmp_path / 'resumed'
+    code, out, timing = run(spec, resumed, 20, checkpoint)
+    assert code == 0, out[-2000:]
+    assert timing['completed_steps'] == 20
+    a = (cont / 'field.0000000040.vti').read_bytes()
+    b = (resumed / 'field.0000000040.vti').read_bytes()
+    assert a == b
+    assert (cont / 'field.0000000040.rst').read_bytes() == (resumed / 'field.0000000040.rst').read_bytes()
+    # failure contracts (ci_smoke.py:64-78)
+    pops = Path(str(checkpoint) + '.rst').read_bytes()
+    meta = Path(str(checkpoint) + '.rst.meta').read_text()
+    cases = {'missing-stage': (pops, 'NO_VALID_STAGE\n', 'Missing or incompatible restart metadata'),
+             'short-populations': (pops[:-8], meta, 'Restart population file has an invalid length.'),
+             'nonfinite-population': (struct.pack('<d', float('nan')) + pops[8:], meta,
+                                      'Non-finite or non-positive total fluid density.')}
+    for name, (data, m, message) in cases.items():
+        folder = tmp_path / name
+        folder.mkdir()
+        src = folder / 'src'
+        Path(str(src) + '.rst').write_bytes(data)
+        Path(str(src) + '.rst.meta').write_text(m)
+        code, out, timing = run(spec, folder, 20, src)
+        assert code != 0 and '0 Failed to excute: ' in out and message in out, (name, out[-500:])
+        assert timing is None
+
+
+def report_lines(out):
+    return [l for l in out.splitlines() if l.startswith(' - ')]
+
+
+def test_reports_and_convergence_match_oracle(tmp_path):
+    """Report values and the stop step follow SCSP::report / Simulator::updateConvergence,
+    evaluated here with the oracle's fields (sums to rounding, decisions exactly)."""
+    spec = [s for s in H.channel_cases() if s['name'] == 'channel-scsp-d3-MRT-tau1.4-H8'][0]
+    spec = dict(spec, steps=3000, write=3000, report_interval=50, report_dir=1, stop_error=1e-4, restart=3000,
+                name='converge')
+    code, out, timing = run(spec, tmp_path)
+    assert code == 0, out[-2000:]
+    lines = report_lines(out)
+    # oracle emulation of the report loop
+    o, case = oracle_run(spec, 0)
+    N, D = o.N, case['D']
+    nx = spec['size'][0]
+    first, last = 1, nx - 2
+    prev, has_prev, stable, kold, t = None, False, 0, 0.0, 0
+    nu = (spec['tau'][0] - 0.5) / 3
+    kpar = (last - first) * nu / (spec['size'][1] * spec['size'][2]) * 1e12
+    expected = []
+    adj = o.fields()[2].ravel().copy()
+    while t < spec['steps']:
+        o.step(1)
+        t += 1
+        if t % 50:
+            adj = o.fields()[2].ravel().copy()
+            continue
+        s = o.report_sums(2, first, last)
+        dP = (s[3] / s[7] - s[5] / s[8]) / 3.0
+        Q = s[0] / (last - first + 1)
+        drop = abs(dP) > 1e-14
+        K = kpar * Q / dP if drop else 0.0
+        errK = abs(K - kold) / max(abs(K), 1e-30) if drop else 0.0
+        kold = K
+        expected.append(Q)
+        u = o.fields()[2].ravel().copy()
+        if has_prev:
+            c = o.convergence_sums(prev, adj)
+            rms = math.sqrt(c[0] / c[2])
+            res = math.sqrt(c[0] / c[1]) if c[1] > 0 else 0.0
+            arms = math.sqrt(c[3] / c[2])
+            ares = math.sqrt(c[3] / c[1]) if c[1] > 0 else 0.0
+            ok = (not drop or errK < 1e-4) and (rms <= 1e-12 or res < 1e-4) and (arms <= 1e-12 or ares < 1e-4)
+            stable = stable + 1 if ok else 0
+        has_prev = True
+        prev = u
+        adj = u
+        if stable >= 3:
+            break
+    assert timing['completed_steps'] == t
+    q_printed = [float(m.group(1)) for m in (re.search(r'Q = ([-+0-9.e]+)\(lu\^3/ts\)', l) for l in lines) if m]
+    assert len(q_printed) == len(expected)
+    np.testing.assert_allclose(q_printed, expected, rtol=1e-5)
+    assert (tmp_path / f'field.{t:010d}.rst').exists()
+    assert any('pressure-based permeability unavailable' in l for l in lines)
+
+
+def test_fp32_sparse_engine_option(tmp_path):
+    """`--fp32 --sparse` runs the FP32 shifted sparse engine; the Poiseuille profile stays
+    within 1e-6 relative of the reference's recorded FP64 profile."""
+    spec = [s for s in H.channel_cases() if s['name'] == 'channel-scsp-d3-MRT-tau0.6-H8'][0]
+    code, out, timing = run(spec, tmp_path, extra=('--fp32', '--sparse'))
+    assert code == 0, out[-2000:]
+    _, f = H.read_vti(tmp_path / f'field.{spec["steps"]:010d}.vti')
+    prof = f['velocity'][..., 0].mean(axis=(0, 2))[1:-1]
+    import json as _j
+    want = np.array(_j.loads((Path(__file__).parent / 'golden' / 'reference_records.json').read_text())
+                    ['channel'][spec['name'] + '-1rank']['measured_profile'])
+    assert np.max(np.abs(prof - want) / np.abs(want)) < 1e-6
