@@ -320,14 +320,16 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     rc |= dalloc(c, &c->d_B32, Q2 * 4);
     rc |= dalloc(c, &c->d_rho0, N * 8);
     rc |= dalloc(c, &c->d_u, size_t(d.D) * N * 8);
-    rc |= dalloc(c, &c->d_uadj, size_t(d.D) * N * 8);
+    // adjacent history, two buffers by step parity: step t writes buffer t & 1,
+    // the report of step t reads buffer (t - 1) & 1
+    rc |= dalloc(c, &c->d_uadj, 2 * size_t(d.D) * N * 8);
     rc |= dalloc(c, &c->d_uprev, size_t(d.D) * N * 8);
     rc |= dalloc(c, &c->d_status, sizeof(DevStatus));
     rc |= dalloc(c, &c->d_partial, size_t(report_blocks(N)) * NREP * 8);
     rc |= dalloc(c, &c->d_out, NREP * 8);
     if (NP == 2) {
         rc |= dalloc(c, &c->d_rho1, N * 8);
-        rc |= dalloc(c, &c->d_chiadj, N * 8);
+        rc |= dalloc(c, &c->d_chiadj, 2 * N * 8);
         rc |= dalloc(c, &c->d_chiprev, N * 8);
         rc |= dalloc(c, &c->d_densN, N * 8);
         rc |= dalloc(c, &c->d_wall, N * 4);
@@ -399,11 +401,11 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     for (void* p : {c->f[0], c->f[1]}) (void)hipMemset(p, 0, pop);
     (void)hipMemset(c->d_rho0, 0, N * 8);
     (void)hipMemset(c->d_u, 0, size_t(d.D) * N * 8);
-    (void)hipMemset(c->d_uadj, 0, size_t(d.D) * N * 8);
+    (void)hipMemset(c->d_uadj, 0, 2 * size_t(d.D) * N * 8);
     (void)hipMemset(c->d_uprev, 0, size_t(d.D) * N * 8);
     if (NP == 2) {
         (void)hipMemset(c->d_rho1, 0, N * 8);
-        (void)hipMemset(c->d_chiadj, 0, N * 8);
+        (void)hipMemset(c->d_chiadj, 0, 2 * N * 8);
         (void)hipMemset(c->d_chiprev, 0, N * 8);
     }
     DevStatus st{};
@@ -465,7 +467,8 @@ static ScspBuffers scsp_buffers(lbf_ctx* c, bool init)
     b.B = c->d_B;
     b.rho = c->d_rho0;
     b.u = c->d_u;
-    b.uadj = c->d_uadj;
+    const uint64_t step = init ? 0 : c->t + 1;
+    b.uadj = c->d_uadj + (step & 1) * uint64_t(c->D) * c->N;
     b.status = c->d_status;
     return b;
 }
@@ -485,8 +488,6 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
     a.rho0 = c->d_rho0;
     a.rho1 = c->d_rho1;
     a.u = c->d_u;
-    a.uadj = c->d_uadj;
-    a.chiadj = c->d_chiadj;
     a.status = c->d_status;
     a.flags = flags;
     a.mrt = c->d.equil == LBF_MRT;
@@ -536,6 +537,8 @@ static int run_step(lbf_ctx* c, bool init, uint32_t flags)
     } else {
         ColorArgs a = color_args(c, flags);
         a.t = init ? 0 : c->t + 1;
+        a.uadj = c->d_uadj + (a.t & 1) * uint64_t(c->D) * c->N;
+        a.chiadj = c->d_chiadj + (a.t & 1) * c->N;
         a.fin = static_cast<const double*>(c->f[c->cur]);
         a.fout = static_cast<double*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
         e = launch_color(c->Q, mrt, init, stream_solids, a, c->stream);
@@ -657,10 +660,10 @@ int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
     // simulator.cpp:382-383; the macro arrays are fresh zeros until the first step).
     HIP_CHECK(c, hipMemset(c->d_rho0, 0, c->N * 8));
     HIP_CHECK(c, hipMemset(c->d_u, 0, size_t(c->D) * c->N * 8));
-    HIP_CHECK(c, hipMemset(c->d_uadj, 0, size_t(c->D) * c->N * 8));
+    HIP_CHECK(c, hipMemset(c->d_uadj, 0, 2 * size_t(c->D) * c->N * 8));
     if (c->NP == 2) {
         HIP_CHECK(c, hipMemset(c->d_rho1, 0, c->N * 8));
-        HIP_CHECK(c, hipMemset(c->d_chiadj, 0, c->N * 8));
+        HIP_CHECK(c, hipMemset(c->d_chiadj, 0, 2 * c->N * 8));
     }
     reset_status(c);
     c->t = t;
@@ -680,9 +683,11 @@ int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
     a.rho1 = c->NP == 2 ? c->d_rho1 : c->d_rho0;
     a.u = c->d_u;
     a.uprev = c->d_uprev;
-    a.uadj = c->d_uadj;
+    // the adjacent history of a report at step t was written by step t - 1
+    const uint64_t adjacent_buffer = (c->t + 1) & 1;
+    a.uadj = c->d_uadj + adjacent_buffer * uint64_t(c->D) * c->N;
     a.chiprev = c->d_chiprev;
-    a.chiadj = c->d_chiadj;
+    a.chiadj = c->NP == 2 ? c->d_chiadj + adjacent_buffer * c->N : nullptr;
     a.partial = c->d_partial;
     a.D = c->D;
     a.axis = req->axis;

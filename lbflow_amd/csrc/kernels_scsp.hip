@@ -38,6 +38,7 @@
 // precision and the arithmetic is about 400 FP32 operations per site.
 #include <type_traits>
 
+#include "collide_f32.h"
 #include "device.h"
 
 namespace lbf {
@@ -46,47 +47,6 @@ constexpr uint32_t F_FIELDS = 1, F_ADJ = 2;
 constexpr double C1 = 3.0, C2 = 4.5, C3 = 1.5;   // 3/c^2, 4.5/c^4, 1.5/c^2 with c = 1
 enum : int { DENSE_ALL = 0, DENSE_FLUID = 1, SPARSE = 2 };
 
-// Moment tables of the transform M (compile time): for row k,
-//   T0[k] = sum_q M_kq w_q,  T1[k][a] = sum_q M_kq w_q e_qa,
-//   T2[k][a][b] = sum_q M_kq w_q e_qa e_qb,  n2[k] = sum_q M_kq^2,
-// parity[k] = +1 if row k is even under q -> op(q), -1 if odd, and
-// rate[k] = 0 (conserved), 1 (s_nu = 1/tau) or 2 (s_q) of the legacy
-// spectrum (solver/src/lbm/simulator.cpp:199-216).
-template <int Q> struct MomentTables {
-    double T0[Q];
-    double T1[Q][3];
-    double T2[Q][3][3];
-    double n2[Q];
-    int parity[Q];
-    int rate[Q];
-    constexpr MomentTables() : T0{}, T1{}, T2{}, n2{}, parity{}, rate{}
-    {
-        using L = Lattice<Q>;
-        for (int k = 0; k < Q; ++k) {
-            bool even = true, odd = true;
-            for (int q = 0; q < Q; ++q) {
-                const double m = L::M[k][q], w = L::w[q];
-                T0[k] += m * w;
-                n2[k] += m * m;
-                for (int a = 0; a < L::D; ++a) {
-                    T1[k][a] += m * w * L::e[q][a];
-                    for (int b = 0; b < L::D; ++b) T2[k][a][b] += m * w * L::e[q][a] * L::e[q][b];
-                }
-                even = even && L::M[k][L::op[q]] == L::M[k][q];
-                odd = odd && L::M[k][L::op[q]] == -L::M[k][q];
-            }
-            parity[k] = even ? 1 : (odd ? -1 : 0);
-        }
-        if constexpr (Q == 9) {
-            const int r[9] = {0, 1, 1, 0, 2, 0, 2, 1, 1};
-            for (int k = 0; k < 9; ++k) rate[k] = r[k];
-        } else {
-            const int r[19] = {0, 1, 1, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2};
-            for (int k = 0; k < 19; ++k) rate[k] = r[k];
-        }
-    }
-};
-template <int Q> inline constexpr MomentTables<Q> kMT{};
 
 // ---------------------------------------------------------------------------
 // site / slot resolution
@@ -369,8 +329,6 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
 #pragma clang fp contract(fast)
     using L = Lattice<Q>;
     constexpr int D = L::D;
-    constexpr int H = L::NHALF;
-    constexpr MomentTables<Q> MT = kMT<Q>;
     using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
     uint32_t i, slot;
     if (!resolve<LAYOUT>(p, list, i, slot)) return;
@@ -403,20 +361,8 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
         for (int q = 1; q < Q; ++q) h[q] = ld(fin, pull_offset<Q, LAYOUT, O>(map, dsc, slot, stn, P, q));
     }
     // macro on shifted populations: rho = rho0 + sum h, j = sum e h
-    float drho = 0.0f;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) drho += h[q];
     float u[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        float s = 0.0f;
-#pragma unroll
-        for (int q = 1; q < Q; ++q) {
-            if (L::e[q][d] == 1) s += h[q];
-            else if (L::e[q][d] == -1) s -= h[q];
-        }
-        u[d] = s * p.inv_rho0 + lb.half_rho[d];
-    }
+    const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
     if (!isfinite(drho) || p.dens0 + double(drho) <= 0.0) raise_status(status, 0, p.t);
     if (p.flags & F_FIELDS) {
         rho_out[i] = p.dens0 + double(drho);
@@ -427,104 +373,8 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
 #pragma unroll
         for (int d = 0; d < D; ++d) uadj[uint64_t(d) * p.g.N + i] = double(u[d]);
     }
-    float usq = 0.0f, uF = 0.0f;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        usq += u[d] * u[d];
-        uF += u[d] * F[d];
-    }
-    const float rho0 = p.rho0_32;
-    if constexpr (!MRT) {
-        // h += omega (h_eq - h) + (1 - omega/2) G,  h_eq = w (drho + rho0 (3eu + 4.5eu^2 - 1.5u^2))
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            float eu = 0.0f, ef = 0.0f;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                if (L::e[q][d] == 1) { eu += u[d]; ef += F[d]; }
-                else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
-            }
-            const float w = float(L::w[q]);
-            const float heq = w * (drho + rho0 * (eu * (3.0f + 4.5f * eu) - 1.5f * usq));
-            const float G = w * (3.0f * (ef - uF) + 9.0f * eu * ef);
-            h[q] += p.omega32 * (heq - h[q]) + p.omh32 * G;
-        }
-    } else {
-        // moments of h, pair-summed by parity
-        float sp[H], sm[H];
-#pragma unroll
-        for (int j = 0; j < H; ++j) {
-            const int q = L::half[j];
-            sp[j] = h[q] + h[L::op[q]];
-            sm[j] = h[q] - h[L::op[q]];
-        }
-        float y[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            // Guo source moment: M G = 3 T1.F - 3 (u.F) T0 + 9 u.T2.F
-            float mg = 0.0f;
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                if (MT.T1[k][a] != 0.0) mg += float(3.0 * MT.T1[k][a]) * F[a];
-#pragma unroll
-                for (int b = 0; b < D; ++b)
-                    if (MT.T2[k][a][b] != 0.0) mg += float(9.0 * MT.T2[k][a][b]) * u[a] * F[b];
-            }
-            if (MT.T0[k] != 0.0) mg -= float(3.0 * MT.T0[k]) * uF;
-            if (MT.rate[k] == 0) {
-                y[k] = mg * float(1.0 / MT.n2[k]);
-                continue;
-            }
-            // m_k = (M h)_k
-            float m = 0.0f;
-            if (MT.parity[k] > 0) {
-                if (L::M[k][0] != 0) m += float(L::M[k][0]) * h[0];
-#pragma unroll
-                for (int j = 0; j < H; ++j)
-                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sp[j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < H; ++j)
-                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sm[j];
-            }
-            // shifted equilibrium moment: drho T0 + rho0 (3 T1.u + 4.5 u.T2.u - 1.5 u^2 T0)
-            float meq = 0.0f;
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                if (MT.T1[k][a] != 0.0) meq += float(3.0 * MT.T1[k][a]) * u[a];
-#pragma unroll
-                for (int b = 0; b < D; ++b)
-                    if (MT.T2[k][a][b] != 0.0) meq += float(4.5 * MT.T2[k][a][b]) * u[a] * u[b];
-            }
-            if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
-            meq *= rho0;
-            if (MT.T0[k] != 0.0) meq += float(MT.T0[k]) * drho;
-            const float c1 = MT.rate[k] == 1 ? p.c1_nu : p.c1_q;      // s_k
-            const float c2 = MT.rate[k] == 1 ? p.c2_nu : p.c2_q;      // 1 - s_k / 2
-            y[k] = (c1 * (meq - m) + c2 * mg) * float(1.0 / MT.n2[k]);
-        }
-        // h += M^T y, pair-wise by parity
-        {
-            float e0 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < Q; ++k)
-                if (MT.parity[k] > 0 && L::M[k][0] != 0) e0 += float(L::M[k][0]) * y[k];
-            h[0] += e0;
-        }
-#pragma unroll
-        for (int j = 0; j < H; ++j) {
-            const int q = L::half[j];
-            float ev = 0.0f, od = 0.0f;
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                if (L::M[k][q] == 0) continue;
-                if (MT.parity[k] > 0) ev += float(L::M[k][q]) * y[k];
-                else od += float(L::M[k][q]) * y[k];
-            }
-            h[q] += ev + od;
-            h[L::op[q]] += ev - od;
-        }
-    }
+    const RelaxF32 relax{p.rho0_32, p.omega32, p.omh32, p.c1_nu, p.c2_nu, p.c1_q, p.c2_q};
+    collide_f32<Q, MRT>(h, drho, u, F, relax);
     bool finite = true;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
