@@ -82,6 +82,63 @@ def cpu_baseline(spec, labels, budget_s):
                        f'step, {dt:.1f} s, 1 thread')
 
 
+def time_to_permeability(spec, labels, interval, max_steps):
+    """Config-2 permeability in FP64 and FP32: wall clock from init to the
+    reference's stop rule (residual 1e-6, three stable reports) and the
+    Darcy K of each, with |K32 - K64| / K64."""
+    from lbflow_amd import harness as H
+    from lbflow_amd.engine import FP32, FP64, Engine
+    from lbflow_amd.permeability import converge
+    case = H.lattice_case(spec)
+    n = labels.shape[2]
+    out = {}
+    for name, prec in (('fp64', FP64), ('fp32', FP32)):
+        t0 = time.perf_counter()
+        eng = Engine(case, labels.ravel(), prec, False).init()
+        r = converge(eng, axis=2, first=1, last=n - 2, tol=1e-6, interval=interval, max_steps=max_steps,
+                     force=1e-6, tau=0.8)
+        wall = time.perf_counter() - t0
+        eng.close()
+        out[name] = dict(steps=r['steps'], converged=r['converged'], K_lattice=r['K'], seconds=round(wall, 3))
+        log('time-to-permeability', name, out[name])
+    out['k_rel_diff_fp32_vs_fp64'] = abs(out['fp32']['K_lattice'] - out['fp64']['K_lattice']) / out['fp64']['K_lattice']
+    out['report_interval'] = interval
+    return out
+
+
+def batch_bench(ncrops, interval, max_steps):
+    """Batched permeability: ncrops independent cubic sphere-pack crops (side
+    32-96, porosity 0.15-0.35, seeds 1000+i), D3Q19 MRT FP32, tau 0.8, F 1e-6
+    along x, every crop run to residual 1e-5 in one persistent hipGraph-captured
+    launch sequence."""
+    from lbflow_amd import geometry as G
+    from lbflow_amd.engine import Batch
+    t0 = time.perf_counter()
+    crops, _ = G.batch_crops(ncrops)
+    log(f'batch: {ncrops} crops generated in {time.perf_counter() - t0:.1f} s')
+    labels = {75: ('GV', 1e-6, 1, 0), 255: ('SD', 90.0, 0, 0)}
+    b = Batch(3, 19, 'MRT', 0.8, 1.0, labels, crops)
+    per_cu, tune_ms = b.tune(steps=10)
+    log(f'batch: {b.sites} sites, {b.fluid} fluid, tuned {per_cu} workgroups/CU, {tune_ms:.3f} ms/step')
+    t0 = time.perf_counter()
+    b.init()
+    res, sec = b.run(axis=2, tol=1e-5, interval=interval, max_steps=max_steps)
+    wall = time.perf_counter() - t0
+    site_updates = sum(int(g.size) * r['steps'] for g, r in zip(crops, res))
+    fluid_updates = sum(int((g == 75).sum()) * r['steps'] for g, r in zip(crops, res))
+    steps = [r['steps'] for r in res]
+    out = dict(crops=ncrops, converged=sum(r['converged'] for r in res), seconds=round(wall, 3),
+               crops_per_s=round(ncrops / wall, 3), site_mlups=round(site_updates / wall / 1e6, 1),
+               fluid_mlups=round(fluid_updates / wall / 1e6, 1), sites=b.sites, fluid_sites=b.fluid,
+               steps_min=min(steps), steps_max=max(steps), steps_mean=round(sum(steps) / len(steps), 1),
+               report_interval=interval, tuned_workgroups_per_cu=per_cu,
+               all_crops_ms_per_step=round(tune_ms, 4),
+               workload='512 cubic sphere-pack crops, side U{32..96}, porosity U[0.15,0.35], seeds 1000+i, '
+                        'D3Q19 MRT FP32, tau 0.8, F 1e-6 x, residual 1e-5')
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument('--gpus', type=int, default=1)
@@ -90,10 +147,16 @@ def main():
     ap.add_argument('--size', type=int, default=256)
     ap.add_argument('--cpu-budget', type=float, default=20.0, help='seconds of CPU oracle timing (0: skip)')
     ap.add_argument('--fp64', action='store_true', help='also time the FP64 reference-order kernel')
+    ap.add_argument('--ttp', action='store_true', help='also run FP64 and FP32 to convergence (time-to-K)')
+    ap.add_argument('--ttp-interval', type=int, default=500)
+    ap.add_argument('--ttp-max-steps', type=int, default=100000)
+    ap.add_argument('--batch', type=int, default=0, help='also run N batched crops to convergence (512)')
+    ap.add_argument('--batch-interval', type=int, default=200)
+    ap.add_argument('--batch-max-steps', type=int, default=60000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported '
-              '(multi-GPU sharding is design-only, see DESIGN.md)', file=sys.stderr)
+              '(sharding across GPUs is not built)', file=sys.stderr)
         sys.exit(2)
 
     from lbflow_amd.engine import FP32, FP64, copy_bandwidth
@@ -116,6 +179,10 @@ def main():
     if args.fp64:
         record('fp64_sparse', measure(spec, labels, FP64, False, max(10, args.steps // 2), 5))
         record('fp64_dense_all_sites', measure(spec, labels, FP64, True, max(10, args.steps // 4), 5))
+    if args.ttp:
+        extra['time_to_permeability'] = time_to_permeability(spec, labels, args.ttp_interval, args.ttp_max_steps)
+    if args.batch:
+        extra['batch'] = batch_bench(args.batch, args.batch_interval, args.batch_max_steps)
     for k, v in extra.items():
         log(k, v)
 

@@ -188,6 +188,66 @@ double lbf_step_bytes(const lbf_ctx* ctx);
 int lbf_timed_advance(lbf_ctx* ctx, uint64_t n, double* ms);
 double lbf_copy_bandwidth(uint64_t bytes, int iters);
 
+/* --- batched crops (no reference counterpart) ---------------------------
+ * Many independent single-phase crops (VD/GV fluid, BB/SD solids, periodic
+ * per crop) advanced by one persistent FP32 kernel.  A crop's trajectory is
+ * the one lbf_create(FP32, fluid-only layout) computes for that crop alone:
+ * same operator (SCSP::equilSRT/equilMRT, scsp.cpp:374-464), same FP32
+ * arithmetic.  The replaced reference workflow is one LBFlow process per
+ * crop (scripts/rock_workflow.py:63-69 loops over crops).  lbf_batch_run
+ * applies the stop rule of Simulator::updateConvergence (simulator.cpp:
+ * 248-325: velocity and adjacent-velocity residual below tol, or their RMS
+ * change <= 1e-12, for 3 consecutive reports) to every crop independently;
+ * converged crops stop stepping. */
+typedef struct lbf_batch lbf_batch;
+
+typedef struct lbf_batch_desc {
+    int32_t  D, Q;             /* 2,9 or 3,19 */
+    int32_t  equil;            /* lbf_equil */
+    uint32_t ncrops;
+    double   tau;              /* relaxation time */
+    double   density;          /* reference density rho0 */
+} lbf_batch_desc;
+
+typedef struct lbf_crop {
+    uint64_t       sz[3];      /* crop sizes, slowest first */
+    const uint8_t* geometry;   /* prod(sz) label bytes, z,y,x order */
+} lbf_crop;
+
+typedef struct lbf_crop_result {
+    uint64_t steps;            /* step of the crop's last report */
+    int32_t  converged;
+    int32_t  reserved;
+    double   residual;         /* velocity residual of the last report */
+    double   adjacent_residual;
+    double   k_darcy;          /* nu <u_axis> / F over all crop voxels, lattice units
+                                  (scripts/rock_workflow.py:63-69) */
+} lbf_crop_result;
+
+int  lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels /* 256 */,
+                      const lbf_crop* crops /* desc->ncrops */, lbf_batch** out);
+void lbf_batch_destroy(lbf_batch* batch);
+const char* lbf_batch_error(const lbf_batch* batch);  /* batch may be NULL: last create error */
+/* Total voxels of all crops; *fluid (if not NULL) = fluid sites. */
+uint64_t lbf_batch_sites(const lbf_batch* batch, uint64_t* fluid);
+/* Times `steps` steps for each persistent grid size (1..32 workgroups per
+ * CU) and keeps the fastest.  Steps the current state: call before
+ * lbf_batch_init. */
+int lbf_batch_tune(lbf_batch* batch, uint32_t steps, uint32_t* per_cu, double* ms_per_step);
+/* t = 0 state of every crop (SCSP::Run initial branch, scsp.cpp:60-72). */
+int lbf_batch_init(lbf_batch* batch);
+/* n steps of every crop, no convergence bookkeeping. */
+int lbf_batch_advance(lbf_batch* batch, uint64_t n);
+/* Runs every crop to convergence along `axis` (reference order), reporting
+ * every `interval` steps (rounded up to an even number >= 2), at most
+ * max_steps; results[ncrops]; *seconds = wall time of the call. */
+int lbf_batch_run(lbf_batch* batch, int axis, double tol, uint32_t interval, uint64_t max_steps,
+                  lbf_crop_result* results, double* seconds);
+/* Crop c's populations, [q][site] over all crop voxels (0 on solids), as FP64. */
+int lbf_batch_crop_populations(lbf_batch* batch, uint32_t c, double* out);
+/* Current step of the batch. */
+uint64_t lbf_batch_time(const lbf_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

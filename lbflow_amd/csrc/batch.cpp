@@ -1,0 +1,491 @@
+// batch.cpp -- C ABI of the batched-crop engine (include/lbflow.h, lbf_batch_*).
+//
+// Hundreds of independent single-phase crops share one slot space and one
+// persistent step kernel (kernels_batch.hip).  A run captures `interval`
+// steps plus the per-crop convergence reduction in one hipGraph, replays it,
+// applies the reference's stop rule per crop on the host (velocity residual
+// and adjacent residual below the tolerance for three consecutive reports,
+// Simulator::updateConvergence, simulator.cpp:248-325) and drops converged
+// crops from the device-resident active-tile list.  The autotuner times the
+// persistent grid size (workgroups per CU) on the actual batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbflow.h"
+#include "batch_types.h"
+#include "setup.h"
+
+namespace lbf {
+hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, const BatchParams& p,
+                             const BatchBuffers& b, hipStream_t s);
+hipError_t launch_batch_report(uint32_t grid, uint64_t S, int D, int axis, const BatchBuffers& b, uint32_t ncrops,
+                               double* partial, double* out, hipStream_t s);
+}  // namespace lbf
+
+using namespace lbf;
+
+struct lbf_batch {
+    lbf_batch_desc d{};
+    int D = 0, Q = 0;
+    uint32_t ncrops = 0;
+    uint64_t S = 0, T = 0, sites = 0, fluid = 0;
+    std::vector<CropDev> crops;
+    std::vector<uint64_t> crop_sites;
+    std::vector<double> crop_force;      // force component along each axis of the crop's first fluid label
+    std::vector<uint32_t> tile_crop;
+    hipStream_t stream = nullptr;
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint32_t* d_tiles = nullptr;
+    uint32_t* d_ntiles = nullptr;
+    uint32_t* d_tile_crop = nullptr;
+    CropDev* d_crops = nullptr;
+    uint32_t* d_site = nullptr;
+    uint8_t* d_lab8 = nullptr;
+    int32_t* d_map = nullptr;
+    LabelF* d_lab = nullptr;
+    float *d_ucur = nullptr, *d_uprev = nullptr, *d_uadj = nullptr;
+    DevStatus* d_status = nullptr;
+    double *d_partial = nullptr, *d_out = nullptr;
+    std::vector<void*> allocations;
+    uint32_t grid = 0, cus = 0;
+    std::string err;
+    uint64_t t = 0;
+};
+
+namespace {
+
+thread_local std::string g_batch_error;
+
+int bfail(lbf_batch* b, const std::string& m)
+{
+    if (b) b->err = m;
+    else g_batch_error = m;
+    return -1;
+}
+
+#define BHIP(b, expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return bfail(b, std::string("HIP error: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T> int balloc(lbf_batch* b, T** p, size_t bytes)
+{
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes ? bytes : 8) != hipSuccess) return bfail(b, "HIP error: hipMalloc failed");
+    b->allocations.push_back(q);
+    *p = static_cast<T*>(q);
+    return 0;
+}
+
+RelaxF32 relax_of(double tau, double rho0)
+{
+    const double su = 1.0 / tau, sq = 8.0 * (2.0 - su) / (8.0 - su);
+    return RelaxF32{float(rho0), float(su), float(1.0 - 0.5 * su), float(su), float(1.0 - 0.5 * su), float(sq),
+                    float(1.0 - 0.5 * sq)};
+}
+
+BatchBuffers buffers(lbf_batch* b, bool swap)
+{
+    BatchBuffers x{};
+    x.fin = b->f[b->cur];
+    x.fout = swap ? b->f[b->cur ^ 1] : b->f[b->cur];
+    x.tiles = b->d_tiles;
+    x.ntiles = b->d_ntiles;
+    x.tile_crop = b->d_tile_crop;
+    x.crops = b->d_crops;
+    x.site = b->d_site;
+    x.lab8 = b->d_lab8;
+    x.map = b->d_map;
+    x.lab = b->d_lab;
+    x.ucur = b->d_ucur;
+    x.uprev = b->d_uprev;
+    x.uadj = b->d_uadj;
+    x.status = b->d_status;
+    return x;
+}
+
+BatchParams params(lbf_batch* b, uint32_t flags)
+{
+    BatchParams p{};
+    p.S = b->S;
+    p.flags = flags;
+    p.t = b->t;
+    p.inv_rho0 = float(1.0 / b->d.density);
+    p.relax = relax_of(b->d.tau, b->d.density);
+    return p;
+}
+
+int set_active(lbf_batch* b, const std::vector<char>& active)
+{
+    std::vector<uint32_t> tiles;
+    for (uint64_t t = 0; t < b->T; ++t)
+        if (active[b->tile_crop[t]]) tiles.push_back(uint32_t(t));
+    const uint32_t n = uint32_t(tiles.size());
+    if (n) BHIP(b, hipMemcpy(b->d_tiles, tiles.data(), n * 4, hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(b->d_ntiles, &n, 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// One step on the stream: persistent grid over the active tiles.
+int step(lbf_batch* b, uint32_t flags, bool init)
+{
+    const BatchParams p = params(b, flags);
+    hipError_t e = launch_batch_step(b->Q, b->d.equil == LBF_MRT, init, b->grid, p, buffers(b, !init), b->stream);
+    if (e != hipSuccess) return bfail(b, std::string("HIP error: ") + hipGetErrorString(e));
+    if (!init) {
+        b->cur ^= 1;
+        b->t++;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbf_batch_error(const lbf_batch* b) { return b ? b->err.c_str() : g_batch_error.c_str(); }
+
+void lbf_batch_destroy(lbf_batch* b)
+{
+    if (!b) return;
+    for (void* p : b->allocations) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const lbf_crop* crops, lbf_batch** out)
+{
+    g_batch_error.clear();
+    if (!desc || !labels || !crops || !out) return bfail(nullptr, "lbf_batch_create: null argument");
+    *out = nullptr;
+    const lbf_batch_desc& d = *desc;
+    if (!((d.D == 2 && d.Q == 9) || (d.D == 3 && d.Q == 19)))
+        return bfail(nullptr, "Only D2Q9 and D3Q19 are validated for simulation.");
+    if (!std::isfinite(d.tau) || d.tau <= 0.5 || !(d.density > 0.0))
+        return bfail(nullptr, "Densities must be positive; relaxation times must exceed 0.5.");
+    if (d.ncrops == 0) return bfail(nullptr, "lbf_batch_create: no crops");
+    lbf_batch* b = new lbf_batch();
+    b->d = d;
+    b->D = d.D;
+    b->Q = d.Q;
+    b->ncrops = d.ncrops;
+    // label table: fluid = VD/GV, solids = BB/SD; nothing else is batched
+    std::vector<LabelF> lab(256);
+    for (int l = 0; l < 256; ++l) {
+        std::memset(&lab[l], 0, sizeof(LabelF));
+        const int t = labels[l].type;
+        if (t == LBF_VD || t == LBF_GV) lab[l].kind = KIND_FLUID;
+        if (t == LBF_GV) {
+            lab[l].kind |= KIND_GV;
+            for (int k = 0; k < d.D; ++k) {
+                const int e = d.Q == 9 ? Lattice<9>::e[labels[l].dir][k] : Lattice<19>::e[labels[l].dir][k];
+                const bool diag = d.Q == 9 ? Lattice<9>::diagonal(labels[l].dir) : Lattice<19>::diagonal(labels[l].dir);
+                const double unit = e == 0 ? 0.0 : (diag ? (e > 0 ? std::sqrt(2) / 2 : -std::sqrt(2) / 2) : double(e));
+                const double F = unit * labels[l].value;
+                lab[l].force[k] = float(F);
+                lab[l].half_rho[k] = float(F * 0.5 / d.density);
+            }
+        }
+    }
+    // slot space
+    std::vector<uint32_t> site;
+    std::vector<uint8_t> lab8;
+    std::vector<int32_t> map;
+    b->crops.resize(d.ncrops);
+    b->crop_sites.resize(d.ncrops);
+    b->crop_force.assign(size_t(d.ncrops) * 3, 0.0);
+    for (uint32_t c = 0; c < d.ncrops; ++c) {
+        const lbf_crop& cr = crops[c];
+        uint64_t N = 1;
+        for (int k = 0; k < d.D; ++k) N *= cr.sz[k];
+        const uint32_t nx = uint32_t(cr.sz[d.D - 1]), ny = uint32_t(cr.sz[d.D - 2]);
+        const uint32_t nz = d.D == 3 ? uint32_t(cr.sz[0]) : 1u;
+        CropDev cd{};
+        cd.g.nx = nx;
+        cd.g.ny = ny;
+        cd.g.nz = nz;
+        cd.g.nxy = nx * ny;
+        cd.g.N = N;
+        const FastDiv fx = make_fastdiv(nx), fxy = make_fastdiv(nx * ny);
+        cd.g.div_nx = DevFastDiv{fx.d, fx.m, fx.s};
+        cd.g.div_nxy = DevFastDiv{fxy.d, fxy.m, fxy.s};
+        cd.slot_base = uint32_t(site.size());
+        cd.map_base = uint32_t(map.size());
+        cd.tile_begin = uint32_t(site.size() / TILE);
+        uint32_t nf = 0;
+        bool have_force = false;
+        for (uint64_t i = 0; i < N; ++i) {
+            const uint8_t v = cr.geometry[i];
+            const int t = labels[v].type;
+            if (!t) {
+                lbf_batch_destroy(b);
+                return bfail(nullptr, "map::at");
+            }
+            if (lab[v].kind & KIND_FLUID) {
+                map.push_back(int32_t(nf++));
+                site.push_back(uint32_t(i));
+                lab8.push_back(v);
+                if (!have_force) {
+                    for (int k = 0; k < d.D; ++k) b->crop_force[size_t(c) * 3 + k] = lab[v].force[k];
+                    have_force = true;
+                }
+            } else if (t == LBF_BB || t == LBF_SD) {
+                map.push_back(-1);
+            } else {
+                lbf_batch_destroy(b);
+                return bfail(nullptr, "lbf_batch_create: batched crops support VD/GV fluid and BB/SD solids only");
+            }
+        }
+        while (site.size() % TILE) {
+            site.push_back(NO_SITE);
+            lab8.push_back(0);
+        }
+        cd.nfluid = nf;
+        cd.tile_end = uint32_t(site.size() / TILE);
+        b->crops[c] = cd;
+        b->crop_sites[c] = N;
+        b->sites += N;
+        b->fluid += nf;
+        for (uint32_t t = cd.tile_begin; t < cd.tile_end; ++t) b->tile_crop.push_back(c);
+    }
+    if (site.size() >= (uint64_t(1) << 32) || map.size() >= (uint64_t(1) << 31)) {
+        lbf_batch_destroy(b);
+        return bfail(nullptr, "lbf_batch_create: batch too large for 32-bit slot indices");
+    }
+    b->S = site.size();
+    b->T = b->S / TILE;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        lbf_batch_destroy(b);
+        return bfail(nullptr, "HIP error: cannot create a stream (no GPU?)");
+    }
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipGetDeviceProperties(&prop, dev);
+    b->cus = uint32_t(prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256);
+    b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * 8)));
+    const size_t pop = size_t(b->Q) * b->S * 4;
+    int rc = 0;
+    rc |= balloc(b, &b->f[0], pop);
+    rc |= balloc(b, &b->f[1], pop);
+    rc |= balloc(b, &b->d_tiles, b->T * 4);
+    rc |= balloc(b, &b->d_ntiles, 4);
+    rc |= balloc(b, &b->d_tile_crop, b->T * 4);
+    rc |= balloc(b, &b->d_crops, b->crops.size() * sizeof(CropDev));
+    rc |= balloc(b, &b->d_site, b->S * 4);
+    rc |= balloc(b, &b->d_lab8, b->S);
+    rc |= balloc(b, &b->d_map, map.size() * 4);
+    rc |= balloc(b, &b->d_lab, 256 * sizeof(LabelF));
+    rc |= balloc(b, &b->d_ucur, size_t(b->D) * b->S * 4);
+    rc |= balloc(b, &b->d_uprev, size_t(b->D) * b->S * 4);
+    rc |= balloc(b, &b->d_uadj, size_t(b->D) * b->S * 4);
+    rc |= balloc(b, &b->d_status, sizeof(DevStatus));
+    rc |= balloc(b, &b->d_partial, b->T * 4 * 8);
+    rc |= balloc(b, &b->d_out, size_t(b->ncrops) * 4 * 8);
+    if (rc) {
+        std::string e = b->err;
+        lbf_batch_destroy(b);
+        return bfail(nullptr, e);
+    }
+    auto up = [&](void* dst, const void* src, size_t n) { return hipMemcpy(dst, src, n, hipMemcpyHostToDevice); };
+    bool ok = up(b->d_tile_crop, b->tile_crop.data(), b->T * 4) == hipSuccess &&
+              up(b->d_crops, b->crops.data(), b->crops.size() * sizeof(CropDev)) == hipSuccess &&
+              up(b->d_site, site.data(), b->S * 4) == hipSuccess && up(b->d_lab8, lab8.data(), b->S) == hipSuccess &&
+              up(b->d_map, map.data(), map.size() * 4) == hipSuccess &&
+              up(b->d_lab, lab.data(), 256 * sizeof(LabelF)) == hipSuccess;
+    DevStatus st{};
+    for (auto& v : st.first) v = ~0ull;
+    ok = ok && up(b->d_status, &st, sizeof st) == hipSuccess;
+    for (float* p : {b->d_ucur, b->d_uprev, b->d_uadj})
+        ok = ok && hipMemset(p, 0, size_t(b->D) * b->S * 4) == hipSuccess;
+    ok = ok && hipMemset(b->f[0], 0, pop) == hipSuccess && hipMemset(b->f[1], 0, pop) == hipSuccess;
+    std::vector<char> all(b->ncrops, 1);
+    if (!ok || set_active(b, all)) {
+        lbf_batch_destroy(b);
+        return bfail(nullptr, "HIP error while uploading the batch");
+    }
+    *out = b;
+    return 0;
+}
+
+uint64_t lbf_batch_time(const lbf_batch* b) { return b ? b->t : 0; }
+
+uint64_t lbf_batch_sites(const lbf_batch* b, uint64_t* fluid)
+{
+    if (!b) return 0;
+    if (fluid) *fluid = b->fluid;
+    return b->sites;
+}
+
+int lbf_batch_init(lbf_batch* b)
+{
+    if (!b) return -1;
+    std::vector<char> all(b->ncrops, 1);
+    if (set_active(b, all)) return -1;
+    b->t = 0;
+    b->cur = 0;
+    BHIP(b, hipMemsetAsync(b->d_uprev, 0, size_t(b->D) * b->S * 4, b->stream));
+    if (step(b, 0u, true)) return -1;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int lbf_batch_advance(lbf_batch* b, uint64_t n)
+{
+    if (!b) return -1;
+    for (uint64_t k = 0; k < n; ++k)
+        if (step(b, 0u, false)) return -1;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int lbf_batch_tune(lbf_batch* b, uint32_t steps, uint32_t* best_per_cu, double* best_ms)
+{
+    if (!b) return -1;
+    static const uint32_t candidates[] = {1, 2, 4, 8, 16, 32};
+    double best = 1e300;
+    uint32_t pick = 8;
+    hipEvent_t e0, e1;
+    BHIP(b, hipEventCreate(&e0));
+    BHIP(b, hipEventCreate(&e1));
+    for (uint32_t per : candidates) {
+        b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * per)));
+        if (step(b, 0u, false) || step(b, 0u, false)) return -1;   // warm-up, keeps buffer parity
+        BHIP(b, hipEventRecord(e0, b->stream));
+        for (uint32_t k = 0; k < steps; ++k)
+            if (step(b, 0u, false)) return -1;
+        BHIP(b, hipEventRecord(e1, b->stream));
+        BHIP(b, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+        if (ms / steps < best) {
+            best = ms / steps;
+            pick = per;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * pick)));
+    if (best_per_cu) *best_per_cu = pick;
+    if (best_ms) *best_ms = best;
+    return 0;
+}
+
+int lbf_batch_run(lbf_batch* b, int axis, double tol, uint32_t interval, uint64_t max_steps,
+                  lbf_crop_result* results, double* seconds)
+{
+    if (!b || !results) return -1;
+    if (axis < 0 || axis >= b->D) return bfail(b, "lbf_batch_run: axis outside the lattice");
+    if (interval < 2) interval = 2;
+    if (interval % 2) ++interval;   // keeps the population buffers in the same role per replay
+    const auto start = std::chrono::steady_clock::now();
+    // capture: interval steps (the last two keep adjacent and current velocity) + reduction
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const uint64_t t0 = b->t;
+    for (uint32_t k = 0; k < interval; ++k) {
+        const uint32_t flags = k == interval - 2 ? 2u : (k == interval - 1 ? 1u : 0u);
+        if (step(b, flags, false)) {
+            (void)hipStreamEndCapture(b->stream, &graph);
+            return -1;
+        }
+    }
+    hipError_t e = launch_batch_report(b->grid, b->S, b->D, axis, buffers(b, false), b->ncrops, b->d_partial,
+                                       b->d_out, b->stream);
+    BHIP(b, hipStreamEndCapture(b->stream, &graph));
+    BHIP(b, e);
+    BHIP(b, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    b->t = t0;   // capture advanced the host counters without running anything
+    std::vector<char> active(b->ncrops, 1);
+    if (set_active(b, active)) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        return -1;
+    }
+    std::vector<int> stable(b->ncrops, 0);
+    std::vector<char> has_prev(b->ncrops, 0);
+    std::vector<double> out(size_t(b->ncrops) * 4);
+    for (uint32_t c = 0; c < b->ncrops; ++c) std::memset(&results[c], 0, sizeof(lbf_crop_result));
+    uint32_t remaining = b->ncrops;
+    const double nu = (b->d.tau - 0.5) / 3.0;
+    while (remaining && b->t < max_steps) {
+        BHIP(b, hipGraphLaunch(exec, b->stream));
+        b->t += interval;
+        BHIP(b, hipMemcpyAsync(out.data(), b->d_out, out.size() * 8, hipMemcpyDeviceToHost, b->stream));
+        BHIP(b, hipStreamSynchronize(b->stream));
+        bool changed = false;
+        for (uint32_t c = 0; c < b->ncrops; ++c) {
+            if (!active[c]) continue;
+            const double du2 = out[c * 4 + 0], u2 = out[c * 4 + 1], da2 = out[c * 4 + 2], ux = out[c * 4 + 3];
+            const double count = double(b->crop_sites[c]) * b->D;
+            const double res = u2 > 0.0 ? std::sqrt(du2 / u2) : 0.0, rms = std::sqrt(du2 / count);
+            const double ares = u2 > 0.0 ? std::sqrt(da2 / u2) : 0.0, arms = std::sqrt(da2 / count);
+            lbf_crop_result& r = results[c];
+            r.steps = b->t;
+            r.residual = res;
+            r.adjacent_residual = ares;
+            const double F = b->crop_force[size_t(c) * 3 + axis];
+            r.k_darcy = F != 0.0 ? nu * (ux / double(b->crop_sites[c])) / F : 0.0;
+            if (!std::isfinite(du2) || !std::isfinite(u2)) {
+                (void)hipGraphExecDestroy(exec);
+                (void)hipGraphDestroy(graph);
+                return bfail(b, "Non-finite velocity in convergence report.");
+            }
+            if (has_prev[c]) {
+                const bool ok = (rms <= 1e-12 || res < tol) && (arms <= 1e-12 || ares < tol);
+                stable[c] = ok ? stable[c] + 1 : 0;
+            }
+            has_prev[c] = 1;
+            if (stable[c] >= 3) {
+                r.converged = 1;
+                active[c] = 0;
+                --remaining;
+                changed = true;
+            }
+        }
+        if (changed && remaining && set_active(b, active)) {
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            return -1;
+        }
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    DevStatus st{};
+    BHIP(b, hipMemcpy(&st, b->d_status, sizeof st, hipMemcpyDeviceToHost));
+    if (st.bits) return bfail(b, "Non-finite post-collision population in the batch.");
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return 0;
+}
+
+int lbf_batch_crop_populations(lbf_batch* b, uint32_t c, double* out)
+{
+    if (!b || !out || c >= b->ncrops) return -1;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    const CropDev& cd = b->crops[c];
+    const uint64_t N = b->crop_sites[c];
+    const uint64_t n = uint64_t(cd.tile_end - cd.tile_begin) * TILE;
+    std::vector<float> h(n);
+    std::vector<uint32_t> site(n);
+    BHIP(b, hipMemcpy(site.data(), b->d_site + cd.slot_base, n * 4, hipMemcpyDeviceToHost));
+    std::memset(out, 0, size_t(b->Q) * N * sizeof(double));
+    for (int q = 0; q < b->Q; ++q) {
+        BHIP(b, hipMemcpy(h.data(), static_cast<const float*>(b->f[b->cur]) + uint64_t(q) * b->S + cd.slot_base,
+                          n * 4, hipMemcpyDeviceToHost));
+        const double shift = (b->Q == 9 ? Lattice<9>::w[q] : Lattice<19>::w[q]) * b->d.density;
+        for (uint64_t k = 0; k < n; ++k)
+            if (site[k] != NO_SITE) out[uint64_t(q) * N + site[k]] = double(h[k]) + shift;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -56,6 +56,17 @@ struct GridDev {
     DevFastDiv div_nx, div_nxy;
 };
 
+// Workgroup index remapped so that each of the 8 XCDs (each with its own L2)
+// processes one contiguous range of the grid: the hardware hands workgroup b
+// to XCD b % 8, and neighbouring sites then share that XCD's L2 lines.
+// A bijection on [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    const uint32_t per = nb >> 3, rem = nb & 7u, x = b & 7u;
+    return x * per + (x < rem ? x : rem) + (b >> 3);
+}
+
 __device__ __forceinline__ void site_coords(const GridDev& g, uint32_t i, int& x, int& y, int& z)
 {
     const uint32_t zz = fdiv(i, g.div_nxy);

@@ -1,0 +1,199 @@
+// kernels_batch.hip -- many independent single-phase crops in one launch.
+//
+// Every crop's fluid sites get slots in one shared slot space, padded per
+// crop to whole 256-slot tiles, so a tile belongs to exactly one crop and a
+// workgroup's crop parameters are uniform (scalar loads).  The step kernel is
+// persistent: a fixed grid walks a device-resident list of active tiles, so
+// crops that have converged drop out by rewriting that list, and the same
+// captured hipGraph keeps working.  Per crop: periodic wrap inside the crop,
+// halfway bounce-back on BB/SD, Guo force, SRT/MRT collision on FP32 shifted
+// populations (collide_f32.h: SCSP::equilSRT/equilMRT, scsp.cpp:374-464).
+// The per-crop convergence residuals are reduced per tile in FP64 and summed
+// per crop in tile order (deterministic).
+#include <type_traits>
+
+#include "batch_types.h"
+
+namespace lbf {
+
+template <int Q, bool MRT, bool INIT, bool WIDE>
+__global__ void __launch_bounds__(TILE)
+batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fout,
+           const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ntiles,
+           const uint32_t* __restrict__ tile_crop, const CropDev* __restrict__ crops,
+           const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8, const int32_t* __restrict__ map,
+           const LabelF* __restrict__ lab, float* __restrict__ ucur, float* __restrict__ uadj,
+           DevStatus* __restrict__ status)
+{
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+    const uint32_t n = *ntiles;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint32_t tile = tiles[k];
+        const CropDev cr = crops[tile_crop[tile]];
+        const uint32_t slot = tile * TILE + threadIdx.x;
+        const uint32_t s = site[slot];
+        if (s == NO_SITE) continue;
+        const LabelF lb = lab[lab8[slot]];
+        const float F[3] = {lb.force[0], lb.force[1], D == 3 ? lb.force[2] : 0.0f};
+        float h[Q];
+        if (INIT) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                float ef = 0.0f;
+#pragma unroll
+                for (int d = 0; d < D; ++d) ef += float(L::e[q][d]) * F[d];
+                h[q] = float(-1.5 * L::w[q]) * ef;
+            }
+        } else {
+            int x, y, z;
+            site_coords(cr.g, s, x, y, z);
+            const uint32_t xs[3] = {x == 0 ? cr.g.nx - 1 : uint32_t(x - 1), uint32_t(x),
+                                    uint32_t(x) + 1 == cr.g.nx ? 0u : uint32_t(x + 1)};
+            const uint32_t ys[3] = {(y == 0 ? cr.g.ny - 1 : uint32_t(y - 1)) * cr.g.nx, uint32_t(y) * cr.g.nx,
+                                    (uint32_t(y) + 1 == cr.g.ny ? 0u : uint32_t(y + 1)) * cr.g.nx};
+            const uint32_t zs[3] = {(z == 0 ? cr.g.nz - 1 : uint32_t(z - 1)) * cr.g.nxy, uint32_t(z) * cr.g.nxy,
+                                    (uint32_t(z) + 1 == cr.g.nz ? 0u : uint32_t(z + 1)) * cr.g.nxy};
+            h[0] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(fin) + O(slot) * O(4));
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                int ex, ey, ez;
+                if constexpr (Q == 9) {
+                    ez = 0; ey = L::e[q][0]; ex = L::e[q][1];
+                } else {
+                    ez = L::e[q][0]; ey = L::e[q][1]; ex = L::e[q][2];
+                }
+                const int32_t m = map[cr.map_base + zs[1 - ez] + ys[1 - ey] + xs[1 - ex]];
+                const O off = m < 0 ? O(L::op[q]) * O(p.S) + slot : O(q) * O(p.S) + (cr.slot_base + uint32_t(m));
+                h[q] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(fin) + off * O(4));
+            }
+        }
+        float u[D];
+        const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
+        if (!isfinite(drho)) raise_status(status, 0, p.t);
+        if (p.flags & 1u) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) ucur[uint64_t(d) * p.S + slot] = u[d];
+        }
+        if (p.flags & 2u) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) uadj[uint64_t(d) * p.S + slot] = u[d];
+        }
+        collide_f32<Q, MRT>(h, drho, u, F, p.relax);
+        bool finite = true;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            finite = finite && isfinite(h[q]);
+            *reinterpret_cast<float*>(reinterpret_cast<char*>(fout) + (O(q) * O(p.S) + slot) * O(4)) = h[q];
+        }
+        if (!finite) raise_status(status, 4, p.t);
+    }
+}
+
+// Per tile: sum (u - u_prev)^2, sum u^2, sum (u - u_adj)^2, sum u_axis,
+// then u_prev = u.  One workgroup per active tile (persistent loop).
+__global__ void __launch_bounds__(TILE)
+batch_report(uint64_t S, int D, int axis, const uint32_t* __restrict__ tiles,
+             const uint32_t* __restrict__ ntiles, const uint32_t* __restrict__ site,
+             const float* __restrict__ ucur, float* __restrict__ uprev, const float* __restrict__ uadj,
+             double* __restrict__ partial)
+{
+    __shared__ double lds[TILE / 64][4];
+    const uint32_t n = *ntiles;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint32_t tile = tiles[k];
+        const uint32_t slot = tile * TILE + threadIdx.x;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (site[slot] != NO_SITE) {
+            for (int d = 0; d < D; ++d) {
+                const uint64_t i = uint64_t(d) * S + slot;
+                const double v = ucur[i], pv = uprev[i];
+                const double a = uadj[i];
+                acc[0] += (v - pv) * (v - pv);
+                acc[1] += v * v;
+                acc[2] += (v - a) * (v - a);
+                if (d == axis) acc[3] += v;
+                uprev[i] = float(v);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double v = acc[j];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            acc[j] = v;
+        }
+        if ((threadIdx.x & 63) == 0)
+            for (int j = 0; j < 4; ++j) lds[threadIdx.x >> 6][j] = acc[j];
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double v = 0.0;
+            for (int w = 0; w < int(TILE / 64); ++w) v += lds[w][threadIdx.x];
+            partial[uint64_t(tile) * 4 + threadIdx.x] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Per crop: sum its tiles' partials in tile order.
+__global__ void __launch_bounds__(64)
+batch_crop_sums(uint32_t ncrops, const CropDev* __restrict__ crops, const double* __restrict__ partial,
+                double* __restrict__ out)
+{
+    const uint32_t c = blockIdx.x;
+    if (c >= ncrops) return;
+    const CropDev cr = crops[c];
+    for (int j = 0; j < 4; ++j) {
+        double v = 0.0;
+        for (uint32_t t = cr.tile_begin + threadIdx.x; t < cr.tile_end; t += 64) v += partial[uint64_t(t) * 4 + j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (threadIdx.x == 0) out[uint64_t(c) * 4 + j] = v;
+    }
+}
+
+template <int Q, bool MRT, bool INIT>
+static void launch_batch_q(uint32_t grid, const BatchParams& p, const BatchBuffers& b, bool wide, hipStream_t s)
+{
+    if (wide)
+        hipLaunchKernelGGL((batch_step<Q, MRT, INIT, true>), dim3(grid), dim3(TILE), 0, s, p,
+                           static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
+                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.map, b.lab, b.ucur,
+                           b.uadj, b.status);
+    else
+        hipLaunchKernelGGL((batch_step<Q, MRT, INIT, false>), dim3(grid), dim3(TILE), 0, s, p,
+                           static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
+                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.map, b.lab, b.ucur,
+                           b.uadj, b.status);
+}
+
+hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, const BatchParams& p,
+                             const BatchBuffers& b, hipStream_t s)
+{
+    const bool wide = p.S * uint64_t(Q) * 4u >= (uint64_t(1) << 32);
+    if (Q == 19) {
+        if (mrt) init ? launch_batch_q<19, true, true>(grid, p, b, wide, s)
+                      : launch_batch_q<19, true, false>(grid, p, b, wide, s);
+        else init ? launch_batch_q<19, false, true>(grid, p, b, wide, s)
+                  : launch_batch_q<19, false, false>(grid, p, b, wide, s);
+    } else {
+        if (mrt) init ? launch_batch_q<9, true, true>(grid, p, b, wide, s)
+                      : launch_batch_q<9, true, false>(grid, p, b, wide, s);
+        else init ? launch_batch_q<9, false, true>(grid, p, b, wide, s)
+                  : launch_batch_q<9, false, false>(grid, p, b, wide, s);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_report(uint32_t grid, uint64_t S, int D, int axis, const BatchBuffers& b, uint32_t ncrops,
+                               double* partial, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_report, dim3(grid), dim3(TILE), 0, s, S, D, axis, b.tiles, b.ntiles,
+                       b.site, b.ucur, b.uprev, b.uadj, partial);
+    hipLaunchKernelGGL(batch_crop_sums, dim3(ncrops), dim3(64), 0, s, ncrops,
+                       static_cast<const CropDev*>(b.crops), partial, out);
+    return hipGetLastError();
+}
+
+}  // namespace lbf

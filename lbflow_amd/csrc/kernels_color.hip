@@ -79,7 +79,7 @@ template <int Q, bool INIT>
 __global__ void __launch_bounds__(256) color_density(ColorArgs a)
 {
 #pragma clang fp contract(off)
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = xcd_block() * 256u + threadIdx.x;
     if (i >= a.g.N) return;
     const uint32_t desc = a.desc[i];
     const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
 #pragma clang fp contract(off)
     using L = Lattice<Q>;
     constexpr int D = L::D;
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = xcd_block() * 256u + threadIdx.x;
     if (i >= a.g.N) return;
     const uint64_t N = a.g.N;
     const uint32_t desc = a.desc[i];

@@ -55,7 +55,7 @@ template <int LAYOUT>
 __device__ __forceinline__ bool resolve(const ScspParams& p, const uint32_t* __restrict__ list, uint32_t& site,
                                         uint32_t& slot)
 {
-    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t k = xcd_block() * 256u + threadIdx.x;
     if constexpr (LAYOUT == SPARSE) {
         if (k >= p.nfluid) return false;
         site = list[k];

@@ -23,7 +23,9 @@ SYMBOLS = ('lbf_create', 'lbf_destroy', 'lbf_last_error', 'lbf_init', 'lbf_step'
            'lbf_time', 'lbf_synchronize', 'lbf_status', 'lbf_status_message', 'lbf_fields',
            'lbf_get_populations', 'lbf_set_populations', 'lbf_report', 'lbf_links', 'lbf_sites',
            'lbf_device_bytes', 'lbf_step_kernel', 'lbf_step_bytes', 'lbf_timed_advance',
-           'lbf_copy_bandwidth')
+           'lbf_copy_bandwidth', 'lbf_batch_create', 'lbf_batch_destroy', 'lbf_batch_error',
+           'lbf_batch_sites', 'lbf_batch_tune', 'lbf_batch_init', 'lbf_batch_advance', 'lbf_batch_run',
+           'lbf_batch_crop_populations', 'lbf_batch_time')
 
 
 class Label(C.Structure):
@@ -49,6 +51,20 @@ class ReportOut(C.Structure):
                 ('rho_slice', (C.c_double * 2) * 2), ('count_slice', C.c_double * 2),
                 ('conv', C.c_double * 4), ('phase_change', C.c_double * 2),
                 ('has_previous', C.c_int32), ('nonfinite', C.c_int32)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [('D', C.c_int32), ('Q', C.c_int32), ('equil', C.c_int32), ('ncrops', C.c_uint32),
+                ('tau', C.c_double), ('density', C.c_double)]
+
+
+class Crop(C.Structure):
+    _fields_ = [('sz', C.c_uint64 * 3), ('geometry', C.c_void_p)]
+
+
+class CropResult(C.Structure):
+    _fields_ = [('steps', C.c_uint64), ('converged', C.c_int32), ('reserved', C.c_int32),
+                ('residual', C.c_double), ('adjacent_residual', C.c_double), ('k_darcy', C.c_double)]
 
 
 _LIB = None
@@ -105,6 +121,26 @@ def load_library(path=None):
     L.lbf_timed_advance.restype = i32
     L.lbf_copy_bandwidth.argtypes = [u64, C.c_int]
     L.lbf_copy_bandwidth.restype = C.c_double
+    L.lbf_batch_create.argtypes = [C.POINTER(BatchDesc), C.POINTER(Label), C.POINTER(Crop), C.POINTER(vp)]
+    L.lbf_batch_create.restype = i32
+    L.lbf_batch_destroy.argtypes = [vp]
+    L.lbf_batch_error.argtypes = [vp]
+    L.lbf_batch_error.restype = C.c_char_p
+    L.lbf_batch_sites.argtypes = [vp, C.POINTER(u64)]
+    L.lbf_batch_sites.restype = u64
+    L.lbf_batch_tune.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+    L.lbf_batch_tune.restype = i32
+    L.lbf_batch_init.argtypes = [vp]
+    L.lbf_batch_init.restype = i32
+    L.lbf_batch_advance.argtypes = [vp, u64]
+    L.lbf_batch_advance.restype = i32
+    L.lbf_batch_run.argtypes = [vp, i32, C.c_double, C.c_uint32, u64, C.POINTER(CropResult),
+                                C.POINTER(C.c_double)]
+    L.lbf_batch_run.restype = i32
+    L.lbf_batch_crop_populations.argtypes = [vp, C.c_uint32, vp]
+    L.lbf_batch_crop_populations.restype = i32
+    L.lbf_batch_time.argtypes = [vp]
+    L.lbf_batch_time.restype = u64
     if path is None:
         _LIB = L
     return L
@@ -259,3 +295,84 @@ class Engine:
     @property
     def device_bytes(self):
         return int(self.L.lbf_device_bytes(self.h))
+
+
+class Batch:
+    """Many independent single-phase crops stepped together (lbf_batch_*).
+
+    ``labels`` is the {value: (TYPE, value, dir, phase)} map of a lattice case;
+    ``crops`` a list of uint8 label arrays (z, y, x) or (y, x) of any sizes.
+    """
+
+    def __init__(self, D, Q, eq, tau, density, labels, crops):
+        self.L = load_library()
+        d = BatchDesc(D=D, Q=Q, equil=1 if eq == 'MRT' else 0, ncrops=len(crops), tau=tau, density=density)
+        lab = (Label * 256)()
+        for key, (typ, value, direction, phase) in labels.items():
+            lab[key].type = TYPES[typ]
+            lab[key].value = value
+            lab[key].dir = direction
+            lab[key].phase = phase
+        self._geoms = [np.ascontiguousarray(g, dtype=np.uint8) for g in crops]
+        arr = (Crop * len(crops))()
+        for i, g in enumerate(self._geoms):
+            if g.ndim != D:
+                raise ValueError('crop dimensionality does not match D')
+            for k, n in enumerate(g.shape):
+                arr[i].sz[k] = n
+            arr[i].geometry = g.ctypes.data
+        h = C.c_void_p()
+        rc = self.L.lbf_batch_create(C.byref(d), lab, arr, C.byref(h))
+        if rc:
+            raise EngineError(self.L.lbf_batch_error(None).decode())
+        self.h = h
+        self.D, self.Q = D, Q
+        self.shapes = [g.shape for g in self._geoms]
+        fluid = C.c_uint64()
+        self.sites = int(self.L.lbf_batch_sites(h, C.byref(fluid)))
+        self.fluid = int(fluid.value)
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.lbf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise EngineError(self.L.lbf_batch_error(self.h).decode())
+
+    def tune(self, steps=10):
+        per = C.c_uint32()
+        ms = C.c_double()
+        self._check(self.L.lbf_batch_tune(self.h, steps, C.byref(per), C.byref(ms)))
+        return int(per.value), ms.value
+
+    def init(self):
+        self._check(self.L.lbf_batch_init(self.h))
+        return self
+
+    def advance(self, n):
+        self._check(self.L.lbf_batch_advance(self.h, n))
+        return self
+
+    @property
+    def t(self):
+        return int(self.L.lbf_batch_time(self.h))
+
+    def run(self, axis, tol, interval, max_steps):
+        """Run every crop to convergence; returns (list of result dicts, seconds)."""
+        res = (CropResult * len(self.shapes))()
+        sec = C.c_double()
+        self._check(self.L.lbf_batch_run(self.h, axis, tol, interval, max_steps, res, C.byref(sec)))
+        out = [dict(steps=int(r.steps), converged=bool(r.converged), residual=r.residual,
+                    adjacent=r.adjacent_residual, K=r.k_darcy) for r in res]
+        return out, sec.value
+
+    def populations(self, c):
+        n = int(np.prod(self.shapes[c]))
+        out = np.zeros(self.Q * n)
+        self._check(self.L.lbf_batch_crop_populations(self.h, c, _ptr(out)))
+        return out

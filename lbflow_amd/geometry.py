@@ -60,3 +60,18 @@ def random_blocks(labels, block, fraction, seed, phase0=75, phase1=76, pore=75):
     out[poremask & grid] = phase0
     out[poremask & ~grid] = phase1
     return out
+
+
+def batch_crops(n=512, seed0=1000, side=(32, 96), phi=(0.15, 0.35), rmin=4.0, rmax=10.0):
+    """Independent cubic sphere-pack crops: crop i draws its side uniformly from
+    the integers side[0]..side[1] and its porosity uniformly from phi with
+    seed seed0 + i, then packs spheres (radius uniform in [rmin, rmax]) with
+    that seed.  Returns (list of uint8 volumes, list of target porosities)."""
+    crops, targets = [], []
+    for i in range(n):
+        rng = np.random.default_rng(seed0 + i)
+        s = int(rng.integers(side[0], side[1] + 1))
+        p = float(rng.uniform(phi[0], phi[1]))
+        crops.append(sphere_pack((s, s, s), p, rmin, rmax, seed0 + i))
+        targets.append(p)
+    return crops, targets

@@ -27,7 +27,9 @@ def test_bundle_permeability_closed_form(precision):
     case = H.lattice_case(spec)
     eng = Engine(case, H.geometry(spec), precision, stream_solids=False).init()
     nx = spec['size'][0]
-    res = converge(eng, axis=2, first=1, last=nx - 2, tol=1e-9, interval=500, max_steps=40000,
+    # FP32 residuals level off near 1e-7, so its stop tolerance is the FP32 one
+    tol = 1e-9 if precision == FP64 else 1e-6
+    res = converge(eng, axis=2, first=1, last=nx - 2, tol=tol, interval=500, max_steps=40000,
                    force=spec['force'], tau=spec['tau'][0])
     widths, ny = spec['widths'], spec['size'][1]
     expected = sum(h ** 3 / 12 + h / 24 for h in widths) / ny
