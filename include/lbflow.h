@@ -247,6 +247,56 @@ int lbf_batch_run(lbf_batch* batch, int axis, double tol, uint32_t interval, uin
 int lbf_batch_crop_populations(lbf_batch* batch, uint32_t c, double* out);
 /* Current step of the batch. */
 uint64_t lbf_batch_time(const lbf_batch* batch);
+/* Macro fields of crop c after its last report: fields[site][0..D-1] = u
+ * (reference axis order), fields[site][D] = rho, over all crop voxels
+ * (solids 0).  `fields` is DEVICE memory of prod(sz) * (D+1) floats. */
+int lbf_batch_crop_fields(lbf_batch* batch, uint32_t c, float* fields);
+/* Warm start: set crop c to the equilibrium of the given fields (same layout,
+ * DEVICE memory), shifted by half the body force so that the first macro
+ * step returns u; the FP32 counterpart of SCSP::initialize (scsp.cpp:116-134)
+ * with a velocity.  Call after lbf_batch_init and before lbf_batch_run. */
+int lbf_batch_set_fields(lbf_batch* batch, uint32_t c, const float* fields);
+
+/* --- learned warm-start network (no reference counterpart) -------------
+ * A periodic 3D U-Net (4 levels, 32-256 channels, 5.8 M parameters) mapping
+ * a crop's fluid mask to its steady (u, rho); BF16 MFMA convolutions with
+ * FP32 accumulation, FP32 master weights, Adam.  Masks, targets and outputs
+ * are DEVICE memory: mask[sample][site] (1 = fluid), target/out
+ * [sample][site][4] FP32 in the channel order the caller chose (the driver
+ * uses u_z, u_y, u_x, rho, normalised).  Sites are z,y,x with side `side`. */
+typedef struct lbf_net lbf_net;
+typedef struct lbf_adam {
+    float lr, beta1, beta2, eps;
+    float weights[4];          /* loss weight per output channel */
+} lbf_adam;
+
+int  lbf_net_create(uint32_t side, uint32_t capacity /* samples per call */, uint64_t seed, lbf_net** out);
+void lbf_net_destroy(lbf_net* net);
+const char* lbf_net_error(const lbf_net* net);   /* net may be NULL: last create error */
+uint64_t lbf_net_param_count(const lbf_net* net);
+/* Parameters in layer order; per layer W[cout][tap][cin] then bias[cout]
+ * (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)), FP32, host memory. */
+int lbf_net_get_params(lbf_net* net, float* host);
+int lbf_net_set_params(lbf_net* net, const float* host);
+int lbf_net_forward(lbf_net* net, const uint8_t* mask, uint32_t count, float* out);
+/* Loss (mean over sites of sum_c w_c (out_c - target_c)^2) and its gradient
+ * in the parameter layout (host memory, may be NULL); no update. */
+int lbf_net_gradients(lbf_net* net, const uint8_t* mask, const float* target, uint32_t count,
+                      const float* weights /* 4 */, double* loss, float* grad_host);
+int lbf_net_train_step(lbf_net* net, const uint8_t* mask, const float* target, uint32_t count,
+                       const lbf_adam* opt, double* loss);
+/* `steps` training steps timed with HIP events; *ms per step and the
+ * convolution FLOPs of one step (forward, input and weight gradients). */
+int lbf_net_timed_train(lbf_net* net, const uint8_t* mask, const float* target, uint32_t count,
+                        const lbf_adam* opt, uint32_t steps, double* ms, double* flops_per_step);
+/* One periodic convolution on DEVICE buffers (kernel tests): channels-last
+ * BF16 in/out with row strides, weights [cout][roundup(taps*cin, 32)] BF16,
+ * optional FP32 bias, ReLU, FP32 output; wgrad writes dW [cout][kp] FP32. */
+int lbf_conv3d_forward(int cin, int cout, int taps, uint32_t side, uint32_t samples, const void* in,
+                       uint32_t in_stride, const void* w, const float* bias, void* out, uint32_t out_stride,
+                       int relu, int out_f32);
+int lbf_conv3d_wgrad(int cin, int cout, int taps, uint32_t side, uint32_t samples, const void* in,
+                     uint32_t in_stride, const void* dz, uint32_t dz_stride, uint32_t splits, float* dw);
 
 #ifdef __cplusplus
 }

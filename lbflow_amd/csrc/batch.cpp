@@ -24,6 +24,11 @@
 namespace lbf {
 hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, const BatchParams& p,
                              const BatchBuffers& b, hipStream_t s);
+hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begin, uint32_t ntiles,
+                               const uint32_t* site, const float* f, const float* ucur, float* fields, hipStream_t s);
+hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t tile_begin, uint32_t ntiles,
+                             const uint32_t* site, const uint8_t* lab8, const LabelF* lab, const float* fields,
+                             float* f, hipStream_t s);
 hipError_t launch_batch_report(uint32_t grid, uint64_t S, int D, int axis, const BatchBuffers& b, uint32_t ncrops,
                                double* partial, double* out, hipStream_t s);
 }  // namespace lbf
@@ -37,7 +42,7 @@ struct lbf_batch {
     uint64_t S = 0, T = 0, sites = 0, fluid = 0;
     std::vector<CropDev> crops;
     std::vector<uint64_t> crop_sites;
-    std::vector<double> crop_force;      // force component along each axis of the crop's first fluid label
+    std::vector<double> crop_force;      // FP64 force of the crop's first GV label, per axis
     std::vector<uint32_t> tile_crop;
     hipStream_t stream = nullptr;
     void* f[2] = {nullptr, nullptr};
@@ -179,6 +184,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     b->ncrops = d.ncrops;
     // label table: fluid = VD/GV, solids = BB/SD; nothing else is batched
     std::vector<LabelF> lab(256);
+    std::vector<double> force64(size_t(256) * 3, 0.0);
     for (int l = 0; l < 256; ++l) {
         std::memset(&lab[l], 0, sizeof(LabelF));
         const int t = labels[l].type;
@@ -190,6 +196,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
                 const bool diag = d.Q == 9 ? Lattice<9>::diagonal(labels[l].dir) : Lattice<19>::diagonal(labels[l].dir);
                 const double unit = e == 0 ? 0.0 : (diag ? (e > 0 ? std::sqrt(2) / 2 : -std::sqrt(2) / 2) : double(e));
                 const double F = unit * labels[l].value;
+                force64[size_t(l) * 3 + k] = F;
                 lab[l].force[k] = float(F);
                 lab[l].half_rho[k] = float(F * 0.5 / d.density);
             }
@@ -233,8 +240,8 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
                 map.push_back(int32_t(nf++));
                 site.push_back(uint32_t(i));
                 lab8.push_back(v);
-                if (!have_force) {
-                    for (int k = 0; k < d.D; ++k) b->crop_force[size_t(c) * 3 + k] = lab[v].force[k];
+                if (!have_force && (lab[v].kind & KIND_GV)) {
+                    for (int k = 0; k < d.D; ++k) b->crop_force[size_t(c) * 3 + k] = force64[size_t(v) * 3 + k];
                     have_force = true;
                 }
             } else if (t == LBF_BB || t == LBF_SD) {
@@ -464,6 +471,28 @@ int lbf_batch_run(lbf_batch* b, int axis, double tol, uint32_t interval, uint64_
     BHIP(b, hipMemcpy(&st, b->d_status, sizeof st, hipMemcpyDeviceToHost));
     if (st.bits) return bfail(b, "Non-finite post-collision population in the batch.");
     if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    return 0;
+}
+
+int lbf_batch_crop_fields(lbf_batch* b, uint32_t c, float* fields)
+{
+    if (!b || !fields || c >= b->ncrops) return -1;
+    const CropDev& cd = b->crops[c];
+    BHIP(b, hipMemsetAsync(fields, 0, b->crop_sites[c] * (b->D + 1) * sizeof(float), b->stream));
+    BHIP(b, launch_batch_fields(b->Q, b->S, float(b->d.density), cd.tile_begin, cd.tile_end - cd.tile_begin,
+                                b->d_site, static_cast<const float*>(b->f[b->cur]), b->d_ucur, fields, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int lbf_batch_set_fields(lbf_batch* b, uint32_t c, const float* fields)
+{
+    if (!b || !fields || c >= b->ncrops) return -1;
+    const CropDev& cd = b->crops[c];
+    const BatchParams p = params(b, 0u);
+    BHIP(b, launch_batch_warm(b->Q, b->d.equil == LBF_MRT, p, cd.tile_begin, cd.tile_end - cd.tile_begin, b->d_site,
+                              b->d_lab8, b->d_lab, fields, static_cast<float*>(b->f[b->cur]), b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
     return 0;
 }
 

@@ -27,4 +27,9 @@ struct ConvGeom {
 
 enum ConvAct { ACT_NONE = 0, ACT_RELU = 1 };
 
+struct Chan {           // a channel range of a channels-last BF16 buffer
+    __bf16* p;
+    uint32_t stride, off;
+};
+
 }  // namespace lbf

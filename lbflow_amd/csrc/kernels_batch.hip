@@ -153,6 +153,94 @@ batch_crop_sums(uint32_t ncrops, const CropDev* __restrict__ crops, const double
     }
 }
 
+// Macro fields of one crop after its last report: fields[site][0..D-1] = u of
+// the last fields step, fields[site][D] = rho0 + sum h (collision conserves
+// mass, so the post-collision sum is the macro density).  One workgroup per tile.
+template <int Q>
+__global__ void __launch_bounds__(TILE)
+batch_fields(uint64_t S, float rho0, uint32_t tile_begin, const uint32_t* __restrict__ site,
+             const float* __restrict__ f, const float* __restrict__ ucur, float* __restrict__ fields)
+{
+    constexpr int D = Lattice<Q>::D;
+    const uint32_t slot = (tile_begin + blockIdx.x) * TILE + threadIdx.x;
+    const uint32_t s = site[slot];
+    if (s == NO_SITE) return;
+    float drho = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) drho += f[uint64_t(q) * S + slot];
+#pragma unroll
+    for (int d = 0; d < D; ++d) fields[uint64_t(s) * (D + 1) + d] = ucur[uint64_t(d) * S + slot];
+    fields[uint64_t(s) * (D + 1) + D] = rho0 + drho;
+}
+
+// Warm start of one crop: the t = 0 state from given fields, i.e. the
+// equilibrium of (rho, u - F/(2 rho0)) followed by the t = 0 macro step and
+// collision, as the initial branch does for the rest state.
+template <int Q, bool MRT>
+__global__ void __launch_bounds__(TILE)
+batch_warm(BatchParams p, uint32_t tile_begin, const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8,
+           const LabelF* __restrict__ lab, const float* __restrict__ fields, float* __restrict__ f)
+{
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    const uint32_t slot = (tile_begin + blockIdx.x) * TILE + threadIdx.x;
+    const uint32_t s = site[slot];
+    if (s == NO_SITE) return;
+    const LabelF lb = lab[lab8[slot]];
+    const float F[3] = {lb.force[0], lb.force[1], D == 3 ? lb.force[2] : 0.0f};
+    float ue[D];
+    float usq = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        ue[d] = fields[uint64_t(s) * (D + 1) + d] - lb.half_rho[d];
+        usq += ue[d] * ue[d];
+    }
+    const float rho0 = p.relax.rho0;
+    const float dr = fields[uint64_t(s) * (D + 1) + D] - rho0;
+    float h[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        float eu = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) eu += float(L::e[q][d]) * ue[d];
+        h[q] = float(L::w[q]) * (dr + rho0 * (eu * (3.0f + 4.5f * eu) - 1.5f * usq));
+    }
+    float u[D];
+    const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
+    collide_f32<Q, MRT>(h, drho, u, F, p.relax);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[uint64_t(q) * p.S + slot] = h[q];
+}
+
+hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begin, uint32_t ntiles,
+                               const uint32_t* site, const float* f, const float* ucur, float* fields, hipStream_t s)
+{
+    if (ntiles == 0) return hipSuccess;
+    if (Q == 19)
+        hipLaunchKernelGGL(batch_fields<19>, dim3(ntiles), dim3(TILE), 0, s, S, rho0, tile_begin, site, f, ucur,
+                           fields);
+    else
+        hipLaunchKernelGGL(batch_fields<9>, dim3(ntiles), dim3(TILE), 0, s, S, rho0, tile_begin, site, f, ucur,
+                           fields);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t tile_begin, uint32_t ntiles,
+                             const uint32_t* site, const uint8_t* lab8, const LabelF* lab, const float* fields,
+                             float* f, hipStream_t s)
+{
+    if (ntiles == 0) return hipSuccess;
+    const dim3 g(ntiles), b(TILE);
+    if (Q == 19) {
+        if (mrt) hipLaunchKernelGGL((batch_warm<19, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
+        else hipLaunchKernelGGL((batch_warm<19, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
+    } else {
+        if (mrt) hipLaunchKernelGGL((batch_warm<9, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
+        else hipLaunchKernelGGL((batch_warm<9, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
+    }
+    return hipGetLastError();
+}
+
 template <int Q, bool MRT, bool INIT>
 static void launch_batch_q(uint32_t grid, const BatchParams& p, const BatchBuffers& b, bool wide, hipStream_t s)
 {

@@ -81,8 +81,12 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     uint32_t vox[4];
+    bool live[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) vox[i] = vbase + 16u * i + r16;
+    for (int i = 0; i < 4; ++i) {
+        vox[i] = vbase + 16u * i + r16;
+        live[i] = vox[i] < g.nvox;
+    }
 
     for (int kc = 0; kc < KP; kc += KC) {
         __syncthreads();
@@ -103,7 +107,9 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
                 const int tap = k / CIN, ci = k - tap * CIN;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const bf16x8*>(in + size_t(nbr<TAPS>(vox[i], tap, g.log_s)) * g.in_stride + ci);
+                    a[i] = live[i] ? *reinterpret_cast<const bf16x8*>(in + size_t(nbr<TAPS>(vox[i], tap, g.log_s)) *
+                                                                         g.in_stride + ci)
+                                   : bf16_zero();
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) a[i] = bf16_zero();
@@ -126,6 +132,7 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t v = vbase + 16u * i + 4u * grp + r;
+                if (v >= g.nvox) continue;
                 float x = acc[i][j][r] + bb;
                 if (ACT == ACT_RELU) x = x > 0.0f ? x : 0.0f;
                 if (OUTF32) static_cast<float*>(out)[size_t(v) * g.out_stride + co] = x;
@@ -233,14 +240,14 @@ sum_splits(const float* __restrict__ partial, uint32_t n, uint32_t splits, float
 // ---------------------------------------------------------------------------
 template <int CIN, int COUT, int TAPS>
 struct ConvLaunch {
-    static constexpr int BN_FWD = COUT >= 128 ? 128 : COUT;
+    static constexpr int BN_FWD = COUT >= 64 ? 64 : COUT;
     static constexpr int BM_W = COUT >= 64 ? 64 : COUT;
     static constexpr int BN_W = 64;
 
     static void forward(const ConvGeom& g, const __bf16* in, const __bf16* w, const float* bias, void* out, int act,
                         bool out_f32, hipStream_t s)
     {
-        const dim3 grid(g.nvox / 256u, (COUT + BN_FWD - 1) / BN_FWD);
+        const dim3 grid((g.nvox + 255u) / 256u, (COUT + BN_FWD - 1) / BN_FWD);
         if (out_f32)
             hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_NONE, true, true>), grid, dim3(256), 0, s, g,
                                in, w, bias, out);

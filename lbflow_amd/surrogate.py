@@ -1,0 +1,319 @@
+"""Learned warm-start for single-phase permeability runs.
+
+A periodic 3D U-Net (lbf_net_* in include/lbflow.h; BF16 MFMA kernels with
+FP32 accumulation) maps a crop's fluid mask to its steady velocity and
+density.  Its prediction initialises the populations at equilibrium
+(lbf_batch_set_fields), and the plain lattice Boltzmann iteration then runs
+to the unchanged stop rule (Simulator::updateConvergence,
+solver/src/lbm/simulator.cpp:248-325), which stays the judge: a poor
+prediction costs iterations, never correctness.
+
+Pipeline: generate sphere-pack crops -> solve them to steady state with the
+batched engine -> train the network on (mask, fields) -> evaluate cold vs
+warm starts on unseen crops.  Device memory is held in torch tensors
+(plumbing only); all arithmetic runs in the engine's HIP kernels.
+"""
+import ctypes as C
+import json
+import os
+import time
+from multiprocessing import get_context
+
+import numpy as np
+
+from . import geometry as G
+from .engine import Batch, EngineError, load_library
+
+PORE, SOLID = 75, 255
+
+
+class Adam(C.Structure):
+    _fields_ = [('lr', C.c_float), ('beta1', C.c_float), ('beta2', C.c_float), ('eps', C.c_float),
+                ('weights', C.c_float * 4)]
+
+
+def _bind(L):
+    if getattr(L, '_net_bound', False):
+        return L
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    L.lbf_net_create.argtypes = [u32, u32, u64, C.POINTER(vp)]
+    L.lbf_net_create.restype = i32
+    L.lbf_net_destroy.argtypes = [vp]
+    L.lbf_net_error.argtypes = [vp]
+    L.lbf_net_error.restype = C.c_char_p
+    L.lbf_net_param_count.argtypes = [vp]
+    L.lbf_net_param_count.restype = u64
+    L.lbf_net_get_params.argtypes = [vp, vp]
+    L.lbf_net_get_params.restype = i32
+    L.lbf_net_set_params.argtypes = [vp, vp]
+    L.lbf_net_set_params.restype = i32
+    L.lbf_net_forward.argtypes = [vp, vp, u32, vp]
+    L.lbf_net_forward.restype = i32
+    L.lbf_net_gradients.argtypes = [vp, vp, vp, u32, vp, C.POINTER(C.c_double), vp]
+    L.lbf_net_gradients.restype = i32
+    L.lbf_net_train_step.argtypes = [vp, vp, vp, u32, C.POINTER(Adam), C.POINTER(C.c_double)]
+    L.lbf_net_train_step.restype = i32
+    L.lbf_net_timed_train.argtypes = [vp, vp, vp, u32, C.POINTER(Adam), u32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+    L.lbf_net_timed_train.restype = i32
+    L.lbf_conv3d_forward.argtypes = [i32, i32, i32, u32, u32, vp, u32, vp, vp, vp, u32, i32, i32]
+    L.lbf_conv3d_forward.restype = i32
+    L.lbf_conv3d_wgrad.argtypes = [i32, i32, i32, u32, u32, vp, u32, vp, u32, u32, vp]
+    L.lbf_conv3d_wgrad.restype = i32
+    L.lbf_batch_crop_fields.argtypes = [vp, u32, vp]
+    L.lbf_batch_crop_fields.restype = i32
+    L.lbf_batch_set_fields.argtypes = [vp, u32, vp]
+    L.lbf_batch_set_fields.restype = i32
+    L._net_bound = True
+    return L
+
+
+def lib():
+    return _bind(load_library())
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class Net:
+    """The warm-start network on the GPU (device buffers are torch CUDA tensors)."""
+
+    def __init__(self, side=64, capacity=4, seed=1):
+        self.L = lib()
+        h = C.c_void_p()
+        if self.L.lbf_net_create(side, capacity, seed, C.byref(h)):
+            raise EngineError(self.L.lbf_net_error(None).decode())
+        self.h = h
+        self.side, self.capacity = side, capacity
+        self.n_params = int(self.L.lbf_net_param_count(h))
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.lbf_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise EngineError(self.L.lbf_net_error(self.h).decode())
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self._check(self.L.lbf_net_get_params(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.size == self.n_params
+        self._check(self.L.lbf_net_set_params(self.h, flat.ctypes.data_as(C.c_void_p)))
+
+    def forward(self, mask, out):
+        """mask: uint8 CUDA tensor [n, S, S, S]; out: float32 CUDA tensor [n, S^3, 4]."""
+        import torch
+        torch.cuda.synchronize()
+        self._check(self.L.lbf_net_forward(self.h, _ptr(mask), mask.shape[0], _ptr(out)))
+
+    def gradients(self, mask, target, weights=(1, 1, 1, 1)):
+        import torch
+        torch.cuda.synchronize()
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        self._check(self.L.lbf_net_gradients(self.h, _ptr(mask), _ptr(target), mask.shape[0],
+                                             w.ctypes.data_as(C.c_void_p), C.byref(loss),
+                                             g.ctypes.data_as(C.c_void_p)))
+        return loss.value, g
+
+    def train_step(self, mask, target, opt):
+        loss = C.c_double()
+        self._check(self.L.lbf_net_train_step(self.h, _ptr(mask), _ptr(target), mask.shape[0], C.byref(opt),
+                                              C.byref(loss)))
+        return loss.value
+
+    def timed_train(self, mask, target, opt, steps):
+        ms, fl = C.c_double(), C.c_double()
+        self._check(self.L.lbf_net_timed_train(self.h, _ptr(mask), _ptr(target), mask.shape[0], C.byref(opt),
+                                               steps, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
+
+def adam(lr=1e-3, weights=(1.0, 1.0, 1.0, 1.0), beta1=0.9, beta2=0.999, eps=1e-8):
+    o = Adam(lr=lr, beta1=beta1, beta2=beta2, eps=eps)
+    for i, w in enumerate(weights):
+        o.weights[i] = w
+    return o
+
+
+# ---------------------------------------------------------------------------
+# data
+# ---------------------------------------------------------------------------
+def _crop(args):
+    seed, side, phi, rmin, rmax = args
+    rng = np.random.default_rng(seed)
+    p = float(rng.uniform(*phi))
+    return G.sphere_pack((side, side, side), p, rmin, rmax, seed)
+
+
+def make_crops(n, seed0, side=64, phi=(0.15, 0.35), rmin=4.0, rmax=10.0, workers=16):
+    """n cubic sphere-pack crops of the given side; crop i uses seed seed0 + i for
+    its porosity (uniform in phi) and its spheres.  Call before the GPU is used
+    (the pool forks)."""
+    args = [(seed0 + i, side, phi, rmin, rmax) for i in range(n)]
+    if workers > 1 and n > 8:
+        with get_context('fork').Pool(workers) as pool:
+            return pool.map(_crop, args, chunksize=max(1, n // (workers * 4)))
+    return [_crop(a) for a in args]
+
+
+def labels(force):
+    return {PORE: ('GV', force, 1, 0), SOLID: ('SD', 90.0, 0, 0)}
+
+
+def solve(crops, tau=0.8, force=1e-5, tol=1e-5, interval=100, max_steps=40000, log=None):
+    """Steady fields of every crop with the batched engine.  Returns (fields as a
+    float32 CUDA tensor [n, S^3, 4] = (u_z, u_y, u_x, rho), per-crop results)."""
+    import torch
+    b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), crops)
+    b.tune(steps=4)
+    b.init()
+    res, sec = b.run(axis=2, tol=tol, interval=interval, max_steps=max_steps)
+    if log:
+        log(f'solved {len(crops)} crops in {sec:.1f} s, steps {min(r["steps"] for r in res)}-'
+            f'{max(r["steps"] for r in res)}, converged {sum(r["converged"] for r in res)}')
+    s3 = crops[0].size
+    fields = torch.empty((len(crops), s3, 4), dtype=torch.float32, device='cuda')
+    L = lib()
+    torch.cuda.synchronize()
+    for c in range(len(crops)):
+        if L.lbf_batch_crop_fields(b.h, c, C.c_void_p(fields[c].data_ptr())):
+            raise EngineError(L.lbf_batch_error(b.h).decode())
+    b.close()
+    return fields, res
+
+
+class Scaling:
+    """Targets are (u * nu / F, (rho - rho0) / F) divided by their RMS over fluid sites."""
+
+    def __init__(self, rms, tau, force):
+        self.rms = np.asarray(rms, dtype=np.float64)
+        self.nu = (tau - 0.5) / 3.0
+        self.force = force
+
+    @classmethod
+    def fit(cls, fields, masks, tau, force):
+        import torch
+        nu = (tau - 0.5) / 3.0
+        m = masks.reshape(masks.shape[0], -1).bool()
+        raw = fields.clone()
+        raw[..., :3] *= nu / force
+        raw[..., 3] = (raw[..., 3] - 1.0) / force
+        rms = [float(torch.sqrt((raw[..., c][m] ** 2).mean())) for c in range(4)]
+        return cls(rms, tau, force)
+
+    def normalise(self, fields):
+        out = fields.clone()
+        out[..., :3] *= self.nu / self.force
+        out[..., 3] = (out[..., 3] - 1.0) / self.force
+        for c in range(4):
+            out[..., c] /= self.rms[c]
+        return out
+
+    def fields(self, pred, tau, force, mask):
+        """Network output -> physical (u, rho) for a run with (tau, force)."""
+        nu = (tau - 0.5) / 3.0
+        out = pred.clone()
+        for c in range(4):
+            out[..., c] *= self.rms[c]
+        out[..., :3] *= force / nu
+        out[..., 3] = 1.0 + out[..., 3] * force
+        m = mask.reshape(mask.shape[0], -1, 1).to(out.dtype)
+        out[..., :3] *= m
+        return out
+
+    def to_json(self):
+        return dict(rms=self.rms.tolist(), tau=(self.nu * 3.0 + 0.5), force=self.force)
+
+    @classmethod
+    def from_json(cls, d):
+        return cls(d['rms'], d['tau'], d['force'])
+
+
+def train(net, masks, targets, epochs, lr=1e-3, batch=None, log=None, seed=0, lr_final=None):
+    """Adam over shuffled minibatches; cosine learning-rate decay to lr_final."""
+    import torch
+    batch = batch or net.capacity
+    n = masks.shape[0]
+    rng = np.random.default_rng(seed)
+    steps = epochs * (n // batch)
+    lr_final = lr * 0.05 if lr_final is None else lr_final
+    opt = adam(lr)
+    k = 0
+    history = []
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = torch.as_tensor(rng.permutation(n), device='cuda')
+        tot = 0.0
+        for i in range(n // batch):
+            idx = order[i * batch:(i + 1) * batch]
+            mb, tb = masks[idx].contiguous(), targets[idx].contiguous()
+            torch.cuda.synchronize()
+            opt.lr = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * k / max(1, steps)))
+            tot += net.train_step(mb, tb, opt)
+            k += 1
+        history.append(tot / max(1, n // batch))
+        if log:
+            log(f'epoch {ep + 1}/{epochs} loss {history[-1]:.5f} ({time.perf_counter() - t0:.1f} s)')
+    return history
+
+
+def save(path, net, scaling, half=True):
+    """Parameters (float16 when half, else float32), the target scaling and the side."""
+    p = net.params()
+    np.savez_compressed(path, params=p.astype(np.float16) if half else p, scaling=json.dumps(scaling.to_json()),
+                        side=net.side)
+
+
+def load(path, capacity=4):
+    d = np.load(path, allow_pickle=False)
+    net = Net(side=int(d['side']), capacity=capacity)
+    net.set_params(d['params'].astype(np.float32))
+    return net, Scaling.from_json(json.loads(str(d['scaling'])))
+
+
+# ---------------------------------------------------------------------------
+# warm vs cold
+# ---------------------------------------------------------------------------
+def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=None):
+    """Runs all crops to the stop rule in one batch, cold (rest state) or warm
+    (network prediction).  Returns (results, seconds from init to the stop,
+    inference seconds)."""
+    import torch
+    b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), crops)
+    b.tune(steps=4)
+    L = lib()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.init()
+    t_inf = 0.0
+    if net is not None:
+        t1 = time.perf_counter()
+        side = crops[0].shape[0]
+        masks = torch.as_tensor(np.stack([(c == PORE) for c in crops]).astype(np.uint8), device='cuda')
+        pred = torch.empty((net.capacity, side ** 3, 4), dtype=torch.float32, device='cuda')
+        for i in range(0, len(crops), net.capacity):
+            m = masks[i:i + net.capacity].contiguous()
+            p = pred[:m.shape[0]]
+            net.forward(m, p)
+            f = scaling.fields(p, tau, force, m).contiguous()
+            torch.cuda.synchronize()
+            for j in range(m.shape[0]):
+                if L.lbf_batch_set_fields(b.h, i + j, C.c_void_p(f[j].data_ptr())):
+                    raise EngineError(L.lbf_batch_error(b.h).decode())
+        t_inf = time.perf_counter() - t1
+    res, _ = b.run(axis=2, tol=tol, interval=interval, max_steps=max_steps)
+    sec = time.perf_counter() - t0
+    b.close()
+    return res, sec, t_inf

@@ -1,0 +1,199 @@
+"""Warm-start network on the GPU against its CPU reference (oracle/surrogate_ref.py).
+
+* single convolutions (forward with bias/ReLU/strided channel ranges, the
+  1x1x1 head, the weight gradient) against float64 PyTorch on the same
+  BF16-rounded operands: only FP32 accumulation order differs;
+* the whole network's output and parameter gradients against the float64
+  reference that rounds to BF16 where the device stores BF16 (activations,
+  weights); the device also keeps backward tensors in BF16, so gradients
+  agree to BF16 accuracy;
+* training reduces the loss; the batch engine's field export / warm start
+  round-trips a converged state.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from lbflow_amd import geometry as G
+from lbflow_amd import surrogate as S
+from oracle import surrogate_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kp(cin, taps):
+    return (taps * cin + 31) // 32 * 32
+
+
+def rand_bf16(rng, shape, scale=1.0):
+    return torch.from_numpy((rng.standard_normal(shape) * scale).astype(np.float32)).to(torch.bfloat16)
+
+
+def packed_weights(w):
+    """[cout][taps][cin] float -> device layout [cout][kp] bf16 (zero padded)."""
+    cout, taps, cin = w.shape
+    out = torch.zeros((cout, kp(cin, taps)), dtype=torch.bfloat16)
+    out[:, :taps * cin] = w.reshape(cout, taps * cin)
+    return out
+
+
+@pytest.mark.parametrize('cin,cout,taps,side,n,in_extra,out_extra,relu,f32', [
+    (8, 32, 27, 8, 2, 0, 0, True, True),
+    (32, 96, 27, 8, 1, 32, 0, False, True),
+    (384, 128, 27, 4, 2, 0, 0, False, True),
+    (64, 32, 27, 16, 1, 0, 64, True, False),
+    (32, 16, 1, 8, 2, 0, 0, False, True),
+    (16, 32, 1, 8, 1, 0, 0, False, True),
+])
+def test_conv_forward(cin, cout, taps, side, n, in_extra, out_extra, relu, f32):
+    L = S.lib()
+    rng = np.random.default_rng(cin + cout + side)
+    nv = n * side ** 3
+    x = rand_bf16(rng, (nv, cin + in_extra))
+    w = rand_bf16(rng, (cout, taps, cin), 1.0 / np.sqrt(taps * cin))
+    b = torch.from_numpy(rng.standard_normal(cout).astype(np.float32))
+    say('conv forward', cin, cout, taps, side, n)
+    dx = x.cuda()
+    dw = packed_weights(w).cuda()
+    db = b.cuda()
+    ostride = cout + out_extra
+    out = torch.zeros((nv, ostride), dtype=torch.float32 if f32 else torch.bfloat16, device='cuda')
+    # read channels [in_extra, in_extra + cin) of the wider input, write channels [out_extra, ...)
+    in_ptr = dx.data_ptr() + 2 * in_extra
+    out_ptr = out.data_ptr() + out.element_size() * out_extra
+    torch.cuda.synchronize()
+    rc = L.lbf_conv3d_forward(cin, cout, taps, side, n, in_ptr, cin + in_extra, dw.data_ptr(), db.data_ptr(),
+                              out_ptr, ostride, int(relu), int(f32))
+    assert rc == 0, L.lbf_net_error(None)
+    got = out.float().cpu().numpy()[:, out_extra:]
+    xs = x.float().numpy()[:, in_extra:].reshape(n, side, side, side, cin)
+    want = R.conv_bruteforce(xs, w.float().numpy().astype(np.float64), b.numpy()).reshape(nv, cout)
+    if relu:
+        want = np.maximum(want, 0)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    say('  max rel err', err)
+    assert err < (1e-5 if f32 else 8e-3)
+    if out_extra:
+        assert np.all(out.float().cpu().numpy()[:, :out_extra] == 0)   # channels outside the range untouched
+
+
+@pytest.mark.parametrize('cin,cout,taps,side,n,splits', [
+    (8, 32, 27, 8, 2, 4), (32, 64, 27, 8, 1, 3), (128, 256, 27, 4, 2, 1), (96, 32, 27, 8, 1, 7), (32, 16, 1, 8, 2, 2)])
+def test_conv_weight_gradient(cin, cout, taps, side, n, splits):
+    L = S.lib()
+    rng = np.random.default_rng(3 * cin + cout)
+    nv = n * side ** 3
+    x = rand_bf16(rng, (nv, cin))
+    dz = rand_bf16(rng, (nv, cout))
+    say('conv wgrad', cin, cout, taps, side, n, splits)
+    dx, ddz = x.cuda(), dz.cuda()
+    dw = torch.zeros((cout, kp(cin, taps)), dtype=torch.float32, device='cuda')
+    torch.cuda.synchronize()
+    rc = L.lbf_conv3d_wgrad(cin, cout, taps, side, n, dx.data_ptr(), cin, ddz.data_ptr(), cout, splits,
+                            dw.data_ptr())
+    assert rc == 0, L.lbf_net_error(None)
+    got = dw.cpu().numpy()
+    xs = x.float().numpy().astype(np.float64).reshape(n, side, side, side, cin)
+    g = dz.float().numpy().astype(np.float64)
+    want = np.zeros((cout, taps, cin))
+    for t in range(taps):
+        dzz, dy, dxx = (t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1) if taps == 27 else (0, 0, 0)
+        sh = np.roll(xs, shift=(-dzz, -dy, -dxx), axis=(1, 2, 3)).reshape(nv, cin)
+        want[:, t, :] = g.T @ sh
+    want = want.reshape(cout, taps * cin)
+    err = np.max(np.abs(got[:, :taps * cin] - want)) / np.max(np.abs(want))
+    say('  max rel err', err)
+    assert err < 1e-5
+    assert np.all(got[:, taps * cin:] == 0)
+
+
+def masks_and_targets(rng, n, side):
+    crops = [G.sphere_pack((side, side, side), 0.5, 1.5, 3.0, 100 + i) for i in range(n)]
+    mask = np.stack([(c == 75) for c in crops]).astype(np.uint8)
+    target = (rng.standard_normal((n, side ** 3, 4)) * 0.3).astype(np.float32)
+    return mask, target
+
+
+def test_network_forward_matches_reference():
+    rng = np.random.default_rng(11)
+    side, n = 16, 2
+    net = S.Net(side=side, capacity=n, seed=5)
+    p = net.params()
+    assert p.size == R.param_count()
+    p = p + rng.standard_normal(p.size).astype(np.float32) * 0.01   # non-zero biases and head rows
+    net.set_params(p)
+    mask, _ = masks_and_targets(rng, n, side)
+    dm = torch.as_tensor(mask, device='cuda')
+    out = torch.empty((n, side ** 3, 4), dtype=torch.float32, device='cuda')
+    net.forward(dm, out)
+    got = out.cpu().numpy().astype(np.float64)
+    p64 = p.astype(np.float64)
+    want = R.forward(mask, R.unpack(p64), round_bf16=True).numpy()
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    say('forward rel L2', rel)
+    assert rel < 2e-2
+
+
+def test_network_gradients_match_reference():
+    rng = np.random.default_rng(12)
+    side, n = 16, 2
+    net = S.Net(side=side, capacity=n, seed=6)
+    p = net.params() + rng.standard_normal(R.param_count()).astype(np.float32) * 0.01
+    net.set_params(p)
+    mask, target = masks_and_targets(rng, n, side)
+    w4 = (1.0, 0.5, 2.0, 1.0)
+    loss, g = net.gradients(torch.as_tensor(mask, device='cuda'), torch.as_tensor(target, device='cuda'), w4)
+    lref, gref = R.gradients(mask, target, p.astype(np.float64), w4, round_bf16=True)
+    say('loss', loss, lref)
+    assert abs(loss - lref) <= 2e-2 * lref
+    offs = np.cumsum([0] + [co * (t * ci + 1) for ci, co, t in R.LAYERS])
+    for i in range(len(R.LAYERS)):
+        a, b = g[offs[i]:offs[i + 1]].astype(np.float64), gref[offs[i]:offs[i + 1]]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        say(f'layer {i}: rel {rel:.3e} cos {cos:.6f}')
+        assert rel < 6e-2 and cos > 0.998, i
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(13)
+    side, n = 16, 4
+    net = S.Net(side=side, capacity=n, seed=7)
+    mask, _ = masks_and_targets(rng, n, side)
+    # a learnable target: a smooth function of the mask
+    m = mask.astype(np.float32).reshape(n, side, side, side)
+    blur = (m + np.roll(m, 1, 1) + np.roll(m, -1, 1) + np.roll(m, 1, 2) + np.roll(m, -1, 2)) / 5
+    target = np.stack([blur, 0.5 * blur, m, 1 - blur], axis=-1).reshape(n, side ** 3, 4).astype(np.float32)
+    dm, dt = torch.as_tensor(mask, device='cuda'), torch.as_tensor(target, device='cuda')
+    opt = S.adam(3e-3)
+    first = net.train_step(dm, dt, opt)
+    for _ in range(60):
+        last = net.train_step(dm, dt, opt)
+    say('loss', first, last)
+    assert last < 0.2 * first
+
+
+def test_field_export_and_warm_start_roundtrip():
+    """Warm-starting from a converged crop's own fields stops after the minimum
+    number of reports with the same permeability."""
+    crops = [G.sphere_pack((16, 16, 16), 0.4, 1.5, 3.0, 200 + i) for i in range(3)]
+    force, tau, tol, interval = 1e-5, 0.8, 1e-5, 40
+    fields, res = S.solve(crops, tau=tau, force=force, tol=tol, interval=interval, max_steps=20000)
+    assert all(r['converged'] for r in res)
+    from lbflow_amd.engine import Batch
+    b = Batch(3, 19, 'MRT', tau, 1.0, S.labels(force), crops).init()
+    L = S.lib()
+    torch.cuda.synchronize()
+    for c in range(len(crops)):
+        assert L.lbf_batch_set_fields(b.h, c, fields[c].data_ptr()) == 0
+    warm, _ = b.run(axis=2, tol=tol, interval=interval, max_steps=20000)
+    for r0, r1 in zip(res, warm):
+        say('cold', r0['steps'], r0['K'], 'warm', r1['steps'], r1['K'])
+        assert r1['converged'] and r1['steps'] <= 4 * interval
+        assert abs(r1['K'] - r0['K']) <= 2e-5 * r0['K']
