@@ -28,7 +28,7 @@ hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begi
                                const uint32_t* site, const float* f, const float* ucur, float* fields, hipStream_t s);
 hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t tile_begin, uint32_t ntiles,
                              const uint32_t* site, const uint8_t* lab8, const LabelF* lab, const float* fields,
-                             float* f, hipStream_t s);
+                             float* f, float* upres, hipStream_t s);
 hipError_t launch_batch_report(uint32_t grid, uint64_t S, int D, int axis, const BatchBuffers& b, uint32_t ncrops,
                                double* partial, double* out, hipStream_t s);
 }  // namespace lbf
@@ -55,7 +55,7 @@ struct lbf_batch {
     uint8_t* d_lab8 = nullptr;
     int32_t* d_map = nullptr;
     LabelF* d_lab = nullptr;
-    float *d_ucur = nullptr, *d_uprev = nullptr, *d_uadj = nullptr;
+    float *d_ucur = nullptr, *d_uprev = nullptr, *d_uadj = nullptr, *d_upres = nullptr;
     DevStatus* d_status = nullptr;
     double *d_partial = nullptr, *d_out = nullptr;
     std::vector<void*> allocations;
@@ -113,6 +113,7 @@ BatchBuffers buffers(lbf_batch* b, bool swap)
     x.ucur = b->d_ucur;
     x.uprev = b->d_uprev;
     x.uadj = b->d_uadj;
+    x.upres = b->d_upres;
     x.status = b->d_status;
     return x;
 }
@@ -294,6 +295,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     rc |= balloc(b, &b->d_ucur, size_t(b->D) * b->S * 4);
     rc |= balloc(b, &b->d_uprev, size_t(b->D) * b->S * 4);
     rc |= balloc(b, &b->d_uadj, size_t(b->D) * b->S * 4);
+    rc |= balloc(b, &b->d_upres, size_t(b->D) * b->S * 4);
     rc |= balloc(b, &b->d_status, sizeof(DevStatus));
     rc |= balloc(b, &b->d_partial, b->T * 4 * 8);
     rc |= balloc(b, &b->d_out, size_t(b->ncrops) * 4 * 8);
@@ -311,7 +313,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     DevStatus st{};
     for (auto& v : st.first) v = ~0ull;
     ok = ok && up(b->d_status, &st, sizeof st) == hipSuccess;
-    for (float* p : {b->d_ucur, b->d_uprev, b->d_uadj})
+    for (float* p : {b->d_ucur, b->d_uprev, b->d_uadj, b->d_upres})
         ok = ok && hipMemset(p, 0, size_t(b->D) * b->S * 4) == hipSuccess;
     ok = ok && hipMemset(b->f[0], 0, pop) == hipSuccess && hipMemset(b->f[1], 0, pop) == hipSuccess;
     std::vector<char> all(b->ncrops, 1);
@@ -491,7 +493,17 @@ int lbf_batch_set_fields(lbf_batch* b, uint32_t c, const float* fields)
     const CropDev& cd = b->crops[c];
     const BatchParams p = params(b, 0u);
     BHIP(b, launch_batch_warm(b->Q, b->d.equil == LBF_MRT, p, cd.tile_begin, cd.tile_end - cd.tile_begin, b->d_site,
-                              b->d_lab8, b->d_lab, fields, static_cast<float*>(b->f[b->cur]), b->stream));
+                              b->d_lab8, b->d_lab, fields, static_cast<float*>(b->f[b->cur]), b->d_upres,
+                              b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int lbf_batch_relax(lbf_batch* b, uint32_t steps)
+{
+    if (!b) return -1;
+    for (uint32_t k = 0; k < steps; ++k)
+        if (step(b, 4u, false)) return -1;
     BHIP(b, hipStreamSynchronize(b->stream));
     return 0;
 }

@@ -19,7 +19,8 @@ struct CropDev {
 
 struct BatchParams {
     uint64_t S;           // slots per population plane (all crops, padded)
-    uint32_t flags;       // bit0: write velocity, bit1: write adjacent velocity
+    uint32_t flags;       // bit0: write velocity, bit1: write adjacent velocity,
+                          // bit2: collide with the prescribed velocity upres
     uint32_t pad;
     unsigned long long t;
     float inv_rho0;
@@ -40,6 +41,7 @@ struct BatchBuffers {
     float* ucur;                // velocity of the last fields step [d][slot]
     float* uprev;               // velocity of the previous report
     float* uadj;                // velocity of the step before a report
+    float* upres;               // prescribed velocity of the consistent-initialisation sweeps
     DevStatus* status;
 };
 

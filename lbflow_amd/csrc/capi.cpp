@@ -22,6 +22,7 @@ namespace lbf {
 hipError_t launch_scsp(int Q, bool mrt, bool init, bool fp32, int layout, const ScspParams& p,
                        const ScspBuffers& b, hipStream_t s);
 hipError_t launch_color(int Q, bool mrt, bool init, bool stream_solids, const ColorArgs& a, hipStream_t s);
+hipError_t launch_color32(int Q, bool mrt, bool init, const ColorArgs& a, hipStream_t s);
 hipError_t launch_report(const ReportArgs& a, double* out, hipStream_t s);
 uint32_t report_blocks(uint64_t N);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
@@ -181,8 +182,6 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         return fail(nullptr, "Only scsp and color fluid models are implemented.");
     if (d.equil != LBF_SRT && d.equil != LBF_MRT) return fail(nullptr, "Equilibrium model must be SRT or MRT.");
     if (d.precision != LBF_FP64 && d.precision != LBF_FP32) return fail(nullptr, "lbf_create: unknown precision");
-    if (d.model == LBF_MODEL_COLOR && d.precision != LBF_FP64)
-        return fail(nullptr, "lbf_create: the colour model runs in FP64 in this build");
     const int NP = d.model == LBF_MODEL_COLOR ? 2 : 1;
     uint64_t N = 1;
     for (int k = 0; k < d.D; ++k) {
@@ -274,7 +273,9 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     // Layout: the colour model and checkpoint-exact runs stream every site;
     // otherwise only fluid sites are stored (sparse) unless the dense
     // fluid-only layout is asked for.
-    if (NP == 2 || (d.flags & LBF_STREAM_SOLIDS)) c->layout = 0;
+    // The FP32 colour model keeps the dense layout and skips solid sites.
+    if (NP == 2) c->layout = d.precision == LBF_FP32 ? 1 : 0;
+    else if (d.flags & LBF_STREAM_SOLIDS) c->layout = 0;
     else c->layout = (d.flags & LBF_LAYOUT_DENSE) ? 1 : 2;
     c->P = c->layout == 2 ? c->nfluid : N;
 
@@ -377,8 +378,13 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         rc |= upload(c, c->d_map, map.data(), N * 4);
     }
     if (NP == 2) {
-        std::vector<double> dn(N, 86.0);
-        rc |= upload(c, c->d_densN, dn.data(), N * 8);
+        if (c->elem == 4) {
+            std::vector<float> dn(N, 86.0f);   // solid sentinel, colorfluid.cpp:49
+            rc |= upload(c, c->d_densN, dn.data(), N * 4);
+        } else {
+            std::vector<double> dn(N, 86.0);
+            rc |= upload(c, c->d_densN, dn.data(), N * 8);
+        }
         rc |= upload(c, c->d_wall, c->links.wall.data(), N * 4);
         std::vector<DevWall> walls(c->links.nwall);
         const int DD = d.D * d.D;
@@ -415,7 +421,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         lbf_destroy(c);
         return fail(nullptr, "HIP error during context set-up");
     }
-    if (NP == 2) c->kernel_name = "color_step";
+    if (NP == 2) c->kernel_name = d.precision == LBF_FP32 ? "color_step_f32" : "color_step";
     else c->kernel_name = d.precision == LBF_FP32 ? "scsp_step_f32" : "scsp_step_f64";
     static const char* layout_names[3] = {"dense", "dense-fluid", "sparse"};
     c->kernel_name += std::string(" (") + layout_names[c->layout] + ")";
@@ -521,6 +527,18 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
             a.bi[q] = q == 0 ? b0 : (q <= 6 ? b1 : b7);
         }
     }
+    // FP32 form: rest-state moments M ai_k and the shift density
+    for (int k = 0; k < c->Q; ++k) {
+        double m0 = 0.0, m1 = 0.0;
+        for (int q = 0; q < c->Q; ++q) {
+            const double mk = c->Q == 9 ? Lattice<9>::M[k][q] : Lattice<19>::M[k][q];
+            m0 += mk * a.ai0[q];
+            m1 += mk * a.ai1[q];
+        }
+        a.mai0[k] = m0;
+        a.mai1[k] = m1;
+    }
+    a.rho_ref = c->d.density[0];
     return a;
 }
 
@@ -539,9 +557,16 @@ static int run_step(lbf_ctx* c, bool init, uint32_t flags)
         a.t = init ? 0 : c->t + 1;
         a.uadj = c->d_uadj + (a.t & 1) * uint64_t(c->D) * c->N;
         a.chiadj = c->d_chiadj + (a.t & 1) * c->N;
-        a.fin = static_cast<const double*>(c->f[c->cur]);
-        a.fout = static_cast<double*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
-        e = launch_color(c->Q, mrt, init, stream_solids, a, c->stream);
+        if (c->elem == 4) {
+            a.fin32 = static_cast<const float*>(c->f[c->cur]);
+            a.fout32 = static_cast<float*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
+            a.densN32 = reinterpret_cast<float*>(c->d_densN);
+            e = launch_color32(c->Q, mrt, init, a, c->stream);
+        } else {
+            a.fin = static_cast<const double*>(c->f[c->cur]);
+            a.fout = static_cast<double*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
+            e = launch_color(c->Q, mrt, init, stream_solids, a, c->stream);
+        }
     }
     if (e != hipSuccess) return fail(c, std::string("HIP error: kernel launch: ") + hipGetErrorString(e));
     if (!init) {
@@ -626,6 +651,19 @@ int lbf_get_populations(lbf_ctx* c, double* out)
     else HIP_CHECK(c, hipMemcpy(h32.data(), c->f[c->cur], n * 4, hipMemcpyDeviceToHost));
     std::memset(out, 0, planes * c->N * sizeof(double));
     const std::vector<uint32_t>& desc = c->links.desc;
+    if (c->NP == 2 && c->elem == 4) {
+        // device planes: h = fT - s (Q planes), then f0 (Q planes); checkpoint: f0, f1
+        const ColorArgs a = color_args(c, 0u);
+        const int Q = c->Q;
+        for (int q = 0; q < Q; ++q)
+            for (uint64_t i = 0; i < c->N; ++i) {
+                if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
+                const double h = h32[size_t(q) * c->P + i], f0 = h32[size_t(Q + q) * c->P + i];
+                out[size_t(q) * c->N + i] = f0;
+                out[size_t(Q + q) * c->N + i] = h + a.dist0[q] - f0;
+            }
+        return 0;
+    }
     for (size_t plane = 0; plane < planes; ++plane)
         for (uint64_t s = 0; s < c->P; ++s) {
             const uint64_t i = c->layout == 2 ? c->list[s] : s;
@@ -647,6 +685,18 @@ int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
     } else {
         std::vector<double> h64(c->elem == 8 ? n : 0);
         std::vector<float> h32(c->elem == 4 ? n : 0);
+        if (c->NP == 2 && c->elem == 4) {
+            const ColorArgs a = color_args(c, 0u);
+            const int Q = c->Q;
+            const std::vector<uint32_t>& desc = c->links.desc;
+            for (int q = 0; q < Q; ++q)
+                for (uint64_t i = 0; i < c->N; ++i) {
+                    if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
+                    const double f0 = in[size_t(q) * c->N + i], f1 = in[size_t(Q + q) * c->N + i];
+                    h32[size_t(q) * c->P + i] = float(f0 + f1 - a.dist0[q]);
+                    h32[size_t(Q + q) * c->P + i] = float(f0);
+                }
+        } else
         for (size_t plane = 0; plane < planes; ++plane)
             for (uint64_t s = 0; s < c->P; ++s) {
                 const double v = in[plane * c->N + (c->layout == 2 ? c->list[s] : s)];
@@ -740,6 +790,10 @@ double lbf_step_bytes(const lbf_ctx* c)
 {
     if (!c) return 0.0;
     const double pops = 2.0 * c->NP * c->Q * double(c->elem);   // read + write per streamed site
+    // the colour model also writes and reads its indicator once per site
+    const double indicator = c->NP == 2 ? 2.0 * double(c->elem) : 0.0;
+    if (c->NP == 2 && c->layout == 0) return double(c->N) * (pops + indicator + 4.0);
+    if (c->NP == 2) return double(c->nfluid) * (pops + indicator) + double(c->N) * 4.0;
     if (c->layout == 0) return double(c->N) * (pops + 4.0);       // + descriptor
     if (c->layout == 1) return double(c->nfluid) * pops + double(c->N) * 4.0;
     // sparse: per fluid site its populations, its list entry, its descriptor

@@ -185,6 +185,13 @@ struct ColorArgs {
     double ak, beta;
     double ai0[19], ai1[19], bi[19];
     double dist0[19], dist1[19];   // ai_k * rho_k
+    // FP32 form: planes [0,Q) hold h = fT - dist0 (the total population shifted by
+    // phase 0's rest state), planes [Q,2Q) hold f0; densN32 is the FP32 indicator.
+    const float* fin32;
+    float* fout32;
+    float* densN32;
+    double mai0[19], mai1[19];     // M ai_k: rest-state moments of each phase
+    double rho_ref;                // rho_0, the density of the shift s_q = ai0_q rho_0
 };
 
 struct ReportArgs {

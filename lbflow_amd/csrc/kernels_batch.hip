@@ -23,7 +23,7 @@ batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fou
            const uint32_t* __restrict__ tile_crop, const CropDev* __restrict__ crops,
            const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8, const int32_t* __restrict__ map,
            const LabelF* __restrict__ lab, float* __restrict__ ucur, float* __restrict__ uadj,
-           DevStatus* __restrict__ status)
+           const float* __restrict__ upres, DevStatus* __restrict__ status)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
@@ -72,6 +72,14 @@ batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fou
         float u[D];
         const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
         if (!isfinite(drho)) raise_status(status, 0, p.t);
+        if (p.flags & 4u) {
+            // consistent initialisation (Mei, Luo, Lallemand & d'Humieres, Computers &
+            // Fluids 35 (2006) 855-862): collide towards the equilibrium of the current
+            // density and the prescribed velocity, so density and non-equilibrium parts
+            // relax to the given flow field before the run starts
+#pragma unroll
+            for (int d = 0; d < D; ++d) u[d] = upres[uint64_t(d) * p.S + slot];
+        }
         if (p.flags & 1u) {
 #pragma unroll
             for (int d = 0; d < D; ++d) ucur[uint64_t(d) * p.S + slot] = u[d];
@@ -179,7 +187,8 @@ batch_fields(uint64_t S, float rho0, uint32_t tile_begin, const uint32_t* __rest
 template <int Q, bool MRT>
 __global__ void __launch_bounds__(TILE)
 batch_warm(BatchParams p, uint32_t tile_begin, const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8,
-           const LabelF* __restrict__ lab, const float* __restrict__ fields, float* __restrict__ f)
+           const LabelF* __restrict__ lab, const float* __restrict__ fields, float* __restrict__ f,
+           float* __restrict__ upres)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
@@ -192,7 +201,9 @@ batch_warm(BatchParams p, uint32_t tile_begin, const uint32_t* __restrict__ site
     float usq = 0.0f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        ue[d] = fields[uint64_t(s) * (D + 1) + d] - lb.half_rho[d];
+        const float ud = fields[uint64_t(s) * (D + 1) + d];
+        upres[uint64_t(d) * p.S + slot] = ud;
+        ue[d] = ud - lb.half_rho[d];
         usq += ue[d] * ue[d];
     }
     const float rho0 = p.relax.rho0;
@@ -227,16 +238,16 @@ hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begi
 
 hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t tile_begin, uint32_t ntiles,
                              const uint32_t* site, const uint8_t* lab8, const LabelF* lab, const float* fields,
-                             float* f, hipStream_t s)
+                             float* f, float* upres, hipStream_t s)
 {
     if (ntiles == 0) return hipSuccess;
     const dim3 g(ntiles), b(TILE);
     if (Q == 19) {
-        if (mrt) hipLaunchKernelGGL((batch_warm<19, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
-        else hipLaunchKernelGGL((batch_warm<19, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
+        if (mrt) hipLaunchKernelGGL((batch_warm<19, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
+        else hipLaunchKernelGGL((batch_warm<19, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
     } else {
-        if (mrt) hipLaunchKernelGGL((batch_warm<9, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
-        else hipLaunchKernelGGL((batch_warm<9, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f);
+        if (mrt) hipLaunchKernelGGL((batch_warm<9, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
+        else hipLaunchKernelGGL((batch_warm<9, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
     }
     return hipGetLastError();
 }
@@ -248,12 +259,12 @@ static void launch_batch_q(uint32_t grid, const BatchParams& p, const BatchBuffe
         hipLaunchKernelGGL((batch_step<Q, MRT, INIT, true>), dim3(grid), dim3(TILE), 0, s, p,
                            static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
                            b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.map, b.lab, b.ucur,
-                           b.uadj, b.status);
+                           b.uadj, b.upres, b.status);
     else
         hipLaunchKernelGGL((batch_step<Q, MRT, INIT, false>), dim3(grid), dim3(TILE), 0, s, p,
                            static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
                            b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.map, b.lab, b.ucur,
-                           b.uadj, b.status);
+                           b.uadj, b.upres, b.status);
 }
 
 hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, const BatchParams& p,

@@ -1,0 +1,393 @@
+// kernels_color32.hip -- FP32 colour-gradient time step.
+//
+// Same two-launch structure and operators as kernels_color.hip (reference:
+// ColorFluid::calcMacroVar / bodyForce / collision / colorGradient,
+// solver/src/lbm/colorfluid.cpp:238-570, 736-750), on FP32 state:
+//   planes [0, Q):  h_q = fT_q - s_q, the total population shifted by phase
+//                   0's rest state s_q = ai0_q rho_0 (h is the small
+//                   deviation that carries the momentum);
+//   planes [Q, 2Q): f0_q, the phase-0 population (the colour).
+// rho = rho_0 + sum h, rho0 = sum f0, rho1 = rho - rho0.  The collision runs
+// in moment space with the reference's transform M and per-site rates from
+// tau (the non-uniform branch, colorfluid.cpp:374-386; with equal tau it is
+// the uniform A = M^-1 S M form); the shifted equilibrium moments are
+// (rho0 - rho_0) M ai0 + rho1 M ai1 + rho M w (3eu + 4.5 (eu)^2 - 1.5 u^2).
+// Perturbation and recolouring act on the unshifted total fT = h + s.
+// Solid sites are not streamed (their populations stay 0).  Macro fields for
+// output and reports are written in FP64, so the component-mass and flux
+// reductions run in FP64.
+//
+// FP32 tolerance: the negative-component check uses 1e-5 rho (the FP64
+// kernel's 1e-10 rho is below the FP32 rounding of rho1 = rho - rho0).
+#include "collide_f32.h"
+#include "device.h"
+
+namespace lbf {
+
+namespace {
+
+constexpr float SOLID_N32 = 86.0f;   // colorfluid.cpp:49
+constexpr float NEG_TOL32 = 1e-5f;
+constexpr float SQRT2_32 = 1.41421356f;
+
+template <int Q, bool INIT>
+__device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t i, uint32_t desc, const DevLabel& lab,
+                                             float* h, float* f0, float& r0, float& dr)
+{
+    using L = Lattice<Q>;
+    const uint64_t N = a.g.N;
+    r0 = 0.0f;
+    dr = 0.0f;
+    if (INIT) {
+        // colorfluid.cpp:214-236: f[phase] = ai_phase rho_phase - 1.5 w (e.F), other phase 0
+        const int ph = lab.phase;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double f = (ph == 0 ? a.dist0[q] : a.dist1[q]) - 1.5 * L::w[q] * lab.ef[q];
+            h[q] = float(f - a.dist0[q]);
+            f0[q] = ph == 0 ? float(f) : 0.0f;
+            r0 += f0[q];
+            dr += h[q];
+        }
+        return;
+    }
+    int x, y, z;
+    site_coords(a.g, i, x, y, z);
+    const uint32_t links = desc & DESC_LINK_MASK;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint32_t src;
+        int qs = q;
+        if (q == 0) src = i;
+        else if ((links >> L::op[q]) & 1u) { src = i; qs = L::op[q]; }
+        else src = pull_source<Q>(a.g, x, y, z, q);
+        h[q] = a.fin32[uint64_t(qs) * N + src];
+        f0[q] = a.fin32[(uint64_t(Q) + qs) * N + src];
+        r0 += f0[q];
+        dr += h[q];
+    }
+}
+
+}  // namespace
+
+template <int Q, bool INIT>
+__global__ void __launch_bounds__(256) color_density_f32(ColorArgs a)
+{
+    const uint32_t i = xcd_block() * 256u + threadIdx.x;
+    if (i >= a.g.N) return;
+    const uint32_t desc = a.desc[i];
+    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
+    if (!(lab.kind & KIND_FLUID)) return;
+    float h[Q], f0[Q], r0, dr;
+    load_color32<Q, INIT>(a, i, desc, lab, h, f0, r0, dr);
+    const float rho = float(a.rho_ref) + dr;
+    a.densN32[i] = (2.0f * r0 - rho) / rho;   // (rho0 - rho1) / rho
+}
+
+template <int Q, bool MRT, bool INIT>
+__global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
+{
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    constexpr int H = L::NHALF;
+    constexpr MomentTables<Q> MT = kMT<Q>;
+    const uint32_t i = xcd_block() * 256u + threadIdx.x;
+    if (i >= a.g.N) return;
+    const uint64_t N = a.g.N;
+    const uint32_t desc = a.desc[i];
+    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
+    const int kind = lab.kind;
+    if (!(kind & KIND_FLUID)) return;
+
+    float h[Q], f0[Q], r0, dr;
+    load_color32<Q, INIT>(a, i, desc, lab, h, f0, r0, dr);
+
+    // calcMacroVar (colorfluid.cpp:238-290)
+    const float rt = float(a.rho_ref) + dr;
+    const float r1 = rt - r0;
+    if (!isfinite(r0) || !isfinite(rt) || rt <= 0.0f) raise_status(a.status, 0, a.t);
+    else if (r0 < -NEG_TOL32 * rt || r1 < -NEG_TOL32 * rt) raise_status(a.status, 1, a.t);
+    const float inv_rt = 1.0f / rt;
+    const float dn = (r0 - r1) * inv_rt;
+    float tau = (1.0f + dn) * float(a.relax0) + (1.0f - dn) * float(a.relax1);
+    if (a.harmonic) {
+        const float phi = r0 * inv_rt;
+        tau = 0.5f + 3.0f / (phi / float(a.nu0) + (1.0f - phi) / float(a.nu1));
+    }
+    if (!isfinite(tau) || tau <= 0.5f) raise_status(a.status, 2, a.t);
+    float u[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            if (L::e[q][d] == 1) s += h[q];
+            else if (L::e[q][d] == -1) s -= h[q];
+        }
+        u[d] = s * inv_rt;
+        if (kind & KIND_GV) u[d] += float(lab.half[d]) * inv_rt;   // bodyForce (colorfluid.cpp:736-750)
+        if (!isfinite(u[d])) raise_status(a.status, 3, a.t);
+    }
+    if (a.flags & 1u) {
+        a.rho0[i] = double(r0);
+        a.rho1[i] = double(r1);
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = double(u[d]);
+    }
+    if (a.flags & 2u) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = double(u[d]);
+        a.chiadj[i] = double(dn);
+    }
+
+    // collision of the total population (colorfluid.cpp:292-386), on h
+    const bool forced = kind & KIND_FORCED;
+    float F[3] = {0.0f, 0.0f, 0.0f};
+    if (forced)
+#pragma unroll
+        for (int d = 0; d < D; ++d) F[d] = float(lab.force[d]);
+    float usq = 0.0f, uF = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        usq += u[d] * u[d];
+        uF += u[d] * F[d];
+    }
+    const float a0 = r0 - float(a.rho_ref);   // rest-state weights of the shifted equilibrium
+    if constexpr (!MRT) {
+        const float inv_tau = 1.0f / tau, cg = 1.0f - 0.5f * inv_tau;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            float eu = 0.0f, ef = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (L::e[q][d] == 1) { eu += u[d]; ef += F[d]; }
+                else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
+            }
+            const float w = float(L::w[q]);
+            const float heq = a0 * float(a.ai0[q]) + r1 * float(a.ai1[q]) +
+                              rt * w * (eu * (3.0f + 4.5f * eu) - 1.5f * usq);
+            h[q] += (heq - h[q]) * inv_tau;
+            if (forced) h[q] += cg * w * (3.0f * (ef - uF) + 9.0f * eu * ef);
+        }
+    } else {
+        const float su = 1.0f / tau;
+        const float sq = 8.0f * (2.0f - su) / (8.0f - su);
+        float sp[H], sm[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int q = L::half[j];
+            sp[j] = h[q] + h[L::op[q]];
+            sm[j] = h[q] - h[L::op[q]];
+        }
+        float y[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            float mg = 0.0f;
+            if (forced) {
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    if (MT.T1[k][p] != 0.0) mg += float(3.0 * MT.T1[k][p]) * F[p];
+#pragma unroll
+                    for (int b = 0; b < D; ++b)
+                        if (MT.T2[k][p][b] != 0.0) mg += float(9.0 * MT.T2[k][p][b]) * u[p] * F[b];
+                }
+                if (MT.T0[k] != 0.0) mg -= float(3.0 * MT.T0[k]) * uF;
+            }
+            if (MT.rate[k] == 0) {
+                y[k] = mg * float(1.0 / MT.n2[k]);
+                continue;
+            }
+            float m = 0.0f;
+            if (MT.parity[k] > 0) {
+                if (L::M[k][0] != 0) m += float(L::M[k][0]) * h[0];
+#pragma unroll
+                for (int j = 0; j < H; ++j)
+                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sp[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < H; ++j)
+                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sm[j];
+            }
+            float meq = 0.0f;
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                if (MT.T1[k][p] != 0.0) meq += float(3.0 * MT.T1[k][p]) * u[p];
+#pragma unroll
+                for (int b = 0; b < D; ++b)
+                    if (MT.T2[k][p][b] != 0.0) meq += float(4.5 * MT.T2[k][p][b]) * u[p] * u[b];
+            }
+            if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
+            meq = meq * rt + a0 * float(a.mai0[k]) + r1 * float(a.mai1[k]);
+            const float s = MT.rate[k] == 1 ? su : sq;
+            y[k] = (s * (meq - m) + (1.0f - 0.5f * s) * mg) * float(1.0 / MT.n2[k]);
+        }
+        {
+            float e0 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < Q; ++k)
+                if (MT.parity[k] > 0 && L::M[k][0] != 0) e0 += float(L::M[k][0]) * y[k];
+            h[0] += e0;
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int q = L::half[j];
+            float ev = 0.0f, od = 0.0f;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                if (L::M[k][q] == 0) continue;
+                if (MT.parity[k] > 0) ev += float(L::M[k][q]) * y[k];
+                else od += float(L::M[k][q]) * y[k];
+            }
+            h[q] += ev + od;
+            h[L::op[q]] += ev - od;
+        }
+    }
+
+    // colour gradient (legacy branch, colorfluid.cpp:435-570)
+    int x, y, z;
+    site_coords(a.g, i, x, y, z);
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    float norm;
+    const int32_t wi = a.wall[i];
+    if (!wi) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int k = 0; k < L::NREL; ++k) {
+                const int q = L::reld[d][k];
+                const float dnq = a.densN32[push_target<Q>(a.g, x, y, z, q)];
+                if (L::e[q][d] == 1) g[d] += float(L::w[q]) * dnq;
+                else g[d] -= float(L::w[q]) * dnq;
+            }
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] *= 3.0f;
+        float n2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
+        norm = sqrtf(n2);
+    } else {
+        const float dn0 = a.densN32[i];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int k = 0; k < L::NREL; ++k) {
+                const int q = L::reld[d][k];
+                const float dnq = a.densN32[push_target<Q>(a.g, x, y, z, q)];
+                if (dnq == SOLID_N32) continue;
+                if (L::e[q][d] == 1) g[d] += float(L::w[q]) * (dnq - dn0);
+                else g[d] -= float(L::w[q]) * (dnq - dn0);
+            }
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] *= 3.0f;
+        float n2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
+        norm = sqrtf(n2);
+        const DevWall& wl = a.walls[wi];
+        const float cA = float(wl.cosca), sA = float(wl.sinca);
+        if constexpr (D == 3) {
+            float T[9], Ti[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                T[k] = float(wl.tib[k]);
+                Ti[k] = float(wl.tibinv[k]);
+            }
+            float gn[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) gn[r] = T[r * 3] * g[0] + T[r * 3 + 1] * g[1] + T[r * 3 + 2] * g[2];
+            // RotateTx3D(-gn0, gn1) (fn.cpp:130-146)
+            float c = 1.0f, sn = 0.0f;
+            const float so = -gn[0], sa = gn[1];
+            const float hyp = sqrtf(so * so + sa * sa);
+            if (hyp != 0.0f) {
+                c = sa / hyp;
+                sn = so / hyp;
+            }
+            const float tx[9] = {c, -sn, 0.0f, sn, c, 0.0f, 0.0f, 0.0f, 1.0f};
+            const float hv[3] = {0.0f, norm * sA, norm * cA};
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                float s = 0.0f;
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    const float tic = Ti[r * 3] * tx[cc] + Ti[r * 3 + 1] * tx[3 + cc] + Ti[r * 3 + 2] * tx[6 + cc];
+                    s += tic * hv[cc];
+                }
+                g[r] = s;
+            }
+        } else {
+            const float gn0 = float(wl.tib[0]) * g[0] + float(wl.tib[1]) * g[1];
+            const float hv0 = gn0 >= 0.0f ? norm * sA : -norm * sA, hv1 = norm * cA;
+            const float g0 = float(wl.tibinv[0]) * hv0 + float(wl.tibinv[1]) * hv1;
+            const float g1 = float(wl.tibinv[2]) * hv0 + float(wl.tibinv[3]) * hv1;
+            g[0] = g0;
+            g[1] = g1;
+        }
+    }
+
+    // perturbation + recolouring (colorfluid.cpp:389-428)
+    const float dens0T = r0 * inv_rt;
+    bool finite = true;
+    if (norm > 1e-9f) {
+        const float b01 = float(a.beta) * dens0T * r1 * inv_rt;
+        const float akn = float(a.ak) * norm / tau;
+        const float inv_norm = 1.0f / norm;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            float angle = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (L::e[q][d] == 1) angle += g[d];
+                else if (L::e[q][d] == -1) angle -= g[d];
+            }
+            angle *= inv_norm;
+            h[q] += akn * (float(L::w[q]) * angle * angle - float(a.bi[q]));
+            if (L::diagonal(q)) angle *= (1.0f / SQRT2_32);
+            const float feq0 = r0 * float(a.ai0[q]) + r1 * float(a.ai1[q]);
+            const float fo = dens0T * (h[q] + float(a.dist0[q])) + b01 * angle * feq0;
+            finite = finite && isfinite(h[q]) && isfinite(fo);
+            a.fout32[uint64_t(q) * N + i] = h[q];
+            a.fout32[(uint64_t(Q) + q) * N + i] = fo;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const float fo = dens0T * (h[q] + float(a.dist0[q]));
+            finite = finite && isfinite(h[q]) && isfinite(fo);
+            a.fout32[uint64_t(q) * N + i] = h[q];
+            a.fout32[(uint64_t(Q) + q) * N + i] = fo;
+        }
+    }
+    if (!finite) raise_status(a.status, 4, a.t);
+}
+
+// Solid sites carry the indicator sentinel; set once per context.
+__global__ void color_solid_sentinel_f32(ColorArgs a)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.g.N) return;
+    const DevLabel& lab = a.lab[a.desc[i] >> DESC_LABEL_SHIFT];
+    if (!(lab.kind & KIND_FLUID)) a.densN32[i] = SOLID_N32;
+}
+
+template <int Q, bool MRT, bool INIT>
+static hipError_t launch_color32_q(const ColorArgs& a, hipStream_t s)
+{
+    const dim3 grid(uint32_t((a.g.N + 255) / 256)), block(256);
+    if (INIT) hipLaunchKernelGGL(color_solid_sentinel_f32, grid, block, 0, s, a);
+    hipLaunchKernelGGL((color_density_f32<Q, INIT>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((color_step_f32<Q, MRT, INIT>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_color32(int Q, bool mrt, bool init, const ColorArgs& a, hipStream_t s)
+{
+    if (Q == 19) {
+        if (mrt) return init ? launch_color32_q<19, true, true>(a, s) : launch_color32_q<19, true, false>(a, s);
+        return init ? launch_color32_q<19, false, true>(a, s) : launch_color32_q<19, false, false>(a, s);
+    }
+    if (mrt) return init ? launch_color32_q<9, true, true>(a, s) : launch_color32_q<9, true, false>(a, s);
+    return init ? launch_color32_q<9, false, true>(a, s) : launch_color32_q<9, false, false>(a, s);
+}
+
+}  // namespace lbf

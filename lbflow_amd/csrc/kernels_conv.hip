@@ -248,7 +248,10 @@ struct ConvLaunch {
                         bool out_f32, hipStream_t s)
     {
         const dim3 grid((g.nvox + 255u) / 256u, (COUT + BN_FWD - 1) / BN_FWD);
-        if (out_f32)
+        if (out_f32 && act == ACT_RELU)
+            hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_RELU, true, true>), grid, dim3(256), 0, s, g,
+                               in, w, bias, out);
+        else if (out_f32)
             hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_NONE, true, true>), grid, dim3(256), 0, s, g,
                                in, w, bias, out);
         else if (act == ACT_RELU)

@@ -15,7 +15,6 @@ warm starts on unseen crops.  Device memory is held in torch tensors
 """
 import ctypes as C
 import json
-import os
 import time
 from multiprocessing import get_context
 
@@ -64,6 +63,8 @@ def _bind(L):
     L.lbf_batch_crop_fields.restype = i32
     L.lbf_batch_set_fields.argtypes = [vp, u32, vp]
     L.lbf_batch_set_fields.restype = i32
+    L.lbf_batch_relax.argtypes = [vp, u32]
+    L.lbf_batch_relax.restype = i32
     L._net_bound = True
     return L
 
@@ -286,7 +287,7 @@ def load(path, capacity=4):
 # ---------------------------------------------------------------------------
 # warm vs cold
 # ---------------------------------------------------------------------------
-def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=None):
+def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=None, relax=0):
     """Runs all crops to the stop rule in one batch, cold (rest state) or warm
     (network prediction).  Returns (results, seconds from init to the stop,
     inference seconds)."""
@@ -312,8 +313,101 @@ def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=Non
             for j in range(m.shape[0]):
                 if L.lbf_batch_set_fields(b.h, i + j, C.c_void_p(f[j].data_ptr())):
                     raise EngineError(L.lbf_batch_error(b.h).decode())
+        if relax and L.lbf_batch_relax(b.h, relax):
+            raise EngineError(L.lbf_batch_error(b.h).decode())
         t_inf = time.perf_counter() - t1
     res, _ = b.run(axis=2, tol=tol, interval=interval, max_steps=max_steps)
     sec = time.perf_counter() - t0
     b.close()
     return res, sec, t_inf
+
+
+def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, tau=0.8, force=1e-6,
+             train_force=1e-5, train_tol=1e-5, eval_tol=1e-3, interval=50, max_steps=60000, out=None, log=print,
+             relax_sweeps=(0, 50)):
+    """Generate, solve, train, evaluate.  Returns a report dict."""
+    import torch
+    rep = dict(side=side, n_train=n_train, n_eval=n_eval, epochs=epochs, capacity=capacity, lr=lr, tau=tau,
+               force=force, train_force=train_force, train_tol=train_tol, eval_tol=eval_tol,
+               report_interval=interval)
+    t0 = time.perf_counter()
+    train_crops = make_crops(n_train, 10000, side)
+    eval_crops = make_crops(n_eval, 20000, side)
+    rep['crop_generation_s'] = round(time.perf_counter() - t0, 2)
+    log(f'crops generated in {rep["crop_generation_s"]} s')
+    t0 = time.perf_counter()
+    fields, res = solve(train_crops, tau=tau, force=train_force, tol=train_tol, interval=100, max_steps=max_steps,
+                        log=log)
+    rep['training_data'] = dict(seconds=round(time.perf_counter() - t0, 2),
+                                converged=int(sum(r['converged'] for r in res)),
+                                steps_mean=float(np.mean([r['steps'] for r in res])),
+                                fluid_sites=int(sum(int((c == PORE).sum()) for c in train_crops)))
+    masks = torch.as_tensor(np.stack([(c == PORE) for c in train_crops]).astype(np.uint8), device='cuda')
+    del train_crops
+    scaling = Scaling.fit(fields, masks, tau, train_force)
+    targets = scaling.normalise(fields).contiguous()
+    del fields
+    torch.cuda.synchronize()
+    net = Net(side=side, capacity=capacity, seed=1)
+    ms, flops = net.timed_train(masks[:capacity].contiguous(), targets[:capacity].contiguous(), adam(0.0), 3)
+    rep['train_step'] = dict(ms=round(ms, 3), tflops=round(flops / ms / 1e9, 1), samples_per_step=capacity,
+                             conv_flops_per_step=flops)
+    log(f'train step {ms:.2f} ms, {flops / ms / 1e9:.1f} TFLOP/s (convolutions)')
+    net.close()   # the timing steps ran with lr 0; train a fresh network
+    net = Net(side=side, capacity=capacity, seed=1)
+    t0 = time.perf_counter()
+    hist = train(net, masks, targets, epochs, lr=lr, batch=capacity, log=log)
+    rep['training'] = dict(seconds=round(time.perf_counter() - t0, 2), loss=[round(h, 5) for h in hist])
+    del masks, targets
+    torch.cuda.empty_cache()
+    if out:
+        save(out, net, scaling)
+    cold, t_cold, _ = run_crops(eval_crops, tau, force, eval_tol, interval, max_steps)
+    ref, _, _ = run_crops(eval_crops, tau, force, 1e-5, interval, max_steps)
+    kc = np.array([r['K'] for r in cold])
+    kr = np.array([r['K'] for r in ref])
+    sc = np.array([r['steps'] for r in cold])
+    ev = dict(cold=dict(seconds=round(t_cold, 3), steps_mean=float(sc.mean()), steps_total=int(sc.sum()),
+                        converged=int(sum(r['converged'] for r in cold))),
+              k_rel_cold_vs_tight_mean=float(np.mean(np.abs(kc - kr) / kr)))
+    for relax in relax_sweeps:
+        warm, t_warm, t_inf = run_crops(eval_crops, tau, force, eval_tol, interval, max_steps, net, scaling, relax)
+        kw = np.array([r['K'] for r in warm])
+        sw = np.array([r['steps'] for r in warm])
+        ev[f'warm_relax{relax}'] = dict(
+            seconds=round(t_warm, 3), inference_and_relax_seconds=round(t_inf, 3), steps_mean=float(sw.mean()),
+            steps_total=int(sw.sum()), converged=int(sum(r['converged'] for r in warm)),
+            speedup_wall=round(t_cold / t_warm, 3), speedup_iterations=round(float(sc.sum() / sw.sum()), 3),
+            k_rel_vs_cold_max=float(np.max(np.abs(kw - kc) / kc)),
+            k_rel_vs_cold_mean=float(np.mean(np.abs(kw - kc) / kc)),
+            k_rel_vs_tight_mean=float(np.mean(np.abs(kw - kr) / kr)))
+        log(f'relax {relax}: ' + json.dumps(ev[f'warm_relax{relax}']))
+    rep['evaluation'] = ev
+    log(json.dumps(ev['cold']))
+    return rep
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description='train and evaluate the warm-start network')
+    ap.add_argument('--train', type=int, default=4096)
+    ap.add_argument('--eval', type=int, default=256)
+    ap.add_argument('--side', type=int, default=64)
+    ap.add_argument('--epochs', type=int, default=8)
+    ap.add_argument('--capacity', type=int, default=8)
+    ap.add_argument('--lr', type=float, default=1e-3)
+    ap.add_argument('--out', default=None, help='weights file (.npz)')
+    ap.add_argument('--report', default=None, help='report file (.json)')
+    a = ap.parse_args(argv)
+
+    def log(*x):
+        print(*x, flush=True)
+
+    rep = pipeline(a.train, a.eval, a.side, a.epochs, a.capacity, a.lr, out=a.out, log=log)
+    if a.report:
+        with open(a.report, 'w') as f:
+            json.dump(rep, f, indent=1)
+
+
+if __name__ == '__main__':
+    main()

@@ -57,7 +57,7 @@ def pack_grads(params):
 
 def bf16(x):
     """Round to bfloat16 (nearest even) and back, keeping the dtype."""
-    return x.to(torch.bfloat16).to(x.dtype)
+    return x.to(torch.bfloat16).to(x.dtype)   # autograd rounds the gradient through it as well
 
 
 def conv(x, w, b, relu, round_bf16):
@@ -88,14 +88,24 @@ def forward(mask, params, round_bf16=False):
     def up(x):
         return x.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
 
+    def cat(a, b):
+        # the device stores the gradient of a concatenation (a convolution's input
+        # gradient) in BF16; bf16() above rounds gradients only where it is applied
+        c = torch.cat([a, b], 1)
+        if r and c.requires_grad:
+            c.register_hook(bf16)
+        return c
+
     e1 = conv(conv(x0, *P[0], True, r), *P[1], True, r)
     e2 = conv(conv(pool(e1), *P[2], True, r), *P[3], True, r)
     e3 = conv(conv(pool(e2), *P[4], True, r), *P[5], True, r)
     e4 = conv(conv(pool(e3), *P[6], True, r), *P[7], True, r)
-    d3 = conv(conv(torch.cat([up(e4), e3], 1), *P[8], True, r), *P[9], True, r)
-    d2 = conv(conv(torch.cat([up(d3), e2], 1), *P[10], True, r), *P[11], True, r)
-    d1 = conv(conv(torch.cat([up(d2), e1], 1), *P[12], True, r), *P[13], True, r)
+    d3 = conv(conv(cat(up(e4), e3), *P[8], True, r), *P[9], True, r)
+    d2 = conv(conv(cat(up(d3), e2), *P[10], True, r), *P[11], True, r)
+    d1 = conv(conv(cat(up(d2), e1), *P[12], True, r), *P[13], True, r)
     out = conv(d1, *P[14], False, False)
+    if r and out.requires_grad:
+        out.register_hook(bf16)   # the loss gradient is stored in BF16 too
     return out.permute(0, 2, 3, 4, 1).reshape(n, s * s * s, 4)
 
 

@@ -1,0 +1,93 @@
+"""FP32 colour-gradient model against the FP64 one.
+
+The FP64 colour kernels are bit-exact with the CPU oracle (test_gpu_parity);
+the FP32 form (shifted total population + phase-0 population, FP64 macro
+fields and reductions) is pinned by its distance to them on the same
+two-phase sphere packs, by exact conservation of each component's mass
+(bounce-back and recolouring conserve it), and by a bit-exact restart.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from lbflow_amd import harness as H
+from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Engine
+from tests.helpers import sphere_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True),
+    dict(model='color', eq='MRT', tau=(0.7, 1.2), two_phase=True),
+    dict(model='color', eq='SRT', tau=(1.0, 0.8), two_phase=True, angle=120),
+    dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, dims=2, angle=45),
+]
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run(spec, steps, precision):
+    case = H.lattice_case(spec)
+    e = Engine(case, H.geometry(spec), precision, stream_solids=precision == FP64).init()
+    for k in range(steps):
+        e.step(STEP_FIELDS if k == steps - 1 else 0)
+    e.synchronize()
+    e.check_status()
+    return e
+
+
+def masses(e):
+    r0, r1, _ = e.fields()
+    return float(r0.sum()), float(r1.sum())
+
+
+@pytest.mark.parametrize('kw', CASES, ids=lambda k: f"{k['eq']}-{k['tau']}-{k.get('dims', 3)}d")
+def test_fp32_colour_tracks_fp64(kw):
+    spec = sphere_case(**kw)
+    say('case', spec['name'], kw)
+    a = run(spec, 200, FP64)
+    b = run(spec, 200, FP32)
+    ra0, ra1, ua = a.fields()
+    rb0, rb1, ub = b.fields()
+    d0 = np.max(np.abs(ra0 - rb0))
+    du = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
+    say('max |rho0 diff|', d0, 'rel L2 u', du)
+    assert d0 < 1e-4
+    assert du < 1e-2
+    # component masses: FP64 conserves them to round-off; FP32 to its own
+    m64, m32 = masses(a), masses(b)
+    say('masses', m64, m32)
+    for x, y in zip(m64, m32):
+        assert abs(x - y) <= 1e-5 * max(abs(x), 1.0)
+
+
+def test_fp32_colour_conserves_component_mass():
+    spec = sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True)
+    e = run(spec, 1, FP32)
+    m_first = masses(e)
+    for _ in range(400):
+        e.step(0)
+    e.step(STEP_FIELDS)
+    e.synchronize()
+    m_last = masses(e)
+    say('masses', m_first, m_last)
+    for x, y in zip(m_first, m_last):
+        assert abs(x - y) <= 1e-5 * abs(x)
+
+
+def test_fp32_colour_restart_is_exact():
+    spec = sphere_case(model='color', eq='MRT', tau=(0.7, 1.2), two_phase=True)
+    a = run(spec, 50, FP32)
+    pops = a.populations()
+    b = Engine(H.lattice_case(spec), H.geometry(spec), FP32, stream_solids=False)
+    b.set_populations(pops, a.t)
+    for e in (a, b):
+        for _ in range(30):
+            e.step(0)
+        e.step(STEP_FIELDS)
+        e.synchronize()
+    pa, pb = a.populations(), b.populations()
+    assert np.array_equal(pa, pb)
