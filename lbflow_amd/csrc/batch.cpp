@@ -53,7 +53,7 @@ struct lbf_batch {
     CropDev* d_crops = nullptr;
     uint32_t* d_site = nullptr;
     uint8_t* d_lab8 = nullptr;
-    int32_t* d_map = nullptr;
+    uint2* d_rank = nullptr;
     LabelF* d_lab = nullptr;
     float *d_ucur = nullptr, *d_uprev = nullptr, *d_uadj = nullptr, *d_upres = nullptr;
     DevStatus* d_status = nullptr;
@@ -108,7 +108,7 @@ BatchBuffers buffers(lbf_batch* b, bool swap)
     x.crops = b->d_crops;
     x.site = b->d_site;
     x.lab8 = b->d_lab8;
-    x.map = b->d_map;
+    x.rank = b->d_rank;
     x.lab = b->d_lab;
     x.ucur = b->d_ucur;
     x.uprev = b->d_uprev;
@@ -206,7 +206,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     // slot space
     std::vector<uint32_t> site;
     std::vector<uint8_t> lab8;
-    std::vector<int32_t> map;
+    std::vector<uint2> rank;
     b->crops.resize(d.ncrops);
     b->crop_sites.resize(d.ncrops);
     b->crop_force.assign(size_t(d.ncrops) * 3, 0.0);
@@ -226,7 +226,8 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
         cd.g.div_nx = DevFastDiv{fx.d, fx.m, fx.s};
         cd.g.div_nxy = DevFastDiv{fxy.d, fxy.m, fxy.s};
         cd.slot_base = uint32_t(site.size());
-        cd.map_base = uint32_t(map.size());
+        cd.rank_base = uint32_t(rank.size());
+        rank.resize(rank.size() + (N + 31) / 32, uint2{0u, 0u});
         cd.tile_begin = uint32_t(site.size() / TILE);
         uint32_t nf = 0;
         bool have_force = false;
@@ -238,19 +239,24 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
                 return bfail(nullptr, "map::at");
             }
             if (lab[v].kind & KIND_FLUID) {
-                map.push_back(int32_t(nf++));
+                uint2& r = rank[cd.rank_base + i / 32];
+                r.y |= 1u << (i % 32);
+                ++nf;
                 site.push_back(uint32_t(i));
                 lab8.push_back(v);
                 if (!have_force && (lab[v].kind & KIND_GV)) {
                     for (int k = 0; k < d.D; ++k) b->crop_force[size_t(c) * 3 + k] = force64[size_t(v) * 3 + k];
                     have_force = true;
                 }
-            } else if (t == LBF_BB || t == LBF_SD) {
-                map.push_back(-1);
-            } else {
+            } else if (t != LBF_BB && t != LBF_SD) {
                 lbf_batch_destroy(b);
                 return bfail(nullptr, "lbf_batch_create: batched crops support VD/GV fluid and BB/SD solids only");
             }
+        }
+        uint32_t before = 0;
+        for (size_t w = cd.rank_base; w < rank.size(); ++w) {
+            rank[w].x = before;
+            before += uint32_t(__builtin_popcount(rank[w].y));
         }
         while (site.size() % TILE) {
             site.push_back(NO_SITE);
@@ -264,7 +270,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
         b->fluid += nf;
         for (uint32_t t = cd.tile_begin; t < cd.tile_end; ++t) b->tile_crop.push_back(c);
     }
-    if (site.size() >= (uint64_t(1) << 32) || map.size() >= (uint64_t(1) << 31)) {
+    if (site.size() >= (uint64_t(1) << 32) || rank.size() >= (uint64_t(1) << 31)) {
         lbf_batch_destroy(b);
         return bfail(nullptr, "lbf_batch_create: batch too large for 32-bit slot indices");
     }
@@ -290,7 +296,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     rc |= balloc(b, &b->d_crops, b->crops.size() * sizeof(CropDev));
     rc |= balloc(b, &b->d_site, b->S * 4);
     rc |= balloc(b, &b->d_lab8, b->S);
-    rc |= balloc(b, &b->d_map, map.size() * 4);
+    rc |= balloc(b, &b->d_rank, rank.size() * sizeof(uint2));
     rc |= balloc(b, &b->d_lab, 256 * sizeof(LabelF));
     rc |= balloc(b, &b->d_ucur, size_t(b->D) * b->S * 4);
     rc |= balloc(b, &b->d_uprev, size_t(b->D) * b->S * 4);
@@ -308,7 +314,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     bool ok = up(b->d_tile_crop, b->tile_crop.data(), b->T * 4) == hipSuccess &&
               up(b->d_crops, b->crops.data(), b->crops.size() * sizeof(CropDev)) == hipSuccess &&
               up(b->d_site, site.data(), b->S * 4) == hipSuccess && up(b->d_lab8, lab8.data(), b->S) == hipSuccess &&
-              up(b->d_map, map.data(), map.size() * 4) == hipSuccess &&
+              up(b->d_rank, rank.data(), rank.size() * sizeof(uint2)) == hipSuccess &&
               up(b->d_lab, lab.data(), 256 * sizeof(LabelF)) == hipSuccess;
     DevStatus st{};
     for (auto& v : st.first) v = ~0ull;

@@ -11,7 +11,7 @@ constexpr uint32_t NO_SITE = 0xffffffffu;
 struct CropDev {
     GridDev g;            // crop grid (nx, ny, nz, fast divisions)
     uint32_t slot_base;   // first slot of the crop (multiple of TILE)
-    uint32_t map_base;    // offset of the crop's site -> slot map
+    uint32_t rank_base;   // first word of the crop's rank index
     uint32_t nfluid;
     uint32_t tile_begin, tile_end;
     uint32_t pad;
@@ -36,7 +36,7 @@ struct BatchBuffers {
     const CropDev* crops;
     const uint32_t* site;       // crop-local site of every slot, NO_SITE on padding
     const uint8_t* lab8;        // geometry label of every slot
-    const int32_t* map;         // crop-local site -> crop-local slot, -1 on solids
+    const uint2* rank;          // per crop, per 32 sites: {fluid sites before, fluid bit mask}
     const LabelF* lab;
     float* ucur;                // velocity of the last fields step [d][slot]
     float* uprev;               // velocity of the previous report

@@ -7,6 +7,7 @@
 // for the colour model, a wall index with per-wall rotation tables.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -48,7 +49,8 @@ struct lbf_ctx {
     size_t elem = 8;
     uint32_t* d_desc = nullptr;
     uint32_t* d_list = nullptr;
-    int32_t* d_map = nullptr;
+    uint2* d_rank = nullptr;         // sparse: per 32 sites {fluid sites before, fluid mask}
+    uint32_t* d_desc_s = nullptr;    // sparse: descriptor per slot
     DevLabel* d_lab = nullptr;
     LabelF* d_lab32 = nullptr;
     double *d_AT = nullptr, *d_A = nullptr, *d_B = nullptr, *d_MInv = nullptr;
@@ -311,7 +313,8 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     rc |= dalloc(c, &c->d_lab32, 256 * sizeof(LabelF));
     if (c->layout == 2) {
         rc |= dalloc(c, &c->d_list, c->nfluid * 4);
-        rc |= dalloc(c, &c->d_map, N * 4);
+        rc |= dalloc(c, &c->d_rank, (N + 31) / 32 * sizeof(uint2));
+        rc |= dalloc(c, &c->d_desc_s, c->nfluid * 4);
     }
     rc |= dalloc(c, &c->d_AT, Q2 * 8);
     rc |= dalloc(c, &c->d_A, Q2 * 8);
@@ -363,19 +366,30 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         rc |= upload(c, c->d_lab32, l32.data(), 256 * sizeof(LabelF));
     }
     if (c->layout == 2) {
-        std::vector<int32_t> map(N, -1);
-        for (uint64_t s = 0; s < c->nfluid; ++s) map[c->list[s]] = int32_t(s);
-        // bounce-back in the sparse layout is map < 0: exactly the BB/SD sites
-        for (uint64_t i = 0; i < N; ++i)
-            if (map[i] < 0) {
-                const int t = c->labels[geometry[i]].type;
-                if (t != T_BB && t != T_SD) {
-                    lbf_destroy(c);
-                    return fail(nullptr, "lbf_create: sparse layout needs every non-fluid label to be BB or SD");
+        // Every non-fluid site must bounce back (BB/SD), so a pull from it is a
+        // link of the descriptor and the rank index only ever sees fluid sites.
+        std::vector<uint2> rank((N + 31) / 32, uint2{0u, 0u});
+        uint32_t before = 0;
+        for (uint64_t w = 0; w < rank.size(); ++w) {
+            rank[w].x = before;
+            for (uint64_t i = w * 32; i < std::min<uint64_t>(N, w * 32 + 32); ++i) {
+                if (c->hlab[geometry[i]].kind & KIND_FLUID) {
+                    rank[w].y |= 1u << (i - w * 32);
+                    ++before;
+                } else {
+                    const int t = c->labels[geometry[i]].type;
+                    if (t != T_BB && t != T_SD) {
+                        lbf_destroy(c);
+                        return fail(nullptr, "lbf_create: sparse layout needs every non-fluid label to be BB or SD");
+                    }
                 }
             }
+        }
+        std::vector<uint32_t> desc_s(c->nfluid);
+        for (uint64_t k = 0; k < c->nfluid; ++k) desc_s[k] = c->links.desc[c->list[k]];
         rc |= upload(c, c->d_list, c->list.data(), c->nfluid * 4);
-        rc |= upload(c, c->d_map, map.data(), N * 4);
+        rc |= upload(c, c->d_rank, rank.data(), rank.size() * sizeof(uint2));
+        rc |= upload(c, c->d_desc_s, desc_s.data(), c->nfluid * 4);
     }
     if (NP == 2) {
         if (c->elem == 4) {
@@ -464,9 +478,9 @@ static ScspBuffers scsp_buffers(lbf_ctx* c, bool init)
     ScspBuffers b{};
     b.fin = c->f[c->cur];
     b.fout = init ? c->f[c->cur] : c->f[c->cur ^ 1];
-    b.desc = c->d_desc;
+    b.desc = c->layout == 2 ? c->d_desc_s : c->d_desc;
     b.list = c->d_list;
-    b.map = c->d_map;
+    b.rank = c->d_rank;
     b.lab = c->d_lab;
     b.lab32 = c->d_lab32;
     b.AT = c->d_AT;
@@ -796,9 +810,9 @@ double lbf_step_bytes(const lbf_ctx* c)
     if (c->NP == 2) return double(c->nfluid) * (pops + indicator) + double(c->N) * 4.0;
     if (c->layout == 0) return double(c->N) * (pops + 4.0);       // + descriptor
     if (c->layout == 1) return double(c->nfluid) * pops + double(c->N) * 4.0;
-    // sparse: per fluid site its populations, its list entry, its descriptor
-    // and its own map entry (read as a neighbour of other fluid sites)
-    return double(c->nfluid) * (pops + 12.0);
+    // sparse: per fluid site its populations, its list entry and its
+    // descriptor, plus one pass over the rank index (8 bytes per 32 sites)
+    return double(c->nfluid) * (pops + 8.0) + double((c->N + 31) / 32) * 8.0;
 }
 
 int lbf_timed_advance(lbf_ctx* c, uint64_t n, double* ms)

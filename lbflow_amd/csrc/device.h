@@ -146,9 +146,9 @@ struct ScspParams {
 struct ScspBuffers {
     const void* fin;
     void* fout;
-    const uint32_t* desc;
+    const uint32_t* desc;    // descriptor per site (SPARSE: per slot)
     const uint32_t* list;    // SPARSE: fluid site of each slot
-    const int32_t* map;      // SPARSE: slot of each site, -1 on solids
+    const uint2* rank;       // SPARSE: per 32 sites {fluid sites before them, fluid bit mask}
     const DevLabel* lab;
     const LabelF* lab32;
     const double* AT;        // A transposed (MInvSMT of scsp.cpp:425)

@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(TILE)
 batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fout,
            const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ntiles,
            const uint32_t* __restrict__ tile_crop, const CropDev* __restrict__ crops,
-           const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8, const int32_t* __restrict__ map,
+           const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8, const uint2* __restrict__ rank,
            const LabelF* __restrict__ lab, float* __restrict__ ucur, float* __restrict__ uadj,
            const float* __restrict__ upres, DevStatus* __restrict__ status)
 {
@@ -64,8 +64,14 @@ batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fou
                 } else {
                     ez = L::e[q][0]; ey = L::e[q][1]; ex = L::e[q][2];
                 }
-                const int32_t m = map[cr.map_base + zs[1 - ez] + ys[1 - ey] + xs[1 - ex]];
-                const O off = m < 0 ? O(L::op[q]) * O(p.S) + slot : O(q) * O(p.S) + (cr.slot_base + uint32_t(m));
+                // the upstream site's slot is its rank among the crop's fluid
+                // sites; a solid upstream site bounces back into op(q)
+                const uint32_t src = zs[1 - ez] + ys[1 - ey] + xs[1 - ex];
+                const uint2 r = rank[cr.rank_base + (src >> 5)];
+                const uint32_t bit = src & 31u;
+                const O off = ((r.y >> bit) & 1u)
+                                  ? O(q) * O(p.S) + (cr.slot_base + r.x + uint32_t(__popc(r.y & ((1u << bit) - 1u))))
+                                  : O(L::op[q]) * O(p.S) + slot;
                 h[q] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(fin) + off * O(4));
             }
         }
@@ -258,12 +264,12 @@ static void launch_batch_q(uint32_t grid, const BatchParams& p, const BatchBuffe
     if (wide)
         hipLaunchKernelGGL((batch_step<Q, MRT, INIT, true>), dim3(grid), dim3(TILE), 0, s, p,
                            static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
-                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.map, b.lab, b.ucur,
+                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.rank, b.lab, b.ucur,
                            b.uadj, b.upres, b.status);
     else
         hipLaunchKernelGGL((batch_step<Q, MRT, INIT, false>), dim3(grid), dim3(TILE), 0, s, p,
                            static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
-                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.map, b.lab, b.ucur,
+                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.rank, b.lab, b.ucur,
                            b.uadj, b.upres, b.status);
 }
 

@@ -19,9 +19,10 @@
 //                but never collided, exactly like the reference, so
 //                checkpoints are byte-identical);
 //   DENSE_FLUID  slot = site, solid sites skipped;
-//   SPARSE       slot = fluid index: populations of fluid sites only, a
-//                sorted fluid-site list and a dense site -> slot map (-1 on
-//                solids), so solid voxels cost neither HBM traffic nor lanes.
+//   SPARSE       slot = fluid index: populations and descriptors of fluid
+//                sites only, a sorted fluid-site list and a 32-site rank
+//                index that turns a neighbour site into its slot, so solid
+//                voxels cost neither HBM traffic nor lanes.
 //
 // FP64 kernels keep the reference's floating-point operation order (the file
 // is built with -ffp-contract=off) and are bit-identical to it.  Terms whose
@@ -108,18 +109,24 @@ __device__ __forceinline__ uint32_t upstream(const Stencil& s, int q)
 // Element offset (from the start of the population array) of the value that
 // streams into direction q of this slot: the upstream neighbour's f_q, or
 // this slot's own f_op(q) when the link bounces back.
+//
+// Sparse layout: the slot of a fluid site is its rank among the fluid sites
+// in z,y,x order, read from a 32-site rank index {fluid sites before the
+// word, fluid bit mask of the word}: 8 bytes per 32 sites (4 MB at 256^3),
+// small enough to stay in L2, instead of a 4-byte map entry per site.
 template <int Q, int LAYOUT, typename O>
-__device__ __forceinline__ O pull_offset(const int32_t* __restrict__ map, uint32_t links, uint32_t slot,
+__device__ __forceinline__ O pull_offset(const uint2* __restrict__ rank, uint32_t links, uint32_t slot,
                                          const Stencil& st, uint32_t P, int q)
 {
     using L = Lattice<Q>;
+    // site x - e_q solid <=> site in ibS[op(q)] (simulator.cpp:103-111)
+    if ((links >> L::op[q]) & 1u) return O(L::op[q]) * P + slot;
     const uint32_t src = upstream<Q>(st, q);
     if constexpr (LAYOUT == SPARSE) {
-        const int32_t m = map[src];
-        return m < 0 ? O(L::op[q]) * P + slot : O(q) * P + uint32_t(m);
+        const uint2 r = rank[src >> 5];
+        return O(q) * P + (r.x + uint32_t(__popc(r.y & ((1u << (src & 31u)) - 1u))));
     } else {
-        // site x - e_q solid <=> site in ibS[op(q)] (simulator.cpp:103-111)
-        return ((links >> L::op[q]) & 1u) ? O(L::op[q]) * P + slot : O(q) * P + src;
+        return O(q) * P + src;
     }
 }
 
@@ -143,7 +150,7 @@ template <int Q, bool MRT, bool INIT, int LAYOUT, bool WIDE>
 __global__ void __launch_bounds__(256)
 scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__ fout,
               const uint32_t* __restrict__ desc, const uint32_t* __restrict__ list,
-              const int32_t* __restrict__ map, const DevLabel* __restrict__ lab,
+              const uint2* __restrict__ rank, const DevLabel* __restrict__ lab,
               const double* __restrict__ AT, const double* __restrict__ B, double* __restrict__ rho_out,
               double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status)
 {
@@ -164,7 +171,7 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
     uint32_t i, slot;
     if (!resolve<LAYOUT>(p, list, i, slot)) return;
     const uint32_t P = LAYOUT == SPARSE ? p.nfluid : uint32_t(p.g.N);   // plane length
-    const uint32_t dsc = desc[i];
+    const uint32_t dsc = desc[LAYOUT == SPARSE ? slot : i];
     const DevLabel* lb = lab + (dsc >> DESC_LABEL_SHIFT);
     const int kind = lb->kind;
     double f[Q];
@@ -204,7 +211,7 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
     } else {
         f[0] = ld(fin, O(slot));
 #pragma unroll
-        for (int q = 1; q < Q; ++q) f[q] = ld(fin, pull_offset<Q, LAYOUT, O>(map, dsc, slot, stn, P, q));
+        for (int q = 1; q < Q; ++q) f[q] = ld(fin, pull_offset<Q, LAYOUT, O>(rank, dsc, slot, stn, P, q));
     }
 
     // calcMacroVar: rho = sum_q f_q (q ascending); u = (sum +-f) / rho0
@@ -323,7 +330,7 @@ template <int Q, bool MRT, bool INIT, int LAYOUT, bool WIDE>
 __global__ void __launch_bounds__(256)
 scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ fout,
               const uint32_t* __restrict__ desc, const uint32_t* __restrict__ list,
-              const int32_t* __restrict__ map, const LabelF* __restrict__ lab, double* __restrict__ rho_out,
+              const uint2* __restrict__ rank, const LabelF* __restrict__ lab, double* __restrict__ rho_out,
               double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status)
 {
 #pragma clang fp contract(fast)
@@ -333,7 +340,7 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
     uint32_t i, slot;
     if (!resolve<LAYOUT>(p, list, i, slot)) return;
     const uint32_t P = LAYOUT == SPARSE ? p.nfluid : uint32_t(p.g.N);
-    const uint32_t dsc = desc[i];
+    const uint32_t dsc = desc[LAYOUT == SPARSE ? slot : i];
     const LabelF lb = lab[dsc >> DESC_LABEL_SHIFT];
     float h[Q];
     Stencil stn{};
@@ -358,7 +365,7 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
     } else {
         h[0] = ld(fin, O(slot));
 #pragma unroll
-        for (int q = 1; q < Q; ++q) h[q] = ld(fin, pull_offset<Q, LAYOUT, O>(map, dsc, slot, stn, P, q));
+        for (int q = 1; q < Q; ++q) h[q] = ld(fin, pull_offset<Q, LAYOUT, O>(rank, dsc, slot, stn, P, q));
     }
     // macro on shifted populations: rho = rho0 + sum h, j = sum e h
     float u[D];
@@ -395,12 +402,12 @@ static void launch_scsp_w(const ScspParams& p, const ScspBuffers& b, bool fp32, 
     if (n == 0) return;
     if (fp32)
         hipLaunchKernelGGL((scsp_step_f32<Q, MRT, INIT, LAYOUT, WIDE>), grid, block, 0, s, p,
-                           static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.desc, b.list, b.map,
+                           static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.desc, b.list, b.rank,
                            b.lab32, b.rho, b.u, b.uadj, b.status);
     else
         hipLaunchKernelGGL((scsp_step_f64<Q, MRT, INIT, LAYOUT, WIDE>), grid, block, 0, s, p,
                            static_cast<const double*>(b.fin), static_cast<double*>(b.fout), b.desc, b.list,
-                           b.map, b.lab, b.AT, b.B, b.rho, b.u, b.uadj, b.status);
+                           b.rank, b.lab, b.AT, b.B, b.rho, b.u, b.uadj, b.status);
 }
 
 template <int Q, bool MRT, bool INIT, int LAYOUT>
