@@ -256,11 +256,11 @@ int lbf_batch_crop_fields(lbf_batch* batch, uint32_t c, float* fields);
  * step returns u; the FP32 counterpart of SCSP::initialize (scsp.cpp:116-134)
  * with a velocity.  Call after lbf_batch_init and before lbf_batch_run. */
 int lbf_batch_set_fields(lbf_batch* batch, uint32_t c, const float* fields);
-/* `steps` consistent-initialisation sweeps after lbf_batch_set_fields
- * (Mei, Luo, Lallemand & d'Humieres, Computers & Fluids 35 (2006) 855):
- * stream and collide towards the equilibrium of the current density and the
- * prescribed velocity, so that density and non-equilibrium populations
- * settle to the given flow before lbf_batch_run.  The sweeps count as steps. */
+/* `steps` consistent-initialisation sweeps after lbf_batch_set_fields (after
+ * Mei, Luo, Lallemand & d'Humieres, Computers & Fluids 35 (2006) 855):
+ * stream, give the streamed populations the momentum of the prescribed
+ * velocity, collide; density and non-equilibrium populations settle to the
+ * given flow before lbf_batch_run.  The sweeps count as steps. */
 int lbf_batch_relax(lbf_batch* batch, uint32_t steps);
 
 /* --- learned warm-start network (no reference counterpart) -------------

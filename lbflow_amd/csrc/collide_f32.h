@@ -66,9 +66,13 @@ struct RelaxF32 {
 };
 
 // Macro values of shifted populations: drho = sum h, u = (sum e h)/rho0 + F/(2 rho0).
+// Every product-sum below is an explicit fmaf and nothing else is contracted,
+// so the rounding is fixed by this source alone: kernels that inline these
+// functions (single domain, batched crops) produce identical bits.
 template <int Q>
 __device__ __forceinline__ float macro_f32(const float* h, float inv_rho0, const float* half_rho, float* u)
 {
+#pragma clang fp contract(off)
     using L = Lattice<Q>;
     float drho = 0.0f;
 #pragma unroll
@@ -81,7 +85,7 @@ __device__ __forceinline__ float macro_f32(const float* h, float inv_rho0, const
             if (L::e[q][d] == 1) s += h[q];
             else if (L::e[q][d] == -1) s -= h[q];
         }
-        u[d] = s * inv_rho0 + half_rho[d];
+        u[d] = fmaf(s, inv_rho0, half_rho[d]);
     }
     return drho;
 }
@@ -89,7 +93,7 @@ __device__ __forceinline__ float macro_f32(const float* h, float inv_rho0, const
 template <int Q, bool MRT>
 __device__ __forceinline__ void collide_f32(float* h, float drho, const float* u, const float* F, const RelaxF32& r)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
     using L = Lattice<Q>;
     constexpr int D = L::D;
     constexpr int H = L::NHALF;
@@ -97,8 +101,8 @@ __device__ __forceinline__ void collide_f32(float* h, float drho, const float* u
     float usq = 0.0f, uF = 0.0f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        usq += u[d] * u[d];
-        uF += u[d] * F[d];
+        usq = fmaf(u[d], u[d], usq);
+        uF = fmaf(u[d], F[d], uF);
     }
     const float rho0 = r.rho0;
     if constexpr (!MRT) {
@@ -112,9 +116,10 @@ __device__ __forceinline__ void collide_f32(float* h, float drho, const float* u
                 else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
             }
             const float w = float(L::w[q]);
-            const float heq = w * (drho + rho0 * (eu * (3.0f + 4.5f * eu) - 1.5f * usq));
-            const float G = w * (3.0f * (ef - uF) + 9.0f * eu * ef);
-            h[q] += r.omega * (heq - h[q]) + r.omh * G;
+            const float quad = fmaf(eu, fmaf(4.5f, eu, 3.0f), -1.5f * usq);
+            const float heq = w * fmaf(rho0, quad, drho);
+            const float G = w * fmaf(9.0f * eu, ef, 3.0f * (ef - uF));
+            h[q] = fmaf(r.omega, heq - h[q], fmaf(r.omh, G, h[q]));
         }
     } else {
         float sp[H], sm[H];
@@ -131,48 +136,48 @@ __device__ __forceinline__ void collide_f32(float* h, float drho, const float* u
             float mg = 0.0f;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                if (MT.T1[k][a] != 0.0) mg += float(3.0 * MT.T1[k][a]) * F[a];
+                if (MT.T1[k][a] != 0.0) mg = fmaf(float(3.0 * MT.T1[k][a]), F[a], mg);
 #pragma unroll
                 for (int b = 0; b < D; ++b)
-                    if (MT.T2[k][a][b] != 0.0) mg += float(9.0 * MT.T2[k][a][b]) * u[a] * F[b];
+                    if (MT.T2[k][a][b] != 0.0) mg = fmaf(float(9.0 * MT.T2[k][a][b]) * u[a], F[b], mg);
             }
-            if (MT.T0[k] != 0.0) mg -= float(3.0 * MT.T0[k]) * uF;
+            if (MT.T0[k] != 0.0) mg = fmaf(float(-3.0 * MT.T0[k]), uF, mg);
             if (MT.rate[k] == 0) {
                 y[k] = mg * float(1.0 / MT.n2[k]);
                 continue;
             }
             float m = 0.0f;
             if (MT.parity[k] > 0) {
-                if (L::M[k][0] != 0) m += float(L::M[k][0]) * h[0];
+                if (L::M[k][0] != 0) m = fmaf(float(L::M[k][0]), h[0], m);
 #pragma unroll
                 for (int j = 0; j < H; ++j)
-                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sp[j];
+                    if (L::M[k][L::half[j]] != 0) m = fmaf(float(L::M[k][L::half[j]]), sp[j], m);
             } else {
 #pragma unroll
                 for (int j = 0; j < H; ++j)
-                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sm[j];
+                    if (L::M[k][L::half[j]] != 0) m = fmaf(float(L::M[k][L::half[j]]), sm[j], m);
             }
             // shifted equilibrium moment: drho T0 + rho0 (3 T1.u + 4.5 u.T2.u - 1.5 u^2 T0)
             float meq = 0.0f;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                if (MT.T1[k][a] != 0.0) meq += float(3.0 * MT.T1[k][a]) * u[a];
+                if (MT.T1[k][a] != 0.0) meq = fmaf(float(3.0 * MT.T1[k][a]), u[a], meq);
 #pragma unroll
                 for (int b = 0; b < D; ++b)
-                    if (MT.T2[k][a][b] != 0.0) meq += float(4.5 * MT.T2[k][a][b]) * u[a] * u[b];
+                    if (MT.T2[k][a][b] != 0.0) meq = fmaf(float(4.5 * MT.T2[k][a][b]) * u[a], u[b], meq);
             }
-            if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
+            if (MT.T0[k] != 0.0) meq = fmaf(float(-1.5 * MT.T0[k]), usq, meq);
             meq *= rho0;
-            if (MT.T0[k] != 0.0) meq += float(MT.T0[k]) * drho;
+            if (MT.T0[k] != 0.0) meq = fmaf(float(MT.T0[k]), drho, meq);
             const float c1 = MT.rate[k] == 1 ? r.c1_nu : r.c1_q;
             const float c2 = MT.rate[k] == 1 ? r.c2_nu : r.c2_q;
-            y[k] = (c1 * (meq - m) + c2 * mg) * float(1.0 / MT.n2[k]);
+            y[k] = fmaf(c1, meq - m, c2 * mg) * float(1.0 / MT.n2[k]);
         }
         {
             float e0 = 0.0f;
 #pragma unroll
             for (int k = 0; k < Q; ++k)
-                if (MT.parity[k] > 0 && L::M[k][0] != 0) e0 += float(L::M[k][0]) * y[k];
+                if (MT.parity[k] > 0 && L::M[k][0] != 0) e0 = fmaf(float(L::M[k][0]), y[k], e0);
             h[0] += e0;
         }
 #pragma unroll
@@ -182,8 +187,8 @@ __device__ __forceinline__ void collide_f32(float* h, float drho, const float* u
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 if (L::M[k][q] == 0) continue;
-                if (MT.parity[k] > 0) ev += float(L::M[k][q]) * y[k];
-                else od += float(L::M[k][q]) * y[k];
+                if (MT.parity[k] > 0) ev = fmaf(float(L::M[k][q]), y[k], ev);
+                else od = fmaf(float(L::M[k][q]), y[k], od);
             }
             h[q] += ev + od;
             h[L::op[q]] += ev - od;

@@ -79,12 +79,28 @@ batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fou
         const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
         if (!isfinite(drho)) raise_status(status, 0, p.t);
         if (p.flags & 4u) {
-            // consistent initialisation (Mei, Luo, Lallemand & d'Humieres, Computers &
-            // Fluids 35 (2006) 855-862): collide towards the equilibrium of the current
-            // density and the prescribed velocity, so density and non-equilibrium parts
-            // relax to the given flow field before the run starts
+            // Consistent initialisation in the spirit of Mei, Luo, Lallemand &
+            // d'Humieres (Computers & Fluids 35 (2006) 855-862): the streamed
+            // populations get the momentum of the prescribed velocity (added as
+            // 3 w_q e_q . dj, which carries momentum dj and no mass) before the
+            // ordinary collision, so the non-equilibrium parts and the density
+            // settle to the given flow.  With MRT the momentum moments do not
+            // relax, so the velocity is imposed on the populations themselves
+            // rather than only through the equilibrium.
+            float dj[D];
 #pragma unroll
-            for (int d = 0; d < D; ++d) u[d] = upres[uint64_t(d) * p.S + slot];
+            for (int d = 0; d < D; ++d) {
+                const float up = upres[uint64_t(d) * p.S + slot];
+                dj[d] = p.relax.rho0 * (up - u[d]);
+                u[d] = up;
+            }
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                float ej = 0.0f;
+#pragma unroll
+                for (int d = 0; d < D; ++d) ej += float(L::e[q][d]) * dj[d];
+                h[q] += float(3.0 * L::w[q]) * ej;
+            }
         }
         if (p.flags & 1u) {
 #pragma unroll

@@ -333,7 +333,6 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
               const uint2* __restrict__ rank, const LabelF* __restrict__ lab, double* __restrict__ rho_out,
               double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status)
 {
-#pragma clang fp contract(fast)
     using L = Lattice<Q>;
     constexpr int D = L::D;
     using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;

@@ -214,13 +214,14 @@ class Scaling:
         rms = [float(torch.sqrt((raw[..., c][m] ** 2).mean())) for c in range(4)]
         return cls(rms, tau, force)
 
-    def normalise(self, fields):
+    def normalise(self, fields, masks):
+        """Training targets; solid sites (no flow, no density) are 0."""
         out = fields.clone()
         out[..., :3] *= self.nu / self.force
         out[..., 3] = (out[..., 3] - 1.0) / self.force
         for c in range(4):
             out[..., c] /= self.rms[c]
-        return out
+        return out * masks.reshape(masks.shape[0], -1, 1).to(out.dtype)
 
     def fields(self, pred, tau, force, mask):
         """Network output -> physical (u, rho) for a run with (tau, force)."""
@@ -324,7 +325,7 @@ def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=Non
 
 def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, tau=0.8, force=1e-6,
              train_force=1e-5, train_tol=1e-5, eval_tol=1e-3, interval=50, max_steps=60000, out=None, log=print,
-             relax_sweeps=(0, 50)):
+             relax_sweeps=(0, 5)):
     """Generate, solve, train, evaluate.  Returns a report dict."""
     import torch
     rep = dict(side=side, n_train=n_train, n_eval=n_eval, epochs=epochs, capacity=capacity, lr=lr, tau=tau,
@@ -345,7 +346,7 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
     masks = torch.as_tensor(np.stack([(c == PORE) for c in train_crops]).astype(np.uint8), device='cuda')
     del train_crops
     scaling = Scaling.fit(fields, masks, tau, train_force)
-    targets = scaling.normalise(fields).contiguous()
+    targets = scaling.normalise(fields, masks).contiguous()
     del fields
     torch.cuda.synchronize()
     net = Net(side=side, capacity=capacity, seed=1)

@@ -150,20 +150,25 @@ def test_network_gradients_match_reference():
     w4 = (1.0, 0.5, 2.0, 1.0)
     loss, g = net.gradients(torch.as_tensor(mask, device='cuda'), torch.as_tensor(target, device='cuda'), w4)
     lref, gref = R.gradients(mask, target, p.astype(np.float64), w4, round_bf16=True)
+    _, gexact = R.gradients(mask, target, p.astype(np.float64), w4, round_bf16=False)
     say('loss', loss, lref)
     assert abs(loss - lref) <= 2e-2 * lref
     offs = np.cumsum([0] + [co * (t * ci + 1) for ci, co, t in R.LAYERS])
-    stats = []
+
+    def compare(a, b):
+        return np.linalg.norm(a - b) / np.linalg.norm(b), a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+
     for i in range(len(R.LAYERS)):
-        a, b = g[offs[i]:offs[i + 1]].astype(np.float64), gref[offs[i]:offs[i + 1]]
-        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
-        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
-        say(f'layer {i}: rel {rel:.3e} cos {cos:.6f}')
-        stats.append((rel, cos))
-    # BF16 storage of activations and gradients on both sides; what remains is FP32 vs
-    # FP64 accumulation moving values across BF16 rounding boundaries
-    for i, (rel, cos) in enumerate(stats):
-        assert rel < 5e-2 and cos > 0.998, i
+        sl = slice(offs[i], offs[i + 1])
+        rel, cos = compare(g[sl].astype(np.float64), gref[sl])
+        bf, _ = compare(gref[sl], gexact[sl])   # what BF16 rounding alone does to this layer
+        say(f'layer {i}: device vs reference rel {rel:.3e} cos {cos:.6f}; BF16 effect {bf:.3e}')
+        # The device and the reference both keep activations and gradients in BF16;
+        # their FP32 / FP64 accumulations round differently, so the two BF16
+        # trajectories differ by about the BF16 effect itself (independent
+        # rounding: up to ~sqrt(2) times it).  A wrong index, sign or operand gives
+        # O(1) errors and a cosine far from 1.
+        assert rel < max(3.0 * bf, 1e-2) and cos > 0.995, i
 
 
 def test_training_reduces_loss():
@@ -185,9 +190,10 @@ def test_training_reduces_loss():
 
 
 def test_field_export_and_warm_start_roundtrip():
-    """Warm-starting from a converged crop's own fields: the equilibrium start
-    plus consistent-initialisation sweeps reaches the stop rule in fewer steps
-    than the cold start, with the same permeability."""
+    """Warm-starting from a converged crop's own fields.  The bare equilibrium
+    start lacks the non-equilibrium populations and converges slower than the
+    cold start; a few consistent-initialisation sweeps restore them, after which
+    the run stops after the minimum number of reports, with the same K."""
     crops = [G.sphere_pack((16, 16, 16), 0.4, 1.5, 3.0, 200 + i) for i in range(3)]
     force, tau, tol, interval = 1e-5, 0.8, 1e-5, 40
     fields, res = S.solve(crops, tau=tau, force=force, tol=tol, interval=interval, max_steps=20000)
@@ -195,7 +201,7 @@ def test_field_export_and_warm_start_roundtrip():
     from lbflow_amd.engine import Batch
     L = S.lib()
     out = {}
-    for relax in (0, 20, 60):
+    for relax in (0, 5, 20):
         b = Batch(3, 19, 'MRT', tau, 1.0, S.labels(force), crops).init()
         torch.cuda.synchronize()
         for c in range(len(crops)):
@@ -209,5 +215,6 @@ def test_field_export_and_warm_start_roundtrip():
         for v in out.values():
             assert v[c]['converged']
             assert abs(v[c]['K'] - r0['K']) <= 2e-5 * r0['K']
-    best = min(out, key=lambda k: sum(v['steps'] for v in out[k]))
-    assert sum(v['steps'] for v in out[best]) < sum(r['steps'] for r in res)
+        # minimum: first report + 3 stable reports; the sweeps shift the report grid
+        assert out[5][c]['steps'] <= 5 + 5 * interval
+        assert out[5][c]['steps'] < r0['steps']
