@@ -254,7 +254,10 @@ int lbf_batch_crop_fields(lbf_batch* batch, uint32_t c, float* fields);
 /* Warm start: set crop c to the equilibrium of the given fields (same layout,
  * DEVICE memory), shifted by half the body force so that the first macro
  * step returns u; the FP32 counterpart of SCSP::initialize (scsp.cpp:116-134)
- * with a velocity.  Call after lbf_batch_init and before lbf_batch_run. */
+ * with a velocity.  Fluid sites whose cluster does not wrap around the box
+ * along the force (no steady flow: isolated pockets, dead clusters) keep the
+ * rest state of lbf_batch_init.  Call after lbf_batch_init and before
+ * lbf_batch_run. */
 int lbf_batch_set_fields(lbf_batch* batch, uint32_t c, const float* fields);
 /* `steps` consistent-initialisation sweeps after lbf_batch_set_fields (after
  * Mei, Luo, Lallemand & d'Humieres, Computers & Fluids 35 (2006) 855):

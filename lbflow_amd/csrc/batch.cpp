@@ -27,8 +27,8 @@ hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, const Ba
 hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begin, uint32_t ntiles,
                                const uint32_t* site, const float* f, const float* ucur, float* fields, hipStream_t s);
 hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t tile_begin, uint32_t ntiles,
-                             const uint32_t* site, const uint8_t* lab8, const LabelF* lab, const float* fields,
-                             float* f, float* upres, hipStream_t s);
+                             const uint32_t* site, const uint8_t* lab8, const uint8_t* flow8, const LabelF* lab,
+                             const float* fields, float* f, float* upres, hipStream_t s);
 hipError_t launch_batch_report(uint32_t grid, uint64_t S, int D, int axis, const BatchBuffers& b, uint32_t ncrops,
                                double* partial, double* out, hipStream_t s);
 }  // namespace lbf
@@ -53,6 +53,7 @@ struct lbf_batch {
     CropDev* d_crops = nullptr;
     uint32_t* d_site = nullptr;
     uint8_t* d_lab8 = nullptr;
+    uint8_t* d_flow8 = nullptr;      // 1 on fluid sites of clusters that carry the steady flow
     uint2* d_rank = nullptr;
     LabelF* d_lab = nullptr;
     float *d_ucur = nullptr, *d_uprev = nullptr, *d_uadj = nullptr, *d_upres = nullptr;
@@ -153,6 +154,95 @@ int step(lbf_batch* b, uint32_t flags, bool init)
     return 0;
 }
 
+// Fluid sites that carry the steady flow: those whose cluster (fluid sites
+// joined by lattice links, periodic) wraps around the periodic box along an
+// axis the body force has a component on.  In every other cluster the steady
+// velocity is exactly 0 (a uniform force there is balanced by pressure), and
+// a warm start must not put flow into them: a pocket whose links all bounce
+// back would keep a prescribed momentum forever, flipping its sign each step.
+// Union-find with each site's winding (box crossings) relative to its root;
+// a link that closes a cycle with a non-zero net winding marks a wrap.
+std::vector<uint8_t> flowing_sites(const uint8_t* geom, const uint64_t* sz, int D, int Q, const LabelF* lab,
+                                   const double* force)
+{
+    const uint32_t nx = uint32_t(sz[D - 1]), ny = uint32_t(sz[D - 2]), nz = D == 3 ? uint32_t(sz[0]) : 1u;
+    const uint64_t N = uint64_t(nx) * ny * nz;
+    std::vector<int32_t> parent(N);
+    std::vector<int16_t> off(N * 3, 0);        // winding of a site relative to its parent (z, y, x)
+    std::vector<uint8_t> wrap(N, 0);           // at roots: bit a = the cluster wraps along axis a
+    for (uint64_t i = 0; i < N; ++i) parent[i] = int32_t(i);
+    auto fluid = [&](uint64_t i) { return (lab[geom[i]].kind & KIND_FLUID) != 0; };
+    // root of i and the winding of i relative to it (path compression keeps windings)
+    auto find = [&](uint64_t i, int* w) {
+        w[0] = w[1] = w[2] = 0;
+        uint64_t r = i;
+        while (uint64_t(parent[r]) != r) {
+            for (int a = 0; a < 3; ++a) w[a] += off[r * 3 + a];
+            r = uint64_t(parent[r]);
+        }
+        // compress: point every node on the path at r with its total winding
+        int acc[3] = {w[0], w[1], w[2]};
+        uint64_t c = i;
+        while (uint64_t(parent[c]) != r && c != r) {
+            const uint64_t nxt = uint64_t(parent[c]);
+            int here[3];
+            for (int a = 0; a < 3; ++a) here[a] = off[c * 3 + a];
+            for (int a = 0; a < 3; ++a) off[c * 3 + a] = int16_t(acc[a]);
+            parent[c] = int32_t(r);
+            for (int a = 0; a < 3; ++a) acc[a] -= here[a];
+            c = nxt;
+        }
+        return r;
+    };
+    const int H = Q == 9 ? Lattice<9>::NHALF : Lattice<19>::NHALF;
+    const int* half = Q == 9 ? Lattice<9>::half : Lattice<19>::half;
+    for (uint32_t z = 0; z < nz; ++z)
+        for (uint32_t y = 0; y < ny; ++y)
+            for (uint32_t x = 0; x < nx; ++x) {
+                const uint64_t i = (uint64_t(z) * ny + y) * nx + x;
+                if (!fluid(i)) continue;
+                for (int j = 0; j < H; ++j) {
+                    const int q = half[j];
+                    int e[3] = {0, 0, 0};   // (z, y, x)
+                    for (int a = 0; a < D; ++a)
+                        e[3 - D + a] = Q == 9 ? Lattice<9>::e[q][a] : Lattice<19>::e[q][a];
+                    const int64_t c[3] = {int64_t(z) + e[0], int64_t(y) + e[1], int64_t(x) + e[2]};
+                    const int64_t n[3] = {nz, ny, nx};
+                    int cross[3];
+                    int64_t t[3];
+                    for (int a = 0; a < 3; ++a) {
+                        cross[a] = c[a] < 0 ? -1 : (c[a] >= n[a] ? 1 : 0);
+                        t[a] = c[a] - cross[a] * n[a];
+                    }
+                    const uint64_t k = (uint64_t(t[0]) * ny + uint64_t(t[1])) * nx + uint64_t(t[2]);
+                    if (!fluid(k)) continue;
+                    int wi[3], wk[3];
+                    const uint64_t ri = find(i, wi), rk = find(k, wk);
+                    // image of k reached from i lies `cross` boxes away
+                    int d[3];
+                    for (int a = 0; a < 3; ++a) d[a] = wi[a] + cross[a] - wk[a];
+                    if (ri == rk) {
+                        for (int a = 0; a < 3; ++a)
+                            if (d[a] != 0) wrap[ri] |= uint8_t(1u << a);
+                    } else {
+                        parent[rk] = int32_t(ri);
+                        for (int a = 0; a < 3; ++a) off[rk * 3 + a] = int16_t(d[a]);
+                        wrap[ri] |= wrap[rk];
+                    }
+                }
+            }
+    uint8_t need = 0;
+    for (int a = 0; a < D; ++a)
+        if (force[a] != 0.0) need |= uint8_t(1u << (3 - D + a));
+    std::vector<uint8_t> flow(N, 0);
+    for (uint64_t i = 0; i < N; ++i) {
+        if (!fluid(i)) continue;
+        int w[3];
+        flow[i] = (wrap[find(i, w)] & need) ? 1 : 0;
+    }
+    return flow;
+}
+
 }  // namespace
 
 extern "C" {
@@ -205,7 +295,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     }
     // slot space
     std::vector<uint32_t> site;
-    std::vector<uint8_t> lab8;
+    std::vector<uint8_t> lab8, flow8;
     std::vector<uint2> rank;
     b->crops.resize(d.ncrops);
     b->crop_sites.resize(d.ncrops);
@@ -244,6 +334,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
                 ++nf;
                 site.push_back(uint32_t(i));
                 lab8.push_back(v);
+                flow8.push_back(0);
                 if (!have_force && (lab[v].kind & KIND_GV)) {
                     for (int k = 0; k < d.D; ++k) b->crop_force[size_t(c) * 3 + k] = force64[size_t(v) * 3 + k];
                     have_force = true;
@@ -253,6 +344,11 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
                 return bfail(nullptr, "lbf_batch_create: batched crops support VD/GV fluid and BB/SD solids only");
             }
         }
+        {
+            const std::vector<uint8_t> flow = flowing_sites(cr.geometry, cr.sz, d.D, d.Q, lab.data(),
+                                                            &b->crop_force[size_t(c) * 3]);
+            for (uint64_t k = cd.slot_base; k < site.size(); ++k) flow8[k] = flow[site[k]];
+        }
         uint32_t before = 0;
         for (size_t w = cd.rank_base; w < rank.size(); ++w) {
             rank[w].x = before;
@@ -261,6 +357,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
         while (site.size() % TILE) {
             site.push_back(NO_SITE);
             lab8.push_back(0);
+            flow8.push_back(0);
         }
         cd.nfluid = nf;
         cd.tile_end = uint32_t(site.size() / TILE);
@@ -296,6 +393,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     rc |= balloc(b, &b->d_crops, b->crops.size() * sizeof(CropDev));
     rc |= balloc(b, &b->d_site, b->S * 4);
     rc |= balloc(b, &b->d_lab8, b->S);
+    rc |= balloc(b, &b->d_flow8, b->S);
     rc |= balloc(b, &b->d_rank, rank.size() * sizeof(uint2));
     rc |= balloc(b, &b->d_lab, 256 * sizeof(LabelF));
     rc |= balloc(b, &b->d_ucur, size_t(b->D) * b->S * 4);
@@ -314,6 +412,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     bool ok = up(b->d_tile_crop, b->tile_crop.data(), b->T * 4) == hipSuccess &&
               up(b->d_crops, b->crops.data(), b->crops.size() * sizeof(CropDev)) == hipSuccess &&
               up(b->d_site, site.data(), b->S * 4) == hipSuccess && up(b->d_lab8, lab8.data(), b->S) == hipSuccess &&
+              up(b->d_flow8, flow8.data(), b->S) == hipSuccess &&
               up(b->d_rank, rank.data(), rank.size() * sizeof(uint2)) == hipSuccess &&
               up(b->d_lab, lab.data(), 256 * sizeof(LabelF)) == hipSuccess;
     DevStatus st{};
@@ -499,7 +598,7 @@ int lbf_batch_set_fields(lbf_batch* b, uint32_t c, const float* fields)
     const CropDev& cd = b->crops[c];
     const BatchParams p = params(b, 0u);
     BHIP(b, launch_batch_warm(b->Q, b->d.equil == LBF_MRT, p, cd.tile_begin, cd.tile_end - cd.tile_begin, b->d_site,
-                              b->d_lab8, b->d_lab, fields, static_cast<float*>(b->f[b->cur]), b->d_upres,
+                              b->d_lab8, b->d_flow8, b->d_lab, fields, static_cast<float*>(b->f[b->cur]), b->d_upres,
                               b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     return 0;

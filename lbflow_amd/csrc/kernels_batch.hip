@@ -205,12 +205,14 @@ batch_fields(uint64_t S, float rho0, uint32_t tile_begin, const uint32_t* __rest
 
 // Warm start of one crop: the t = 0 state from given fields, i.e. the
 // equilibrium of (rho, u - F/(2 rho0)) followed by the t = 0 macro step and
-// collision, as the initial branch does for the rest state.
+// collision, as the initial branch does for the rest state.  Sites outside the
+// clusters that carry the steady flow (flow8 = 0; their steady velocity is 0)
+// start from the rest state instead.
 template <int Q, bool MRT>
 __global__ void __launch_bounds__(TILE)
 batch_warm(BatchParams p, uint32_t tile_begin, const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8,
-           const LabelF* __restrict__ lab, const float* __restrict__ fields, float* __restrict__ f,
-           float* __restrict__ upres)
+           const uint8_t* __restrict__ flow8, const LabelF* __restrict__ lab, const float* __restrict__ fields,
+           float* __restrict__ f, float* __restrict__ upres)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
@@ -221,15 +223,17 @@ batch_warm(BatchParams p, uint32_t tile_begin, const uint32_t* __restrict__ site
     const float F[3] = {lb.force[0], lb.force[1], D == 3 ? lb.force[2] : 0.0f};
     float ue[D];
     float usq = 0.0f;
+    // sites off the flowing clusters start at rest, as in the cold start
+    const bool flows = flow8[slot] != 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        const float ud = fields[uint64_t(s) * (D + 1) + d];
+        const float ud = flows ? fields[uint64_t(s) * (D + 1) + d] : 0.0f;
         upres[uint64_t(d) * p.S + slot] = ud;
         ue[d] = ud - lb.half_rho[d];
         usq += ue[d] * ue[d];
     }
     const float rho0 = p.relax.rho0;
-    const float dr = fields[uint64_t(s) * (D + 1) + D] - rho0;
+    const float dr = flows ? fields[uint64_t(s) * (D + 1) + D] - rho0 : 0.0f;
     float h[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -259,17 +263,17 @@ hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begi
 }
 
 hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t tile_begin, uint32_t ntiles,
-                             const uint32_t* site, const uint8_t* lab8, const LabelF* lab, const float* fields,
-                             float* f, float* upres, hipStream_t s)
+                             const uint32_t* site, const uint8_t* lab8, const uint8_t* flow8, const LabelF* lab,
+                             const float* fields, float* f, float* upres, hipStream_t s)
 {
     if (ntiles == 0) return hipSuccess;
     const dim3 g(ntiles), b(TILE);
     if (Q == 19) {
-        if (mrt) hipLaunchKernelGGL((batch_warm<19, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
-        else hipLaunchKernelGGL((batch_warm<19, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
+        if (mrt) hipLaunchKernelGGL((batch_warm<19, true>), g, b, 0, s, p, tile_begin, site, lab8, flow8, lab, fields, f, upres);
+        else hipLaunchKernelGGL((batch_warm<19, false>), g, b, 0, s, p, tile_begin, site, lab8, flow8, lab, fields, f, upres);
     } else {
-        if (mrt) hipLaunchKernelGGL((batch_warm<9, true>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
-        else hipLaunchKernelGGL((batch_warm<9, false>), g, b, 0, s, p, tile_begin, site, lab8, lab, fields, f, upres);
+        if (mrt) hipLaunchKernelGGL((batch_warm<9, true>), g, b, 0, s, p, tile_begin, site, lab8, flow8, lab, fields, f, upres);
+        else hipLaunchKernelGGL((batch_warm<9, false>), g, b, 0, s, p, tile_begin, site, lab8, flow8, lab, fields, f, upres);
     }
     return hipGetLastError();
 }

@@ -1,0 +1,43 @@
+"""What the trained warm-start network gains over the cold start.
+
+Uses the network trained on the GPU by `python -m lbflow_amd.surrogate`
+(4096 solver-generated 64^3 sphere-pack crops, seeds 10000+i), stored in
+lbflow_amd/weights/warm64.npz.  On unseen crops (seeds 20000+i) the same
+batched solver runs to the same stop rule (Simulator::updateConvergence,
+simulator.cpp:248-325, tolerance 1e-3) from rest and from the network's
+prediction; the warm runs must need fewer iterations and give the same
+permeability within the tolerance.
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from lbflow_amd import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = Path(S.__file__).resolve().parent / 'weights' / 'warm64.npz'
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason='no trained weights in lbflow_amd/weights')
+def test_warm_start_beats_cold_start():
+    net, scaling = S.load(WEIGHTS, capacity=4)
+    crops = S.make_crops(8, 20000, 64, workers=1)
+    tau, force, tol, interval = 0.8, 1e-6, 1e-3, 50
+    cold, t_cold, _ = S.run_crops(crops, tau, force, tol, interval, 60000)
+    warm, t_warm, t_inf = S.run_crops(crops, tau, force, tol, interval, 60000, net, scaling, relax=5)
+    sc = np.array([r['steps'] for r in cold])
+    sw = np.array([r['steps'] for r in warm])
+    kc = np.array([r['K'] for r in cold])
+    kw = np.array([r['K'] for r in warm])
+    say('cold steps', sc, 'warm steps', sw, 'K rel', np.abs(kw - kc) / kc)
+    say('cold s', t_cold, 'warm s', t_warm, 'inference', t_inf)
+    assert all(r['converged'] for r in cold) and all(r['converged'] for r in warm)
+    assert sw.sum() < sc.sum()
+    assert np.max(np.abs(kw - kc) / kc) < 1e-2
