@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/lbflow.h"
@@ -344,11 +345,6 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
                 return bfail(nullptr, "lbf_batch_create: batched crops support VD/GV fluid and BB/SD solids only");
             }
         }
-        {
-            const std::vector<uint8_t> flow = flowing_sites(cr.geometry, cr.sz, d.D, d.Q, lab.data(),
-                                                            &b->crop_force[size_t(c) * 3]);
-            for (uint64_t k = cd.slot_base; k < site.size(); ++k) flow8[k] = flow[site[k]];
-        }
         uint32_t before = 0;
         for (size_t w = cd.rank_base; w < rank.size(); ++w) {
             rank[w].x = before;
@@ -366,6 +362,22 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
         b->sites += N;
         b->fluid += nf;
         for (uint32_t t = cd.tile_begin; t < cd.tile_end; ++t) b->tile_crop.push_back(c);
+    }
+    // flowing clusters of every crop, crops in parallel (disjoint slot ranges)
+    {
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nt; ++t)
+            pool.emplace_back([&, t]() {
+                for (uint32_t c = t; c < d.ncrops; c += nt) {
+                    const std::vector<uint8_t> flow = flowing_sites(crops[c].geometry, crops[c].sz, d.D, d.Q,
+                                                                    lab.data(), &b->crop_force[size_t(c) * 3]);
+                    const CropDev& cd = b->crops[c];
+                    for (uint64_t k = cd.slot_base; k < uint64_t(cd.slot_base) + cd.nfluid; ++k)
+                        flow8[k] = flow[site[k]];
+                }
+            });
+        for (auto& th : pool) th.join();
     }
     if (site.size() >= (uint64_t(1) << 32) || rank.size() >= (uint64_t(1) << 31)) {
         lbf_batch_destroy(b);
