@@ -259,11 +259,14 @@ int lbf_batch_crop_fields(lbf_batch* batch, uint32_t c, float* fields);
  * rest state of lbf_batch_init.  Call after lbf_batch_init and before
  * lbf_batch_run. */
 int lbf_batch_set_fields(lbf_batch* batch, uint32_t c, const float* fields);
-/* `steps` consistent-initialisation sweeps after lbf_batch_set_fields (after
- * Mei, Luo, Lallemand & d'Humieres, Computers & Fluids 35 (2006) 855):
+/* Consistent initialisation after lbf_batch_set_fields: `steps` sweeps that
  * stream, give the streamed populations the momentum of the prescribed
- * velocity, collide; density and non-equilibrium populations settle to the
- * given flow before lbf_batch_run.  The sweeps count as steps. */
+ * velocity and collide (after Mei, Luo, Lallemand & d'Humieres, Computers &
+ * Fluids 35 (2006) 855), so density and non-equilibrium populations settle
+ * to the given flow; then two free steps, each averaged with the state before
+ * it, which cancels the non-decaying mode that flips sign every step (a
+ * prediction that is not smooth on the lattice scale excites it).  All of
+ * these count as steps. */
 int lbf_batch_relax(lbf_batch* batch, uint32_t steps);
 
 /* --- learned warm-start network (no reference counterpart) -------------

@@ -30,6 +30,7 @@ hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begi
 hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t tile_begin, uint32_t ntiles,
                              const uint32_t* site, const uint8_t* lab8, const uint8_t* flow8, const LabelF* lab,
                              const float* fields, float* f, float* upres, hipStream_t s);
+hipError_t launch_batch_average(float* a, const float* b, uint64_t n, hipStream_t s);
 hipError_t launch_batch_report(uint32_t grid, uint64_t S, int D, int axis, const BatchBuffers& b, uint32_t ncrops,
                                double* partial, double* out, hipStream_t s);
 }  // namespace lbf
@@ -621,6 +622,17 @@ int lbf_batch_relax(lbf_batch* b, uint32_t steps)
     if (!b) return -1;
     for (uint32_t k = 0; k < steps; ++k)
         if (step(b, 4u, false)) return -1;
+    // Two free steps, each averaged with the state before it.  A start that is
+    // not smooth on the lattice scale (any prediction) excites a global mode of
+    // the lattice Boltzmann iteration that flips sign every step and does not
+    // decay; it never lets the adjacent-velocity residual drop.  The average of
+    // two consecutive states cancels it.
+    const uint64_t n = uint64_t(b->Q) * b->S;
+    for (int k = 0; k < 2; ++k) {
+        if (step(b, 0u, false)) return -1;
+        BHIP(b, launch_batch_average(static_cast<float*>(b->f[b->cur]), static_cast<const float*>(b->f[b->cur ^ 1]),
+                                     n, b->stream));
+    }
     BHIP(b, hipStreamSynchronize(b->stream));
     return 0;
 }

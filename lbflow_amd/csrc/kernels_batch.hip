@@ -249,6 +249,20 @@ batch_warm(BatchParams p, uint32_t tile_begin, const uint32_t* __restrict__ site
     for (int q = 0; q < Q; ++q) f[uint64_t(q) * p.S + slot] = h[q];
 }
 
+// a = (a + b) / 2 over n floats: the average of two consecutive states, which
+// removes the populations' period-2 component.
+__global__ void __launch_bounds__(256) batch_average(float* __restrict__ a, const float* __restrict__ b, uint64_t n)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (i < n) a[i] = 0.5f * (a[i] + b[i]);
+}
+
+hipError_t launch_batch_average(float* a, const float* b, uint64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_average, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begin, uint32_t ntiles,
                                const uint32_t* site, const float* f, const float* ucur, float* fields, hipStream_t s)
 {

@@ -325,7 +325,7 @@ def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=Non
 
 def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, tau=0.8, force=1e-6,
              train_force=1e-5, train_tol=1e-5, eval_tol=1e-3, interval=50, max_steps=60000, out=None, log=print,
-             relax_sweeps=(0, 5)):
+             relax_sweeps=(5,)):
     """Generate, solve, train, evaluate.  Returns a report dict."""
     import torch
     rep = dict(side=side, n_train=n_train, n_eval=n_eval, epochs=epochs, capacity=capacity, lr=lr, tau=tau,
@@ -363,6 +363,15 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
     torch.cuda.empty_cache()
     if out:
         save(out, net, scaling)
+    rep['evaluation'] = evaluate(net, scaling, eval_crops, tau, force, eval_tol, interval, max_steps, relax_sweeps,
+                                 log)
+    return rep
+
+
+def evaluate(net, scaling, eval_crops, tau=0.8, force=1e-6, eval_tol=1e-3, interval=50, max_steps=60000,
+             relax_sweeps=(5,), log=print):
+    """Cold vs warm starts of the same crops to the same stop rule, plus a tight
+    (1e-5) cold run as the permeability reference."""
     cold, t_cold, _ = run_crops(eval_crops, tau, force, eval_tol, interval, max_steps)
     ref, _, _ = run_crops(eval_crops, tau, force, 1e-5, interval, max_steps)
     kc = np.array([r['K'] for r in cold])
@@ -383,9 +392,8 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
             k_rel_vs_cold_mean=float(np.mean(np.abs(kw - kc) / kc)),
             k_rel_vs_tight_mean=float(np.mean(np.abs(kw - kr) / kr)))
         log(f'relax {relax}: ' + json.dumps(ev[f'warm_relax{relax}']))
-    rep['evaluation'] = ev
-    log(json.dumps(ev['cold']))
-    return rep
+    log('cold: ' + json.dumps(ev['cold']))
+    return ev
 
 
 def main(argv=None):
@@ -399,12 +407,18 @@ def main(argv=None):
     ap.add_argument('--lr', type=float, default=1e-3)
     ap.add_argument('--out', default=None, help='weights file (.npz)')
     ap.add_argument('--report', default=None, help='report file (.json)')
+    ap.add_argument('--evaluate', default=None, help='only evaluate these weights (.npz) on --eval crops')
     a = ap.parse_args(argv)
 
     def log(*x):
         print(*x, flush=True)
 
-    rep = pipeline(a.train, a.eval, a.side, a.epochs, a.capacity, a.lr, out=a.out, log=log)
+    if a.evaluate:
+        crops = make_crops(a.eval, 20000, a.side)
+        net, scaling = load(a.evaluate, capacity=a.capacity)
+        rep = dict(weights=a.evaluate, n_eval=a.eval, evaluation=evaluate(net, scaling, crops, log=log))
+    else:
+        rep = pipeline(a.train, a.eval, a.side, a.epochs, a.capacity, a.lr, out=a.out, log=log)
     if a.report:
         with open(a.report, 'w') as f:
             json.dump(rep, f, indent=1)
