@@ -21,10 +21,10 @@
 // same kernel on the transposed, tap-mirrored weights.
 //
 // Weight gradient: the reduction runs over voxels, so both operands need 8
-// consecutive voxels per lane; 64-voxel stages of dz and of the gathered
-// input are transposed into LDS ([channel][voxel] rows, padded to 144 B),
-// and every voxel split writes its own FP32 partial, summed in split order by
-// a second kernel (deterministic).
+// consecutive voxels per lane; 128-voxel stages of dz and of the gathered
+// input are fetched into registers one stage ahead, transposed into LDS
+// ([channel][voxel] rows, padded to 272 B), and every voxel split writes its
+// own FP32 partial, summed in split order by a second kernel (deterministic).
 #include "conv_types.h"
 
 namespace lbf {
@@ -88,7 +88,9 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
         live[i] = vox[i] < g.nvox;
     }
 
-    for (int kc = 0; kc < KP; kc += KC) {
+    constexpr int NSTAGE = (KP + KC - 1) / KC;
+    for (int st = 0; st < NSTAGE; ++st) {
+        const int kc = st * KC;
         __syncthreads();
         for (int idx = threadIdx.x; idx < BN * (KC / 8); idx += 256) {
             const int row = idx / (KC / 8), c8 = idx % (KC / 8);
@@ -98,27 +100,34 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
                 val = *reinterpret_cast<const bf16x8*>(w + size_t(co0 + row) * KP + k);
             *reinterpret_cast<bf16x8*>(sw + row * ROW + 8 * c8) = val;
         }
-        __syncthreads();
-        const int ksteps = (KP - kc < KC ? KP - kc : KC) / CONV_K;
-        for (int s = 0; s < ksteps; ++s) {
+        // gather the A operands of all K steps of this stage first, so that
+        // their loads are in flight together and overlap the LDS sync
+        bf16x8 a[KC / CONV_K][4];
+#pragma unroll
+        for (int s = 0; s < KC / CONV_K; ++s) {
             const int k = kc + CONV_K * s + 8 * grp;
-            bf16x8 a[4];
             if (k < KREAL) {
                 const int tap = k / CIN, ci = k - tap * CIN;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    a[i] = live[i] ? *reinterpret_cast<const bf16x8*>(in + size_t(nbr<TAPS>(vox[i], tap, g.log_s)) *
-                                                                         g.in_stride + ci)
-                                   : bf16_zero();
+                    a[s][i] = live[i] ? *reinterpret_cast<const bf16x8*>(in + size_t(nbr<TAPS>(vox[i], tap, g.log_s)) *
+                                                                            g.in_stride + ci)
+                                      : bf16_zero();
             } else {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = bf16_zero();
+                for (int i = 0; i < 4; ++i) a[s][i] = bf16_zero();
             }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < KC / CONV_K; ++s) {
+            if (kc + CONV_K * s >= KP) break;
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
                 const bf16x8 b = *reinterpret_cast<const bf16x8*>(sw + (16 * j + r16) * ROW + CONV_K * s + 8 * grp);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b, acc[i][j], 0, 0, 0);
+                for (int i = 0; i < 4; ++i)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s][i], b, acc[i][j], 0, 0, 0);
             }
         }
     }
@@ -151,11 +160,12 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
 {
     constexpr int KP = conv_kp(CIN, TAPS);
     constexpr int KREAL = TAPS * CIN;
-    constexpr int VC = 64;
-    constexpr int ROW = VC + 8;
+    constexpr int VC = 128;                        // voxels per LDS stage (4 K steps)
+    constexpr int ROW = VC + 8;                    // bf16 per LDS row: 272 B, 16 lanes hit 16 bank quads
     constexpr int WM = BM >= 32 ? 2 : 1, WN = 4 / WM;
     constexpr int MT = BM / WM / 16, NT = BN / WN / 16;
-    static_assert(MT >= 1 && NT >= 1, "tile too small for the wave layout");
+    constexpr int NA = (BM / 8) * VC / 256, NB = (BN / 8) * VC / 256;   // 16-byte loads per thread per stage
+    static_assert(MT >= 1 && NT >= 1 && NA >= 1 && NB >= 1, "tile too small for the wave layout");
     __shared__ __attribute__((aligned(16))) __bf16 sa[BM * ROW];
     __shared__ __attribute__((aligned(16))) __bf16 sb[BN * ROW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -170,18 +180,21 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    for (uint32_t v0 = v_begin; v0 < v_end; v0 += VC) {
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < (BM / 8) * VC; idx += 256) {
+    // one stage's operands in registers: item idx = vv + VC * c8 (voxel vv, 8 channels c8)
+    bf16x8 ra[NA], rb[NB];
+    auto fetch = [&](uint32_t v0) {
+#pragma unroll
+        for (int n = 0; n < NA; ++n) {
+            const int idx = threadIdx.x + 256 * n;
             const int vv = idx % VC, c8 = idx / VC;
             const uint32_t v = v0 + vv;
-            bf16x8 val = bf16_zero();
-            if (v < v_end && co0 + 8 * c8 < COUT)
-                val = *reinterpret_cast<const bf16x8*>(dz + size_t(v) * g.out_stride + co0 + 8 * c8);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) sa[(8 * c8 + j) * ROW + vv] = val[j];
+            ra[n] = (v < v_end && co0 + 8 * c8 < COUT)
+                        ? *reinterpret_cast<const bf16x8*>(dz + size_t(v) * g.out_stride + co0 + 8 * c8)
+                        : bf16_zero();
         }
-        for (int idx = threadIdx.x; idx < (BN / 8) * VC; idx += 256) {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+            const int idx = threadIdx.x + 256 * n;
             const int vv = idx % VC, k8 = idx / VC;
             const uint32_t v = v0 + vv;
             const int k = k0 + 8 * k8;
@@ -190,10 +203,29 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
                 const int tap = k / CIN, ci = k - tap * CIN;
                 val = *reinterpret_cast<const bf16x8*>(in + size_t(nbr<TAPS>(v, tap, g.log_s)) * g.in_stride + ci);
             }
+            rb[n] = val;
+        }
+    };
+    if (v_begin < v_end) fetch(v_begin);
+    for (uint32_t v0 = v_begin; v0 < v_end; v0 += VC) {
+        __syncthreads();   // the previous stage's MFMAs are done with the LDS tiles
+        // transpose the fetched stage into [channel][voxel] rows
 #pragma unroll
-            for (int j = 0; j < 8; ++j) sb[(8 * k8 + j) * ROW + vv] = val[j];
+        for (int n = 0; n < NA; ++n) {
+            const int idx = threadIdx.x + 256 * n;
+            const int vv = idx % VC, c8 = idx / VC;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sa[(8 * c8 + j) * ROW + vv] = ra[n][j];
+        }
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+            const int idx = threadIdx.x + 256 * n;
+            const int vv = idx % VC, k8 = idx / VC;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sb[(8 * k8 + j) * ROW + vv] = rb[n][j];
         }
         __syncthreads();
+        if (v0 + VC < v_end) fetch(v0 + VC);   // next stage's loads overlap this stage's MFMAs
 #pragma unroll
         for (int s = 0; s < VC / CONV_K; ++s) {
             bf16x8 a[MT], b[NT];
@@ -269,7 +301,7 @@ struct ConvLaunch {
                       float* dw, hipStream_t s)
     {
         constexpr int KP = conv_kp(CIN, TAPS);
-        const uint32_t per = ((g.nvox + splits - 1) / splits + 63u) / 64u * 64u;
+        const uint32_t per = ((g.nvox + splits - 1) / splits + 127u) / 128u * 128u;
         const dim3 grid((KP + BN_W - 1) / BN_W, (COUT + BM_W - 1) / BM_W, splits);
         hipLaunchKernelGGL((conv_wgrad<CIN, COUT, TAPS, BM_W, BN_W>), grid, dim3(256), 0, s, g, in, dz, partial, per);
         const uint32_t n = uint32_t(COUT) * KP;

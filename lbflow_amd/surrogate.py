@@ -396,6 +396,24 @@ def evaluate(net, scaling, eval_crops, tau=0.8, force=1e-6, eval_tol=1e-3, inter
     return ev
 
 
+def train_bench(side, capacity, steps, log=print):
+    """Training-step throughput on synthetic masks and targets (no solver):
+    `steps` Adam steps of `capacity` crops, timed with HIP events."""
+    import torch
+    crops = make_crops(capacity, 30000, side, workers=1)
+    masks = torch.as_tensor(np.stack([(c == PORE) for c in crops]).astype(np.uint8), device='cuda')
+    rng = np.random.default_rng(0)
+    targets = torch.as_tensor(rng.standard_normal((capacity, side ** 3, 4)).astype(np.float32), device='cuda')
+    net = Net(side=side, capacity=capacity, seed=1)
+    net.timed_train(masks, targets, adam(1e-4), 2)        # warm-up
+    ms, flops = net.timed_train(masks, targets, adam(1e-4), steps)
+    rep = dict(side=side, samples_per_step=capacity, steps=steps, ms_per_step=round(ms, 3),
+               conv_flops_per_step=flops, tflops=round(flops / ms / 1e9, 1),
+               mfma_frac=round(flops / ms / 1e9 / 2500.0, 4))
+    log(json.dumps(rep))
+    return rep
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description='train and evaluate the warm-start network')
@@ -408,12 +426,16 @@ def main(argv=None):
     ap.add_argument('--out', default=None, help='weights file (.npz)')
     ap.add_argument('--report', default=None, help='report file (.json)')
     ap.add_argument('--evaluate', default=None, help='only evaluate these weights (.npz) on --eval crops')
+    ap.add_argument('--train-bench', type=int, default=0,
+                    help='only time this many training steps on --capacity synthetic crops')
     a = ap.parse_args(argv)
 
     def log(*x):
         print(*x, flush=True)
 
-    if a.evaluate:
+    if a.train_bench:
+        rep = train_bench(a.side, a.capacity, a.train_bench, log)
+    elif a.evaluate:
         crops = make_crops(a.eval, 20000, a.side)
         net, scaling = load(a.evaluate, capacity=a.capacity)
         rep = dict(weights=a.evaluate, n_eval=a.eval, evaluation=evaluate(net, scaling, crops, log=log))
