@@ -23,6 +23,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -82,6 +84,35 @@ def cpu_baseline(spec, labels, budget_s):
                        f'step, {dt:.1f} s, 1 thread')
 
 
+def colour_bench(steps, warmup, fp64=False):
+    """Two-phase colour-gradient D3Q19 MRT on a 128x128x256 periodic sphere pack
+    (z y x = 256 128 128, radius 6-12, porosity 0.30, seed 3), random 8^3 blocks
+    of the two phases (50 %), tau 1/1, densities 1/1, alpha0 1/3, sigma 1e-3,
+    beta 0.7, contact angle 60 deg, force 1e-6 along z (direction 5)."""
+    from lbflow_amd import geometry as G
+    from lbflow_amd import harness as H
+    from lbflow_amd.engine import FP32, FP64
+    labels = G.sphere_pack((256, 128, 128), 0.30, 6.0, 12.0, seed=3)
+    labels = G.random_blocks(labels, 8, 0.5, seed=103, phase0=75, phase1=76, pore=75)
+    spec = H.spec('c3', nx=128, ny=128, nz=256, model='color', eq='MRT', tau=(1.0, 1.0),
+                  geometry=labels.ravel(), sigma=1e-3, beta=0.7, alpha=1 / 3,
+                  indices=['75 "GV" "1e-06" "5" "0"', '76 "GV" "1e-06" "5" "1"', '255 "SD" "60" "N/A" "N/A"'])
+    out = {'workload': 'colour-gradient D3Q19 MRT, 128x128x256 sphere pack (r 6-12, porosity '
+                       f'{float(np.isin(labels, (75, 76)).mean()):.4f}, seed 3), 8^3 phase blocks 50 %, '
+                       'tau 1/1, sigma 1e-3, beta 0.7, 60 deg, F 1e-6 z'}
+    for name, prec, solids in (('fp32', FP32, False), ('fp64', FP64, True)):
+        if name == 'fp64' and not fp64:
+            continue
+        r = measure(spec, labels, prec, solids, steps if name == 'fp32' else max(5, steps // 4),
+                    warmup if name == 'fp32' else 2)
+        out[name] = dict(kernel=r['kernel'], ms_per_step=round(r['ms_per_step'], 5),
+                         site_mlups=round(labels.size / r['ms_per_step'] / 1e3, 1),
+                         bytes_per_launch=r['bytes_per_launch'],
+                         achieved_gbs=round(r['bytes_per_launch'] / r['ms_per_step'] / 1e6, 1))
+        log('colour', name, out[name])
+    return out
+
+
 def time_to_permeability(spec, labels, interval, max_steps):
     """Config-2 permeability in FP64 and FP32: wall clock from init to the
     reference's stop rule (residual 1e-6, three stable reports) and the
@@ -106,16 +137,22 @@ def time_to_permeability(spec, labels, interval, max_steps):
     return out
 
 
-def batch_bench(ncrops, interval, max_steps):
-    """Batched permeability: ncrops independent cubic sphere-pack crops (side
-    32-96, porosity 0.15-0.35, seeds 1000+i), D3Q19 MRT FP32, tau 0.8, F 1e-6
-    along x, every crop run to residual 1e-5 in one persistent hipGraph-captured
-    launch sequence."""
+def batch_crops(ncrops):
+    """The batched workload's crops (before the GPU is touched: a process pool forks)."""
     from lbflow_amd import geometry as G
-    from lbflow_amd.engine import Batch
     t0 = time.perf_counter()
-    crops, _ = G.batch_crops(ncrops)
+    crops, _ = G.batch_crops(ncrops, workers=16)
     log(f'batch: {ncrops} crops generated in {time.perf_counter() - t0:.1f} s')
+    return crops
+
+
+def batch_bench(crops, interval, max_steps):
+    """Batched permeability: independent cubic sphere-pack crops (side 32-96,
+    porosity 0.15-0.35, seeds 1000+i), D3Q19 MRT FP32, tau 0.8, F 1e-6 along x,
+    every crop run to residual 1e-5 in one persistent hipGraph-captured launch
+    sequence."""
+    from lbflow_amd.engine import Batch
+    ncrops = len(crops)
     labels = {75: ('GV', 1e-6, 1, 0), 255: ('SD', 90.0, 0, 0)}
     b = Batch(3, 19, 'MRT', 0.8, 1.0, labels, crops)
     per_cu, tune_ms = b.tune(steps=10)
@@ -150,6 +187,7 @@ def main():
     ap.add_argument('--ttp', action='store_true', help='also run FP64 and FP32 to convergence (time-to-K)')
     ap.add_argument('--ttp-interval', type=int, default=500)
     ap.add_argument('--ttp-max-steps', type=int, default=100000)
+    ap.add_argument('--colour', action='store_true', help='also time the two-phase colour model (128x128x256)')
     ap.add_argument('--batch', type=int, default=0, help='also run N batched crops to convergence (512)')
     ap.add_argument('--batch-interval', type=int, default=200)
     ap.add_argument('--batch-max-steps', type=int, default=60000)
@@ -159,6 +197,7 @@ def main():
               '(sharding across GPUs is not built)', file=sys.stderr)
         sys.exit(2)
 
+    crops = batch_crops(args.batch) if args.batch else None
     from lbflow_amd.engine import FP32, FP64, copy_bandwidth
     spec, labels = workload(args.size)
     porosity = float((labels == 75).mean())
@@ -181,8 +220,10 @@ def main():
         record('fp64_dense_all_sites', measure(spec, labels, FP64, True, max(10, args.steps // 4), 5))
     if args.ttp:
         extra['time_to_permeability'] = time_to_permeability(spec, labels, args.ttp_interval, args.ttp_max_steps)
+    if args.colour:
+        extra['colour'] = colour_bench(max(10, args.steps // 2), 5, args.fp64)
     if args.batch:
-        extra['batch'] = batch_bench(args.batch, args.batch_interval, args.batch_max_steps)
+        extra['batch'] = batch_bench(crops, args.batch_interval, args.batch_max_steps)
     for k, v in extra.items():
         log(k, v)
 

@@ -62,16 +62,25 @@ def random_blocks(labels, block, fraction, seed, phase0=75, phase1=76, pore=75):
     return out
 
 
-def batch_crops(n=512, seed0=1000, side=(32, 96), phi=(0.15, 0.35), rmin=4.0, rmax=10.0):
+def _batch_crop(args):
+    i, seed0, side, phi, rmin, rmax = args
+    rng = np.random.default_rng(seed0 + i)
+    s = int(rng.integers(side[0], side[1] + 1))
+    p = float(rng.uniform(phi[0], phi[1]))
+    return sphere_pack((s, s, s), p, rmin, rmax, seed0 + i), p
+
+
+def batch_crops(n=512, seed0=1000, side=(32, 96), phi=(0.15, 0.35), rmin=4.0, rmax=10.0, workers=1):
     """Independent cubic sphere-pack crops: crop i draws its side uniformly from
     the integers side[0]..side[1] and its porosity uniformly from phi with
     seed seed0 + i, then packs spheres (radius uniform in [rmin, rmax]) with
-    that seed.  Returns (list of uint8 volumes, list of target porosities)."""
-    crops, targets = [], []
-    for i in range(n):
-        rng = np.random.default_rng(seed0 + i)
-        s = int(rng.integers(side[0], side[1] + 1))
-        p = float(rng.uniform(phi[0], phi[1]))
-        crops.append(sphere_pack((s, s, s), p, rmin, rmax, seed0 + i))
-        targets.append(p)
-    return crops, targets
+    that seed.  Returns (list of uint8 volumes, list of target porosities).
+    workers > 1 uses a forked process pool (call before the GPU is used)."""
+    args = [(i, seed0, side, phi, rmin, rmax) for i in range(n)]
+    if workers > 1 and n > 8:
+        from multiprocessing import get_context
+        with get_context('fork').Pool(workers) as pool:
+            out = pool.map(_batch_crop, args, chunksize=max(1, n // (workers * 4)))
+    else:
+        out = [_batch_crop(a) for a in args]
+    return [c for c, _ in out], [p for _, p in out]

@@ -77,10 +77,20 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _torch():
+    """torch with its GPU runtime started.  The torch wheel carries its own HIP
+    runtime next to the engine's; both work in one process only if torch's
+    starts first, so every entry point that mixes them calls this first."""
+    import torch
+    torch.cuda.init()
+    return torch
+
+
 class Net:
     """The warm-start network on the GPU (device buffers are torch CUDA tensors)."""
 
     def __init__(self, side=64, capacity=4, seed=1):
+        _torch()
         self.L = lib()
         h = C.c_void_p()
         if self.L.lbf_net_create(side, capacity, seed, C.byref(h)):
@@ -176,7 +186,7 @@ def labels(force):
 def solve(crops, tau=0.8, force=1e-5, tol=1e-5, interval=100, max_steps=40000, log=None):
     """Steady fields of every crop with the batched engine.  Returns (fields as a
     float32 CUDA tensor [n, S^3, 4] = (u_z, u_y, u_x, rho), per-crop results)."""
-    import torch
+    torch = _torch()
     b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), crops)
     b.tune(steps=4)
     b.init()
@@ -292,7 +302,7 @@ def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=Non
     """Runs all crops to the stop rule in one batch, cold (rest state) or warm
     (network prediction).  Returns (results, seconds from init to the stop,
     inference seconds)."""
-    import torch
+    torch = _torch()
     b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), crops)
     b.tune(steps=4)
     L = lib()
@@ -327,7 +337,6 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
              train_force=1e-5, train_tol=1e-5, eval_tol=1e-3, interval=50, max_steps=60000, out=None, log=print,
              relax_sweeps=(5,)):
     """Generate, solve, train, evaluate.  Returns a report dict."""
-    import torch
     rep = dict(side=side, n_train=n_train, n_eval=n_eval, epochs=epochs, capacity=capacity, lr=lr, tau=tau,
                force=force, train_force=train_force, train_tol=train_tol, eval_tol=eval_tol,
                report_interval=interval)
@@ -336,6 +345,7 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
     eval_crops = make_crops(n_eval, 20000, side)
     rep['crop_generation_s'] = round(time.perf_counter() - t0, 2)
     log(f'crops generated in {rep["crop_generation_s"]} s')
+    torch = _torch()   # after the crop pool has forked
     t0 = time.perf_counter()
     fields, res = solve(train_crops, tau=tau, force=train_force, tol=train_tol, interval=100, max_steps=max_steps,
                         log=log)
@@ -399,7 +409,7 @@ def evaluate(net, scaling, eval_crops, tau=0.8, force=1e-6, eval_tol=1e-3, inter
 def train_bench(side, capacity, steps, log=print):
     """Training-step throughput on synthetic masks and targets (no solver):
     `steps` Adam steps of `capacity` crops, timed with HIP events."""
-    import torch
+    torch = _torch()
     crops = make_crops(capacity, 30000, side, workers=1)
     masks = torch.as_tensor(np.stack([(c == PORE) for c in crops]).astype(np.uint8), device='cuda')
     rng = np.random.default_rng(0)
