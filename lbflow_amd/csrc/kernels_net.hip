@@ -175,27 +175,43 @@ net_sum(const double* __restrict__ partial, uint32_t n, double* __restrict__ out
     if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
 }
 
-// Per-block channel sums of dz over `per` voxels: partial[block][c].
+// Per-block channel sums of dz over `per` voxels: partial[block][c].  The 256
+// threads split into 256/C voxel lanes per channel (C <= 256, C | 256),
+// reduced through LDS in a fixed order.
 __global__ void __launch_bounds__(256)
 net_bias_partial(Chan dz, uint32_t C, uint32_t nvox, uint32_t per, float* __restrict__ partial)
 {
+    __shared__ float red[256];
+    const uint32_t c = threadIdx.x % C, lanes = 256u / C, g = threadIdx.x / C;
     const uint32_t v0 = blockIdx.x * per;
     const uint32_t v1 = v0 + per < nvox ? v0 + per : nvox;
-    for (uint32_t c = threadIdx.x; c < C; c += 256) {
-        float s = 0.0f;
-        for (uint32_t v = v0; v < v1; ++v) s += float(dz.p[size_t(v) * dz.stride + dz.off + c]);
-        partial[size_t(blockIdx.x) * C + c] = s;
+    float s = 0.0f;
+    for (uint32_t v = v0 + g; v < v1; v += lanes) s += float(dz.p[size_t(v) * dz.stride + dz.off + c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.0f;
+        for (uint32_t k = 0; k < lanes; ++k) t += red[k * C + threadIdx.x];
+        partial[size_t(blockIdx.x) * C + threadIdx.x] = t;
     }
 }
 
+// db[c] = sum over blocks of partial[b][c]: one workgroup per channel, strided
+// partial sums then a fixed-order tree.
 __global__ void __launch_bounds__(256)
 net_bias_final(const float* __restrict__ partial, uint32_t C, uint32_t blocks, float* __restrict__ db)
 {
-    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
-    if (c >= C) return;
+    __shared__ float red[256];
+    const uint32_t c = blockIdx.x;
     float s = 0.0f;
-    for (uint32_t b = 0; b < blocks; ++b) s += partial[size_t(b) * C + c];
-    db[c] = s;
+    for (uint32_t b = threadIdx.x; b < blocks; b += 256) s += partial[size_t(b) * C + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
 }
 
 // Adam (bias-corrected) on FP32 master parameters; optionally refreshes the BF16 copy.
@@ -296,7 +312,7 @@ hipError_t launch_net_bias_grad(Chan dz, uint32_t C, uint32_t nvox, float* parti
 {
     const uint32_t per = (nvox + blocks - 1) / blocks;
     hipLaunchKernelGGL(net_bias_partial, dim3(blocks), dim3(256), 0, s, dz, C, nvox, per, partial);
-    hipLaunchKernelGGL(net_bias_final, grid_for(C), dim3(256), 0, s, partial, C, blocks, db);
+    hipLaunchKernelGGL(net_bias_final, dim3(C), dim3(256), 0, s, partial, C, blocks, db);
     return hipGetLastError();
 }
 hipError_t launch_net_adam(float* p, float* m, float* v, const float* g, uint32_t n, float lr, float b1, float b2,

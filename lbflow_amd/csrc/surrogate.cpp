@@ -15,6 +15,7 @@
 // other range.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -154,14 +155,16 @@ int wgrad(lbf_net* n, int i, uint32_t count, const __bf16* in, uint32_t in_strid
     const Layer& l = n->L[i];
     const ConvGeom g = geom(n, l.level, count, in_stride, dz_stride);
     const uint32_t kb = (l.kp + 63) / 64, cb = (uint32_t(l.cout) + 63) / 64;
-    uint32_t splits = (2048u + kb * cb - 1) / (kb * cb);
+    // enough workgroups to fill the chip, but few enough partials that the
+    // fixed-order split sum stays cheap
+    uint32_t splits = std::min(64u, (2048u + kb * cb - 1) / (kb * cb));
     const uint32_t max_splits = (g.nvox + 255) / 256;
     if (splits > max_splits) splits = max_splits;
     if (size_t(splits) * l.cout * l.kp > n->wpart_floats) splits = uint32_t(n->wpart_floats / (size_t(l.cout) * l.kp));
     if (splits < 1) splits = 1;
     NHIP(n, launch_conv_wgrad(l.cin, l.cout, l.taps, g, in, dz, n->wpart, splits, l.gw, n->stream));
-    uint32_t blocks = g.nvox / 64;
-    if (blocks > 4096) blocks = 4096;
+    uint32_t blocks = g.nvox / 1024;
+    if (blocks > 1024) blocks = 1024;
     if (blocks < 1) blocks = 1;
     NHIP(n, launch_net_bias_grad(ch(const_cast<__bf16*>(dz), dz_stride), l.cout, g.nvox, n->bpart, blocks, l.gb,
                                  n->stream));
@@ -328,7 +331,7 @@ int lbf_net_create(uint32_t side, uint32_t capacity, uint64_t seed, lbf_net** ou
         if (hipMemcpy(l.w, hw.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
         // weight-gradient partials: at most (max voxels / 256) splits, capped by the heuristic
         const uint32_t kb = (l.kp + 63) / 64, cb = (uint32_t(l.cout) + 63) / 64;
-        const size_t splits = std::min<size_t>((2048u + kb * cb - 1) / (kb * cb),
+        const size_t splits = std::min<size_t>(std::min(64u, (2048u + kb * cb - 1) / (kb * cb)),
                                                std::max<size_t>(1, (size_t(capacity) << (3 * (ls - l.level))) / 256));
         wmax = std::max(wmax, splits * nw);
     }
