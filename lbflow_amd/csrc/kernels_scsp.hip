@@ -41,12 +41,12 @@
 
 #include "collide_f32.h"
 #include "device.h"
+#include "stream.h"
 
 namespace lbf {
 
 constexpr uint32_t F_FIELDS = 1, F_ADJ = 2;
 constexpr double C1 = 3.0, C2 = 4.5, C3 = 1.5;   // 3/c^2, 4.5/c^4, 1.5/c^2 with c = 1
-enum : int { DENSE_ALL = 0, DENSE_FLUID = 1, SPARSE = 2 };
 
 
 // ---------------------------------------------------------------------------
@@ -66,81 +66,6 @@ __device__ __forceinline__ bool resolve(const ScspParams& p, const uint32_t* __r
     }
     slot = k;
     return true;
-}
-
-// Wrapped coordinate terms of a site: x-1, x, x+1 and the matching row and
-// plane offsets (site index units), computed once per site so that every
-// neighbour index is two integer adds.
-struct Stencil {
-    uint32_t x[3], y[3], z[3];   // [0] = -1, [1] = 0, [2] = +1
-};
-
-__device__ __forceinline__ Stencil make_stencil(const GridDev& g, uint32_t site)
-{
-    int x, y, z;
-    site_coords(g, site, x, y, z);
-    Stencil s;
-    s.x[1] = uint32_t(x);
-    s.x[0] = x == 0 ? g.nx - 1 : uint32_t(x - 1);
-    s.x[2] = uint32_t(x) + 1 == g.nx ? 0u : uint32_t(x + 1);
-    s.y[1] = uint32_t(y) * g.nx;
-    s.y[0] = (y == 0 ? g.ny - 1 : uint32_t(y - 1)) * g.nx;
-    s.y[2] = (uint32_t(y) + 1 == g.ny ? 0u : uint32_t(y + 1)) * g.nx;
-    s.z[1] = uint32_t(z) * g.nxy;
-    s.z[0] = (z == 0 ? g.nz - 1 : uint32_t(z - 1)) * g.nxy;
-    s.z[2] = (uint32_t(z) + 1 == g.nz ? 0u : uint32_t(z + 1)) * g.nxy;
-    return s;
-}
-
-// Site x - e_q (pull source of direction q).
-template <int Q>
-__device__ __forceinline__ uint32_t upstream(const Stencil& s, int q)
-{
-    using L = Lattice<Q>;
-    int ex, ey, ez;
-    if constexpr (Q == 9) {
-        ez = 0; ey = L::e[q][0]; ex = L::e[q][1];
-    } else {
-        ez = L::e[q][0]; ey = L::e[q][1]; ex = L::e[q][2];
-    }
-    return s.z[1 - ez] + s.y[1 - ey] + s.x[1 - ex];
-}
-
-// Element offset (from the start of the population array) of the value that
-// streams into direction q of this slot: the upstream neighbour's f_q, or
-// this slot's own f_op(q) when the link bounces back.
-//
-// Sparse layout: the slot of a fluid site is its rank among the fluid sites
-// in z,y,x order, read from a 32-site rank index {fluid sites before the
-// word, fluid bit mask of the word}: 8 bytes per 32 sites (4 MB at 256^3),
-// small enough to stay in L2, instead of a 4-byte map entry per site.
-template <int Q, int LAYOUT, typename O>
-__device__ __forceinline__ O pull_offset(const uint2* __restrict__ rank, uint32_t links, uint32_t slot,
-                                         const Stencil& st, uint32_t P, int q)
-{
-    using L = Lattice<Q>;
-    // site x - e_q solid <=> site in ibS[op(q)] (simulator.cpp:103-111)
-    if ((links >> L::op[q]) & 1u) return O(L::op[q]) * P + slot;
-    const uint32_t src = upstream<Q>(st, q);
-    if constexpr (LAYOUT == SPARSE) {
-        const uint2 r = rank[src >> 5];
-        return O(q) * P + (r.x + uint32_t(__popc(r.y & ((1u << (src & 31u)) - 1u))));
-    } else {
-        return O(q) * P + src;
-    }
-}
-
-// Load/store with a uniform base and an element offset.  With a 32-bit
-// offset type (population array below 4 GiB) the address is an SGPR base plus
-// a zero-extended 32-bit VGPR byte offset; the 64-bit type serves larger
-// arrays.
-template <typename T, typename O> __device__ __forceinline__ T ld(const T* __restrict__ base, O elem)
-{
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + elem * O(sizeof(T)));
-}
-template <typename T, typename O> __device__ __forceinline__ void st(T* __restrict__ base, O elem, T v)
-{
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + elem * O(sizeof(T))) = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -210,8 +135,10 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
         }
     } else {
         f[0] = ld(fin, O(slot));
+        O off[Q];
+        pull_offsets<Q, LAYOUT, O>(rank, dsc, slot, stn, p.g.nx, P, off);
 #pragma unroll
-        for (int q = 1; q < Q; ++q) f[q] = ld(fin, pull_offset<Q, LAYOUT, O>(rank, dsc, slot, stn, P, q));
+        for (int q = 1; q < Q; ++q) f[q] = ld(fin, off[q]);
     }
 
     // calcMacroVar: rho = sum_q f_q (q ascending); u = (sum +-f) / rho0
@@ -363,8 +290,10 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
         }
     } else {
         h[0] = ld(fin, O(slot));
+        O off[Q];
+        pull_offsets<Q, LAYOUT, O>(rank, dsc, slot, stn, p.g.nx, P, off);
 #pragma unroll
-        for (int q = 1; q < Q; ++q) h[q] = ld(fin, pull_offset<Q, LAYOUT, O>(rank, dsc, slot, stn, P, q));
+        for (int q = 1; q < Q; ++q) h[q] = ld(fin, off[q]);
     }
     // macro on shifted populations: rho = rho0 + sum h, j = sum e h
     float u[D];
