@@ -23,7 +23,7 @@ namespace lbf {
 hipError_t launch_scsp(int Q, bool mrt, bool init, bool fp32, int layout, const ScspParams& p,
                        const ScspBuffers& b, hipStream_t s);
 hipError_t launch_color(int Q, bool mrt, bool init, bool stream_solids, const ColorArgs& a, hipStream_t s);
-hipError_t launch_color32(int Q, bool mrt, bool init, const ColorArgs& a, hipStream_t s);
+hipError_t launch_color32(int Q, bool mrt, bool init, bool sparse, const ColorArgs& a, hipStream_t s);
 hipError_t launch_report(const ReportArgs& a, double* out, hipStream_t s);
 uint32_t report_blocks(uint64_t N);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
@@ -269,14 +269,18 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
             }
         }
     }
-    for (uint64_t i = 0; i < N; ++i)
+    bool all_bounce = true;   // every non-fluid label bounces back (BB or SD)
+    for (uint64_t i = 0; i < N; ++i) {
         if (c->hlab[geometry[i]].kind & KIND_FLUID) c->list.push_back(uint32_t(i));
+        else if (!(c->labels[geometry[i]].type & (T_BB | T_SD))) all_bounce = false;
+    }
     c->nfluid = c->list.size();
-    // Layout: the colour model and checkpoint-exact runs stream every site;
-    // otherwise only fluid sites are stored (sparse) unless the dense
-    // fluid-only layout is asked for.
-    // The FP32 colour model keeps the dense layout and skips solid sites.
-    if (NP == 2) c->layout = d.precision == LBF_FP32 ? 1 : 0;
+    // Layout: the FP64 colour model and checkpoint-exact runs stream every
+    // site; otherwise only fluid sites are stored (sparse) unless the dense
+    // fluid-only layout is asked for.  The FP32 colour model is sparse when
+    // every non-fluid site bounces back, else dense with solid sites skipped.
+    if (NP == 2 && d.precision != LBF_FP32) c->layout = 0;
+    else if (NP == 2) c->layout = (all_bounce && !(d.flags & LBF_LAYOUT_DENSE)) ? 2 : 1;
     else if (d.flags & LBF_STREAM_SOLIDS) c->layout = 0;
     else c->layout = (d.flags & LBF_LAYOUT_DENSE) ? 1 : 2;
     c->P = c->layout == 2 ? c->nfluid : N;
@@ -368,22 +372,19 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     if (c->layout == 2) {
         // Every non-fluid site must bounce back (BB/SD), so a pull from it is a
         // link of the descriptor and the rank index only ever sees fluid sites.
+        if (!all_bounce) {
+            lbf_destroy(c);
+            return fail(nullptr, "lbf_create: sparse layout needs every non-fluid label to be BB or SD");
+        }
         std::vector<uint2> rank((N + 31) / 32, uint2{0u, 0u});
         uint32_t before = 0;
         for (uint64_t w = 0; w < rank.size(); ++w) {
             rank[w].x = before;
-            for (uint64_t i = w * 32; i < std::min<uint64_t>(N, w * 32 + 32); ++i) {
+            for (uint64_t i = w * 32; i < std::min<uint64_t>(N, w * 32 + 32); ++i)
                 if (c->hlab[geometry[i]].kind & KIND_FLUID) {
                     rank[w].y |= 1u << (i - w * 32);
                     ++before;
-                } else {
-                    const int t = c->labels[geometry[i]].type;
-                    if (t != T_BB && t != T_SD) {
-                        lbf_destroy(c);
-                        return fail(nullptr, "lbf_create: sparse layout needs every non-fluid label to be BB or SD");
-                    }
                 }
-            }
         }
         std::vector<uint32_t> desc_s(c->nfluid);
         for (uint64_t k = 0; k < c->nfluid; ++k) desc_s[k] = c->links.desc[c->list[k]];
@@ -575,7 +576,11 @@ static int run_step(lbf_ctx* c, bool init, uint32_t flags)
             a.fin32 = static_cast<const float*>(c->f[c->cur]);
             a.fout32 = static_cast<float*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
             a.densN32 = reinterpret_cast<float*>(c->d_densN);
-            e = launch_color32(c->Q, mrt, init, a, c->stream);
+            a.list = c->d_list;
+            a.rank = c->d_rank;
+            a.P = uint32_t(c->P);
+            if (c->layout == 2) a.desc = c->d_desc_s;
+            e = launch_color32(c->Q, mrt, init, c->layout == 2, a, c->stream);
         } else {
             a.fin = static_cast<const double*>(c->f[c->cur]);
             a.fout = static_cast<double*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
@@ -670,9 +675,10 @@ int lbf_get_populations(lbf_ctx* c, double* out)
         const ColorArgs a = color_args(c, 0u);
         const int Q = c->Q;
         for (int q = 0; q < Q; ++q)
-            for (uint64_t i = 0; i < c->N; ++i) {
+            for (uint64_t s = 0; s < c->P; ++s) {
+                const uint64_t i = c->layout == 2 ? c->list[s] : s;
                 if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
-                const double h = h32[size_t(q) * c->P + i], f0 = h32[size_t(Q + q) * c->P + i];
+                const double h = h32[size_t(q) * c->P + s], f0 = h32[size_t(Q + q) * c->P + s];
                 out[size_t(q) * c->N + i] = f0;
                 out[size_t(Q + q) * c->N + i] = h + a.dist0[q] - f0;
             }
@@ -704,11 +710,12 @@ int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
             const int Q = c->Q;
             const std::vector<uint32_t>& desc = c->links.desc;
             for (int q = 0; q < Q; ++q)
-                for (uint64_t i = 0; i < c->N; ++i) {
+                for (uint64_t s = 0; s < c->P; ++s) {
+                    const uint64_t i = c->layout == 2 ? c->list[s] : s;
                     if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
                     const double f0 = in[size_t(q) * c->N + i], f1 = in[size_t(Q + q) * c->N + i];
-                    h32[size_t(q) * c->P + i] = float(f0 + f1 - a.dist0[q]);
-                    h32[size_t(Q + q) * c->P + i] = float(f0);
+                    h32[size_t(q) * c->P + s] = float(f0 + f1 - a.dist0[q]);
+                    h32[size_t(Q + q) * c->P + s] = float(f0);
                 }
         } else
         for (size_t plane = 0; plane < planes; ++plane)
@@ -807,7 +814,8 @@ double lbf_step_bytes(const lbf_ctx* c)
     // the colour model also writes and reads its indicator once per site
     const double indicator = c->NP == 2 ? 2.0 * double(c->elem) : 0.0;
     if (c->NP == 2 && c->layout == 0) return double(c->N) * (pops + indicator + 4.0);
-    if (c->NP == 2) return double(c->nfluid) * (pops + indicator) + double(c->N) * 4.0;
+    if (c->NP == 2 && c->layout == 1) return double(c->nfluid) * (pops + indicator) + double(c->N) * 4.0;
+    if (c->NP == 2) return double(c->nfluid) * (pops + indicator + 8.0) + double((c->N + 31) / 32) * 8.0;
     if (c->layout == 0) return double(c->N) * (pops + 4.0);       // + descriptor
     if (c->layout == 1) return double(c->nfluid) * pops + double(c->N) * 4.0;
     // sparse: per fluid site its populations, its list entry and its

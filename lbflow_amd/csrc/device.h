@@ -192,6 +192,10 @@ struct ColorArgs {
     float* densN32;
     double mai0[19], mai1[19];     // M ai_k: rest-state moments of each phase
     double rho_ref;                // rho_0, the density of the shift s_q = ai0_q rho_0
+    // FP32 layouts: DENSE_FLUID (slot = site) or SPARSE (slot = fluid index)
+    const uint32_t* list;          // SPARSE: fluid site of each slot
+    const uint2* rank;             // SPARSE: rank index (see stream.h)
+    uint32_t P;                    // slots per plane
 };
 
 struct ReportArgs {

@@ -13,14 +13,18 @@
 // the uniform A = M^-1 S M form); the shifted equilibrium moments are
 // (rho0 - rho_0) M ai0 + rho1 M ai1 + rho M w (3eu + 4.5 (eu)^2 - 1.5 u^2).
 // Perturbation and recolouring act on the unshifted total fT = h + s.
-// Solid sites are not streamed (their populations stay 0).  Macro fields for
-// output and reports are written in FP64, so the component-mass and flux
-// reductions run in FP64.
+// Solid sites are not streamed: the populations live either in the dense
+// layout with solid slots skipped (DENSE_FLUID) or on fluid slots only
+// (SPARSE, pulled through the rank index of stream.h); the indicator, the
+// wall tables and the macro fields stay per site.  Macro fields for output
+// and reports are written in FP64, so the component-mass and flux reductions
+// run in FP64.
 //
 // FP32 tolerance: the negative-component check uses 1e-5 rho (the FP64
 // kernel's 1e-10 rho is below the FP32 rounding of rho1 = rho - rho0).
 #include "collide_f32.h"
 #include "device.h"
+#include "stream.h"
 
 namespace lbf {
 
@@ -30,12 +34,21 @@ constexpr float SOLID_N32 = 86.0f;   // colorfluid.cpp:49
 constexpr float NEG_TOL32 = 1e-5f;
 constexpr float SQRT2_32 = 1.41421356f;
 
-template <int Q, bool INIT>
-__device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t i, uint32_t desc, const DevLabel& lab,
-                                             float* h, float* f0, float& r0, float& dr)
+template <int LAYOUT>
+__device__ __forceinline__ bool resolve32(const ColorArgs& a, uint32_t& site, uint32_t& slot)
+{
+    const uint32_t k = xcd_block() * 256u + threadIdx.x;
+    if (k >= a.P) return false;
+    slot = k;
+    site = LAYOUT == SPARSE ? a.list[k] : k;
+    return true;
+}
+
+template <int Q, bool INIT, int LAYOUT, typename O>
+__device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, uint32_t slot, uint32_t desc,
+                                             const DevLabel& lab, float* h, float* f0, float& r0, float& dr)
 {
     using L = Lattice<Q>;
-    const uint64_t N = a.g.N;
     r0 = 0.0f;
     dr = 0.0f;
     if (INIT) {
@@ -51,18 +64,15 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t i, uin
         }
         return;
     }
-    int x, y, z;
-    site_coords(a.g, i, x, y, z);
-    const uint32_t links = desc & DESC_LINK_MASK;
+    const O P = a.P, f0plane = O(Q) * P;
+    const Stencil stn = make_stencil(a.g, site);
+    O off[Q];
+    off[0] = O(slot);
+    pull_offsets<Q, LAYOUT, O>(a.rank, desc & DESC_LINK_MASK, slot, stn, a.g.nx, uint32_t(P), off);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        uint32_t src;
-        int qs = q;
-        if (q == 0) src = i;
-        else if ((links >> L::op[q]) & 1u) { src = i; qs = L::op[q]; }
-        else src = pull_source<Q>(a.g, x, y, z, q);
-        h[q] = a.fin32[uint64_t(qs) * N + src];
-        f0[q] = a.fin32[(uint64_t(Q) + qs) * N + src];
+        h[q] = ld(a.fin32, off[q]);
+        f0[q] = ld(a.fin32, off[q] + f0plane);
         r0 += f0[q];
         dr += h[q];
     }
@@ -70,37 +80,37 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t i, uin
 
 }  // namespace
 
-template <int Q, bool INIT>
+template <int Q, bool INIT, int LAYOUT, typename O>
 __global__ void __launch_bounds__(256) color_density_f32(ColorArgs a)
 {
-    const uint32_t i = xcd_block() * 256u + threadIdx.x;
-    if (i >= a.g.N) return;
-    const uint32_t desc = a.desc[i];
+    uint32_t site, slot;
+    if (!resolve32<LAYOUT>(a, site, slot)) return;
+    const uint32_t desc = a.desc[slot];
     const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
     if (!(lab.kind & KIND_FLUID)) return;
     float h[Q], f0[Q], r0, dr;
-    load_color32<Q, INIT>(a, i, desc, lab, h, f0, r0, dr);
+    load_color32<Q, INIT, LAYOUT, O>(a, site, slot, desc, lab, h, f0, r0, dr);
     const float rho = float(a.rho_ref) + dr;
-    a.densN32[i] = (2.0f * r0 - rho) / rho;   // (rho0 - rho1) / rho
+    a.densN32[site] = (2.0f * r0 - rho) / rho;   // (rho0 - rho1) / rho
 }
 
-template <int Q, bool MRT, bool INIT>
+template <int Q, bool MRT, bool INIT, int LAYOUT, typename O>
 __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
     constexpr int H = L::NHALF;
     constexpr MomentTables<Q> MT = kMT<Q>;
-    const uint32_t i = xcd_block() * 256u + threadIdx.x;
-    if (i >= a.g.N) return;
+    uint32_t i, slot;
+    if (!resolve32<LAYOUT>(a, i, slot)) return;
     const uint64_t N = a.g.N;
-    const uint32_t desc = a.desc[i];
+    const uint32_t desc = a.desc[slot];
     const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
     const int kind = lab.kind;
     if (!(kind & KIND_FLUID)) return;
 
     float h[Q], f0[Q], r0, dr;
-    load_color32<Q, INIT>(a, i, desc, lab, h, f0, r0, dr);
+    load_color32<Q, INIT, LAYOUT, O>(a, i, slot, desc, lab, h, f0, r0, dr);
 
     // calcMacroVar (colorfluid.cpp:238-290)
     const float rt = float(a.rho_ref) + dr;
@@ -346,48 +356,52 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
             const float feq0 = r0 * float(a.ai0[q]) + r1 * float(a.ai1[q]);
             const float fo = dens0T * (h[q] + float(a.dist0[q])) + b01 * angle * feq0;
             finite = finite && isfinite(h[q]) && isfinite(fo);
-            a.fout32[uint64_t(q) * N + i] = h[q];
-            a.fout32[(uint64_t(Q) + q) * N + i] = fo;
+            st(a.fout32, O(q) * O(a.P) + slot, h[q]);
+            st(a.fout32, (O(Q) + q) * O(a.P) + slot, fo);
         }
     } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const float fo = dens0T * (h[q] + float(a.dist0[q]));
             finite = finite && isfinite(h[q]) && isfinite(fo);
-            a.fout32[uint64_t(q) * N + i] = h[q];
-            a.fout32[(uint64_t(Q) + q) * N + i] = fo;
+            st(a.fout32, O(q) * O(a.P) + slot, h[q]);
+            st(a.fout32, (O(Q) + q) * O(a.P) + slot, fo);
         }
     }
     if (!finite) raise_status(a.status, 4, a.t);
 }
 
-// Solid sites carry the indicator sentinel; set once per context.
-__global__ void color_solid_sentinel_f32(ColorArgs a)
+template <int Q, bool MRT, bool INIT, int LAYOUT, typename O>
+static hipError_t launch_color32_l(const ColorArgs& a, hipStream_t s)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= a.g.N) return;
-    const DevLabel& lab = a.lab[a.desc[i] >> DESC_LABEL_SHIFT];
-    if (!(lab.kind & KIND_FLUID)) a.densN32[i] = SOLID_N32;
-}
-
-template <int Q, bool MRT, bool INIT>
-static hipError_t launch_color32_q(const ColorArgs& a, hipStream_t s)
-{
-    const dim3 grid(uint32_t((a.g.N + 255) / 256)), block(256);
-    if (INIT) hipLaunchKernelGGL(color_solid_sentinel_f32, grid, block, 0, s, a);
-    hipLaunchKernelGGL((color_density_f32<Q, INIT>), grid, block, 0, s, a);
-    hipLaunchKernelGGL((color_step_f32<Q, MRT, INIT>), grid, block, 0, s, a);
+    const dim3 grid(uint32_t((a.P + 255) / 256)), block(256);
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL((color_density_f32<Q, INIT, LAYOUT, O>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((color_step_f32<Q, MRT, INIT, LAYOUT, O>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_color32(int Q, bool mrt, bool init, const ColorArgs& a, hipStream_t s)
+template <int Q, bool MRT, bool INIT>
+static hipError_t launch_color32_q(const ColorArgs& a, bool sparse, hipStream_t s)
+{
+    // 32-bit element offsets while both planes sets fit below 4 GiB
+    const bool wide = uint64_t(2) * Q * a.P * 4u >= (uint64_t(1) << 32);
+    if (sparse)
+        return wide ? launch_color32_l<Q, MRT, INIT, SPARSE, uint64_t>(a, s)
+                    : launch_color32_l<Q, MRT, INIT, SPARSE, uint32_t>(a, s);
+    return wide ? launch_color32_l<Q, MRT, INIT, DENSE_FLUID, uint64_t>(a, s)
+                : launch_color32_l<Q, MRT, INIT, DENSE_FLUID, uint32_t>(a, s);
+}
+
+hipError_t launch_color32(int Q, bool mrt, bool init, bool sparse, const ColorArgs& a, hipStream_t s)
 {
     if (Q == 19) {
-        if (mrt) return init ? launch_color32_q<19, true, true>(a, s) : launch_color32_q<19, true, false>(a, s);
-        return init ? launch_color32_q<19, false, true>(a, s) : launch_color32_q<19, false, false>(a, s);
+        if (mrt) return init ? launch_color32_q<19, true, true>(a, sparse, s)
+                             : launch_color32_q<19, true, false>(a, sparse, s);
+        return init ? launch_color32_q<19, false, true>(a, sparse, s) : launch_color32_q<19, false, false>(a, sparse, s);
     }
-    if (mrt) return init ? launch_color32_q<9, true, true>(a, s) : launch_color32_q<9, true, false>(a, s);
-    return init ? launch_color32_q<9, false, true>(a, s) : launch_color32_q<9, false, false>(a, s);
+    if (mrt) return init ? launch_color32_q<9, true, true>(a, sparse, s) : launch_color32_q<9, true, false>(a, sparse, s);
+    return init ? launch_color32_q<9, false, true>(a, sparse, s) : launch_color32_q<9, false, false>(a, sparse, s);
 }
 
 }  // namespace lbf

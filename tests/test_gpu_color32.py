@@ -4,7 +4,10 @@ The FP64 colour kernels are bit-exact with the CPU oracle (test_gpu_parity);
 the FP32 form (shifted total population + phase-0 population, FP64 macro
 fields and reductions) is pinned by its distance to them on the same
 two-phase sphere packs, by exact conservation of each component's mass
-(bounce-back and recolouring conserve it), and by a bit-exact restart.
+(bounce-back and recolouring conserve it), and by a bit-exact restart.  The
+sparse layout (fluid slots only, neighbours through the rank index) and the
+dense fluid-only layout run the same arithmetic, so they must agree bit for
+bit.
 """
 import sys
 
@@ -29,9 +32,9 @@ def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run(spec, steps, precision):
+def run(spec, steps, precision, dense=False):
     case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), precision, stream_solids=precision == FP64).init()
+    e = Engine(case, H.geometry(spec), precision, stream_solids=precision == FP64, dense=dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
     e.synchronize()
@@ -92,3 +95,16 @@ def test_fp32_colour_restart_is_exact():
         e.synchronize()
     pa, pb = a.populations(), b.populations()
     assert np.array_equal(pa, pb)
+
+
+@pytest.mark.parametrize('kw', CASES, ids=lambda k: f"{k['eq']}-{k['tau']}-{k.get('dims', 3)}d")
+def test_fp32_colour_sparse_matches_dense(kw):
+    spec = sphere_case(**kw)
+    say('case', spec['name'], kw)
+    a = run(spec, 120, FP32, dense=True)
+    b = run(spec, 120, FP32)
+    say('kernels', a.step_kernel, '|', b.step_kernel)
+    assert 'dense-fluid' in a.step_kernel and 'sparse' in b.step_kernel
+    assert np.array_equal(a.populations(), b.populations())
+    for x, y in zip(a.fields(), b.fields()):
+        assert np.array_equal(x, y)
