@@ -46,6 +46,21 @@ def workload(size):
     return spec, labels
 
 
+PMC_PROFILE = ROOT / 'profiles' / 'scsp_step_f32_sparse_256cubed_pmc.md'
+
+
+def measured_traffic(kernel, size):
+    """HBM bytes per launch of the sparse FP32 step at 256^3, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy
+    kernel's counters), or None for another kernel or size."""
+    if size != 256 or 'sparse' not in kernel or not PMC_PROFILE.exists():
+        return None
+    for line in PMC_PROFILE.read_text().splitlines():
+        if line.startswith('traffic_bytes_per_launch:'):
+            return int(line.split(':', 1)[1])
+    return None
+
+
 def measure(spec, labels, precision, stream_solids, steps, warmup, dense=False):
     from lbflow_amd import harness as H
     from lbflow_amd.engine import Engine
@@ -245,7 +260,8 @@ def main():
                    'fluid_mlups': round(site_mlups * porosity, 1), 'porosity': porosity,
                    'kernel': main_run['kernel'], 'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'frac': round(achieved / HBM_PEAK_GBS, 4),
+                     'traffic': measured_traffic(main_run['kernel'], args.size),
                      'bytes_per_launch': main_run['bytes_per_launch'], 'copy_kernel_gbs': round(copy_gbs, 1)},
         'cpu_baseline': cpu,
         'extra': extra,
