@@ -53,6 +53,7 @@ struct lbf_ctx {
     uint32_t* d_desc_s = nullptr;    // sparse: descriptor per slot
     DevLabel* d_lab = nullptr;
     LabelF* d_lab32 = nullptr;
+    DevZouHe* d_zh = nullptr;
     double *d_AT = nullptr, *d_A = nullptr, *d_B = nullptr, *d_MInv = nullptr;
     float *d_A32 = nullptr, *d_B32 = nullptr;
     int32_t* d_wall = nullptr;
@@ -216,10 +217,19 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         for (int l = 0; l < 256; ++l) {
             if (!used[l] || !labels[l].type) continue;
             const int t = labels[l].type;
-            if (t == LBF_PB || t == LBF_VB || t == LBF_AB || t == LBF_HF) {
+            if (t == LBF_AB || t == LBF_HF || ((t == LBF_PB || t == LBF_VB) && NP == 2)) {
                 lbf_destroy(c);
                 return fail(nullptr, "lbf_create: label " + std::to_string(l) +
-                                         " uses a boundary type (PB/VB/AB/HF) this engine does not implement");
+                                         " uses a boundary type this engine does not implement "
+                                         "(AB, HF; PB/VB in the colour model)");
+            }
+            if ((t == LBF_PB || t == LBF_VB) && (labels[l].dir < 1 || labels[l].dir > 2 * d.D)) {
+                lbf_destroy(c);
+                return fail(nullptr, "Pressure and velocity boundaries require an axial direction.");
+            }
+            if (t == LBF_PB && !(labels[l].value > 0.0)) {
+                lbf_destroy(c);
+                return fail(nullptr, "Pressure-boundary density must be positive.");
             }
             if ((t == LBF_VD || t == LBF_GV) && labels[l].phase >= NP) {
                 lbf_destroy(c);
@@ -241,6 +251,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         std::memset(&L, 0, sizeof L);
         const LabelInfo& li = c->labels[l];
         if (li.type == T_VD || li.type == T_GV) L.kind = KIND_FLUID;
+        if (li.type == T_PB || li.type == T_VB) L.kind = KIND_FLUID | KIND_ZOUHE;
         L.phase = li.phase;
         if (li.type == T_GV) {
             if (li.dir < 0 || li.dir >= d.Q || !std::isfinite(li.value)) {
@@ -315,6 +326,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     rc |= dalloc(c, &c->d_desc, N * 4);
     rc |= dalloc(c, &c->d_lab, 256 * sizeof(DevLabel));
     rc |= dalloc(c, &c->d_lab32, 256 * sizeof(LabelF));
+    rc |= dalloc(c, &c->d_zh, 256 * sizeof(DevZouHe));
     if (c->layout == 2) {
         rc |= dalloc(c, &c->d_list, c->nfluid * 4);
         rc |= dalloc(c, &c->d_rank, (N + 31) / 32 * sizeof(uint2));
@@ -350,6 +362,31 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     }
     rc |= upload(c, c->d_desc, c->links.desc.data(), N * 4);
     rc |= upload(c, c->d_lab, c->hlab.data(), 256 * sizeof(DevLabel));
+    {
+        // Zou-He boundaries (scsp.cpp:188-256); FP32 constants as in DevZouHe
+        std::vector<DevZouHe> zh(256);
+        for (int l = 0; l < 256; ++l) {
+            std::memset(&zh[l], 0, sizeof(DevZouHe));
+            const LabelInfo& li = c->labels[l];
+            if ((li.type != T_PB && li.type != T_VB) || li.dir < 1 || li.dir > 2 * d.D) continue;
+            const int sgn = d.D == 3 ? ZouHe<19>::sign[li.dir] : ZouHe<9>::sign[li.dir];
+            const double rho0 = d.density[0];
+            zh[l].dir = li.dir;
+            zh[l].value = li.value;
+            if (li.type == T_VB) {
+                const double v = sgn == 1 ? li.value : -li.value;
+                zh[l].type = ZH_VB;
+                zh[l].a = float(v);
+                zh[l].b = float(rho0 * v);
+                zh[l].c = float(1.0 / (1.0 - v));
+            } else {
+                zh[l].type = ZH_PB;
+                zh[l].b = float(rho0 - li.value);
+                zh[l].c = float(1.0 / li.value);
+            }
+        }
+        rc |= upload(c, c->d_zh, zh.data(), 256 * sizeof(DevZouHe));
+    }
     rc |= upload(c, c->d_AT, c->ops.AT.data(), Q2 * 8);
     rc |= upload(c, c->d_A, c->ops.A.data(), Q2 * 8);
     rc |= upload(c, c->d_B, c->ops.B.data(), Q2 * 8);
@@ -482,6 +519,7 @@ static ScspBuffers scsp_buffers(lbf_ctx* c, bool init)
     b.desc = c->layout == 2 ? c->d_desc_s : c->d_desc;
     b.list = c->d_list;
     b.rank = c->d_rank;
+    b.zh = c->d_zh;
     b.lab = c->d_lab;
     b.lab32 = c->d_lab32;
     b.AT = c->d_AT;

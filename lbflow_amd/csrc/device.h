@@ -13,7 +13,22 @@ enum : int32_t {
     KIND_FLUID = 0x1,    // macro + collision site (VD, GV)
     KIND_FORCED = 0x2,   // GV with a non-zero force vector
     KIND_GV = 0x4,       // GV: body force shifts the velocity
+    KIND_ZOUHE = 0x8,    // PB / VB: density and velocity from the Zou-He boundary
 };
+
+// Zou-He boundary of one label (SCSP::zouHeBoundary, scsp.cpp:188-256).
+// FP64 uses type/dir/value; FP32 works on shifted populations h = f - w rho0:
+//   VB: drho = (sum_h + b) c, u_axis = a   (a = velocity with the sign of
+//       dir, b = rho0 a, c = 1 / (1 - a));
+//   PB: s = (sum_h + b) c, u_axis = -+s, drho = -b   (b = rho0 - rho_b,
+//       c = 1 / rho_b).
+struct DevZouHe {
+    int32_t type;        // 1 VB, 2 PB
+    int32_t dir;         // axial lattice direction 1..2D
+    double value;        // VB speed / PB density
+    float a, b, c, pad;
+};
+enum : int32_t { ZH_VB = 1, ZH_PB = 2 };
 
 // One entry of the 256-entry device label table.
 struct DevLabel {
@@ -157,6 +172,7 @@ struct ScspBuffers {
     double* u;               // fields: velocity [d][site]
     double* uadj;            // adjacent velocity history
     DevStatus* status;
+    const DevZouHe* zh;      // Zou-He boundary per label
 };
 
 struct ColorArgs {

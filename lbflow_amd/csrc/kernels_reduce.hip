@@ -40,7 +40,8 @@ __global__ void __launch_bounds__(256) report_partial(ReportArgs a)
     const uint32_t stride = gridDim.x * 256u;
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < N; i += stride) {
         const uint32_t desc = a.desc[i];
-        const bool space = a.lab[desc >> DESC_LABEL_SHIFT].kind & KIND_FLUID;
+        const int kind = a.lab[desc >> DESC_LABEL_SHIFT].kind;
+        const bool space = kind & KIND_FLUID;
         int c[3];
         site_coords(a.g, i, c[2], c[1], c[0]);   // x, y, z
         // coordinate along the report axis (reference order, slowest first)
@@ -66,7 +67,9 @@ __global__ void __launch_bounds__(256) report_partial(ReportArgs a)
                 acc[S_U2] += v * v;
                 acc[S_COUNT] += 1.0;
                 if (a.adjacent) {
-                    const double adj = a.uadj[k];
+                    // PB/VB sites: the reference copies the adjacent velocity after
+                    // the boundary has set this step's value (scsp.cpp:79-83)
+                    const double adj = (kind & KIND_ZOUHE) ? v : a.uadj[k];
                     if (!isfinite(adj)) acc[S_NONFINITE] += 1.0;
                     acc[S_DA2] += (v - adj) * (v - adj);
                 }

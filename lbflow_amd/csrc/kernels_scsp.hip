@@ -13,6 +13,7 @@
 //   forceSource              solver/src/lbm/simulator.cpp:230-246
 //   initialize               solver/src/lbm/scsp.cpp:116-134
 //   validatePopulations      solver/src/lbm/simulator.cpp:327-333
+//   zouHeBoundary (PB/VB)    solver/src/lbm/scsp.cpp:188-256 (zouhe.h)
 //
 // Layouts (template parameter):
 //   DENSE_ALL    slot = site, every site streamed (solid sites are streamed
@@ -42,6 +43,7 @@
 #include "collide_f32.h"
 #include "device.h"
 #include "stream.h"
+#include "zouhe.h"
 
 namespace lbf {
 
@@ -77,7 +79,8 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
               const uint32_t* __restrict__ desc, const uint32_t* __restrict__ list,
               const uint2* __restrict__ rank, const DevLabel* __restrict__ lab,
               const double* __restrict__ AT, const double* __restrict__ B, double* __restrict__ rho_out,
-              double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status)
+              double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status,
+              const DevZouHe* __restrict__ zh)
 {
 #pragma clang fp contract(off)
     using L = Lattice<Q>;
@@ -141,27 +144,39 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
         for (int q = 1; q < Q; ++q) f[q] = ld(fin, off[q]);
     }
 
-    // calcMacroVar: rho = sum_q f_q (q ascending); u = (sum +-f) / rho0
-    double rho = f[0];
+    double rho, u[D];
+    if (kind & KIND_ZOUHE) {
+        // PB/VB: density and velocity come from the boundary, not from
+        // calcMacroVar; at t = 0 they are initialize's rho0 and 0.
+        if (INIT) {
+            rho = p.dens0;
 #pragma unroll
-    for (int q = 1; q < Q; ++q) rho += f[q];
-    if (!isfinite(rho) || rho <= 0.0) raise_status(status, 0, p.t);
-    double u[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        double s = 0;
-#pragma unroll
-        for (int q = 1; q < Q; ++q) {
-            if (L::e[q][d] == 1) s += f[q];
-            else if (L::e[q][d] == -1) s -= f[q];
+            for (int d = 0; d < D; ++d) u[d] = 0.0;
+        } else {
+            zouhe_f64<Q>(f, zh[dsc >> DESC_LABEL_SHIFT], rho, u);
         }
-        s /= p.dens0;
-        u[d] = s;
-    }
-    // bodyForce: u += (F/2)/rho0 on GV sites
-    if (kind & KIND_GV) {
+    } else {
+        // calcMacroVar: rho = sum_q f_q (q ascending); u = (sum +-f) / rho0
+        rho = f[0];
 #pragma unroll
-        for (int d = 0; d < D; ++d) u[d] += lb->half_rho[d];
+        for (int q = 1; q < Q; ++q) rho += f[q];
+        if (!isfinite(rho) || rho <= 0.0) raise_status(status, 0, p.t);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double s = 0;
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                if (L::e[q][d] == 1) s += f[q];
+                else if (L::e[q][d] == -1) s -= f[q];
+            }
+            s /= p.dens0;
+            u[d] = s;
+        }
+        // bodyForce: u += (F/2)/rho0 on GV sites
+        if (kind & KIND_GV) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) u[d] += lb->half_rho[d];
+        }
     }
     if (p.flags & F_FIELDS) {
         rho_out[i] = rho;
@@ -258,7 +273,8 @@ __global__ void __launch_bounds__(256)
 scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ fout,
               const uint32_t* __restrict__ desc, const uint32_t* __restrict__ list,
               const uint2* __restrict__ rank, const LabelF* __restrict__ lab, double* __restrict__ rho_out,
-              double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status)
+              double* __restrict__ u_out, double* __restrict__ uadj, DevStatus* __restrict__ status,
+              const DevZouHe* __restrict__ zh)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
@@ -296,9 +312,15 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
         for (int q = 1; q < Q; ++q) h[q] = ld(fin, off[q]);
     }
     // macro on shifted populations: rho = rho0 + sum h, j = sum e h
+    // (PB/VB: from the boundary; at t = 0 every h is 0, so drho = 0, u = 0)
     float u[D];
-    const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
-    if (!isfinite(drho) || p.dens0 + double(drho) <= 0.0) raise_status(status, 0, p.t);
+    float drho;
+    if (!INIT && (lb.kind & KIND_ZOUHE)) {
+        zouhe_f32<Q>(h, zh[dsc >> DESC_LABEL_SHIFT], p.rho0_32, drho, u);
+    } else {
+        drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
+        if (!isfinite(drho) || p.dens0 + double(drho) <= 0.0) raise_status(status, 0, p.t);
+    }
     if (p.flags & F_FIELDS) {
         rho_out[i] = p.dens0 + double(drho);
 #pragma unroll
@@ -331,11 +353,11 @@ static void launch_scsp_w(const ScspParams& p, const ScspBuffers& b, bool fp32, 
     if (fp32)
         hipLaunchKernelGGL((scsp_step_f32<Q, MRT, INIT, LAYOUT, WIDE>), grid, block, 0, s, p,
                            static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.desc, b.list, b.rank,
-                           b.lab32, b.rho, b.u, b.uadj, b.status);
+                           b.lab32, b.rho, b.u, b.uadj, b.status, b.zh);
     else
         hipLaunchKernelGGL((scsp_step_f64<Q, MRT, INIT, LAYOUT, WIDE>), grid, block, 0, s, p,
                            static_cast<const double*>(b.fin), static_cast<double*>(b.fout), b.desc, b.list,
-                           b.rank, b.lab, b.AT, b.B, b.rho, b.u, b.uadj, b.status);
+                           b.rank, b.lab, b.AT, b.B, b.rho, b.u, b.uadj, b.status, b.zh);
 }
 
 template <int Q, bool MRT, bool INIT, int LAYOUT>

@@ -80,6 +80,35 @@ template <> struct Lattice<19> {
     LBF_HD static constexpr bool diagonal(int q) { return q > 6; }
 };
 
+// Zou-He tables of the axial directions dir = 1..2D (gridspace.cpp:79-85
+// D2Q9, :123-137 D3Q19): rel[dir] = directions with a component along dir
+// (Qrel), perp[dir] = directions without a component along dir's axis
+// (Qprep), axis[dir] = that axis in reference component order (Qaxis),
+// sign[dir] = its sign (Qsign).  List order is the reference's: the FP64
+// boundary sums run in it.
+template <int Q> struct ZouHe;
+
+template <> struct ZouHe<9> {
+    static constexpr int NREL = 3, NPERP = 3;
+    static constexpr int rel[5][3] = {{0, 0, 0}, {1, 5, 8}, {2, 5, 6}, {3, 6, 7}, {4, 7, 8}};
+    static constexpr int perp[5][3] = {{0, 0, 0}, {0, 2, 4}, {0, 1, 3}, {0, 2, 4}, {0, 1, 3}};
+    static constexpr int axis[5] = {0, 1, 0, 1, 0};
+    static constexpr int sign[5] = {0, 1, 1, -1, -1};
+};
+
+template <> struct ZouHe<19> {
+    static constexpr int NREL = 5, NPERP = 9;
+    static constexpr int rel[7][5] = {{0, 0, 0, 0, 0},    {1, 7, 9, 11, 13},  {2, 8, 10, 12, 14},
+                                      {3, 8, 7, 15, 17},  {4, 9, 10, 16, 18}, {5, 11, 12, 15, 16},
+                                      {6, 13, 14, 17, 18}};
+    static constexpr int perp[7][9] = {{0, 0, 0, 0, 0, 0, 0, 0, 0},     {0, 3, 4, 5, 6, 15, 16, 17, 18},
+                                       {0, 3, 4, 5, 6, 15, 16, 17, 18}, {0, 1, 2, 5, 6, 11, 12, 13, 14},
+                                       {0, 1, 2, 5, 6, 11, 12, 13, 14}, {0, 1, 2, 3, 4, 7, 8, 9, 10},
+                                       {0, 1, 2, 3, 4, 7, 8, 9, 10}};
+    static constexpr int axis[7] = {0, 2, 2, 1, 1, 0, 0};
+    static constexpr int sign[7] = {0, 1, -1, 1, -1, 1, -1};
+};
+
 // Label type bits (solver/inc/file/fileprm.h:44-53).
 enum : int { T_VD = 0x1, T_GV = 0x2, T_PB = 0x4, T_VB = 0x8, T_AB = 0x10, T_BB = 0x20,
              T_HF = 0x40, T_SD = 0x80 };

@@ -84,6 +84,52 @@ def layered_cases(interpolation='arithmetic'):
     return runs
 
 
+def pressure_case(h=16, nx=65, nz=4, dp=1e-5):
+    """Pressure-driven slit between Zou-He pressure boundaries
+    (scripts/porous_media.py:257-264): inlet x = 0 at rho 1 + 3 dp along +x,
+    outlet x = nx - 1 at rho 1 along -x, walls y = 0 and y = h + 1."""
+    return spec('pressure-channel-report', nx=nx, ny=h + 2, nz=nz,
+                model='scsp', tau=(1.,), steps=6000, write=2000, report_dir=1, stop_error=1e-12,
+                label=lambda x, y, z: 255 if y in (0, h + 1) else 50 if x == 0 else 200 if x == nx - 1 else 75,
+                overrides={'latticeLen': 1e-6}, height=h, pressure_drop=dp,
+                indices=[f'50 "PB" "{1 + 3 * dp:.17g}" "1" "0"', '200 "PB" "1" "2" "0"',
+                         '75 "VD" "N/A" "N/A" "0"', '255 "SD" "90" "N/A" "N/A"'])
+
+
+def zouhe_cases(nx=20, ny=10, nz=6, u0=1e-3, rho_in=1.0003):
+    """Small Zou-He boundary cases, every axial direction and both signs:
+    'zh-x': velocity inlet (VB, +x) at x = 0, pressure outlet (PB, -x) at
+    x = nx - 1, walls at y = 0, ny - 1 and a 2-voxel obstacle block;
+    'zh-t': pressure inlet along +z (2D: +y) at the first plane, velocity
+    boundary along -z (2D: -y) at the last plane, walls at x = 0, nx - 1."""
+    cases = []
+    for dims in (3, 2):
+        zdim = nz if dims == 3 else 1
+        for eq in ('SRT', 'MRT'):
+            def lab_x(x, y, z):
+                if y in (0, ny - 1) or (nx // 2 <= x < nx // 2 + 2 and ny // 2 - 1 <= y < ny // 2 + 1):
+                    return 255
+                return 50 if x == 0 else 200 if x == nx - 1 else 75
+
+            cases.append(spec(f'zh-x-{dims}d-{eq}', nx=nx, ny=ny, nz=zdim, dimensions=dims, model='scsp',
+                              eq=eq, tau=(0.8,), steps=300, write=100, label=lab_x,
+                              indices=[f'50 "VB" "{u0:.17g}" "1" "0"', '200 "PB" "1" "2" "0"',
+                                       '75 "VD" "N/A" "N/A" "0"', '255 "SD" "90" "N/A" "N/A"']))
+            last = (nz if dims == 3 else ny) - 1
+            din, dout = (5, 6) if dims == 3 else (2, 4)
+
+            def lab_t(x, y, z, dims=dims, last=last):
+                c = z if dims == 3 else y
+                return 255 if x in (0, nx - 1) else 50 if c == 0 else 200 if c == last else 75
+
+            cases.append(spec(f'zh-t-{dims}d-{eq}', nx=nx, ny=ny, nz=zdim, dimensions=dims, model='scsp',
+                              eq=eq, tau=(0.8,), steps=300, write=100, label=lab_t,
+                              indices=[f'50 "PB" "{rho_in:.17g}" "{din}" "0"',
+                                       f'200 "VB" "{u0:.17g}" "{dout}" "0"',
+                                       '75 "VD" "N/A" "N/A" "0"', '255 "BB" "N/A" "N/A" "N/A"']))
+    return cases
+
+
 def bundle_cases(scales=(1, 2, 3), dimensions=(2, 3)):
     """Two isolated slit pores (scripts/porous_media.py:67-100)."""
     cases = []

@@ -33,6 +33,22 @@ static const int RELD19[3][10] = {{5,11,12,15,16,6,13,14,17,18},
                                   {3,8,7,15,17,4,9,10,16,18},
                                   {1,7,9,11,13,2,8,10,12,14}};
 
+/* Qrel / Qprep / Qaxis / Qsign of the axial directions, in the reference's
+ * list order (gridspace.cpp:79-85 D2Q9, :123-137 D3Q19); used by the
+ * Zou-He boundaries. */
+static const int REL9[5][3] = {{0,0,0},{1,5,8},{2,5,6},{3,6,7},{4,7,8}};
+static const int PERP9[5][3] = {{0,0,0},{0,2,4},{0,1,3},{0,2,4},{0,1,3}};
+static const int AXIS9[5] = {0,1,0,1,0};
+static const int SIGN9[5] = {0,1,1,-1,-1};
+static const int REL19[7][5] = {{0,0,0,0,0},
+    {1,7,9,11,13},{2,8,10,12,14},{3,8,7,15,17},{4,9,10,16,18},{5,11,12,15,16},{6,13,14,17,18}};
+static const int PERP19[7][9] = {{0,0,0,0,0,0,0,0,0},
+    {0,3,4,5,6,15,16,17,18},{0,3,4,5,6,15,16,17,18},
+    {0,1,2,5,6,11,12,13,14},{0,1,2,5,6,11,12,13,14},
+    {0,1,2,3,4,7,8,9,10},{0,1,2,3,4,7,8,9,10}};
+static const int AXIS19[7] = {0,2,2,1,1,0,0};
+static const int SIGN19[7] = {0,1,-1,1,-1,1,-1};
+
 /* simulator.cpp:162-197 */
 static const double M9[81] = {
      1, 1, 1, 1, 1, 1, 1, 1, 1,
@@ -353,7 +369,7 @@ orc_sim* orc_new(const orc_desc* desc, const orc_label* labels, const uint8_t* g
         }
     for (int l = 0; l < 256; ++l) {
         int t = s->lab[l].type;
-        if (t == ORC_PB || t == ORC_VB || t == ORC_AB || t == ORC_HF) {
+        if (t == ORC_AB || t == ORC_HF || ((t == ORC_PB || t == ORC_VB) && desc->model == 1)) {
             snprintf(err, errlen, "oracle: label type %d is outside the restated scope", t);
             free(s->raw); free(s); return NULL;
         }
@@ -495,6 +511,75 @@ static void scsp_body_force(orc_sim* s)
         for (int d = 0; d < s->D; ++d)
             for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k)
                 V(s, d, s->bylabel[k]) += fi[d] / rhoRef;
+    }
+}
+
+/* Zou-He pressure (PB) and velocity (VB) boundaries, SCSP::zouHeBoundary
+ * (scsp.cpp:188-256): density/velocity of the boundary site from the known
+ * populations, then the unknown populations along the boundary direction.
+ * wiT = w * 6 / latticeSpeed (simulator.cpp:28) with latticeSpeed 1. */
+static int zh_rel(const orc_sim* s, int dir, int k) { return s->Q == 9 ? REL9[dir][k] : REL19[dir][k]; }
+static int zh_perp(const orc_sim* s, int dir, int k) { return s->Q == 9 ? PERP9[dir][k] : PERP19[dir][k]; }
+
+static void scsp_zouhe(orc_sim* s)
+{
+    const int nrel = s->Q == 9 ? 3 : 5, nperp = s->Q == 9 ? 3 : 9;
+    for (int pass = 0; pass < 2; ++pass)      /* VB labels first, then PB */
+        for (int l = 0; l < 256; ++l) {
+            if (!label_in(s, l, pass == 0 ? ORC_VB : ORC_PB)) continue;
+            const int dir = s->lab[l].dir;
+            const int axis = s->Q == 9 ? AXIS9[dir] : AXIS19[dir];
+            const int sign = s->Q == 9 ? SIGN9[dir] : SIGN19[dir];
+            const int opdir = s->op[dir];
+            double velocity = 0.0, vel_1 = 0.0;
+            const double density = s->lab[l].value;
+            if (pass == 0) {
+                velocity = sign == 1 ? s->lab[l].value : -s->lab[l].value;
+                vel_1 = 1.0 / (1.0 - velocity);
+            }
+            for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+                const uint64_t i = s->bylabel[k];
+                double sumF = 0;
+                for (int r = 0; r < nrel; ++r) { int q = zh_rel(s, opdir, r); sumF += F(s, 0, q, i) + F(s, 0, q, i); }
+                for (int r = 0; r < nperp; ++r) sumF += F(s, 0, zh_perp(s, dir, r), i);
+                if (pass == 0) {
+                    for (int d = 0; d < s->D; ++d)
+                        if (d != axis) V(s, d, i) = 0;
+                        else { V(s, d, i) = velocity; s->dens[0][i] = sumF * vel_1; }
+                } else {
+                    s->dens[0][i] = density;
+                    for (int d = 0; d < s->D; ++d)
+                        if (d != axis) V(s, d, i) = 0;
+                        else {
+                            sumF /= density; sumF--;
+                            V(s, d, i) = sign == 1 ? -sumF : sumF;
+                        }
+                }
+            }
+        }
+    for (int l = 0; l < 256; ++l) {
+        if (!label_in(s, l, ORC_PB | ORC_VB)) continue;
+        const int dir = s->lab[l].dir;
+        const int axis = s->Q == 9 ? AXIS9[dir] : AXIS19[dir];
+        const int sign = s->Q == 9 ? SIGN9[dir] : SIGN19[dir];
+        for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+            const uint64_t i = s->bylabel[k];
+            for (int r = 0; r < nrel; ++r) {
+                const int q = zh_rel(s, dir, r);
+                F(s, 0, q, i) = F(s, 0, s->op[q], i);
+                F(s, 0, q, i) += sign * (s->wi[q] * 6.0 / 1.0) * s->dens[0][i] * V(s, axis, i);
+                double sumF = 0;
+                for (int n = 0; n < nperp; ++n) {
+                    const int qn = zh_perp(s, dir, n);
+                    for (int d = 0; d < s->D; ++d) {
+                        if (s->qi[qn][d] == 0 || s->qi[q][d] == 0) continue;
+                        else if (s->qi[qn][d] == s->qi[q][d]) sumF += F(s, 0, qn, i);
+                        else sumF -= F(s, 0, qn, i);
+                    }
+                }
+                F(s, 0, q, i) -= sumF * 0.5;
+            }
+        }
     }
 }
 
@@ -842,6 +927,7 @@ int orc_step(orc_sim* s)
     s->t++;
     for (int ph = 0; ph < s->NP; ++ph) stream_and_bounce(s, ph);
     if (s->d.model == 0) {
+        scsp_zouhe(s);
         if (scsp_macro(s)) return 1;
         scsp_body_force(s);
         return scsp_collision(s);

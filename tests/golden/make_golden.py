@@ -48,6 +48,10 @@ def main():
                                'force': m['force'], 'tau': m['tau'],
                                'geometry_sha256': m['geometry_sha256']}
     out['rock_scsp'] = rock
+    pr = porous['pressure_report']
+    out['pressure_report'] = {k: pr[k] for k in ('name', 'prescribed_pressure_drop', 'report_slice_pressure_drop',
+                                                 'expected_report_slice_pressure', 'reported_K_um2',
+                                                 'expected_K_um2')}
     (HERE / 'reference_records.json').write_text(json.dumps(out, indent=1) + '\n')
 
     raw = ref / 'examples/bentheimer48/single-phase.raw'

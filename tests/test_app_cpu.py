@@ -90,7 +90,7 @@ def test_missing_key(tmp_path):
 def test_unsupported_boundary_type(tmp_path):
     spec = dict(base(), indices=['75 "PB" "1.0" "1" "1"'])
     code, out = run_case(tmp_path, spec)
-    assert code != 0 and 'PB/VB/AB/HF' in out
+    assert code != 0 and 'PB/VB in the colour model' in out
 
 
 def test_library_exports_every_declared_symbol():

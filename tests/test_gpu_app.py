@@ -186,3 +186,18 @@ def test_fp32_sparse_engine_option(tmp_path):
     want = np.array(_j.loads((Path(__file__).parent / 'golden' / 'reference_records.json').read_text())
                     ['channel'][spec['name'] + '-1rank']['measured_profile'])
     assert np.max(np.abs(prof - want) / np.abs(want)) < 1e-5
+
+
+def test_pressure_slit_report_line(tmp_path, records):
+    """scripts/porous_media.py:257-279 through the drop-in executable: the last
+    ' - dP = ' and ' - K = ' report lines carry the values the reference printed."""
+    spec = H.pressure_case()
+    code, out, timing = run(spec, tmp_path)
+    assert code == 0, out[-2000:]
+    dP = re.findall(r' - dP = ([\deE+.-]+)', out)[-1]
+    K = re.findall(r' - K = ([\deE+.-]+)', out)[-1]
+    rec = records['pressure_report']
+    say('dP', dP, 'K', K)
+    assert float(dP) == float(f"{rec['report_slice_pressure_drop']:.6e}")
+    assert float(K) == float(f"{rec['reported_K_um2']:.6e}")
+    assert timing['completed_steps'] == 6000

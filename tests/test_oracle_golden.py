@@ -71,6 +71,32 @@ def test_bundle_history(spec, records):
         assert float(rho.sum()) == h['total_mass']
 
 
+def pressure_report(sums, case, tau, first, last, lattice_len=1e-6):
+    """dP and K of SCSP::report (scsp.cpp:325-352, simulator.cpp:151-160) from the report sums."""
+    vel, _, _, s0, _, s1, _, c0, c1 = sums
+    dP = (s0 / c0 - s1 / c1) / 3.0
+    Q = vel / float(last - first + 1)
+    nz, ny, nx = case['size']
+    kpar = (last - first) * ((tau - 0.5) * 1 / 3) / (ny * nz) * lattice_len ** 2 * 1.0e12
+    return dP, kpar * Q / dP
+
+
+def test_pressure_boundaries_report(records):
+    """Zou-He pressure inlet/outlet slit (porous_media.py:257-279): the report line the
+    reference recorded (dP and K printed with 7 significant digits) is reproduced, and
+    both lie within 2e-4 of the analytic slit."""
+    spec = H.pressure_case()
+    o, case = oracle_run(spec, spec['steps'])
+    nx = spec['size'][0]
+    dP, K = pressure_report(o.report_sums(2, 1, nx - 2), case, 1.0, 1, nx - 2)
+    rec = records['pressure_report']
+    assert f'{dP:.6e}' == f"{rec['report_slice_pressure_drop']:.6e}"
+    assert f'{K:.6e}' == f"{rec['reported_K_um2']:.6e}"
+    h, dp = spec['height'], spec['pressure_drop']
+    assert abs(dP / (dp * (nx - 3) / (nx - 1)) - 1) < 2e-4
+    assert abs(K / ((h ** 3 / 12 + h / 24) / (h + 2)) - 1) < 2e-4
+
+
 def test_c1_checkpoint_digest():
     """8x10x4 SRT channel, 20000 steps: checkpoint bytes equal the reference's (solid entries included)."""
     s = c1_spec()

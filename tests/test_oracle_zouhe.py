@@ -1,0 +1,69 @@
+"""Pin the oracle's Zou-He boundaries (SCSP::zouHeBoundary, scsp.cpp:188-256)
+where no recorded reference value exists (CPU only).
+
+The pressure boundary is pinned by the reference's recorded slit report
+(tests/test_oracle_golden.py::test_pressure_boundaries_report).  Beyond that
+the Zou-He construction fixes, at every boundary site and step:
+  - PB: the populations sum to the prescribed density (the unknowns are
+    built so that sum f = rho_b, and the collision conserves mass);
+  - VB on a +axis face: the populations sum to the boundary density and the
+    velocity is exactly the prescribed one, transverse components 0;
+  - VB on a -axis face: the velocity is exactly minus the prescribed speed
+    (the reference's density there uses 1 / (1 - velocity) with the signed
+    velocity, so the mass identity holds only on +axis faces);
+and at steady state the volume flux through every interior cross-section
+equals the inlet's to within the inlet density excess (2e-3 here).
+"""
+import numpy as np
+import pytest
+
+from lbflow_amd import harness as H
+from tests.helpers import oracle_run
+
+CASES = H.zouhe_cases()
+
+
+def grids(o, case):
+    r0, _, u = o.fields()
+    sz = case['size']
+    Q, D = case['Q'], case['D']
+    return r0.reshape(sz), u.reshape((D,) + tuple(sz)), o.populations().reshape((Q,) + tuple(sz))
+
+
+@pytest.mark.parametrize('spec', CASES, ids=lambda s: s['name'])
+def test_zouhe_site_identities(spec):
+    o, case = oracle_run(spec, 200)
+    g = H.geometry(spec).reshape(case['size'])
+    rho, u, f = grids(o, case)
+    labels = case['labels']
+    for lab, (typ, value, direction, _) in labels.items():
+        if typ not in ('PB', 'VB'):
+            continue
+        m = g == lab
+        assert m.any()
+        axis = {1: -1, 2: -1, 3: -2, 4: -2, 5: -3, 6: -3}[direction] if case['D'] == 3 else \
+            {1: -1, 3: -1, 2: -2, 4: -2}[direction]
+        sign = 1 if (direction % 2 == 1 if case['D'] == 3 else direction in (1, 2)) else -1
+        ax = case['D'] + axis          # reference component index of the boundary axis
+        total = f.sum(axis=0)[m]
+        if typ == 'PB':
+            assert np.max(np.abs(total - value)) < 1e-14
+            assert np.all(rho[m] == value)
+        else:
+            assert np.all(u[ax][m] == sign * value)
+            if sign == 1:
+                assert np.max(np.abs(total - rho[m])) < 1e-14
+        for d in range(case['D']):
+            if d != ax:
+                assert np.all(u[d][m] == 0.0)
+
+
+def test_zouhe_flux_continuity():
+    spec = [s for s in CASES if s['name'] == 'zh-x-3d-SRT'][0]
+    o, case = oracle_run(spec, 3000)
+    g = H.geometry(spec).reshape(case['size'])
+    _, u, _ = grids(o, case)
+    flux = u[2].sum(axis=(0, 1))                       # sum of u_x per x-slice
+    inlet = np.count_nonzero(g[..., 0] == 50) * 1e-3
+    assert abs(flux[0] / inlet - 1) < 1e-15
+    assert np.max(np.abs(flux[1:-1] / inlet - 1)) < 2e-3
