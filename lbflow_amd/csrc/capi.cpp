@@ -217,11 +217,10 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         for (int l = 0; l < 256; ++l) {
             if (!used[l] || !labels[l].type) continue;
             const int t = labels[l].type;
-            if (t == LBF_AB || t == LBF_HF || ((t == LBF_PB || t == LBF_VB) && NP == 2)) {
+            if (t == LBF_AB || t == LBF_HF) {
                 lbf_destroy(c);
                 return fail(nullptr, "lbf_create: label " + std::to_string(l) +
-                                         " uses a boundary type this engine does not implement "
-                                         "(AB, HF; PB/VB in the colour model)");
+                                         " uses a boundary type (AB/HF) this engine does not implement");
             }
             if ((t == LBF_PB || t == LBF_VB) && (labels[l].dir < 1 || labels[l].dir > 2 * d.D)) {
                 lbf_destroy(c);
@@ -231,7 +230,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
                 lbf_destroy(c);
                 return fail(nullptr, "Pressure-boundary density must be positive.");
             }
-            if ((t == LBF_VD || t == LBF_GV) && labels[l].phase >= NP) {
+            if ((t == LBF_VD || t == LBF_GV || t == LBF_PB || t == LBF_VB) && labels[l].phase >= NP) {
                 lbf_destroy(c);
                 return fail(nullptr, "Boundary phase is outside the configured fluid model.");
             }
@@ -548,6 +547,7 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
     a.rho1 = c->d_rho1;
     a.u = c->d_u;
     a.status = c->d_status;
+    a.zh = c->d_zh;
     a.flags = flags;
     a.mrt = c->d.equil == LBF_MRT;
     a.uniform = c->d.tau[0] == c->d.tau[1];
@@ -610,6 +610,8 @@ static int run_step(lbf_ctx* c, bool init, uint32_t flags)
         a.t = init ? 0 : c->t + 1;
         a.uadj = c->d_uadj + (a.t & 1) * uint64_t(c->D) * c->N;
         a.chiadj = c->d_chiadj + (a.t & 1) * c->N;
+        a.uadj_prev = c->d_uadj + ((a.t + 1) & 1) * uint64_t(c->D) * c->N;
+        a.chiadj_prev = c->d_chiadj + ((a.t + 1) & 1) * c->N;
         if (c->elem == 4) {
             a.fin32 = static_cast<const float*>(c->f[c->cur]);
             a.fout32 = static_cast<float*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);

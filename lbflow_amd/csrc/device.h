@@ -192,6 +192,11 @@ struct ColorArgs {
     double* u;
     double* uadj;
     double* chiadj;
+    // adjacent history read by this step's report (written by the previous
+    // step); PB/VB sites put their boundary values there on field steps
+    double* uadj_prev;
+    double* chiadj_prev;
+    const DevZouHe* zh;      // Zou-He boundary per label
     DevStatus* status;
     unsigned long long t;
     uint32_t flags;

@@ -19,10 +19,12 @@
 //   collision               solver/src/lbm/colorfluid.cpp:292-433
 //   colorGradient (legacy)  solver/src/lbm/colorfluid.cpp:435-570
 //   RotateTx3D etc.         solver/src/algorithm/fn.cpp:118-164
+//   zouHeBoundary (PB/VB)   solver/src/lbm/colorfluid.cpp:638-734 (zouhe.h)
 //
 // FP64, reference operation order (built with -ffp-contract=off); zero
 // lattice coefficients are skipped exactly as in kernels_scsp.hip.
 #include "device.h"
+#include "zouhe.h"
 
 namespace lbf {
 
@@ -32,10 +34,13 @@ constexpr double SQRT2 = 1.4142135623730951;  // sqrt(2.0), Qnorm of diagonal li
 namespace {
 
 // Pull (or initialise) both phases; returns the total fT_q = f0_q + f1_q and
-// the phase densities summed in direction order.
+// the phase densities summed in direction order.  PB/VB sites apply the
+// Zou-He boundary to the pulled populations first and return its density and
+// velocity in rzh / uzh.
 template <int Q, bool INIT>
 __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint32_t desc,
-                                           const DevLabel& lab, double* fT, double& r0, double& r1)
+                                           const DevLabel& lab, double* fT, double& r0, double& r1,
+                                           double& rzh, double* uzh)
 {
 #pragma clang fp contract(off)
     using L = Lattice<Q>;
@@ -58,6 +63,27 @@ __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint3
     int x, y, z;
     site_coords(a.g, i, x, y, z);
     const uint32_t links = desc & DESC_LINK_MASK;
+    if (lab.kind & KIND_ZOUHE) {
+        double f0[Q], f1[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            uint64_t src;
+            int qs = q;
+            if (q == 0) src = i;
+            else if ((links >> L::op[q]) & 1u) { src = i; qs = L::op[q]; }
+            else src = pull_source<Q>(a.g, x, y, z, q);
+            f0[q] = a.fin[uint64_t(qs) * N + src];
+            f1[q] = a.fin[(uint64_t(Q) + qs) * N + src];
+        }
+        zouhe_color_f64<Q>(f0, f1, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, rzh, uzh);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            if (q == 0) { r0 = f0[q]; r1 = f1[q]; }
+            else { r0 += f0[q]; r1 += f1[q]; }
+            fT[q] = f0[q] + f1[q];
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         uint64_t src;
@@ -84,8 +110,8 @@ __global__ void __launch_bounds__(256) color_density(ColorArgs a)
     const uint32_t desc = a.desc[i];
     const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
     if (!(lab.kind & KIND_FLUID)) return;
-    double fT[Q], r0, r1;
-    load_color<Q, INIT>(a, i, desc, lab, fT, r0, r1);
+    double fT[Q], r0, r1, rzh, uzh[Lattice<Q>::D];
+    load_color<Q, INIT>(a, i, desc, lab, fT, r0, r1, rzh, uzh);
     const double rt = r0 + r1;
     a.densN[i] = (r0 - r1) / rt;
 }
@@ -121,8 +147,8 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
         return;
     }
 
-    double fT[Q], r0, r1;
-    load_color<Q, INIT>(a, i, desc, lab, fT, r0, r1);
+    double fT[Q], r0, r1, rzh = 0.0, uzh[D];
+    load_color<Q, INIT>(a, i, desc, lab, fT, r0, r1, rzh, uzh);
 
     // calcMacroVar
     const double rt = r0 + r1;
@@ -159,10 +185,20 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
 #pragma unroll
         for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = u[d];
     }
+    // adjacent report history (colorfluid.cpp:166-171): copied after the
+    // boundaries, so a PB/VB site carries its Zou-He velocity and
+    // (rho0 - rho1 of the previous step) / (Zou-He density).  The previous
+    // step keeps the numerator; this step's field pass completes it.
     if (a.flags & 2u) {
 #pragma unroll
         for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = u[d];
-        a.chiadj[i] = (r0 - r1) / rt;
+        a.chiadj[i] = (kind & KIND_ZOUHE) ? r0 - r1 : (r0 - r1) / rt;
+    }
+    if (!INIT && (a.flags & 1u) && (kind & KIND_ZOUHE)) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.uadj_prev[uint64_t(d) * N + i] = uzh[d];
+        const double num = a.chiadj_prev[i];
+        a.chiadj_prev[i] = rzh > 0.0 ? num / rzh : 0.0;
     }
 
     // collision of the total population

@@ -25,6 +25,7 @@
 #include "collide_f32.h"
 #include "device.h"
 #include "stream.h"
+#include "zouhe.h"
 
 namespace lbf {
 
@@ -44,9 +45,12 @@ __device__ __forceinline__ bool resolve32(const ColorArgs& a, uint32_t& site, ui
     return true;
 }
 
+// PB/VB sites apply the Zou-He boundary to the pulled populations and return
+// its density deviation and velocity in dzh / uzh.
 template <int Q, bool INIT, int LAYOUT, typename O>
 __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, uint32_t slot, uint32_t desc,
-                                             const DevLabel& lab, float* h, float* f0, float& r0, float& dr)
+                                             const DevLabel& lab, float* h, float* f0, float& r0, float& dr,
+                                             float& dzh, float* uzh)
 {
     using L = Lattice<Q>;
     r0 = 0.0f;
@@ -76,6 +80,16 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
         r0 += f0[q];
         dr += h[q];
     }
+    if (lab.kind & KIND_ZOUHE) {
+        zouhe_color_f32<Q>(h, f0, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, float(a.rho_ref), a.dist0, dzh, uzh);
+        r0 = 0.0f;
+        dr = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            r0 += f0[q];
+            dr += h[q];
+        }
+    }
 }
 
 }  // namespace
@@ -88,8 +102,8 @@ __global__ void __launch_bounds__(256) color_density_f32(ColorArgs a)
     const uint32_t desc = a.desc[slot];
     const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
     if (!(lab.kind & KIND_FLUID)) return;
-    float h[Q], f0[Q], r0, dr;
-    load_color32<Q, INIT, LAYOUT, O>(a, site, slot, desc, lab, h, f0, r0, dr);
+    float h[Q], f0[Q], r0, dr, dzh, uzh[Lattice<Q>::D];
+    load_color32<Q, INIT, LAYOUT, O>(a, site, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
     const float rho = float(a.rho_ref) + dr;
     a.densN32[site] = (2.0f * r0 - rho) / rho;   // (rho0 - rho1) / rho
 }
@@ -109,8 +123,8 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     const int kind = lab.kind;
     if (!(kind & KIND_FLUID)) return;
 
-    float h[Q], f0[Q], r0, dr;
-    load_color32<Q, INIT, LAYOUT, O>(a, i, slot, desc, lab, h, f0, r0, dr);
+    float h[Q], f0[Q], r0, dr, dzh = 0.0f, uzh[D];
+    load_color32<Q, INIT, LAYOUT, O>(a, i, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
 
     // calcMacroVar (colorfluid.cpp:238-290)
     const float rt = float(a.rho_ref) + dr;
@@ -144,10 +158,18 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
 #pragma unroll
         for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = double(u[d]);
     }
+    // adjacent report history; PB/VB sites as in kernels_color.hip
     if (a.flags & 2u) {
 #pragma unroll
         for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = double(u[d]);
-        a.chiadj[i] = double(dn);
+        a.chiadj[i] = (kind & KIND_ZOUHE) ? double(r0 - r1) : double(dn);
+    }
+    if (!INIT && (a.flags & 1u) && (kind & KIND_ZOUHE)) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.uadj_prev[uint64_t(d) * N + i] = double(uzh[d]);
+        const double rzh = a.rho_ref + double(dzh);
+        const double num = a.chiadj_prev[i];
+        a.chiadj_prev[i] = rzh > 0.0 ? num / rzh : 0.0;
     }
 
     // collision of the total population (colorfluid.cpp:292-386), on h

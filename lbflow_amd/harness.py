@@ -113,7 +113,7 @@ def zouhe_cases(nx=20, ny=10, nz=6, u0=1e-3, rho_in=1.0003):
 
             cases.append(spec(f'zh-x-{dims}d-{eq}', nx=nx, ny=ny, nz=zdim, dimensions=dims, model='scsp',
                               eq=eq, tau=(0.8,), steps=300, write=100, label=lab_x,
-                              indices=[f'50 "VB" "{u0:.17g}" "1" "0"', '200 "PB" "1" "2" "0"',
+                              indices=[f'50 "VB" "{u0:.17g}" "1" "0"', f'200 "PB" "1" "{3 if dims == 2 else 2}" "0"',
                                        '75 "VD" "N/A" "N/A" "0"', '255 "SD" "90" "N/A" "N/A"']))
             last = (nz if dims == 3 else ny) - 1
             din, dout = (5, 6) if dims == 3 else (2, 4)
@@ -127,6 +127,71 @@ def zouhe_cases(nx=20, ny=10, nz=6, u0=1e-3, rho_in=1.0003):
                               indices=[f'50 "PB" "{rho_in:.17g}" "{din}" "0"',
                                        f'200 "VB" "{u0:.17g}" "{dout}" "0"',
                                        '75 "VD" "N/A" "N/A" "0"', '255 "BB" "N/A" "N/A" "N/A"']))
+    return cases
+
+
+def zouhe_color_cases(nx=20, ny=10, nz=6, u0=1e-3, rho_in=1.0003):
+    """Colour-model Zou-He cases: phase 0 enters through a velocity inlet (VB,
+    +x) at x = 0 into a channel held by phase 1 with a pressure outlet (PB,
+    -x, phase 1) at x = nx - 1; and a pressure inlet (+z / 2D +y, phase 0)
+    against a velocity boundary (-z / 2D -y, phase 1).  60-degree walls."""
+    cases = []
+    for dims in (3, 2):
+        zdim = nz if dims == 3 else 1
+        for eq in ('SRT', 'MRT'):
+            def lab_x(x, y, z):
+                if y in (0, ny - 1):
+                    return 255
+                return 50 if x == 0 else 200 if x == nx - 1 else 0 if x < 5 else 100
+
+            cases.append(spec(f'zhc-x-{dims}d-{eq}', nx=nx, ny=ny, nz=zdim, dimensions=dims, model='color',
+                              eq=eq, tau=(1.0, 0.8), steps=300, write=100, sigma=0.005, label=lab_x,
+                              indices=[f'50 "VB" "{u0:.17g}" "1" "0"', f'200 "PB" "1" "{3 if dims == 2 else 2}" "1"',
+                                       '0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"',
+                                       '255 "SD" "60" "N/A" "N/A"']))
+            last = (nz if dims == 3 else ny) - 1
+            din, dout = (5, 6) if dims == 3 else (2, 4)
+
+            def lab_t(x, y, z, dims=dims, last=last):
+                c = z if dims == 3 else y
+                return (255 if x in (0, nx - 1) else 50 if c == 0 else 200 if c == last else
+                        0 if c < 3 else 100)
+
+            cases.append(spec(f'zhc-t-{dims}d-{eq}', nx=nx, ny=ny, nz=zdim, dimensions=dims, model='color',
+                              eq=eq, tau=(1.0, 1.0), steps=300, write=100, sigma=0.005, label=lab_t,
+                              indices=[f'50 "PB" "{rho_in:.17g}" "{din}" "0"',
+                                       f'200 "VB" "{u0:.17g}" "{dout}" "1"',
+                                       '0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"',
+                                       '255 "SD" "60" "N/A" "N/A"']))
+    return cases
+
+
+def entry_cases(scales=(1,), dimensions=(2,)):
+    """Capillary entry between Zou-He pressure boundaries in the colour model
+    (scripts/porous_media.py:153-182, legacy wall gradient): phase 0 enters a
+    60-degree slit (2D) or square tube (3D) held by phase 1, at 0.5 and 1.5
+    times the sharp entry pressure."""
+    cases = []
+    for ndim in dimensions:
+        for scale in scales:
+            h, nx = 16 * scale, 64 * scale + 1
+            sigma = .005 / scale
+            pc = (2 if ndim == 2 else 4) * sigma * math.cos(math.radians(60)) / h
+            for ratio in (.5, 1.5):
+                dp = ratio * pc
+
+                def label(x, y, z, h=h, nx=nx, ndim=ndim):
+                    return (255 if y in (0, h + 1) or (ndim == 3 and z in (0, h + 1)) else 50 if x == 0 else
+                            200 if x == nx - 1 else 0 if x < h else 100)
+
+                cases.append(spec(
+                    'entry-' + ('d3-' if ndim == 3 else '') + f's{scale}-legacy-ratio{ratio}',
+                    nx=nx, ny=h + 2, nz=1 if ndim == 2 else h + 2, dimensions=ndim,
+                    steps=8000 * scale ** 2, write=2000 * scale ** 2, tau=(1., 1.), sigma=sigma,
+                    height=h, pressure_drop=dp, label=label,
+                    indices=[f'50 "PB" "{1 + 3 * dp:.17g}" "1" "0"',
+                             f'200 "PB" "1" "{3 if ndim == 2 else 2}" "1"', '0 "VD" "N/A" "N/A" "0"',
+                             '100 "VD" "N/A" "N/A" "1"', '255 "SD" "60" "N/A" "N/A"']))
     return cases
 
 

@@ -196,6 +196,7 @@ struct orc_sim {
     double* tib; double* tibInv;   /* per wall index D*D */
     double* f[2]; double* tmp[2];  /* [q][N] per phase */
     double* dens[2]; double* densT; double* densN; double* tau; double* v;
+    double* adjV; double* adjChi;  /* adjacent report history of the last step */
     double A[MAXQ * MAXQ], AT[MAXQ * MAXQ], B[MAXQ * MAXQ];
     double M[MAXQ * MAXQ], MInv[MAXQ * MAXQ];
     double ai[2][MAXQ], bi[MAXQ];
@@ -369,7 +370,7 @@ orc_sim* orc_new(const orc_desc* desc, const orc_label* labels, const uint8_t* g
         }
     for (int l = 0; l < 256; ++l) {
         int t = s->lab[l].type;
-        if (t == ORC_AB || t == ORC_HF || ((t == ORC_PB || t == ORC_VB) && desc->model == 1)) {
+        if (t == ORC_AB || t == ORC_HF) {
             snprintf(err, errlen, "oracle: label type %d is outside the restated scope", t);
             free(s->raw); free(s); return NULL;
         }
@@ -396,6 +397,8 @@ orc_sim* orc_new(const orc_desc* desc, const orc_label* labels, const uint8_t* g
     }
     s->v = calloc(s->N * s->D, sizeof(double));
     s->densT = calloc(s->N, sizeof(double));
+    s->adjV = calloc(s->N * s->D, sizeof(double));
+    s->adjChi = calloc(s->N, sizeof(double));
     s->tau = calloc(s->N, sizeof(double));
     s->densN = malloc(sizeof(double) * s->N);
     for (uint64_t i = 0; i < s->N; ++i) s->densN[i] = SOLID_DENS_N;
@@ -445,7 +448,7 @@ void orc_free(orc_sim* s)
     for (int q = 0; q < MAXQ; ++q) free(s->ibs[q]);
     free(s->ibNorm); free(s->ibSin); free(s->ibCos); free(s->tib); free(s->tibInv);
     for (int ph = 0; ph < 2; ++ph) { free(s->f[ph]); free(s->tmp[ph]); free(s->dens[ph]); }
-    free(s->v); free(s->densT); free(s->densN); free(s->tau);
+    free(s->v); free(s->densT); free(s->densN); free(s->tau); free(s->adjV); free(s->adjChi);
     free(s);
 }
 
@@ -669,6 +672,92 @@ static void color_initialize(orc_sim* s)
             s->densN[i] = phs == 0 ? 1 : -1;
         }
     }
+}
+
+/* ColorFluid::zouHeBoundary (colorfluid.cpp:638-734): the Zou-He boundary on
+ * the total population fT = f0 + f1; the unknowns go to the label's phase,
+ * the other phase gets 0.  densT and v are set here but calcMacroVar
+ * recomputes them at every PB/VB site right after; only the unknown update
+ * and the adjacent report history (colorfluid.cpp:166-171) see them. */
+static void color_zouhe(orc_sim* s)
+{
+    const int nrel = s->Q == 9 ? 3 : 5, nperp = s->Q == 9 ? 3 : 9;
+    for (int pass = 0; pass < 2; ++pass)      /* VB labels first, then PB */
+        for (int l = 0; l < 256; ++l) {
+            if (!label_in(s, l, pass == 0 ? ORC_VB : ORC_PB)) continue;
+            const int dir = s->lab[l].dir;
+            const int axis = s->Q == 9 ? AXIS9[dir] : AXIS19[dir];
+            const int sign = s->Q == 9 ? SIGN9[dir] : SIGN19[dir];
+            const int opdir = s->op[dir];
+            double velocity = 0.0, vel_1 = 0.0;
+            const double density = s->lab[l].value;
+            if (pass == 0) {
+                velocity = sign == 1 ? s->lab[l].value : -s->lab[l].value;
+                vel_1 = 1.0 / (1.0 - velocity);
+            }
+            for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+                const uint64_t i = s->bylabel[k];
+                double sumF = 0;
+                for (int r = 0; r < nrel; ++r) {
+                    const int q = zh_rel(s, opdir, r);
+                    sumF += F(s, 0, q, i) + F(s, 0, q, i) + F(s, 1, q, i) + F(s, 1, q, i);
+                }
+                for (int r = 0; r < nperp; ++r) {
+                    const int q = zh_perp(s, dir, r);
+                    sumF += F(s, 0, q, i) + F(s, 1, q, i);
+                }
+                if (pass == 0) {
+                    for (int d = 0; d < s->D; ++d)
+                        if (d != axis) V(s, d, i) = 0;
+                        else { V(s, d, i) = velocity; s->densT[i] = sumF * vel_1; }
+                } else {
+                    s->densT[i] = density;
+                    for (int d = 0; d < s->D; ++d)
+                        if (d != axis) V(s, d, i) = 0;
+                        else {
+                            sumF /= density; sumF--;
+                            V(s, d, i) = sign == 1 ? -sumF : sumF;
+                        }
+                }
+            }
+        }
+    for (int l = 0; l < 256; ++l) {
+        if (!label_in(s, l, ORC_PB | ORC_VB)) continue;
+        const int dir = s->lab[l].dir;
+        const int axis = s->Q == 9 ? AXIS9[dir] : AXIS19[dir];
+        const int sign = s->Q == 9 ? SIGN9[dir] : SIGN19[dir];
+        const int phs = s->lab[l].phase, phsOp = 1 - phs;
+        for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+            const uint64_t i = s->bylabel[k];
+            for (int r = 0; r < nrel; ++r) {
+                const int q = zh_rel(s, dir, r);
+                double fT = F(s, 0, s->op[q], i) + F(s, 1, s->op[q], i);
+                fT += sign * (s->wi[q] * 6.0 / 1.0) * s->densT[i] * V(s, axis, i);
+                double sumF = 0;
+                for (int n = 0; n < nperp; ++n) {
+                    const int qn = zh_perp(s, dir, n);
+                    for (int d = 0; d < s->D; ++d) {
+                        if (s->qi[qn][d] == 0 || s->qi[q][d] == 0) continue;
+                        else if (s->qi[qn][d] == s->qi[q][d]) sumF += F(s, 0, qn, i) + F(s, 1, qn, i);
+                        else sumF -= F(s, 0, qn, i) + F(s, 1, qn, i);
+                    }
+                }
+                fT -= sumF * 0.5;
+                F(s, phs, q, i) = fT;
+                F(s, phsOp, q, i) = 0;
+            }
+        }
+    }
+}
+
+/* Adjacent report history, copied after the boundaries and before the macro
+ * variables of each step (scsp.cpp:82-83, colorfluid.cpp:166-171). */
+static void record_adjacent(orc_sim* s)
+{
+    memcpy(s->adjV, s->v, sizeof(double) * s->N * s->D);
+    if (s->NP == 2)
+        for (uint64_t i = 0; i < s->N; ++i)
+            s->adjChi[i] = s->densT[i] > 0.0 ? (s->dens[0][i] - s->dens[1][i]) / s->densT[i] : 0.0;
 }
 
 /* colorfluid.cpp:238-290 (arithmetic and harmonic viscosity) */
@@ -928,10 +1017,13 @@ int orc_step(orc_sim* s)
     for (int ph = 0; ph < s->NP; ++ph) stream_and_bounce(s, ph);
     if (s->d.model == 0) {
         scsp_zouhe(s);
+        record_adjacent(s);
         if (scsp_macro(s)) return 1;
         scsp_body_force(s);
         return scsp_collision(s);
     }
+    color_zouhe(s);
+    record_adjacent(s);
     if (color_macro(s)) return 1;
     color_body_force(s);
     return color_collision(s);
@@ -953,6 +1045,12 @@ void orc_fields(const orc_sim* s, double* rho0, double* rho1, double* u)
     if (rho0) memcpy(rho0, s->dens[0], sizeof(double) * s->N);
     if (rho1 && s->NP == 2) memcpy(rho1, s->dens[1], sizeof(double) * s->N);
     if (u) memcpy(u, s->v, sizeof(double) * s->N * s->D);
+}
+
+void orc_adjacent(const orc_sim* s, double* u, double* chi)
+{
+    if (u) memcpy(u, s->adjV, sizeof(double) * s->N * s->D);
+    if (chi && s->NP == 2) memcpy(chi, s->adjChi, sizeof(double) * s->N);
 }
 
 void orc_get_populations(const orc_sim* s, double* out)

@@ -81,6 +81,11 @@ uint64_t orc_sites(const orc_sim* s);
 /* Macro fields of the last step (after body force, before collision):
  * rho per phase (scsp: rho1 ignored) and velocity [d][site], d in reference order. */
 void orc_fields(const orc_sim* s, double* rho0, double* rho1, double* u);
+/* Adjacent report history of the last step: the velocity (and colour
+ * indicator (rho0 - rho1) / rhoT, 0 where rhoT <= 0) as the reference copies
+ * them after the boundaries and before the macro variables (scsp.cpp:82-83,
+ * colorfluid.cpp:166-171). */
+void orc_adjacent(const orc_sim* s, double* u, double* chi);
 /* Post-collision populations in checkpoint layout [phase][q][site]. */
 void orc_get_populations(const orc_sim* s, double* out);
 void orc_set_populations(orc_sim* s, const double* in, uint64_t t);

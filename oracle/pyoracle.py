@@ -51,6 +51,7 @@ def lib():
         L.orc_sites.restype = u64
         L.orc_fields.argtypes = [vp, vp, vp, vp]
         L.orc_get_populations.argtypes = [vp, vp]
+        L.orc_adjacent.argtypes = [vp, vp, vp]
         L.orc_set_populations.argtypes = [vp, vp, u64]
         L.orc_links.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_links.restype = u64
@@ -137,6 +138,13 @@ class Oracle:
         u = np.zeros(self.D * self.N)
         lib().orc_fields(self.h, _ptr(r0), _ptr(r1), _ptr(u))
         return r0, (r1 if self.NP == 2 else None), u.reshape(self.D, self.N)
+
+    def adjacent(self):
+        """Adjacent report history of the last step: (u [D*N], chi [N] or None)."""
+        u = np.zeros(self.N * self.D)
+        chi = np.zeros(self.N) if self.NP == 2 else None
+        lib().orc_adjacent(self.h, _ptr(u), _ptr(chi))
+        return u, chi
 
     def populations(self):
         out = np.zeros(self.NP * self.Q * self.N)

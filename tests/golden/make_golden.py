@@ -48,6 +48,12 @@ def main():
                                'force': m['force'], 'tau': m['tau'],
                                'geometry_sha256': m['geometry_sha256']}
     out['rock_scsp'] = rock
+    out['entry'] = {m['name']: {'history': [{k: h[k] for k in ('step', 'phase0_equivalent_length', 'pore_mean_ux',
+                                                               'total_mass', 'phase_masses')}
+                                            for h in m['history']],
+                                'height': m['height'], 'pressure_drop': m['pressure_drop']}
+                    for key in ('entry', 'entry3d') for m in porous[key]['measurements']
+                    if m['wall_gradient'] == 'legacy' and m['scale'] == 1}
     pr = porous['pressure_report']
     out['pressure_report'] = {k: pr[k] for k in ('name', 'prescribed_pressure_drop', 'report_slice_pressure_drop',
                                                  'expected_report_slice_pressure', 'reported_K_um2',

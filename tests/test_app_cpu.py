@@ -88,9 +88,9 @@ def test_missing_key(tmp_path):
 
 
 def test_unsupported_boundary_type(tmp_path):
-    spec = dict(base(), indices=['75 "PB" "1.0" "1" "1"'])
+    spec = dict(base(), indices=['75 "HF" "N/A" "N/A" "1"'])
     code, out = run_case(tmp_path, spec)
-    assert code != 0 and 'PB/VB in the colour model' in out
+    assert code != 0 and 'AB/HF' in out
 
 
 def test_library_exports_every_declared_symbol():

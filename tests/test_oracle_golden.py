@@ -97,6 +97,39 @@ def test_pressure_boundaries_report(records):
     assert abs(K / ((h ** 3 / 12 + h / 24) / (h + 2)) - 1) < 2e-4
 
 
+def entry_measure(r0, r1, u, case, height):
+    """phase0_equivalent_length, pore_mean_ux and total_mass as scripts/porous_media.py:
+    187-197 and :20-31 compute them from the VTI fields (x-fastest grids, interior
+    without the boundary planes)."""
+    D = case['D']
+    shape = (1,) + tuple(case['size']) if D == 2 else tuple(case['size'])
+    r0, r1 = r0.reshape(shape), r1.reshape(shape)
+    vx = u[D - 1].reshape(shape)
+    rho = r0 + r1
+    phi = np.divide(r0, rho, out=np.zeros_like(rho), where=rho > 0)
+    interior = (0 if D == 2 else slice(1, -1), slice(1, -1), slice(1, -1))
+    return (float(phi[interior].sum() / height ** (D - 1)), float(vx[interior].mean()), float(rho.sum()),
+            [float(r0.sum()), float(r1.sum())])
+
+
+@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2,)), ids=lambda s: s['name'])
+def test_entry_history(spec, records):
+    """Colour-model capillary entry between Zou-He pressure boundaries
+    (porous_media.py:153-253): the recorded displacement history is reproduced exactly."""
+    from oracle.pyoracle import Oracle
+    case = H.lattice_case(spec)
+    o = Oracle(case, H.geometry(spec)).init()
+    rec = records['entry'][spec['name'] + '-1rank']
+    for h in rec['history'][1:]:
+        o.step(h['step'] - o.t)
+        r0, r1, u = o.fields()
+        L, ux, mass, phases = entry_measure(r0, r1, u, case, spec['height'])
+        assert L == h['phase0_equivalent_length']
+        assert ux == h['pore_mean_ux']
+        assert mass == h['total_mass']
+        assert phases == h['phase_masses']
+
+
 def test_c1_checkpoint_digest():
     """8x10x4 SRT channel, 20000 steps: checkpoint bytes equal the reference's (solid entries included)."""
     s = c1_spec()

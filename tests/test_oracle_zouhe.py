@@ -13,6 +13,12 @@ the Zou-He construction fixes, at every boundary site and step:
     velocity, so the mass identity holds only on +axis faces);
 and at steady state the volume flux through every interior cross-section
 equals the inlet's to within the inlet density excess (2e-3 here).
+
+The colour model's pressure boundary is pinned by the reference's recorded
+capillary-entry histories (tests/test_oracle_golden.py::test_entry_history);
+its velocity boundary by the same construction on the total population: after
+calcMacroVar a PB site holds rho0 + rho1 = rho_b and a +axis VB site the
+prescribed velocity, transverse components 0 (to rounding).
 """
 import numpy as np
 import pytest
@@ -67,3 +73,35 @@ def test_zouhe_flux_continuity():
     inlet = np.count_nonzero(g[..., 0] == 50) * 1e-3
     assert abs(flux[0] / inlet - 1) < 1e-15
     assert np.max(np.abs(flux[1:-1] / inlet - 1)) < 2e-3
+
+
+def axis_sign(D, direction):
+    if D == 3:
+        return {1: 2, 2: 2, 3: 1, 4: 1, 5: 0, 6: 0}[direction], 1 if direction % 2 else -1
+    return {1: 1, 3: 1, 2: 0, 4: 0}[direction], 1 if direction in (1, 2) else -1
+
+
+@pytest.mark.parametrize('spec', H.zouhe_color_cases(), ids=lambda s: s['name'])
+def test_zouhe_color_site_identities(spec):
+    o, case = oracle_run(spec, 200)
+    g = H.geometry(spec).reshape(case['size'])
+    r0, r1, u = o.fields()
+    D = case['D']
+    u = u.reshape((D,) + tuple(case['size']))
+    rt = (r0 + r1).reshape(case['size'])
+    seen = set()
+    for lab, (typ, value, direction, _) in case['labels'].items():
+        if typ not in ('PB', 'VB'):
+            continue
+        m = g == lab
+        assert m.any()
+        ax, sign = axis_sign(D, direction)
+        seen.add((typ, sign))
+        if typ == 'PB':
+            assert np.max(np.abs(rt[m] - value)) < 1e-14
+        elif sign == 1:
+            assert np.max(np.abs(u[ax][m] - value)) < 1e-15
+        for d in range(D):
+            if d != ax:
+                assert np.max(np.abs(u[d][m])) < 1e-15
+    assert len(seen) == 2
