@@ -40,10 +40,12 @@ enum lbf_label_type {
  * solver/src/file/fileprm.cpp:633-670).  type 0 = label not defined. */
 typedef struct lbf_label {
     int32_t type;      /* lbf_label_type */
-    int32_t dir;       /* lattice direction of a GV force (PB/VB flow direction) */
+    int32_t dir;       /* lattice direction of a GV force; PB/VB: axial direction
+                          1..2D of the unknown populations (scsp.cpp:188-256) */
     int32_t phase;     /* phase of VD/GV/PB/VB/HF sites */
     int32_t reserved;
-    double  value;     /* GV force magnitude, SD contact angle in degrees */
+    double  value;     /* GV force magnitude, SD contact angle in degrees,
+                          PB density, VB speed */
 } lbf_label;
 
 enum lbf_model { LBF_MODEL_SCSP = 0, LBF_MODEL_COLOR = 1 };
