@@ -26,6 +26,7 @@
 // ([channel][voxel] rows, padded to 272 B), and every voxel split writes its
 // own FP32 partial, summed in split order by a second kernel (deterministic).
 #include "conv_types.h"
+#include "device.h"
 
 namespace lbf {
 
@@ -151,6 +152,143 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
 }
 
 // ---------------------------------------------------------------------------
+// forward (and input gradient), 3x3x3 with Cin a multiple of 32: input tile
+// with its one-voxel halo staged in LDS once per 32-channel chunk and reused
+// by all 27 taps (the direct kernel above gathers every tap from L2).
+// A workgroup covers a BZ x BY x BX block of 256 voxels (BX = min(side, 32));
+// the halo tile is (BZ+2)(BY+2)(BX+2) voxels x 32 channels, 80 B per voxel so
+// that the 16 lanes of an MFMA row group read 16 distinct bank quads.  Each
+// wave owns 64 voxels (4 row tiles) x BN output channels; the weight
+// fragments come from global memory (the [Cout][KP] panel of a layer is
+// shared by every workgroup and stays in the caches).
+// ---------------------------------------------------------------------------
+constexpr int HALO_VS = 40;          // bf16 per halo voxel in LDS
+constexpr int HALO_MAX = 34 * 6 * 4; // largest halo tile (side >= 32: 32 x 4 x 2 block)
+
+struct HaloShape {
+    int lbx, lby, lbz;               // log2 of the block extent along x, y, z
+    int hx, hy, hz;                  // halo tile extents
+};
+
+__device__ __forceinline__ HaloShape halo_shape(int ls)
+{
+    HaloShape h;
+    h.lbx = ls < 5 ? ls : 5;
+    const int rem = 8 - h.lbx;
+    h.lby = ls < (rem + 1) / 2 ? ls : (rem + 1) / 2;
+    h.lbz = rem - h.lby;
+    h.hx = (1 << h.lbx) + 2;
+    h.hy = (1 << h.lby) + 2;
+    h.hz = (1 << h.lbz) + 2;
+    return h;
+}
+
+template <int CIN, int COUT, int BN, int ACT, bool BIAS, bool OUTF32>
+__global__ void __launch_bounds__(256)
+conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w,
+              const float* __restrict__ bias, void* __restrict__ out)
+{
+    constexpr int KP = conv_kp(CIN, 27);
+    constexpr int NT = BN / 16;
+    constexpr int CC = 32;
+    static_assert(CIN % CC == 0 && BN % 16 == 0, "Cin must be a multiple of 32");
+    __shared__ __attribute__((aligned(16))) __bf16 tile[HALO_MAX * HALO_VS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, grp = lane >> 4;
+    const int ls = int(g.log_s);
+    const uint32_t S = 1u << ls, m = S - 1u;
+    const HaloShape h = halo_shape(ls);
+    const int hv_n = h.hx * h.hy * h.hz;
+    // block -> sample and block origin
+    const uint32_t bps = 1u << (3 * ls - 8);                    // blocks per sample
+    const uint32_t smp = blockIdx.x / bps, b = blockIdx.x % bps;
+    const uint32_t nbx = S >> h.lbx, nby = S >> h.lby;
+    const uint32_t x0 = (b % nbx) << h.lbx, y0 = ((b / nbx) % nby) << h.lby, z0 = (b / (nbx * nby)) << h.lbz;
+    const size_t sbase = size_t(smp) << (3 * ls);
+    const int co0 = blockIdx.y * BN;
+    // this lane's A voxels (4 row tiles) as halo-tile indices
+    int hv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int lv = wave * 64 + 16 * i + r16;
+        const int lx = lv & ((1 << h.lbx) - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1), lz = lv >> (h.lbx + h.lby);
+        hv[i] = ((lz + 1) * h.hy + (ly + 1)) * h.hx + (lx + 1);
+    }
+    const int oy = h.hx, oz = h.hx * h.hy;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const __bf16* wrow[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) wrow[j] = w + size_t(co0 + 16 * j + r16) * KP + 8 * grp;
+
+    for (int c0 = 0; c0 < CIN; c0 += CC) {
+        __syncthreads();   // the previous chunk's MFMAs are done with the tile
+        for (int t = threadIdx.x; t < hv_n * 4; t += 256) {
+            const int v = t >> 2, q = t & 3;
+            const int hx = v % h.hx, hy = (v / h.hx) % h.hy, hz = v / (h.hx * h.hy);
+            const uint32_t x = (x0 + uint32_t(hx) - 1u) & m, y = (y0 + uint32_t(hy) - 1u) & m,
+                           z = (z0 + uint32_t(hz) - 1u) & m;
+            const size_t gv = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls) | x);
+            *reinterpret_cast<bf16x8*>(tile + v * HALO_VS + 8 * q) =
+                *reinterpret_cast<const bf16x8*>(in + gv * g.in_stride + c0 + 8 * q);
+        }
+        __syncthreads();
+        bf16x8 bnext[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) bnext[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + c0);
+        // 9 (dz, dy) rows of 3 taps; the next tap's weight fragments are
+        // loaded one tap ahead
+#pragma unroll 1
+        for (int t9 = 0; t9 < 9; ++t9) {
+            const int row = (t9 / 3 - 1) * oz + (t9 % 3 - 1) * oy;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int tap = 3 * t9 + dx;
+                bf16x8 bcur[NT];
+#pragma unroll
+                for (int j = 0; j < NT; ++j) bcur[j] = bnext[j];
+                if (tap < 26) {
+#pragma unroll
+                    for (int j = 0; j < NT; ++j)
+                        bnext[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + (tap + 1) * CIN + c0);
+                }
+                bf16x8 a[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    a[i] = *reinterpret_cast<const bf16x8*>(tile + (hv[i] + row + dx - 1) * HALO_VS + 8 * grp);
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bcur[j], acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int co = co0 + 16 * j + r16;
+        if (co >= COUT) continue;
+        const float bb = BIAS ? bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int lv = wave * 64 + 16 * i + 4 * grp + r;
+                const uint32_t lx = lv & ((1 << h.lbx) - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1),
+                               lz = lv >> (h.lbx + h.lby);
+                const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
+                float x = acc[i][j][r] + bb;
+                if (ACT == ACT_RELU) x = x > 0.0f ? x : 0.0f;
+                if (OUTF32) static_cast<float*>(out)[v * g.out_stride + co] = x;
+                else static_cast<__bf16*>(out)[v * g.out_stride + co] = __bf16(x);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // weight gradient
 // ---------------------------------------------------------------------------
 template <int CIN, int COUT, int TAPS, int BM, int BN>
@@ -256,6 +394,176 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
         }
 }
 
+// ---------------------------------------------------------------------------
+// weight gradient, 3x3x3 with Cin a multiple of 32: a workgroup owns 32
+// output channels x 16 input channels x all 27 taps and runs over the
+// 256-voxel blocks of its voxel split (the direct kernel above re-gathers
+// every tap of the input for every 64-wide k tile).  Per block it stages
+//   dz^T   [32 channels][256 voxels]                     (rows 264 bf16)
+//   input  [16 channels][(BZ+2)(BY+2) rows][x]  twice:   copy 0 holds halo
+//          x positions 0..BX+1, copy 1 the same row shifted by one
+// so that the 8 consecutive voxels of an MFMA lane group start 16-byte
+// aligned for tap dx = -1 (copy 0) and dx = 0 (copy 1) and 4-byte aligned for
+// dx = +1 (copy 0, read as 4 dwords).  One MFMA reduces 32 voxels; wave w
+// accumulates the taps w, w+4, ... for both 16-channel output tiles.  Tiles
+// are written as dwords (voxel pairs), and the next block's operands are
+// fetched into registers while the current block runs on MFMA.
+// ---------------------------------------------------------------------------
+constexpr int WG_DZS = 256 + 8;         // bf16 per dz^T row (132 dwords: 16 rows hit 16 bank quads)
+constexpr int WG_XPLANE = 968;          // bf16 per channel plane of one copy (max rows * (BX + 8) + 8)
+
+__device__ __forceinline__ uint32_t pack2(__bf16 lo, __bf16 hi)
+{
+    return uint32_t(__builtin_bit_cast(uint16_t, lo)) | (uint32_t(__builtin_bit_cast(uint16_t, hi)) << 16);
+}
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256)
+conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ dz,
+                float* __restrict__ partial, uint32_t blocks_per_split)
+{
+    constexpr int KP = conv_kp(CIN, 27);
+    constexpr int CC = 16, BM = 32;
+    constexpr int TW = 7;                                  // taps per wave (27 over 4 waves)
+    constexpr int NCH = CIN / CC, NCO = COUT / BM;
+    constexpr int NXI = 4;                                 // input-tile items per thread (<= 816 / 256)
+    static_assert(CIN % 32 == 0 && COUT % BM == 0, "shape");
+    __shared__ __attribute__((aligned(16))) __bf16 sdz[BM * WG_DZS];
+    __shared__ __attribute__((aligned(16))) __bf16 sx[2 * CC * WG_XPLANE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, grp = lane >> 4;
+    // workgroups of one voxel split (all channel tiles) run on one XCD
+    const uint32_t wg = xcd_block();
+    const uint32_t split = wg / (NCH * NCO), rest = wg % (NCH * NCO);
+    const int c0 = int(rest % NCH) * CC, co0 = int(rest / NCH) * BM;
+    const int ls = int(g.log_s);
+    const uint32_t S = 1u << ls, m = S - 1u;
+    const HaloShape h = halo_shape(ls);
+    const int bx = 1 << h.lbx, rw = bx + 8;                // copy row width (bf16), 16-byte multiple
+    const int rows = h.hy * h.hz, hp = h.hx / 2;           // halo rows, x pairs per row
+    const int nxi = rows * hp * 2;                         // input items: (row, pair, 8-channel group)
+    const uint32_t nblocks = g.nvox >> 8;
+    const uint32_t b_begin = split * blocks_per_split;
+    const uint32_t b_end = b_begin + blocks_per_split < nblocks ? b_begin + blocks_per_split : nblocks;
+    const uint32_t bps = 1u << (3 * ls - 8);
+    const uint32_t nbx = S >> h.lbx, nby = S >> h.lby;
+    f32x4 acc[TW][2];
+#pragma unroll
+    for (int t = 0; t < TW; ++t) acc[t][0] = acc[t][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    // one block's operands in registers: dz of 2 voxel pairs x 8 channels,
+    // input of up to NXI (row, x pair) items x 8 channels (3 voxels each)
+    bf16x8 rdz[2][2], rx[NXI][3];
+    auto fetch = [&](uint32_t blk) {
+        const uint32_t smp = blk / bps, b = blk % bps;
+        const uint32_t x0 = (b % nbx) << h.lbx, y0 = ((b / nbx) % nby) << h.lby, z0 = (b / (nbx * nby)) << h.lbz;
+        const size_t sbase = size_t(smp) << (3 * ls);
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            const int t = threadIdx.x + 256 * n;
+            const int c8 = t >> 7, pv = t & 127;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int lv = 2 * pv + e;
+                const uint32_t lx = lv & (bx - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1),
+                               lz = lv >> (h.lbx + h.lby);
+                const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
+                rdz[n][e] = *reinterpret_cast<const bf16x8*>(dz + v * g.out_stride + co0 + 8 * c8);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < NXI; ++n) {
+            const int t = threadIdx.x + 256 * n;
+            if (t >= nxi) break;
+            const int p = t % hp, row = (t / hp) % rows, c8 = t / (hp * rows);
+            const int hy = row % h.hy, hz = row / h.hy;
+            const uint32_t y = (y0 + uint32_t(hy) - 1u) & m, z = (z0 + uint32_t(hz) - 1u) & m;
+            const size_t rbase = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls));
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const int hx = 2 * p + e;
+                if (e == 2 && 2 * p + 1 >= bx) break;      // the last pair feeds copy 0 only
+                const uint32_t x = (x0 + uint32_t(hx) - 1u) & m;
+                rx[n][e] = *reinterpret_cast<const bf16x8*>(in + (rbase + x) * g.in_stride + c0 + 8 * c8);
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            const int t = threadIdx.x + 256 * n;
+            const int c8 = t >> 7, pv = t & 127;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                *reinterpret_cast<uint32_t*>(sdz + (8 * c8 + j) * WG_DZS + 2 * pv) = pack2(rdz[n][0][j], rdz[n][1][j]);
+        }
+#pragma unroll
+        for (int n = 0; n < NXI; ++n) {
+            const int t = threadIdx.x + 256 * n;
+            if (t >= nxi) break;
+            const int p = t % hp, row = (t / hp) % rows, c8 = t / (hp * rows);
+            const bool shifted = 2 * p + 1 < bx;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                __bf16* plane0 = sx + (8 * c8 + j) * WG_XPLANE + row * rw + 2 * p;
+                *reinterpret_cast<uint32_t*>(plane0) = pack2(rx[n][0][j], rx[n][1][j]);
+                if (shifted)
+                    *reinterpret_cast<uint32_t*>(plane0 + CC * WG_XPLANE) = pack2(rx[n][1][j], rx[n][2][j]);
+            }
+        }
+    };
+
+    if (b_begin < b_end) fetch(b_begin);
+    for (uint32_t blk = b_begin; blk < b_end; ++blk) {
+        __syncthreads();   // the previous block's MFMAs are done with the tiles
+        store();
+        __syncthreads();
+        if (blk + 1 < b_end) fetch(blk + 1);   // the next block's loads overlap this block's MFMAs
+#pragma unroll 1
+        for (int r = 0; r < 8; ++r) {
+            // this lane's 8 voxels: block positions 32 r + 8 grp .. + 7 (one x run)
+            const int lv = 32 * r + 8 * grp;
+            const int lx = lv & (bx - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1), lz = lv >> (h.lbx + h.lby);
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(sdz + r16 * WG_DZS + lv);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(sdz + (16 + r16) * WG_DZS + lv);
+            const __bf16* xc = sx + r16 * WG_XPLANE;
+#pragma unroll
+            for (int tt = 0; tt < TW; ++tt) {
+                const int tap = wave + 4 * tt;
+                if (tap >= 27) break;
+                const int dzt = tap / 9 - 1, dyt = (tap / 3) % 3 - 1, dxt = tap % 3 - 1;
+                const int row = (lz + dzt + 1) * h.hy + (ly + dyt + 1);
+                bf16x8 bv;
+                if (dxt == -1) {
+                    bv = *reinterpret_cast<const bf16x8*>(xc + row * rw + lx);
+                } else if (dxt == 0) {
+                    bv = *reinterpret_cast<const bf16x8*>(xc + CC * WG_XPLANE + row * rw + lx);
+                } else {
+                    const uint32_t* q = reinterpret_cast<const uint32_t*>(xc + row * rw + lx + 2);
+                    uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
+                    __builtin_memcpy(&bv, w4, 16);
+                }
+                acc[tt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[tt][0], 0, 0, 0);
+                acc[tt][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[tt][1], 0, 0, 0);
+            }
+        }
+    }
+    // C[co][ci]: lane holds column ci = c0 + r16, rows co = 4 grp .. + 3 of each 16-row tile
+#pragma unroll
+    for (int tt = 0; tt < TW; ++tt) {
+        const int tap = wave + 4 * tt;
+        if (tap >= 27) break;
+        const int k = tap * CIN + c0 + r16;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int co = co0 + 16 * i + 4 * grp + rr;
+                partial[(size_t(split) * COUT + co) * KP + k] = acc[tt][i][rr];
+            }
+    }
+}
+
 // dW[i] = sum over splits of partial[s][i], in split order.
 __global__ void __launch_bounds__(256)
 sum_splits(const float* __restrict__ partial, uint32_t n, uint32_t splits, float* __restrict__ out)
@@ -273,34 +581,56 @@ sum_splits(const float* __restrict__ partial, uint32_t n, uint32_t splits, float
 template <int CIN, int COUT, int TAPS>
 struct ConvLaunch {
     static constexpr int BN_FWD = COUT >= 64 ? 64 : COUT;
+    // the halo kernel: 3x3x3 with Cin a multiple of 32; Cout <= 96 in one block
+    static constexpr bool HALO = TAPS == 27 && CIN % 32 == 0;
+    static constexpr int BN_HALO = COUT <= 64 ? COUT : (COUT % 64 == 0 ? 64 : 48);
+
+    template <int ACT, bool BIAS, bool OUTF32>
+    static void fwd(const ConvGeom& g, const __bf16* in, const __bf16* w, const float* bias, void* out, hipStream_t s)
+    {
+        if constexpr (HALO) {
+            // a 256-voxel block must fit one sample: side >= 8
+            if (g.log_s >= 3) {
+                const dim3 grid(g.nvox / 256u, (COUT + BN_HALO - 1) / BN_HALO);
+                hipLaunchKernelGGL((conv_fwd_halo<CIN, COUT, BN_HALO, ACT, BIAS, OUTF32>), grid, dim3(256), 0, s, g,
+                                   in, w, bias, out);
+                return;
+            }
+        }
+        const dim3 grid((g.nvox + 255u) / 256u, (COUT + BN_FWD - 1) / BN_FWD);
+        hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT, BIAS, OUTF32>), grid, dim3(256), 0, s, g, in, w,
+                           bias, out);
+    }
     static constexpr int BM_W = COUT >= 64 ? 64 : COUT;
     static constexpr int BN_W = 64;
 
     static void forward(const ConvGeom& g, const __bf16* in, const __bf16* w, const float* bias, void* out, int act,
                         bool out_f32, hipStream_t s)
     {
-        const dim3 grid((g.nvox + 255u) / 256u, (COUT + BN_FWD - 1) / BN_FWD);
-        if (out_f32 && act == ACT_RELU)
-            hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_RELU, true, true>), grid, dim3(256), 0, s, g,
-                               in, w, bias, out);
-        else if (out_f32)
-            hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_NONE, true, true>), grid, dim3(256), 0, s, g,
-                               in, w, bias, out);
-        else if (act == ACT_RELU)
-            hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_RELU, true, false>), grid, dim3(256), 0, s, g,
-                               in, w, bias, out);
-        else if (bias)
-            hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_NONE, true, false>), grid, dim3(256), 0, s, g,
-                               in, w, bias, out);
-        else
-            hipLaunchKernelGGL((conv_fwd<CIN, COUT, TAPS, BN_FWD, ACT_NONE, false, false>), grid, dim3(256), 0, s, g,
-                               in, w, bias, out);
+        if (out_f32 && act == ACT_RELU) fwd<ACT_RELU, true, true>(g, in, w, bias, out, s);
+        else if (out_f32) fwd<ACT_NONE, true, true>(g, in, w, bias, out, s);
+        else if (act == ACT_RELU) fwd<ACT_RELU, true, false>(g, in, w, bias, out, s);
+        else if (bias) fwd<ACT_NONE, true, false>(g, in, w, bias, out, s);
+        else fwd<ACT_NONE, false, false>(g, in, w, bias, out, s);
     }
+
+    static constexpr bool HALO_W = HALO && COUT % 32 == 0;
 
     static void wgrad(const ConvGeom& g, const __bf16* in, const __bf16* dz, float* partial, uint32_t splits,
                       float* dw, hipStream_t s)
     {
         constexpr int KP = conv_kp(CIN, TAPS);
+        if constexpr (HALO_W) {
+            if (g.log_s >= 3) {
+                const uint32_t nblocks = g.nvox >> 8;
+                const uint32_t per = (nblocks + splits - 1) / splits;
+                hipLaunchKernelGGL((conv_wgrad_halo<CIN, COUT>), dim3((CIN / 16) * (COUT / 32) * splits), dim3(256),
+                                   0, s, g, in, dz, partial, per);
+                const uint32_t n = uint32_t(COUT) * KP;
+                hipLaunchKernelGGL(sum_splits, dim3((n + 255) / 256), dim3(256), 0, s, partial, n, splits, dw);
+                return;
+            }
+        }
         const uint32_t per = ((g.nvox + splits - 1) / splits + 127u) / 128u * 128u;
         const dim3 grid((KP + BN_W - 1) / BN_W, (COUT + BM_W - 1) / BM_W, splits);
         hipLaunchKernelGGL((conv_wgrad<CIN, COUT, TAPS, BM_W, BN_W>), grid, dim3(256), 0, s, g, in, dz, partial, per);
@@ -330,6 +660,25 @@ struct ConvLaunch {
     X(256, 128, 27)        \
     X(128, 64, 27)         \
     X(64, 32, 27)
+
+// Voxel splits of a weight gradient: enough workgroups to fill the chip, few
+// enough partials that the fixed-order split sum stays cheap.
+uint32_t conv_wgrad_splits(int cin, int cout, int taps, uint32_t nvox)
+{
+    const uint32_t max_splits = nvox / 256 > 0 ? nvox / 256 : 1;
+    uint32_t splits;
+    if (taps == 27 && cin % 32 == 0 && cout % 32 == 0 && nvox >= 512) {
+        const uint32_t base = uint32_t(cin / 16) * uint32_t(cout / 32);
+        splits = (1024u + base - 1) / base;
+        if (splits > 256u) splits = 256u;
+    } else {
+        const uint32_t kb = (uint32_t(conv_kp(cin, taps)) + 63) / 64, cb = (uint32_t(cout) + 63) / 64;
+        splits = (2048u + kb * cb - 1) / (kb * cb);
+        if (splits > 64u) splits = 64u;
+    }
+    if (splits > max_splits) splits = max_splits;
+    return splits < 1 ? 1 : splits;
+}
 
 bool conv_shape_supported(int cin, int cout, int taps)
 {

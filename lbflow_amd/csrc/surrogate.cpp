@@ -26,6 +26,7 @@
 #include "conv_types.h"
 
 namespace lbf {
+uint32_t conv_wgrad_splits(int cin, int cout, int taps, uint32_t nvox);
 hipError_t launch_conv_forward(int cin, int cout, int taps, const ConvGeom& g, const __bf16* in, const __bf16* w,
                                const float* bias, void* out, int act, bool out_f32, hipStream_t s);
 hipError_t launch_conv_wgrad(int cin, int cout, int taps, const ConvGeom& g, const __bf16* in, const __bf16* dz,
@@ -154,12 +155,7 @@ int wgrad(lbf_net* n, int i, uint32_t count, const __bf16* in, uint32_t in_strid
 {
     const Layer& l = n->L[i];
     const ConvGeom g = geom(n, l.level, count, in_stride, dz_stride);
-    const uint32_t kb = (l.kp + 63) / 64, cb = (uint32_t(l.cout) + 63) / 64;
-    // enough workgroups to fill the chip, but few enough partials that the
-    // fixed-order split sum stays cheap
-    uint32_t splits = std::min(64u, (2048u + kb * cb - 1) / (kb * cb));
-    const uint32_t max_splits = (g.nvox + 255) / 256;
-    if (splits > max_splits) splits = max_splits;
+    uint32_t splits = conv_wgrad_splits(l.cin, l.cout, l.taps, g.nvox);
     if (size_t(splits) * l.cout * l.kp > n->wpart_floats) splits = uint32_t(n->wpart_floats / (size_t(l.cout) * l.kp));
     if (splits < 1) splits = 1;
     NHIP(n, launch_conv_wgrad(l.cin, l.cout, l.taps, g, in, dz, n->wpart, splits, l.gw, n->stream));
@@ -329,10 +325,8 @@ int lbf_net_create(uint32_t side, uint32_t capacity, uint64_t seed, lbf_net** ou
         for (int co = 0; co < l.real_out; ++co)
             for (int k = 0; k < l.taps * l.cin; ++k) hw[size_t(co) * l.kp + k] = float(nd(rng));
         if (hipMemcpy(l.w, hw.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
-        // weight-gradient partials: at most (max voxels / 256) splits, capped by the heuristic
-        const uint32_t kb = (l.kp + 63) / 64, cb = (uint32_t(l.cout) + 63) / 64;
-        const size_t splits = std::min<size_t>(std::min(64u, (2048u + kb * cb - 1) / (kb * cb)),
-                                               std::max<size_t>(1, (size_t(capacity) << (3 * (ls - l.level))) / 256));
+        // weight-gradient partials at the largest batch
+        const size_t splits = conv_wgrad_splits(l.cin, l.cout, l.taps, uint32_t(size_t(capacity) << (3 * (ls - l.level))));
         wmax = std::max(wmax, splits * nw);
     }
     const uint32_t v[4] = {uint32_t(v0), uint32_t(v0 >> 3), uint32_t(v0 >> 6), uint32_t(v0 >> 9)};

@@ -335,8 +335,9 @@ def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=Non
 
 def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, tau=0.8, force=1e-6,
              train_force=1e-5, train_tol=1e-5, eval_tol=1e-3, interval=50, max_steps=60000, out=None, log=print,
-             relax_sweeps=(5,)):
-    """Generate, solve, train, evaluate.  Returns a report dict."""
+             relax_sweeps=(5,), init=None):
+    """Generate, solve, train (from `init` weights when given), evaluate.
+    Returns a report dict."""
     rep = dict(side=side, n_train=n_train, n_eval=n_eval, epochs=epochs, capacity=capacity, lr=lr, tau=tau,
                force=force, train_force=train_force, train_tol=train_tol, eval_tol=eval_tol,
                report_interval=interval)
@@ -366,6 +367,9 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
     log(f'train step {ms:.2f} ms, {flops / ms / 1e9:.1f} TFLOP/s (convolutions)')
     net.close()   # the timing steps ran with lr 0; train a fresh network
     net = Net(side=side, capacity=capacity, seed=1)
+    if init:
+        net.set_params(np.load(init, allow_pickle=False)['params'].astype(np.float32))
+        rep['init'] = str(init)
     t0 = time.perf_counter()
     hist = train(net, masks, targets, epochs, lr=lr, batch=capacity, log=log)
     rep['training'] = dict(seconds=round(time.perf_counter() - t0, 2), loss=[round(h, 5) for h in hist])
@@ -406,6 +410,39 @@ def evaluate(net, scaling, eval_crops, tau=0.8, force=1e-6, eval_tol=1e-3, inter
     return ev
 
 
+def diagnose(net, scaling, crops, tau=0.8, force=1e-6, log=print):
+    """Prediction error on crops solved to 1e-5: relative L2 error of u over
+    fluid sites, the flux ratio sum(u_x pred) / sum(u_x true), and the error
+    left after the best single scale factor (how much of the error is the
+    global magnitude, the slowest mode of the warm-started run)."""
+    torch = _torch()
+    fields, _ = solve(crops, tau=tau, force=force, tol=1e-5, interval=100)
+    side = crops[0].shape[0]
+    masks = torch.as_tensor(np.stack([(c == PORE) for c in crops]).astype(np.uint8), device='cuda')
+    pred = torch.empty((net.capacity, side ** 3, 4), dtype=torch.float32, device='cuda')
+    rows = []
+    for i in range(0, len(crops), net.capacity):
+        m = masks[i:i + net.capacity].contiguous()
+        p = pred[:m.shape[0]]
+        net.forward(m, p)
+        f = scaling.fields(p, tau, force, m)
+        torch.cuda.synchronize()
+        for j in range(m.shape[0]):
+            up = f[j, :, :3].double()
+            ut = fields[i + j, :, :3].double()
+            err = float(torch.linalg.norm(up - ut) / torch.linalg.norm(ut))
+            alpha = float((up * ut).sum() / (up * up).sum())
+            err_s = float(torch.linalg.norm(alpha * up - ut) / torch.linalg.norm(ut))
+            flux = float(up[:, 2].sum() / ut[:, 2].sum())
+            rows.append((err, flux, alpha, err_s))
+    r = np.array(rows)
+    rep = dict(n=len(rows), rel_l2_mean=float(r[:, 0].mean()), flux_ratio_mean=float(r[:, 1].mean()),
+               flux_ratio_std=float(r[:, 1].std()), best_scale_mean=float(r[:, 2].mean()),
+               rel_l2_after_best_scale_mean=float(r[:, 3].mean()))
+    log('diagnose: ' + json.dumps(rep))
+    return rep
+
+
 def train_bench(side, capacity, steps, log=print):
     """Training-step throughput on synthetic masks and targets (no solver):
     `steps` Adam steps of `capacity` crops, timed with HIP events."""
@@ -434,8 +471,10 @@ def main(argv=None):
     ap.add_argument('--capacity', type=int, default=8)
     ap.add_argument('--lr', type=float, default=1e-3)
     ap.add_argument('--out', default=None, help='weights file (.npz)')
+    ap.add_argument('--init', default=None, help='start training from these weights (.npz)')
     ap.add_argument('--report', default=None, help='report file (.json)')
     ap.add_argument('--evaluate', default=None, help='only evaluate these weights (.npz) on --eval crops')
+    ap.add_argument('--diagnose', default=None, help='only report the prediction error of these weights (.npz)')
     ap.add_argument('--train-bench', type=int, default=0,
                     help='only time this many training steps on --capacity synthetic crops')
     a = ap.parse_args(argv)
@@ -445,12 +484,16 @@ def main(argv=None):
 
     if a.train_bench:
         rep = train_bench(a.side, a.capacity, a.train_bench, log)
+    elif a.diagnose:
+        crops = make_crops(a.eval, 20000, a.side)
+        net, scaling = load(a.diagnose, capacity=a.capacity)
+        rep = dict(weights=a.diagnose, diagnose=diagnose(net, scaling, crops, log=log))
     elif a.evaluate:
         crops = make_crops(a.eval, 20000, a.side)
         net, scaling = load(a.evaluate, capacity=a.capacity)
         rep = dict(weights=a.evaluate, n_eval=a.eval, evaluation=evaluate(net, scaling, crops, log=log))
     else:
-        rep = pipeline(a.train, a.eval, a.side, a.epochs, a.capacity, a.lr, out=a.out, log=log)
+        rep = pipeline(a.train, a.eval, a.side, a.epochs, a.capacity, a.lr, out=a.out, log=log, init=a.init)
     if a.report:
         with open(a.report, 'w') as f:
             json.dump(rep, f, indent=1)

@@ -50,6 +50,11 @@ def packed_weights(w):
     (64, 32, 27, 16, 1, 0, 64, True, False),
     (32, 16, 1, 8, 2, 0, 0, False, True),
     (16, 32, 1, 8, 1, 0, 0, False, True),
+    # LDS halo-tile kernel (3x3x3, Cin % 32 == 0, side >= 8) at every block shape
+    (32, 32, 27, 64, 1, 0, 0, True, False),
+    (96, 32, 27, 32, 1, 32, 0, False, True),
+    (32, 96, 27, 16, 2, 0, 32, False, True),
+    (256, 128, 27, 8, 2, 0, 0, True, True),
 ])
 def test_conv_forward(cin, cout, taps, side, n, in_extra, out_extra, relu, f32):
     L = S.lib()
@@ -84,7 +89,9 @@ def test_conv_forward(cin, cout, taps, side, n, in_extra, out_extra, relu, f32):
 
 
 @pytest.mark.parametrize('cin,cout,taps,side,n,splits', [
-    (8, 32, 27, 8, 2, 4), (32, 64, 27, 8, 1, 3), (128, 256, 27, 4, 2, 1), (96, 32, 27, 8, 1, 7), (32, 16, 1, 8, 2, 2)])
+    (8, 32, 27, 8, 2, 4), (32, 64, 27, 8, 1, 3), (128, 256, 27, 4, 2, 1), (96, 32, 27, 8, 1, 7), (32, 16, 1, 8, 2, 2),
+    # LDS halo-tile kernel at every block shape, splits with empty and partial ranges
+    (32, 32, 27, 32, 1, 16), (64, 32, 27, 16, 2, 5), (32, 96, 27, 64, 1, 300)])
 def test_conv_weight_gradient(cin, cout, taps, side, n, splits):
     L = S.lib()
     rng = np.random.default_rng(3 * cin + cout)
@@ -183,7 +190,7 @@ def test_training_reduces_loss():
     dm, dt = torch.as_tensor(mask, device='cuda'), torch.as_tensor(target, device='cuda')
     opt = S.adam(3e-3)
     first = net.train_step(dm, dt, opt)
-    for _ in range(60):
+    for _ in range(120):
         last = net.train_step(dm, dt, opt)
     say('loss', first, last)
     assert last < 0.2 * first
