@@ -564,6 +564,151 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// weight gradient for sides >= 32, where a block is 2 x 4 x-rows of 32
+// voxels and one MFMA's 32 voxels are one x-row: three waves, one per tap
+// dx.  Per output z-layer a wave keeps the dz^T fragments of its 4 rows in
+// registers and loads the input fragment of each of the 18 halo rows it
+// touches once; that fragment serves every tap dy whose output row lies in
+// the layer (up to 3 taps x 2 output-channel tiles), so LDS reads per MFMA
+// drop from ~0.55 KB to ~0.3 KB.
+// Tiles and copies are those of conv_wgrad_halo.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(2)))
+conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ dz,
+                float* __restrict__ partial, uint32_t blocks_per_split)
+{
+    constexpr int KP = conv_kp(CIN, 27);
+    constexpr int CC = 16, BM = 32, NTH = 192;
+    constexpr int NCH = CIN / CC, NCO = COUT / BM;
+    constexpr int HY = 6, HX = 34, ROWS = 24, HP = 17, RW = 40;   // block 2 x 4 x 32
+    constexpr int NXI = (ROWS * HP * 2 + NTH - 1) / NTH, NDZ = (512 + NTH - 1) / NTH;
+    static_assert(CIN % 32 == 0 && COUT % BM == 0, "shape");
+    __shared__ __attribute__((aligned(16))) __bf16 sdz[BM * WG_DZS];
+    __shared__ __attribute__((aligned(16))) __bf16 sx[2 * CC * WG_XPLANE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;   // wave = dx + 1
+    const int r16 = lane & 15, grp = lane >> 4;
+    const uint32_t wg = xcd_block();
+    const uint32_t split = wg / (NCH * NCO), rest = wg % (NCH * NCO);
+    const int c0 = int(rest % NCH) * CC, co0 = int(rest / NCH) * BM;
+    const int ls = int(g.log_s);
+    const uint32_t S = 1u << ls, m = S - 1u;
+    const uint32_t nblocks = g.nvox >> 8;
+    const uint32_t b_begin = split * blocks_per_split;
+    const uint32_t b_end = b_begin + blocks_per_split < nblocks ? b_begin + blocks_per_split : nblocks;
+    const uint32_t bps = 1u << (3 * ls - 8);
+    const uint32_t nbx = S >> 5, nby = S >> 2;
+    f32x4 acc[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t][0] = acc[t][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    auto load_tiles = [&](uint32_t blk) {
+        const uint32_t smp = blk / bps, b = blk % bps;
+        const uint32_t x0 = (b % nbx) << 5, y0 = ((b / nbx) % nby) << 2, z0 = (b / (nbx * nby)) << 1;
+        const size_t sbase = size_t(smp) << (3 * ls);
+#pragma unroll
+        for (int n = 0; n < NDZ; ++n) {
+            const int t = threadIdx.x + NTH * n;
+            if (t >= 512) break;
+            const int c8 = t >> 7, pv = t & 127;
+            bf16x8 rdz[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int lv = 2 * pv + e;
+                const uint32_t lx = lv & 31, ly = (lv >> 5) & 3, lz = lv >> 7;
+                const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
+                rdz[e] = *reinterpret_cast<const bf16x8*>(dz + v * g.out_stride + co0 + 8 * c8);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                *reinterpret_cast<uint32_t*>(sdz + (8 * c8 + j) * WG_DZS + 2 * pv) = pack2(rdz[0][j], rdz[1][j]);
+        }
+#pragma unroll
+        for (int n = 0; n < NXI; ++n) {
+            const int t = threadIdx.x + NTH * n;
+            if (t >= ROWS * HP * 2) break;
+            const int p = t % HP, row = (t / HP) % ROWS, c8 = t / (HP * ROWS);
+            const int hy = row % HY, hz = row / HY;
+            const uint32_t y = (y0 + uint32_t(hy) - 1u) & m, z = (z0 + uint32_t(hz) - 1u) & m;
+            const size_t rbase = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls));
+            const bool shifted = 2 * p + 1 < 32;
+            bf16x8 rx[3];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const int hx = 2 * p + e;
+                if (e == 2 && !shifted) break;
+                const uint32_t x = (x0 + uint32_t(hx) - 1u) & m;
+                rx[e] = *reinterpret_cast<const bf16x8*>(in + (rbase + x) * g.in_stride + c0 + 8 * c8);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                __bf16* plane0 = sx + (8 * c8 + j) * WG_XPLANE + row * RW + 2 * p;
+                *reinterpret_cast<uint32_t*>(plane0) = pack2(rx[0][j], rx[1][j]);
+                if (shifted)
+                    *reinterpret_cast<uint32_t*>(plane0 + CC * WG_XPLANE) = pack2(rx[1][j], rx[2][j]);
+            }
+        }
+    };
+
+    for (uint32_t blk = b_begin; blk < b_end; ++blk) {
+        // no register prefetch here (the dz^T fragments of all rows take the
+        // registers); the two workgroups per CU overlap loads and MFMAs
+        __syncthreads();   // the previous block's MFMAs are done with the tiles
+        load_tiles(blk);
+        __syncthreads();
+        // this wave's copy and x offset: dx = -1 copy 0 at x, dx = 0 copy 1 at x, dx = +1 copy 0 at x + 2
+        const __bf16* xc = sx + r16 * WG_XPLANE + (wave == 1 ? CC * WG_XPLANE : 0) + 8 * grp + (wave == 2 ? 2 : 0);
+        // one output z-layer at a time: its 4 rows' dz^T fragments in
+        // registers, the 3 x 6 halo rows it touches loaded once each
+#pragma unroll
+        for (int lz = 0; lz < 2; ++lz) {
+            bf16x8 a[4][2];
+#pragma unroll
+            for (int ly = 0; ly < 4; ++ly) {
+                a[ly][0] = *reinterpret_cast<const bf16x8*>(sdz + r16 * WG_DZS + 32 * (4 * lz + ly) + 8 * grp);
+                a[ly][1] = *reinterpret_cast<const bf16x8*>(sdz + (16 + r16) * WG_DZS + 32 * (4 * lz + ly) + 8 * grp);
+            }
+#pragma unroll
+            for (int dzt = -1; dzt <= 1; ++dzt) {
+                const int hz = lz + dzt + 1;
+#pragma unroll
+                for (int hy = 0; hy < HY; ++hy) {
+                    const int rho = hz * HY + hy;
+                    bf16x8 bv;
+                    if (wave == 2) {
+                        const uint32_t* q = reinterpret_cast<const uint32_t*>(xc + rho * RW);
+                        uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
+                        __builtin_memcpy(&bv, w4, 16);
+                    } else {
+                        bv = *reinterpret_cast<const bf16x8*>(xc + rho * RW);
+                    }
+#pragma unroll
+                    for (int dyt = -1; dyt <= 1; ++dyt) {
+                        const int ly = hy - dyt - 1;
+                        if (ly < 0 || ly > 3) continue;
+                        const int t9 = (dzt + 1) * 3 + (dyt + 1);
+                        acc[t9][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ly][0], bv, acc[t9][0], 0, 0, 0);
+                        acc[t9][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ly][1], bv, acc[t9][1], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t9 = 0; t9 < 9; ++t9) {
+        const int tap = t9 * 3 + wave;    // (dz+1)*9 + (dy+1)*3 + (dx+1)
+        const int k = tap * CIN + c0 + r16;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int co = co0 + 16 * i + 4 * grp + rr;
+                partial[(size_t(split) * COUT + co) * KP + k] = acc[t9][i][rr];
+            }
+    }
+}
+
 // dW[i] = sum over splits of partial[s][i], in split order.
 __global__ void __launch_bounds__(256)
 sum_splits(const float* __restrict__ partial, uint32_t n, uint32_t splits, float* __restrict__ out)
@@ -624,8 +769,12 @@ struct ConvLaunch {
             if (g.log_s >= 3) {
                 const uint32_t nblocks = g.nvox >> 8;
                 const uint32_t per = (nblocks + splits - 1) / splits;
-                hipLaunchKernelGGL((conv_wgrad_halo<CIN, COUT>), dim3((CIN / 16) * (COUT / 32) * splits), dim3(256),
-                                   0, s, g, in, dz, partial, per);
+                if (g.log_s >= 5)
+                    hipLaunchKernelGGL((conv_wgrad_rows<CIN, COUT>), dim3((CIN / 16) * (COUT / 32) * splits),
+                                       dim3(192), 0, s, g, in, dz, partial, per);
+                else
+                    hipLaunchKernelGGL((conv_wgrad_halo<CIN, COUT>), dim3((CIN / 16) * (COUT / 32) * splits),
+                                       dim3(256), 0, s, g, in, dz, partial, per);
                 const uint32_t n = uint32_t(COUT) * KP;
                 hipLaunchKernelGGL(sum_splits, dim3((n + 255) / 256), dim3(256), 0, s, partial, n, splits, dw);
                 return;
