@@ -253,8 +253,49 @@ class Scaling:
         return cls(d['rms'], d['tau'], d['force'])
 
 
-def train(net, masks, targets, epochs, lr=1e-3, batch=None, log=None, seed=0, lr_final=None):
-    """Adam over shuffled minibatches; cosine learning-rate decay to lr_final."""
+def symmetry(masks, targets, k, rms):
+    """One of the 16 lattice symmetries that keep a +x body-force problem a
+    +x body-force problem, applied to masks [n, S^3] and normalised targets
+    [n, S^3, 4] = (u_z, u_y, u_x, rho): bit 0 flips z (u_z -> -u_z), bit 1
+    flips y (u_y -> -u_y), bit 2 swaps y and z, bit 3 flips x (the mirrored
+    field of the reversed force, negated: u_y and u_z change sign; the
+    density deviation keeps its sign).  The periodic D3Q19 lattice with
+    halfway bounce-back is invariant under bits 0-2, so the field of the
+    mapped mask is the mapped field exactly; bit 3 holds in the Stokes limit
+    (to relative size ~1e-4 at the training force)."""
+    import torch
+    n = masks.shape[0]
+    s = round(masks.shape[1] ** (1.0 / 3.0))
+    m = masks.reshape(n, s, s, s)
+    t = targets.reshape(n, s, s, s, 4)
+    sign = torch.ones(4, device=t.device, dtype=t.dtype)
+    dims = []
+    if k & 1:
+        dims.append(1)
+        sign[0] = -sign[0]
+    if k & 2:
+        dims.append(2)
+        sign[1] = -sign[1]
+    if k & 8:
+        dims.append(3)
+        sign[0], sign[1] = -sign[0], -sign[1]
+    if dims:
+        m = torch.flip(m, dims)
+        t = torch.flip(t, dims)
+    t = t * sign
+    if k & 4:
+        m = m.transpose(1, 2)
+        t = t.transpose(1, 2)[..., [1, 0, 2, 3]]
+        # channels are normalised per component: carry u_y <-> u_z across their RMS
+        ratio = torch.tensor([rms[1] / rms[0], rms[0] / rms[1], 1.0, 1.0], device=t.device, dtype=t.dtype)
+        t = t * ratio
+    return m.reshape(n, -1).contiguous(), t.reshape(n, -1, 4).contiguous()
+
+
+def train(net, masks, targets, epochs, lr=1e-3, batch=None, log=None, seed=0, lr_final=None, augment=None):
+    """Adam over shuffled minibatches; cosine learning-rate decay to lr_final.
+    With `augment` (the per-channel RMS of the target scaling) every
+    minibatch is mapped by a random one of the 16 lattice symmetries."""
     import torch
     batch = batch or net.capacity
     n = masks.shape[0]
@@ -271,6 +312,8 @@ def train(net, masks, targets, epochs, lr=1e-3, batch=None, log=None, seed=0, lr
         for i in range(n // batch):
             idx = order[i * batch:(i + 1) * batch]
             mb, tb = masks[idx].contiguous(), targets[idx].contiguous()
+            if augment is not None:
+                mb, tb = symmetry(mb, tb, int(rng.integers(16)), augment)
             torch.cuda.synchronize()
             opt.lr = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * k / max(1, steps)))
             tot += net.train_step(mb, tb, opt)
@@ -335,7 +378,7 @@ def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=Non
 
 def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, tau=0.8, force=1e-6,
              train_force=1e-5, train_tol=1e-5, eval_tol=1e-3, interval=50, max_steps=60000, out=None, log=print,
-             relax_sweeps=(5,), init=None):
+             relax_sweeps=(5,), init=None, augment=True):
     """Generate, solve, train (from `init` weights when given), evaluate.
     Returns a report dict."""
     rep = dict(side=side, n_train=n_train, n_eval=n_eval, epochs=epochs, capacity=capacity, lr=lr, tau=tau,
@@ -371,7 +414,9 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
         net.set_params(np.load(init, allow_pickle=False)['params'].astype(np.float32))
         rep['init'] = str(init)
     t0 = time.perf_counter()
-    hist = train(net, masks, targets, epochs, lr=lr, batch=capacity, log=log)
+    hist = train(net, masks, targets, epochs, lr=lr, batch=capacity, log=log,
+                 augment=scaling.rms if augment else None)
+    rep['augment'] = bool(augment)
     rep['training'] = dict(seconds=round(time.perf_counter() - t0, 2), loss=[round(h, 5) for h in hist])
     del masks, targets
     torch.cuda.empty_cache()
