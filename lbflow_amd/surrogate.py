@@ -217,11 +217,16 @@ class Scaling:
     def fit(cls, fields, masks, tau, force):
         import torch
         nu = (tau - 0.5) / 3.0
-        m = masks.reshape(masks.shape[0], -1).bool()
-        raw = fields.clone()
-        raw[..., :3] *= nu / force
-        raw[..., 3] = (raw[..., 3] - 1.0) / force
-        rms = [float(torch.sqrt((raw[..., c][m] ** 2).mean())) for c in range(4)]
+        sq = torch.zeros(4, dtype=torch.float64, device=fields.device)
+        count = 0
+        for i in range(0, fields.shape[0], 256):   # chunks keep every tensor below 2^31 elements
+            m = masks[i:i + 256].reshape(-1).bool()
+            raw = fields[i:i + 256].reshape(-1, 4)[m].double()
+            raw[:, :3] *= nu / force
+            raw[:, 3] = (raw[:, 3] - 1.0) / force
+            sq += (raw ** 2).sum(dim=0)
+            count += raw.shape[0]
+        rms = [float(torch.sqrt(sq[c] / count)) for c in range(4)]
         return cls(rms, tau, force)
 
     def normalise(self, fields, masks):
