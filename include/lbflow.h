@@ -64,6 +64,12 @@ enum lbf_create_flags {
                                 solid sites instead of storing fluid sites only */
 };
 
+/* Colour-gradient treatment at wall sites (input.prm wallGradient,
+ * colorfluid.cpp:458-492): the one-sided stencil as is, or renormalized by
+ * the inverse of its fluid-link second moment (calcWallGradientInverse,
+ * colorfluid.cpp:917-957); both followed by the contact-angle rotation. */
+enum lbf_wall_gradient { LBF_WALL_LEGACY = 0, LBF_WALL_RENORMALIZED = 1 };
+
 /* Simulation description; the fields of input.prm that the step needs
  * (solver/src/file/fileprm.cpp:364-456). */
 typedef struct lbf_desc {
@@ -73,7 +79,7 @@ typedef struct lbf_desc {
     int32_t  precision;        /* lbf_precision */
     int32_t  flags;            /* lbf_create_flags */
     int32_t  harmonic;         /* color viscosityInterpolation: 0 arithmetic, 1 harmonic */
-    int32_t  reserved;
+    int32_t  wall_gradient;    /* color wallGradient: lbf_wall_gradient */
     uint64_t sz[3];            /* grid sizes, slowest first */
     double   tau[2];           /* rexlationTime per phase */
     double   density[2];       /* reference density per phase */

@@ -216,6 +216,7 @@ public:
         d.precision = precision == "fp32" ? LBF_FP32 : LBF_FP64;
         d.flags = layout == "sparse" ? 0 : LBF_STREAM_SOLIDS;
         d.harmonic = color_ && p_.viscosityInterpolation() == "harmonic";
+        d.wall_gradient = color_ && p_.wallGradient() == "renormalized" ? LBF_WALL_RENORMALIZED : LBF_WALL_LEGACY;
         for (size_t k = 0; k < D_; ++k) d.sz[k] = sz_[k];
         for (size_t k = 0; k < NP_; ++k) {
             d.tau[k] = p_.relaxationTime()[k];
@@ -261,7 +262,9 @@ private:
                                             want + "\").");
         };
         if (color_) {
-            need("wallGradient", p_.wallGradient(), "legacy");
+            if (p_.wallGradient() != "legacy" && p_.wallGradient() != "renormalized")
+                throw std::invalid_argument("wallGradient \"" + p_.wallGradient() +
+                                            "\" is not implemented by this engine (only \"legacy\", \"renormalized\").");
             need("capillaryModel", p_.capillaryModel(), "perturbation");
             need("wallRecoloring", p_.wallRecoloring(), "legacy");
             need("phaseBoundary", p_.phaseBoundary(), "bounce");

@@ -238,9 +238,15 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     }
     std::string msg = build_links(d.D, d.Q, d.sz, c->labels.data(), geometry, d.model == LBF_MODEL_COLOR,
                                   c->links);
+    if (msg.empty() && d.model == LBF_MODEL_COLOR && d.wall_gradient == LBF_WALL_RENORMALIZED)
+        msg = wall_gradient_inverse(d.D, d.Q, d.sz, c->labels.data(), geometry, c->links);
     if (!msg.empty()) {
         lbf_destroy(c);
         return fail(nullptr, msg);
+    }
+    if (d.wall_gradient != LBF_WALL_LEGACY && d.wall_gradient != LBF_WALL_RENORMALIZED) {
+        lbf_destroy(c);
+        return fail(nullptr, "wallGradient must be legacy or renormalized in this engine.");
     }
 
     // label table (simulator.cpp:218-228 forceForIndex)
@@ -446,6 +452,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
             for (int k = 0; k < DD; ++k) {
                 walls[w].tib[k] = c->links.tib[w * DD + k];
                 walls[w].tibinv[k] = c->links.tibinv[w * DD + k];
+                if (!c->links.ginv.empty()) walls[w].ginv[k] = c->links.ginv[w * DD + k];
             }
         }
         rc |= upload(c, c->d_walls, walls.data(), walls.size() * sizeof(DevWall));
@@ -539,6 +546,7 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
     a.lab = c->d_lab;
     a.wall = c->d_wall;
     a.walls = c->d_walls;
+    a.renormalized = c->d.wall_gradient == LBF_WALL_RENORMALIZED;
     a.densN = c->d_densN;
     a.A = c->d_A;
     a.B = c->d_B;

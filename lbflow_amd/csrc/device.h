@@ -46,6 +46,7 @@ struct DevWall {
     double sinca, cosca;
     double tib[9];
     double tibinv[9];
+    double ginv[9];      // renormalized wall gradient: inverse fluid-link moment (colorfluid.cpp:917-957)
 };
 
 // Device failure record.
@@ -197,6 +198,7 @@ struct ColorArgs {
     double* uadj_prev;
     double* chiadj_prev;
     const DevZouHe* zh;      // Zou-He boundary per label
+    int32_t renormalized;    // wallGradient renormalized (colorfluid.cpp:487-492)
     DevStatus* status;
     unsigned long long t;
     uint32_t flags;

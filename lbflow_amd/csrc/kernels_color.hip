@@ -17,7 +17,7 @@
 //   calcMacroVar            solver/src/lbm/colorfluid.cpp:238-290
 //   bodyForce               solver/src/lbm/colorfluid.cpp:736-750
 //   collision               solver/src/lbm/colorfluid.cpp:292-433
-//   colorGradient (legacy)  solver/src/lbm/colorfluid.cpp:435-570
+//   colorGradient (legacy / renormalized walls)  solver/src/lbm/colorfluid.cpp:435-570
 //   RotateTx3D etc.         solver/src/algorithm/fn.cpp:118-164
 //   zouHeBoundary (PB/VB)   solver/src/lbm/colorfluid.cpp:638-734 (zouhe.h)
 //
@@ -360,6 +360,23 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
         for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
         norm = sqrt(n2);
         const DevWall& wl = a.walls[w];
+        if (a.renormalized) {
+            // wallGradient renormalized (colorfluid.cpp:487-492): g = inverse g / 3
+            double r[3];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < D; ++b) t = t + g[b] * wl.ginv[c * D + b];
+                r[c] = t;
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) g[c] = r[c] / 3.0;
+            n2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+            norm = sqrt(n2);
+        }
         const double cA = wl.cosca, sA = wl.sinca;
         if constexpr (D == 3) {
             const double* T = wl.tib;

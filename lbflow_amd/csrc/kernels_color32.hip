@@ -316,6 +316,23 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
         for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
         norm = sqrtf(n2);
         const DevWall& wl = a.walls[wi];
+        if (a.renormalized) {
+            // wallGradient renormalized (colorfluid.cpp:487-492): g = inverse g / 3
+            float r[3];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                float t = 0.0f;
+#pragma unroll
+                for (int b = 0; b < D; ++b) t += g[b] * float(wl.ginv[c * D + b]);
+                r[c] = t;
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) g[c] = r[c] * (1.0f / 3.0f);
+            n2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
+            norm = sqrtf(n2);
+        }
         const float cA = float(wl.cosca), sA = float(wl.sinca);
         if constexpr (D == 3) {
             float T[9], Ti[9];

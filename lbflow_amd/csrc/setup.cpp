@@ -2,6 +2,7 @@
 // Compiled without FMA contraction (-ffp-contract=off) like the reference.
 #include "setup.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -208,6 +209,47 @@ std::string build_links(int D, int Q, const uint64_t* sz, const LabelInfo* label
             std::memcpy(&out.tib[size_t(w) * 4], t, sizeof t);
             std::memcpy(&out.tibinv[size_t(w) * 4], ti, sizeof ti);
         }
+    }
+    return "";
+}
+
+std::string wall_gradient_inverse(int D, int Q, const uint64_t* sz, const LabelInfo* labels,
+                                  const uint8_t* geometry, Links& out)
+{
+    uint64_t N = 1;
+    for (int d = 0; d < D; ++d) N *= sz[d];
+    out.ginv.assign(size_t(out.nwall) * D * D, 0.0);
+    for (uint64_t i = 0; i < N; ++i) {
+        if (!out.wall[i]) continue;
+        double aug[3][6] = {};
+        for (int q = 1; q < Q; ++q) {
+            int e[3];
+            for (int d = 0; d < D; ++d) e[d] = Q == 9 ? Lattice<9>::e[q][d] : Lattice<19>::e[q][d];
+            const int type = labels[geometry[neighbour(D, sz, i, e)]].type;
+            if (type == T_SD || type == T_BB) continue;
+            const double w = Q == 9 ? Lattice<9>::w[q] : Lattice<19>::w[q];
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) aug[a][b] += w * e[a] * e[b];
+        }
+        for (int a = 0; a < D; ++a) aug[a][D + a] = 1.0;
+        for (int col = 0; col < D; ++col) {
+            int pivot = col;
+            for (int row = col + 1; row < D; ++row)
+                if (std::abs(aug[row][col]) > std::abs(aug[pivot][col])) pivot = row;
+            if (std::abs(aug[pivot][col]) < 1e-12)
+                return "Rank-deficient fluid wall stencil; reconstructed derivatives are unsupported here.";
+            for (int k = 0; k < 2 * D; ++k) std::swap(aug[col][k], aug[pivot][k]);
+            const double diagonal = aug[col][col];
+            for (int k = 0; k < 2 * D; ++k) aug[col][k] /= diagonal;
+            for (int row = 0; row < D; ++row) {
+                if (row == col) continue;
+                const double scale = aug[row][col];
+                for (int k = 0; k < 2 * D; ++k) aug[row][k] -= scale * aug[col][k];
+            }
+        }
+        double* inv = &out.ginv[size_t(out.wall[i]) * D * D];
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) inv[a * D + b] = aug[a][D + b];
     }
     return "";
 }

@@ -34,8 +34,15 @@ struct Links {
     std::vector<double> normal;     // per wall entry: D components
     std::vector<double> sinca, cosca;
     std::vector<double> tib, tibinv;  // per wall entry: D*D
+    std::vector<double> ginv;       // per wall entry: D*D, renormalized wall gradient
     uint64_t nwall = 1;             // entries incl. the unused entry 0
 };
+
+// Inverse of the fluid-link second moment sum_q w_q e_q e_q^T of every wall
+// site (ColorFluid::calcWallGradientInverse, colorfluid.cpp:917-957) into
+// out.ginv.  Returns an empty string on success, else the reference's message.
+std::string wall_gradient_inverse(int D, int Q, const uint64_t* sz, const LabelInfo* labels,
+                                  const uint8_t* geometry, Links& out);
 
 // Validates the labels used by the geometry and builds the link tables.
 // Returns an empty string on success, else the reference's message.

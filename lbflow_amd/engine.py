@@ -36,7 +36,7 @@ class Label(C.Structure):
 class Desc(C.Structure):
     _fields_ = [('D', C.c_int32), ('Q', C.c_int32), ('model', C.c_int32), ('equil', C.c_int32),
                 ('precision', C.c_int32), ('flags', C.c_int32), ('harmonic', C.c_int32),
-                ('reserved', C.c_int32), ('sz', C.c_uint64 * 3), ('tau', C.c_double * 2),
+                ('wall_gradient', C.c_int32), ('sz', C.c_uint64 * 3), ('tau', C.c_double * 2),
                 ('density', C.c_double * 2), ('alpha0', C.c_double), ('sigma', C.c_double),
                 ('beta', C.c_double)]
 
@@ -176,6 +176,7 @@ class Engine:
         d.precision = precision
         d.flags = (STREAM_SOLIDS if stream_solids else 0) | (LAYOUT_DENSE if dense else 0)
         d.harmonic = 1 if case.get('harmonic') else 0
+        d.wall_gradient = 1 if case.get('wall_gradient') == 'renormalized' else 0
         for k, n in enumerate(case['size']):
             d.sz[k] = n
         tau = list(case['tau']) + [0.0] * (2 - len(case['tau']))

@@ -166,18 +166,18 @@ def zouhe_color_cases(nx=20, ny=10, nz=6, u0=1e-3, rho_in=1.0003):
     return cases
 
 
-def entry_cases(scales=(1,), dimensions=(2,)):
+def entry_cases(scales=(1,), dimensions=(2,), walls=('legacy',)):
     """Capillary entry between Zou-He pressure boundaries in the colour model
-    (scripts/porous_media.py:153-182, legacy wall gradient): phase 0 enters a
-    60-degree slit (2D) or square tube (3D) held by phase 1, at 0.5 and 1.5
-    times the sharp entry pressure."""
+    (scripts/porous_media.py:153-182): phase 0 enters a 60-degree slit (2D) or
+    square tube (3D) held by phase 1, at 0.5 and 1.5 times the sharp entry
+    pressure, with the given wall gradients (legacy, renormalized)."""
     cases = []
     for ndim in dimensions:
         for scale in scales:
             h, nx = 16 * scale, 64 * scale + 1
             sigma = .005 / scale
             pc = (2 if ndim == 2 else 4) * sigma * math.cos(math.radians(60)) / h
-            for ratio in (.5, 1.5):
+            for wall, ratio in [(w, r) for w in walls for r in (.5, 1.5)]:
                 dp = ratio * pc
 
                 def label(x, y, z, h=h, nx=nx, ndim=ndim):
@@ -185,10 +185,10 @@ def entry_cases(scales=(1,), dimensions=(2,)):
                             200 if x == nx - 1 else 0 if x < h else 100)
 
                 cases.append(spec(
-                    'entry-' + ('d3-' if ndim == 3 else '') + f's{scale}-legacy-ratio{ratio}',
+                    'entry-' + ('d3-' if ndim == 3 else '') + f's{scale}-{wall}-ratio{ratio}',
                     nx=nx, ny=h + 2, nz=1 if ndim == 2 else h + 2, dimensions=ndim,
                     steps=8000 * scale ** 2, write=2000 * scale ** 2, tau=(1., 1.), sigma=sigma,
-                    height=h, pressure_drop=dp, label=label,
+                    height=h, pressure_drop=dp, label=label, wall_gradient=wall,
                     indices=[f'50 "PB" "{1 + 3 * dp:.17g}" "1" "0"',
                              f'200 "PB" "1" "{3 if ndim == 2 else 2}" "1"', '0 "VD" "N/A" "N/A" "0"',
                              '100 "VD" "N/A" "N/A" "1"', '255 "SD" "60" "N/A" "N/A"']))
@@ -276,6 +276,7 @@ def lattice_case(s):
                 model=s['model'], eq=s['eq'], tau=tuple(s['tau']), density=tuple(s['density']),
                 alpha=s['alpha'], sigma=s['sigma'], beta=s['beta'],
                 harmonic=s.get('viscosity_interpolation', 'arithmetic') == 'harmonic',
+                wall_gradient=s.get('wall_gradient', 'legacy'),
                 labels=dict(parse_index(line) for line in s['indices']))
 
 

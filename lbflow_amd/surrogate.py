@@ -270,7 +270,8 @@ def symmetry(masks, targets, k, rms):
     (to relative size ~1e-4 at the training force)."""
     import torch
     n = masks.shape[0]
-    s = round(masks.shape[1] ** (1.0 / 3.0))
+    shape = masks.shape
+    s = round(targets.shape[1] ** (1.0 / 3.0))
     m = masks.reshape(n, s, s, s)
     t = targets.reshape(n, s, s, s, 4)
     sign = torch.ones(4, device=t.device, dtype=t.dtype)
@@ -294,7 +295,7 @@ def symmetry(masks, targets, k, rms):
         # channels are normalised per component: carry u_y <-> u_z across their RMS
         ratio = torch.tensor([rms[1] / rms[0], rms[0] / rms[1], 1.0, 1.0], device=t.device, dtype=t.dtype)
         t = t * ratio
-    return m.reshape(n, -1).contiguous(), t.reshape(n, -1, 4).contiguous()
+    return m.reshape(shape).contiguous(), t.reshape(n, -1, 4).contiguous()
 
 
 def train(net, masks, targets, epochs, lr=1e-3, batch=None, log=None, seed=0, lr_final=None, augment=None):

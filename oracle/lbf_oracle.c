@@ -194,6 +194,7 @@ struct orc_sim {
     int32_t* ibI;
     double* ibNorm; double* ibSin; double* ibCos; uint64_t nwall;
     double* tib; double* tibInv;   /* per wall index D*D */
+    double* ginv;                  /* per wall index D*D: renormalized wall gradient */
     double* f[2]; double* tmp[2];  /* [q][N] per phase */
     double* dens[2]; double* densT; double* densN; double* tau; double* v;
     double* adjV; double* adjChi;  /* adjacent report history of the last step */
@@ -344,6 +345,45 @@ static void calc_tib(orc_sim* s)
     }
 }
 
+/* ColorFluid::calcWallGradientInverse (colorfluid.cpp:917-957): per wall
+ * site the inverse of sum over fluid links of w e e^T (Gauss-Jordan with
+ * partial pivoting), for the renormalized wall gradient. */
+static int calc_wall_gradient_inverse(orc_sim* s)
+{
+    const int D = s->D;
+    s->ginv = calloc(s->nwall * D * D, sizeof(double));
+    for (uint64_t i = 0; i < s->N; ++i) {
+        if (!s->ibI[i]) continue;
+        double aug[3][6] = {{0}};
+        for (int q = 1; q < s->Q; ++q) {
+            const int type = s->lab[s->raw[s->adj[i * s->Q + q]]].type;
+            if (type == ORC_SD || type == ORC_BB) continue;
+            for (int a = 0; a < D; ++a)
+                for (int b = 0; b < D; ++b) aug[a][b] += s->wi[q] * s->qi[q][a] * s->qi[q][b];
+        }
+        for (int a = 0; a < D; ++a) aug[a][D + a] = 1.0;
+        for (int col = 0; col < D; ++col) {
+            int pivot = col;
+            for (int row = col + 1; row < D; ++row)
+                if (fabs(aug[row][col]) > fabs(aug[pivot][col])) pivot = row;
+            if (fabs(aug[pivot][col]) < 1e-12)
+                return fail(s, "Rank-deficient fluid wall stencil; reconstructed derivatives are unsupported here.");
+            for (int k = 0; k < 2 * D; ++k) { double t = aug[col][k]; aug[col][k] = aug[pivot][k]; aug[pivot][k] = t; }
+            const double diagonal = aug[col][col];
+            for (int k = 0; k < 2 * D; ++k) aug[col][k] /= diagonal;
+            for (int row = 0; row < D; ++row) {
+                if (row == col) continue;
+                const double scale = aug[row][col];
+                for (int k = 0; k < 2 * D; ++k) aug[row][k] -= scale * aug[col][k];
+            }
+        }
+        double* inv = s->ginv + (uint64_t)s->ibI[i] * D * D;
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) inv[a * D + b] = aug[a][D + b];
+    }
+    return 0;
+}
+
 orc_sim* orc_new(const orc_desc* desc, const orc_label* labels, const uint8_t* geometry,
                  char* err, size_t errlen)
 {
@@ -436,8 +476,12 @@ orc_sim* orc_new(const orc_desc* desc, const orc_label* labels, const uint8_t* g
             for (int q = 0; q < 19; ++q) s->bi[q] = q == 0 ? b0 : (q <= 6 ? b1 : b7);
         }
         calc_tib(s);
+        if (s->d.wall_gradient == 1 && calc_wall_gradient_inverse(s)) {
+            snprintf(err, errlen, "%s", s->err);
+            orc_free(s);
+            return NULL;
+        }
     }
-    (void)err; (void)errlen;
     return s;
 }
 
@@ -446,7 +490,7 @@ void orc_free(orc_sim* s)
     if (!s) return;
     free(s->raw); free(s->bylabel); free(s->adj); free(s->ibI);
     for (int q = 0; q < MAXQ; ++q) free(s->ibs[q]);
-    free(s->ibNorm); free(s->ibSin); free(s->ibCos); free(s->tib); free(s->tibInv);
+    free(s->ibNorm); free(s->ibSin); free(s->ibCos); free(s->tib); free(s->tibInv); free(s->ginv);
     for (int ph = 0; ph < 2; ++ph) { free(s->f[ph]); free(s->tmp[ph]); free(s->dens[ph]); }
     free(s->v); free(s->densT); free(s->densN); free(s->tau); free(s->adjV); free(s->adjChi);
     free(s);
@@ -853,6 +897,20 @@ static void color_gradient(const orc_sim* s, uint64_t i, double* g, double* norm
         norm = sqrt(n2);
         const int c = s->ibI[i];
         const double cA = s->ibCos[c], sA = s->ibSin[c];
+        if (s->d.wall_gradient == 1) {
+            /* colorfluid.cpp:487-492: g = MatMulVec(inverse, g) / 3 (fn.cpp:29-39) */
+            const double* inv = s->ginv + (uint64_t)c * D * D;
+            double r[3];
+            for (int a = 0; a < D; ++a) {
+                double t = 0.0;
+                for (int b = 0; b < D; ++b) t = t + g[b] * inv[a * D + b];
+                r[a] = t;
+            }
+            for (int a = 0; a < D; ++a) g[a] = r[a] / 3.0;
+            n2 = 0.0;
+            for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+            norm = sqrt(n2);
+        }
         if (D == 3) {
             double gn[3], txInv[9], tInv[9], h[3];
             mat_mul(s->tib + c * 9, g, 3, 3, 1, gn);

@@ -52,7 +52,7 @@ typedef struct orc_desc {
     int32_t model;         /* 0 = scsp, 1 = color */
     int32_t equil;         /* 0 = SRT, 1 = MRT */
     int32_t harmonic;      /* color viscosity interpolation: 0 arithmetic, 1 harmonic */
-    int32_t pad;
+    int32_t wall_gradient; /* color wallGradient: 0 legacy, 1 renormalized */
     uint64_t sz[3];        /* grid size in reference order: slowest first (z,y,x / y,x) */
     double tau[2];         /* relaxation time per phase */
     double density[2];     /* reference density per phase */

@@ -53,7 +53,7 @@ def main():
                                             for h in m['history']],
                                 'height': m['height'], 'pressure_drop': m['pressure_drop']}
                     for key in ('entry', 'entry3d') for m in porous[key]['measurements']
-                    if m['wall_gradient'] == 'legacy' and m['scale'] == 1}
+                    if m['wall_gradient'] in ('legacy', 'renormalized') and m['scale'] == 1}
     pr = porous['pressure_report']
     out['pressure_report'] = {k: pr[k] for k in ('name', 'prescribed_pressure_drop', 'report_slice_pressure_drop',
                                                  'expected_report_slice_pressure', 'reported_K_um2',

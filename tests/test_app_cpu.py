@@ -43,7 +43,7 @@ def base():
     ({'reportDir': 3, 'reportSlice': '1 6'}, 'Report slice is outside the selected axis.'),
     ({'fluidModel': '"shanchen"'}, 'Only scsp and color fluid models are implemented.'),
     ({'fluidModel': '"lattice-gas"'}, 'Main(): Unsupport fluid model'),
-    ({'wallGradient': '"renormalized"'}, 'wallGradient "renormalized" is not implemented by this engine'),
+    ({'wallGradient': '"tanh"'}, 'wallGradient "tanh" is not implemented by this engine'),
 ])
 def test_invalid_inputs_fail_with_reference_message(tmp_path, overrides, message):
     spec = dict(base(), overrides=overrides)

@@ -243,3 +243,44 @@ def test_zouhe_color_adjacent_report(precision):
     else:
         np.testing.assert_allclose(list(r.conv)[1:3], ref[1:3], rtol=1e-4)
         assert abs(r.phase_change[1] - np.max(np.abs(chi - adj_chi))) < 1e-4
+
+
+# ------------------------------------------------- renormalized wall gradient
+@pytest.mark.parametrize('kw', [dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, porosity=0.8),
+                                dict(model='color', eq='SRT', tau=(1.0, 0.8), two_phase=True, angle=120,
+                                     porosity=0.8, seed=4),
+                                dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, dims=2, angle=45)],
+                         ids=['mrt-3d', 'srt-3d-120', 'mrt-2d-45'])
+def test_renormalized_wall_bitwise(kw):
+    """wallGradient renormalized (colorfluid.cpp:487-492, 917-957) on curved walls:
+    FP64 populations equal to the oracle's bit for bit; FP32 sparse == dense.  (Denser
+    3D packs have wall sites whose fluid links span fewer than 3 directions; the
+    reference rejects those, see the next test.)"""
+    from tests.helpers import sphere_case
+    spec = dict(sphere_case(**kw), wall_gradient='renormalized')
+    say('case', spec['name'], kw)
+    e, _ = engine_run(spec, 150)
+    o, _ = oracle_run(spec, 150)
+    assert np.array_equal(e.populations().view(np.uint64), o.populations().view(np.uint64))
+    b, _ = engine_run(spec, 100, FP32, stream_solids=False, dense=True)
+    c, _ = engine_run(spec, 100, FP32, stream_solids=False)
+    assert np.array_equal(b.populations(), c.populations())
+
+
+@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2, 3), walls=('renormalized',)), ids=lambda s: s['name'])
+def test_entry_history_renormalized_gpu(spec, records):
+    test_entry_history_gpu(spec, records)
+
+
+def test_renormalized_wall_rejects_rank_deficient_stencils():
+    """A wall site whose fluid links do not span all axes: the reference throws
+    (colorfluid.cpp:940-941); the engine refuses the geometry with the same text."""
+    from tests.helpers import sphere_case
+    from lbflow_amd.engine import EngineError
+    from oracle.pyoracle import Oracle
+    spec = dict(sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True), wall_gradient='renormalized')
+    msg = 'Rank-deficient fluid wall stencil; reconstructed derivatives are unsupported here.'
+    with pytest.raises(Exception, match=msg):
+        Oracle(H.lattice_case(spec), H.geometry(spec))
+    with pytest.raises(EngineError, match=msg):
+        Engine(H.lattice_case(spec), H.geometry(spec), FP64, True)

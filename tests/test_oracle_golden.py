@@ -112,10 +112,12 @@ def entry_measure(r0, r1, u, case, height):
             [float(r0.sum()), float(r1.sum())])
 
 
-@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2,)), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2,), walls=('legacy', 'renormalized')),
+                         ids=lambda s: s['name'])
 def test_entry_history(spec, records):
     """Colour-model capillary entry between Zou-He pressure boundaries
-    (porous_media.py:153-253): the recorded displacement history is reproduced exactly."""
+    (porous_media.py:153-253), legacy and renormalized wall gradients: the recorded
+    displacement history is reproduced exactly."""
     from oracle.pyoracle import Oracle
     case = H.lattice_case(spec)
     o = Oracle(case, H.geometry(spec)).init()

@@ -39,7 +39,8 @@ def base():
 @pytest.mark.parametrize('k,tol', [(1, 1e-10), (2, 1e-10), (4, 1e-10), (7, 1e-10), (8, 2e-3), (15, 2e-3)])
 def test_symmetry_maps_solutions_to_solutions(base, k, tol):
     labels, field = base
-    mask = torch.as_tensor((labels == 75).reshape(1, -1).astype(np.uint8))
+    # masks as the training pipeline holds them: [n, S, S, S]
+    mask = torch.as_tensor((labels == 75).reshape(1, N, N, N).astype(np.uint8))
     tgt = torch.as_tensor(field.reshape(1, -1, 4))
     m2, t2 = symmetry(mask, tgt, k, np.ones(4))
     lab2 = np.where(m2.numpy().reshape(N, N, N) == 1, 75, 255).astype(np.uint8)
