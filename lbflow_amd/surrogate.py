@@ -263,11 +263,11 @@ def symmetry(masks, targets, k, rms):
     +x body-force problem, applied to masks [n, S^3] and normalised targets
     [n, S^3, 4] = (u_z, u_y, u_x, rho): bit 0 flips z (u_z -> -u_z), bit 1
     flips y (u_y -> -u_y), bit 2 swaps y and z, bit 3 flips x (the mirrored
-    field of the reversed force, negated: u_y and u_z change sign; the
-    density deviation keeps its sign).  The periodic D3Q19 lattice with
-    halfway bounce-back is invariant under bits 0-2, so the field of the
-    mapped mask is the mapped field exactly; bit 3 holds in the Stokes limit
-    (to relative size ~1e-4 at the training force)."""
+    field of the reversed force, negated: u_y, u_z and the density
+    deviation change sign).  Targets are 0 on solid sites.  The periodic
+    D3Q19 lattice with halfway bounce-back is invariant under bits 0-2, so
+    the field of the mapped mask is the mapped field exactly; bit 3 holds in
+    the Stokes (linear) limit, to relative size ~1e-4 at the training force."""
     import torch
     n = masks.shape[0]
     shape = masks.shape
@@ -284,7 +284,7 @@ def symmetry(masks, targets, k, rms):
         sign[1] = -sign[1]
     if k & 8:
         dims.append(3)
-        sign[0], sign[1] = -sign[0], -sign[1]
+        sign[0], sign[1], sign[3] = -sign[0], -sign[1], -sign[3]
     if dims:
         m = torch.flip(m, dims)
         t = torch.flip(t, dims)

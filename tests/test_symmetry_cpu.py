@@ -4,9 +4,9 @@
 field.  For flips of z and y and the y <-> z swap the D3Q19 MRT lattice with
 halfway bounce-back is exactly invariant, so running the oracle on the
 mapped geometry must give the mapped field (to summation-order rounding);
-the x flip with the transverse velocity negated is a symmetry of the
-linear (Stokes) limit only, so it holds to the size of the quadratic
-equilibrium terms (1e-4 relative measured at F = 1e-5).
+the x flip with the transverse velocity and the density deviation negated
+is a symmetry of the linear (Stokes) limit only, so it holds to the size of
+the quadratic equilibrium terms (1e-4 relative measured at F = 1e-5).
 """
 import numpy as np
 import pytest
@@ -26,8 +26,10 @@ def run(labels):
     o = Oracle(H.lattice_case(s), H.geometry(s)).init()
     o.step(STEPS)
     r, _, u = o.fields()
-    # (u_z, u_y, u_x, rho - 1) per site, unnormalised (rms 1 per channel)
-    return np.stack([u[0], u[1], u[2], r - 1.0], axis=-1)
+    # (u_z, u_y, u_x, rho - 1) per site, unnormalised (rms 1 per channel),
+    # 0 on solid sites like the training targets
+    fluid = (labels.ravel() == 75)[:, None]
+    return np.stack([u[0], u[1], u[2], r - 1.0], axis=-1) * fluid
 
 
 @pytest.fixture(scope='module')
@@ -48,3 +50,14 @@ def test_symmetry_maps_solutions_to_solutions(base, k, tol):
     got = t2.numpy().reshape(-1, 4)
     err = np.abs(got - want).max(axis=0) / np.abs(want).max(axis=0)
     assert np.all(err < tol), (k, err)
+
+
+def test_density_parity_is_pinned(base):
+    """The x flip must negate the density deviation: keeping its sign is caught."""
+    labels, field = base
+    mask = torch.as_tensor((labels == 75).reshape(1, N, N, N).astype(np.uint8))
+    m2, t2 = symmetry(mask, torch.as_tensor(field.reshape(1, -1, 4)), 8, np.ones(4))
+    lab2 = np.where(m2.numpy().reshape(N, N, N) == 1, 75, 255).astype(np.uint8)
+    want = run(lab2)
+    wrong = t2.numpy().reshape(-1, 4)[:, 3] * -1.0
+    assert np.abs(wrong - want[:, 3]).max() / np.abs(want[:, 3]).max() > 1.0
