@@ -42,10 +42,12 @@ __global__ void __launch_bounds__(256) report_partial(ReportArgs a)
         const uint32_t desc = a.desc[i];
         const int kind = a.lab[desc >> DESC_LABEL_SHIFT].kind;
         const bool space = kind & KIND_FLUID;
-        int c[3];
-        site_coords(a.g, i, c[2], c[1], c[0]);   // x, y, z
-        // coordinate along the report axis (reference order, slowest first)
-        const int along = a.D == 3 ? c[a.axis] : c[a.axis + 1];
+        int x, y, z;
+        site_coords(a.g, i, x, y, z);
+        // coordinate along the report axis (reference order, slowest first:
+        // z y x in 3D, y x in 2D), selected without a runtime-indexed array
+        const int ax = a.D == 3 ? a.axis : a.axis + 1;
+        const int along = ax == 0 ? z : (ax == 1 ? y : x);
         const double r0 = a.rho0[i];
         const double r1 = a.color ? a.rho1[i] : 0.0;
         if (space) {
@@ -67,9 +69,11 @@ __global__ void __launch_bounds__(256) report_partial(ReportArgs a)
                 acc[S_U2] += v * v;
                 acc[S_COUNT] += 1.0;
                 if (a.adjacent) {
-                    // PB/VB sites: the reference copies the adjacent velocity after
-                    // the boundary has set this step's value (scsp.cpp:79-83)
-                    const double adj = (kind & KIND_ZOUHE) ? v : a.uadj[k];
+                    // single-phase PB/VB sites: the reference copies the adjacent
+                    // velocity after the boundary has set this step's value, which
+                    // calcMacroVar keeps (scsp.cpp:79-83); colour PB/VB sites carry
+                    // their boundary velocity in the history buffer itself
+                    const double adj = ((kind & KIND_ZOUHE) && !a.color) ? v : a.uadj[k];
                     if (!isfinite(adj)) acc[S_NONFINITE] += 1.0;
                     acc[S_DA2] += (v - adj) * (v - adj);
                 }

@@ -5,7 +5,13 @@
 //
 // The boundary direction dir (1..2D) is a property of the label, so the
 // tables are selected by a switch and every population index stays a
-// compile-time constant (the populations live in registers).
+// compile-time constant (the populations live in registers).  The arms of
+// the switch only compute: they return the new unknown populations with
+// their directions (ZhSite), and one write-back over compile-time indices
+// stores them.  Arms that each ended by storing to different elements of
+// the population array would be merged by the compiler into one store
+// through a pointer chosen per arm, which moves those elements to scratch
+// memory for the whole kernel.
 //
 // FP64 keeps the reference's operation order: the known-population sum
 // 2 sum_{Qrel[op dir]} f + sum_{Qprep[dir]} f, then the boundary density and
@@ -31,167 +37,168 @@
 
 namespace lbf {
 
-template <int Q, int DIR>
-__device__ __forceinline__ void zouhe_f64_dir(double* f, const DevZouHe& z, double& rho, double* u)
+// Unknown populations of one PB/VB site in the order of ZouHe::rel[dir],
+// with the boundary density (FP32: its deviation from rho0) and the axial
+// velocity.
+template <int Q, typename T>
+struct ZhSite {
+    T val[ZouHe<Q>::NREL];
+    int idx[ZouHe<Q>::NREL];
+    T rho;
+    T uax;
+    int axis;
+};
+
+template <int Q, typename T>
+__device__ __forceinline__ void zh_store(T* f, const ZhSite<Q, T>& z)
+{
+#pragma unroll
+    for (int q = 1; q < Q; ++q)
+#pragma unroll
+        for (int r = 0; r < ZouHe<Q>::NREL; ++r)
+            if (z.idx[r] == q) f[q] = z.val[r];
+}
+
+template <int Q, typename T>
+__device__ __forceinline__ void zh_velocity(const ZhSite<Q, T>& z, T* u)
+{
+#pragma unroll
+    for (int d = 0; d < Lattice<Q>::D; ++d) u[d] = d == z.axis ? z.uax : T(0);
+}
+
+// Total population of the site: f (single-phase) or f0 + f1 (colour).
+template <int Q, int DIR, typename T, typename Pop>
+__device__ __forceinline__ ZhSite<Q, T> zouhe_unknowns_f64(Pop pop, const DevZouHe& z)
 {
 #pragma clang fp contract(off)
     using L = Lattice<Q>;
     using Z = ZouHe<Q>;
     constexpr int D = L::D;
     constexpr int AX = Z::axis[DIR], SG = Z::sign[DIR], OPD = L::op[DIR];
-    double sumF = 0;
+    ZhSite<Q, T> o;
+    T sumF = 0;
 #pragma unroll
-    for (int r = 0; r < Z::NREL; ++r) sumF += f[Z::rel[OPD][r]] + f[Z::rel[OPD][r]];
+    for (int r = 0; r < Z::NREL; ++r) sumF += pop.twice(Z::rel[OPD][r]);
 #pragma unroll
-    for (int r = 0; r < Z::NPERP; ++r) sumF += f[Z::perp[DIR][r]];
-#pragma unroll
-    for (int d = 0; d < D; ++d) u[d] = 0.0;
+    for (int r = 0; r < Z::NPERP; ++r) sumF += pop(Z::perp[DIR][r]);
     if (z.type == ZH_VB) {
-        const double velocity = SG == 1 ? z.value : -z.value;
-        const double vel_1 = 1.0 / (1.0 - velocity);
-        u[AX] = velocity;
-        rho = sumF * vel_1;
+        const T velocity = SG == 1 ? z.value : -z.value;
+        const T vel_1 = 1.0 / (1.0 - velocity);
+        o.uax = velocity;
+        o.rho = sumF * vel_1;
     } else {
-        rho = z.value;
+        o.rho = z.value;
         sumF /= z.value;
         sumF = sumF - 1.0;
-        u[AX] = SG == 1 ? -sumF : sumF;
+        o.uax = SG == 1 ? -sumF : sumF;
     }
+    o.axis = AX;
 #pragma unroll
     for (int r = 0; r < Z::NREL; ++r) {
         const int q = Z::rel[DIR][r];
-        double fq = f[L::op[q]];
-        fq += double(SG) * (L::w[q] * 6.0) * rho * u[AX];
-        double s = 0;
+        T fq = pop(L::op[q]);
+        fq += T(SG) * (L::w[q] * 6.0) * o.rho * o.uax;
+        T s = 0;
 #pragma unroll
         for (int n = 0; n < Z::NPERP; ++n) {
             const int qn = Z::perp[DIR][n];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 if (L::e[qn][d] == 0 || L::e[q][d] == 0) continue;
-                if (L::e[qn][d] == L::e[q][d]) s += f[qn];
-                else s -= f[qn];
+                if (L::e[qn][d] == L::e[q][d]) s += pop(qn);
+                else s -= pop(qn);
             }
         }
         fq -= s * 0.5;
-        f[q] = fq;
+        o.val[r] = fq;
+        o.idx[r] = q;
+    }
+    return o;
+}
+
+// Single-phase populations: sumF adds f + f for the reflected set.
+struct PopF64 {
+    const double* f;
+    __device__ double operator()(int q) const { return f[q]; }
+    __device__ double twice(int q) const { return f[q] + f[q]; }
+};
+
+// Colour populations: the total f0 + f1, the reflected set as f0 + f0 + f1 + f1
+// (colorfluid.cpp:652-657 sums both phases' terms in that order).
+struct PopColorF64 {
+    const double* f0;
+    const double* f1;
+    __device__ double operator()(int q) const { return f0[q] + f1[q]; }
+    __device__ double twice(int q) const { return f0[q] + f0[q] + f1[q] + f1[q]; }
+};
+
+template <int Q, typename Pop>
+__device__ __forceinline__ ZhSite<Q, double> zouhe_dispatch_f64(Pop pop, const DevZouHe& z)
+{
+    switch (z.dir) {
+    case 1: return zouhe_unknowns_f64<Q, 1, double>(pop, z);
+    case 2: return zouhe_unknowns_f64<Q, 2, double>(pop, z);
+    case 3: return zouhe_unknowns_f64<Q, 3, double>(pop, z);
+    default:
+        if constexpr (Q == 19) {
+            if (z.dir == 5) return zouhe_unknowns_f64<Q, 5, double>(pop, z);
+            if (z.dir == 6) return zouhe_unknowns_f64<Q, 6, double>(pop, z);
+        }
+        return zouhe_unknowns_f64<Q, 4, double>(pop, z);
     }
 }
 
 template <int Q>
 __device__ __forceinline__ void zouhe_f64(double* f, const DevZouHe& z, double& rho, double* u)
 {
-    switch (z.dir) {
-    case 1: zouhe_f64_dir<Q, 1>(f, z, rho, u); break;
-    case 2: zouhe_f64_dir<Q, 2>(f, z, rho, u); break;
-    case 3: zouhe_f64_dir<Q, 3>(f, z, rho, u); break;
-    case 4: zouhe_f64_dir<Q, 4>(f, z, rho, u); break;
-    default:
-        if constexpr (Q == 19) {
-            if (z.dir == 5) zouhe_f64_dir<Q, 5>(f, z, rho, u);
-            else zouhe_f64_dir<Q, 6>(f, z, rho, u);
-        }
-        break;
-    }
+    const ZhSite<Q, double> o = zouhe_dispatch_f64<Q>(PopF64{f}, z);
+    zh_store<Q>(f, o);
+    rho = o.rho;
+    zh_velocity<Q>(o, u);
 }
 
-template <int Q, int DIR>
-__device__ __forceinline__ void zouhe_color_f64_dir(double* f0, double* f1, const DevZouHe& z, int phs,
-                                                    double& rhoT, double* u)
-{
-#pragma clang fp contract(off)
-    using L = Lattice<Q>;
-    using Z = ZouHe<Q>;
-    constexpr int D = L::D;
-    constexpr int AX = Z::axis[DIR], SG = Z::sign[DIR], OPD = L::op[DIR];
-    double sumF = 0;
-#pragma unroll
-    for (int r = 0; r < Z::NREL; ++r) {
-        const int q = Z::rel[OPD][r];
-        sumF += f0[q] + f0[q] + f1[q] + f1[q];
-    }
-#pragma unroll
-    for (int r = 0; r < Z::NPERP; ++r) {
-        const int q = Z::perp[DIR][r];
-        sumF += f0[q] + f1[q];
-    }
-#pragma unroll
-    for (int d = 0; d < D; ++d) u[d] = 0.0;
-    if (z.type == ZH_VB) {
-        const double velocity = SG == 1 ? z.value : -z.value;
-        const double vel_1 = 1.0 / (1.0 - velocity);
-        u[AX] = velocity;
-        rhoT = sumF * vel_1;
-    } else {
-        rhoT = z.value;
-        sumF /= z.value;
-        sumF = sumF - 1.0;
-        u[AX] = SG == 1 ? -sumF : sumF;
-    }
-#pragma unroll
-    for (int r = 0; r < Z::NREL; ++r) {
-        const int q = Z::rel[DIR][r];
-        double fT = f0[L::op[q]] + f1[L::op[q]];
-        fT += double(SG) * (L::w[q] * 6.0) * rhoT * u[AX];
-        double s = 0;
-#pragma unroll
-        for (int n = 0; n < Z::NPERP; ++n) {
-            const int qn = Z::perp[DIR][n];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                if (L::e[qn][d] == 0 || L::e[q][d] == 0) continue;
-                if (L::e[qn][d] == L::e[q][d]) s += f0[qn] + f1[qn];
-                else s -= f0[qn] + f1[qn];
-            }
-        }
-        fT -= s * 0.5;
-        f0[q] = phs == 0 ? fT : 0.0;
-        f1[q] = phs == 0 ? 0.0 : fT;
-    }
-}
-
+// Colour: each unknown total goes to the label's phase, the other phase gets 0.
 template <int Q>
-__device__ __forceinline__ void zouhe_color_f64(double* f0, double* f1, const DevZouHe& z, int phs, double& rhoT,
-                                                double* u)
+__device__ __forceinline__ void zouhe_color_f64(double* f0, double* f1, const DevZouHe& z, int phs,
+                                                double& rhoT, double* u)
 {
-    switch (z.dir) {
-    case 1: zouhe_color_f64_dir<Q, 1>(f0, f1, z, phs, rhoT, u); break;
-    case 2: zouhe_color_f64_dir<Q, 2>(f0, f1, z, phs, rhoT, u); break;
-    case 3: zouhe_color_f64_dir<Q, 3>(f0, f1, z, phs, rhoT, u); break;
-    case 4: zouhe_color_f64_dir<Q, 4>(f0, f1, z, phs, rhoT, u); break;
-    default:
-        if constexpr (Q == 19) {
-            if (z.dir == 5) zouhe_color_f64_dir<Q, 5>(f0, f1, z, phs, rhoT, u);
-            else zouhe_color_f64_dir<Q, 6>(f0, f1, z, phs, rhoT, u);
-        }
-        break;
-    }
+    const ZhSite<Q, double> o = zouhe_dispatch_f64<Q>(PopColorF64{f0, f1}, z);
+#pragma unroll
+    for (int q = 1; q < Q; ++q)
+#pragma unroll
+        for (int r = 0; r < ZouHe<Q>::NREL; ++r)
+            if (o.idx[r] == q) {
+                f0[q] = phs == 0 ? o.val[r] : 0.0;
+                f1[q] = phs == 0 ? 0.0 : o.val[r];
+            }
+    rhoT = o.rho;
+    zh_velocity<Q>(o, u);
 }
 
 template <int Q, int DIR>
-__device__ __forceinline__ void zouhe_f32_dir(float* h, const DevZouHe& z, float rho0, float& drho, float* u)
+__device__ __forceinline__ ZhSite<Q, float> zouhe_unknowns_f32(const float* h, const DevZouHe& z, float rho0)
 {
 #pragma clang fp contract(off)
     using L = Lattice<Q>;
     using Z = ZouHe<Q>;
     constexpr int D = L::D;
     constexpr int AX = Z::axis[DIR], SG = Z::sign[DIR], OPD = L::op[DIR];
+    ZhSite<Q, float> o;
     float sumH = 0.0f;
 #pragma unroll
     for (int r = 0; r < Z::NREL; ++r) sumH += h[Z::rel[OPD][r]] + h[Z::rel[OPD][r]];
 #pragma unroll
     for (int r = 0; r < Z::NPERP; ++r) sumH += h[Z::perp[DIR][r]];
-#pragma unroll
-    for (int d = 0; d < D; ++d) u[d] = 0.0f;
     if (z.type == ZH_VB) {
-        u[AX] = z.a;
-        drho = (sumH + z.b) * z.c;
+        o.uax = z.a;
+        o.rho = (sumH + z.b) * z.c;
     } else {
         const float s = (sumH + z.b) * z.c;
-        u[AX] = SG == 1 ? -s : s;
-        drho = -z.b;
+        o.uax = SG == 1 ? -s : s;
+        o.rho = -z.b;
     }
-    const float rho = rho0 + drho;
+    o.axis = AX;
+    const float rho = rho0 + o.rho;
 #pragma unroll
     for (int r = 0; r < Z::NREL; ++r) {
         const int q = Z::rel[DIR][r];
@@ -206,59 +213,55 @@ __device__ __forceinline__ void zouhe_f32_dir(float* h, const DevZouHe& z, float
                 else s -= h[qn];
             }
         }
-        h[q] = h[L::op[q]] + float(SG * 6.0 * L::w[q]) * rho * u[AX] - 0.5f * s;
+        o.val[r] = h[L::op[q]] + float(SG * 6.0 * L::w[q]) * rho * o.uax - 0.5f * s;
+        o.idx[r] = q;
     }
-}
-
-// Colour FP32: the unknowns of the total (shifted) population as in the
-// single-phase form, then the phase-0 population of each unknown: fT if the
-// label is phase 0, else 0 (fT = h + ai0 rho_0).
-template <int Q, int DIR>
-__device__ __forceinline__ void zouhe_color_f32_dir(float* h, float* f0, const DevZouHe& z, int phs, float rho0,
-                                                    const double* dist0, float& drho, float* u)
-{
-    using Z = ZouHe<Q>;
-    zouhe_f32_dir<Q, DIR>(h, z, rho0, drho, u);
-#pragma unroll
-    for (int r = 0; r < Z::NREL; ++r) {
-        const int q = Z::rel[DIR][r];
-        f0[q] = phs == 0 ? h[q] + float(dist0[q]) : 0.0f;
-    }
+    return o;
 }
 
 template <int Q>
-__device__ __forceinline__ void zouhe_color_f32(float* h, float* f0, const DevZouHe& z, int phs, float rho0,
-                                                const double* dist0, float& drho, float* u)
+__device__ __forceinline__ ZhSite<Q, float> zouhe_dispatch_f32(const float* h, const DevZouHe& z, float rho0)
 {
     switch (z.dir) {
-    case 1: zouhe_color_f32_dir<Q, 1>(h, f0, z, phs, rho0, dist0, drho, u); break;
-    case 2: zouhe_color_f32_dir<Q, 2>(h, f0, z, phs, rho0, dist0, drho, u); break;
-    case 3: zouhe_color_f32_dir<Q, 3>(h, f0, z, phs, rho0, dist0, drho, u); break;
-    case 4: zouhe_color_f32_dir<Q, 4>(h, f0, z, phs, rho0, dist0, drho, u); break;
+    case 1: return zouhe_unknowns_f32<Q, 1>(h, z, rho0);
+    case 2: return zouhe_unknowns_f32<Q, 2>(h, z, rho0);
+    case 3: return zouhe_unknowns_f32<Q, 3>(h, z, rho0);
     default:
         if constexpr (Q == 19) {
-            if (z.dir == 5) zouhe_color_f32_dir<Q, 5>(h, f0, z, phs, rho0, dist0, drho, u);
-            else zouhe_color_f32_dir<Q, 6>(h, f0, z, phs, rho0, dist0, drho, u);
+            if (z.dir == 5) return zouhe_unknowns_f32<Q, 5>(h, z, rho0);
+            if (z.dir == 6) return zouhe_unknowns_f32<Q, 6>(h, z, rho0);
         }
-        break;
+        return zouhe_unknowns_f32<Q, 4>(h, z, rho0);
     }
 }
 
 template <int Q>
 __device__ __forceinline__ void zouhe_f32(float* h, const DevZouHe& z, float rho0, float& drho, float* u)
 {
-    switch (z.dir) {
-    case 1: zouhe_f32_dir<Q, 1>(h, z, rho0, drho, u); break;
-    case 2: zouhe_f32_dir<Q, 2>(h, z, rho0, drho, u); break;
-    case 3: zouhe_f32_dir<Q, 3>(h, z, rho0, drho, u); break;
-    case 4: zouhe_f32_dir<Q, 4>(h, z, rho0, drho, u); break;
-    default:
-        if constexpr (Q == 19) {
-            if (z.dir == 5) zouhe_f32_dir<Q, 5>(h, z, rho0, drho, u);
-            else zouhe_f32_dir<Q, 6>(h, z, rho0, drho, u);
-        }
-        break;
-    }
+    const ZhSite<Q, float> o = zouhe_dispatch_f32<Q>(h, z, rho0);
+    zh_store<Q>(h, o);
+    drho = o.rho;
+    zh_velocity<Q>(o, u);
+}
+
+// Colour FP32: the unknowns of the total (shifted) population as in the
+// single-phase form, then the phase-0 population of each unknown: fT if the
+// label is phase 0, else 0 (fT = h + ai0 rho_0).
+template <int Q>
+__device__ __forceinline__ void zouhe_color_f32(float* h, float* f0, const DevZouHe& z, int phs, float rho0,
+                                                const double* dist0, float& drho, float* u)
+{
+    const ZhSite<Q, float> o = zouhe_dispatch_f32<Q>(h, z, rho0);
+#pragma unroll
+    for (int q = 1; q < Q; ++q)
+#pragma unroll
+        for (int r = 0; r < ZouHe<Q>::NREL; ++r)
+            if (o.idx[r] == q) {
+                h[q] = o.val[r];
+                f0[q] = phs == 0 ? o.val[r] + float(dist0[q]) : 0.0f;
+            }
+    drho = o.rho;
+    zh_velocity<Q>(o, u);
 }
 
 }  // namespace lbf
