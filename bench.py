@@ -152,6 +152,31 @@ def time_to_permeability(spec, labels, interval, max_steps):
     return out
 
 
+def ttp_warm_vs_cold(labels, interval, max_steps):
+    """Config-2 time to permeability from rest and from the warm-start
+    network (trained on 64^3 crops; fully convolutional and periodic, so it
+    runs on the whole 256^3 lattice): the batched FP32 engine with one crop,
+    same stop rule (residual 1e-6, reports every `interval` steps), wall
+    clock from init to the stop, inference and set-up included."""
+    from lbflow_amd import surrogate as S
+    weights = ROOT / 'lbflow_amd' / 'weights' / 'warm64.npz'
+    crops = [np.ascontiguousarray(labels)]
+    cold, t_cold, _ = S.run_crops(crops, 0.8, 1e-6, 1e-6, interval, max_steps)
+    net, scaling = S.load(weights, capacity=1, side=labels.shape[0])
+    warm, t_warm, t_inf = S.run_crops(crops, 0.8, 1e-6, 1e-6, interval, max_steps, net, scaling, relax=5)
+    net.close()
+    out = dict(cold=dict(steps=cold[0]['steps'], converged=bool(cold[0]['converged']), K_lattice=cold[0]['K'],
+                         seconds=round(t_cold, 3)),
+               warm=dict(steps=warm[0]['steps'], converged=bool(warm[0]['converged']), K_lattice=warm[0]['K'],
+                         seconds=round(t_warm, 3), inference_and_relax_seconds=round(t_inf, 3)),
+               report_interval=interval)
+    out['speedup_iterations'] = round(cold[0]['steps'] / max(1, warm[0]['steps']), 3)
+    out['speedup_wall'] = round(t_cold / t_warm, 3)
+    out['k_rel_diff_warm_vs_cold'] = abs(warm[0]['K'] - cold[0]['K']) / cold[0]['K']
+    log('time-to-permeability warm vs cold', out)
+    return out
+
+
 def batch_crops(ncrops):
     """The batched workload's crops (before the GPU is touched: a process pool forks)."""
     from lbflow_amd import geometry as G
@@ -202,6 +227,8 @@ def main():
     ap.add_argument('--ttp', action='store_true', help='also run FP64 and FP32 to convergence (time-to-K)')
     ap.add_argument('--ttp-interval', type=int, default=500)
     ap.add_argument('--ttp-max-steps', type=int, default=100000)
+    ap.add_argument('--ttp-warm', action='store_true',
+                    help='also run config 2 to convergence from rest and from the warm-start network')
     ap.add_argument('--colour', action='store_true', help='also time the two-phase colour model (128x128x256)')
     ap.add_argument('--batch', type=int, default=0, help='also run N batched crops to convergence (512)')
     ap.add_argument('--batch-interval', type=int, default=200)
@@ -213,6 +240,11 @@ def main():
         sys.exit(2)
 
     crops = batch_crops(args.batch) if args.batch else None
+    if args.ttp_warm:
+        # the network's tensors live in torch's HIP runtime, which must start
+        # before the engine's (two runtimes share the process)
+        from lbflow_amd import surrogate as S
+        S._torch()
     from lbflow_amd.engine import FP32, FP64, copy_bandwidth
     spec, labels = workload(args.size)
     porosity = float((labels == 75).mean())
@@ -235,6 +267,8 @@ def main():
         record('fp64_dense_all_sites', measure(spec, labels, FP64, True, max(10, args.steps // 4), 5))
     if args.ttp:
         extra['time_to_permeability'] = time_to_permeability(spec, labels, args.ttp_interval, args.ttp_max_steps)
+    if args.ttp_warm:
+        extra['time_to_permeability_warm_vs_cold'] = ttp_warm_vs_cold(labels, args.ttp_interval, args.ttp_max_steps)
     if args.colour:
         extra['colour'] = colour_bench(max(10, args.steps // 2), 5, args.fp64)
     if args.batch:

@@ -285,7 +285,10 @@ int lbf_net_create(uint32_t side, uint32_t capacity, uint64_t seed, lbf_net** ou
     uint32_t ls = 0;
     while ((1u << ls) < side) ++ls;
     const uint64_t v0 = uint64_t(capacity) << (3 * ls);
-    if (v0 * 384 >= (uint64_t(1) << 32)) return nfail(nullptr, "lbf_net_create: batch too large for 32-bit indexing");
+    // voxel indices and per-launch thread counts (voxels x channels / 8, at
+    // most 96 channels at the finest level) stay 32-bit; element offsets are 64-bit
+    if (v0 >= (uint64_t(1) << 31) || v0 * 96 / 8 >= (uint64_t(1) << 31))
+        return nfail(nullptr, "lbf_net_create: batch too large for 32-bit voxel indexing");
     lbf_net* n = new lbf_net();
     n->side = side;
     n->ls = ls;

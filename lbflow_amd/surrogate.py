@@ -183,25 +183,30 @@ def labels(force):
     return {PORE: ('GV', force, 1, 0), SOLID: ('SD', 90.0, 0, 0)}
 
 
-def solve(crops, tau=0.8, force=1e-5, tol=1e-5, interval=100, max_steps=40000, log=None):
-    """Steady fields of every crop with the batched engine.  Returns (fields as a
-    float32 CUDA tensor [n, S^3, 4] = (u_z, u_y, u_x, rho), per-crop results)."""
+def solve(crops, tau=0.8, force=1e-5, tol=1e-5, interval=100, max_steps=40000, log=None, chunk=4096):
+    """Steady fields of every crop with the batched engine, `chunk` crops per
+    batch.  Returns (fields as a float32 CUDA tensor [n, S^3, 4] =
+    (u_z, u_y, u_x, rho), per-crop results)."""
     torch = _torch()
-    b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), crops)
-    b.tune(steps=4)
-    b.init()
-    res, sec = b.run(axis=2, tol=tol, interval=interval, max_steps=max_steps)
-    if log:
-        log(f'solved {len(crops)} crops in {sec:.1f} s, steps {min(r["steps"] for r in res)}-'
-            f'{max(r["steps"] for r in res)}, converged {sum(r["converged"] for r in res)}')
     s3 = crops[0].size
     fields = torch.empty((len(crops), s3, 4), dtype=torch.float32, device='cuda')
     L = lib()
-    torch.cuda.synchronize()
-    for c in range(len(crops)):
-        if L.lbf_batch_crop_fields(b.h, c, C.c_void_p(fields[c].data_ptr())):
-            raise EngineError(L.lbf_batch_error(b.h).decode())
-    b.close()
+    res = []
+    for c0 in range(0, len(crops), chunk):
+        part = crops[c0:c0 + chunk]
+        b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), part)
+        b.tune(steps=4)
+        b.init()
+        r, sec = b.run(axis=2, tol=tol, interval=interval, max_steps=max_steps)
+        res += r
+        if log:
+            log(f'solved crops {c0}-{c0 + len(part) - 1} in {sec:.1f} s, steps {min(x["steps"] for x in r)}-'
+                f'{max(x["steps"] for x in r)}, converged {sum(x["converged"] for x in r)}')
+        torch.cuda.synchronize()
+        for c in range(len(part)):
+            if L.lbf_batch_crop_fields(b.h, c, C.c_void_p(fields[c0 + c].data_ptr())):
+                raise EngineError(L.lbf_batch_error(b.h).decode())
+        b.close()
     return fields, res
 
 
@@ -337,9 +342,11 @@ def save(path, net, scaling, half=True):
                         side=net.side)
 
 
-def load(path, capacity=4):
+def load(path, capacity=4, side=None):
+    """Weights and target scaling; the network is fully convolutional and
+    periodic, so it can be built for another cubic side than it was trained on."""
     d = np.load(path, allow_pickle=False)
-    net = Net(side=int(d['side']), capacity=capacity)
+    net = Net(side=int(d['side']) if side is None else side, capacity=capacity)
     net.set_params(d['params'].astype(np.float32))
     return net, Scaling.from_json(json.loads(str(d['scaling'])))
 

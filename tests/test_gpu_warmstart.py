@@ -5,8 +5,11 @@ Uses the network trained on the GPU by `python -m lbflow_amd.surrogate`
 lbflow_amd/weights/warm64.npz.  On unseen crops (seeds 20000+i) the same
 batched solver runs to the same stop rule (Simulator::updateConvergence,
 simulator.cpp:248-325, tolerance 1e-3) from rest and from the network's
-prediction; the warm runs must need fewer iterations and give the same
-permeability within the tolerance.
+prediction; the warm runs must need fewer iterations, and their
+permeability must be as close to the tightly converged one (tolerance 1e-5)
+as the cold runs' is: a stop at residual 1e-3 leaves K uncertain by about
+1e-3 either way, so the warm K is checked against the tight K with the cold
+K's own error as the yardstick.
 """
 import sys
 from pathlib import Path
@@ -32,12 +35,17 @@ def test_warm_start_beats_cold_start():
     tau, force, tol, interval = 0.8, 1e-6, 1e-3, 50
     cold, t_cold, _ = S.run_crops(crops, tau, force, tol, interval, 60000)
     warm, t_warm, t_inf = S.run_crops(crops, tau, force, tol, interval, 60000, net, scaling, relax=5)
+    tight, _, _ = S.run_crops(crops, tau, force, 1e-5, interval, 60000)
     sc = np.array([r['steps'] for r in cold])
     sw = np.array([r['steps'] for r in warm])
     kc = np.array([r['K'] for r in cold])
     kw = np.array([r['K'] for r in warm])
-    say('cold steps', sc, 'warm steps', sw, 'K rel', np.abs(kw - kc) / kc)
+    kt = np.array([r['K'] for r in tight])
+    ec, ew = np.abs(kc - kt) / kt, np.abs(kw - kt) / kt
+    say('cold steps', sc, 'warm steps', sw)
+    say('K error vs tight: cold', ec, 'warm', ew)
     say('cold s', t_cold, 'warm s', t_warm, 'inference', t_inf)
-    assert all(r['converged'] for r in cold) and all(r['converged'] for r in warm)
+    assert all(r['converged'] for r in cold + warm + tight)
     assert sw.sum() < sc.sum()
-    assert np.max(np.abs(kw - kc) / kc) < 1e-2
+    assert ew.mean() <= 1.5 * ec.mean() + 1e-4
+    assert ew.max() < 2e-2
