@@ -30,7 +30,10 @@ extern "C" {
 
 #define LBF_ABI_VERSION 1
 
-/* Label types: same bit values as FilePrm::IDXTYPE (solver/inc/file/fileprm.h:44-53). */
+/* Label types: same bit values as FilePrm::IDXTYPE (solver/inc/file/fileprm.h:44-53).
+ * lbf_create accepts VD, GV, PB, VB, BB, SD, and HF in the colour model
+ * (half-permeable film: the site holds the label's phase only and reflects
+ * the other phase, simulator.cpp:418-421, colorfluid.cpp:156-159, 752-764). */
 enum lbf_label_type {
     LBF_VD = 0x1, LBF_GV = 0x2, LBF_PB = 0x4, LBF_VB = 0x8,
     LBF_AB = 0x10, LBF_BB = 0x20, LBF_HF = 0x40, LBF_SD = 0x80

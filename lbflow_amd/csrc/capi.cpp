@@ -58,6 +58,7 @@ struct lbf_ctx {
     float *d_A32 = nullptr, *d_B32 = nullptr;
     int32_t* d_wall = nullptr;
     DevWall* d_walls = nullptr;
+    uint64_t* d_film = nullptr;      // per-site film links, only with HF labels
     double* d_densN = nullptr;
     double *d_rho0 = nullptr, *d_rho1 = nullptr, *d_u = nullptr, *d_uadj = nullptr, *d_uprev = nullptr;
     double *d_chiadj = nullptr, *d_chiprev = nullptr;
@@ -217,10 +218,17 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         for (int l = 0; l < 256; ++l) {
             if (!used[l] || !labels[l].type) continue;
             const int t = labels[l].type;
-            if (t == LBF_AB || t == LBF_HF) {
+            if (t == LBF_AB) {
                 lbf_destroy(c);
                 return fail(nullptr, "lbf_create: label " + std::to_string(l) +
-                                         " uses a boundary type (AB/HF) this engine does not implement");
+                                         " uses a boundary type (AB) this engine does not implement");
+            }
+            if (t == LBF_HF && d.model != LBF_MODEL_COLOR) {
+                // the reference's single-phase model never initialises or
+                // collides HF sites (scsp.cpp:116-175, 374-464)
+                lbf_destroy(c);
+                return fail(nullptr, "lbf_create: label " + std::to_string(l) +
+                                         " is a half-permeable film (HF), which this engine supports in the colour model only");
             }
             if ((t == LBF_PB || t == LBF_VB) && (labels[l].dir < 1 || labels[l].dir > 2 * d.D)) {
                 lbf_destroy(c);
@@ -230,7 +238,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
                 lbf_destroy(c);
                 return fail(nullptr, "Pressure-boundary density must be positive.");
             }
-            if ((t == LBF_VD || t == LBF_GV || t == LBF_PB || t == LBF_VB) && labels[l].phase >= NP) {
+            if ((t == LBF_VD || t == LBF_GV || t == LBF_PB || t == LBF_VB || t == LBF_HF) && labels[l].phase >= NP) {
                 lbf_destroy(c);
                 return fail(nullptr, "Boundary phase is outside the configured fluid model.");
             }
@@ -257,6 +265,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         const LabelInfo& li = c->labels[l];
         if (li.type == T_VD || li.type == T_GV) L.kind = KIND_FLUID;
         if (li.type == T_PB || li.type == T_VB) L.kind = KIND_FLUID | KIND_ZOUHE;
+        if (li.type == T_HF) L.kind = KIND_FLUID | KIND_FILM;
         L.phase = li.phase;
         if (li.type == T_GV) {
             if (li.dir < 0 || li.dir >= d.Q || !std::isfinite(li.value)) {
@@ -359,6 +368,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         rc |= dalloc(c, &c->d_densN, N * 8);
         rc |= dalloc(c, &c->d_wall, N * 4);
         rc |= dalloc(c, &c->d_walls, c->links.nwall * sizeof(DevWall));
+        if (!c->links.film.empty()) rc |= dalloc(c, &c->d_film, N * sizeof(uint64_t));
     }
     if (rc) {
         std::string e = c->err;
@@ -456,6 +466,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
             }
         }
         rc |= upload(c, c->d_walls, walls.data(), walls.size() * sizeof(DevWall));
+        if (c->d_film) rc |= upload(c, c->d_film, c->links.film.data(), N * sizeof(uint64_t));
     }
     if (rc) {
         std::string e = c->err;
@@ -546,6 +557,7 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
     a.lab = c->d_lab;
     a.wall = c->d_wall;
     a.walls = c->d_walls;
+    a.film = c->d_film;
     a.renormalized = c->d.wall_gradient == LBF_WALL_RENORMALIZED;
     a.densN = c->d_densN;
     a.A = c->d_A;

@@ -14,6 +14,7 @@ enum : int32_t {
     KIND_FORCED = 0x2,   // GV with a non-zero force vector
     KIND_GV = 0x4,       // GV: body force shifts the velocity
     KIND_ZOUHE = 0x8,    // PB / VB: density and velocity from the Zou-He boundary
+    KIND_FILM = 0x10,    // HF: half-permeable film of the label's phase
 };
 
 // Zou-He boundary of one label (SCSP::zouHeBoundary, scsp.cpp:188-256).
@@ -219,6 +220,11 @@ struct ColorArgs {
     const uint32_t* list;          // SPARSE: fluid site of each slot
     const uint2* rank;             // SPARSE: rank index (see stream.h)
     uint32_t P;                    // slots per plane
+    // Half-permeable films (simulator.cpp:418-421, colorfluid.cpp:156-159):
+    // per site, bit q = phase 0 bounces back at link q (the q-neighbour is a
+    // film of phase 1), bit 32 + q = phase 1 bounces back (film of phase 0);
+    // read only where the descriptor has DESC_FILM.
+    const uint64_t* film;
 };
 
 struct ReportArgs {

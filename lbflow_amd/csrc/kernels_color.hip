@@ -63,7 +63,12 @@ __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint3
     int x, y, z;
     site_coords(a.g, i, x, y, z);
     const uint32_t links = desc & DESC_LINK_MASK;
-    if (lab.kind & KIND_ZOUHE) {
+    if ((lab.kind & (KIND_ZOUHE | KIND_FILM)) || (desc & DESC_FILM)) {
+        // Boundary sites, in the reference's order: bounce-back at solid
+        // links, then each phase bounces back at links into a film of the
+        // other phase, the film empties the other phase (clearHalfFilm,
+        // colorfluid.cpp:752-764), then the Zou-He boundary.
+        const uint64_t film = (desc & DESC_FILM) ? a.film[i] : 0;
         double f0[Q], f1[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -72,10 +77,19 @@ __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint3
             if (q == 0) src = i;
             else if ((links >> L::op[q]) & 1u) { src = i; qs = L::op[q]; }
             else src = pull_source<Q>(a.g, x, y, z, q);
-            f0[q] = a.fin[uint64_t(qs) * N + src];
-            f1[q] = a.fin[(uint64_t(Q) + qs) * N + src];
+            const bool back0 = q != 0 && ((film >> L::op[q]) & 1u);
+            const bool back1 = q != 0 && ((film >> (32 + L::op[q])) & 1u);
+            f0[q] = a.fin[uint64_t(back0 ? L::op[q] : qs) * N + (back0 ? i : src)];
+            f1[q] = a.fin[(uint64_t(Q) + (back1 ? L::op[q] : qs)) * N + (back1 ? i : src)];
         }
-        zouhe_color_f64<Q>(f0, f1, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, rzh, uzh);
+        if (lab.kind & KIND_FILM) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if (lab.phase == 0) f1[q] = 0.0;
+                else f0[q] = 0.0;
+            }
+        }
+        if (lab.kind & KIND_ZOUHE) zouhe_color_f64<Q>(f0, f1, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, rzh, uzh);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             if (q == 0) { r0 = f0[q]; r1 = f1[q]; }

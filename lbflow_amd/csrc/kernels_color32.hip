@@ -73,15 +73,62 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
     O off[Q];
     off[0] = O(slot);
     pull_offsets<Q, LAYOUT, O>(a.rank, desc & DESC_LINK_MASK, slot, stn, a.g.nx, uint32_t(P), off);
+    // The boundary path is chosen before the loads, so that plain sites can
+    // fold their populations into the sums as they arrive.
+    const bool film_links = desc & DESC_FILM;
+    if (!film_links && !(lab.kind & (KIND_FILM | KIND_ZOUHE))) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        h[q] = ld(a.fin32, off[q]);
-        f0[q] = ld(a.fin32, off[q] + f0plane);
-        r0 += f0[q];
-        dr += h[q];
-    }
-    if (lab.kind & KIND_ZOUHE) {
-        zouhe_color_f32<Q>(h, f0, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, float(a.rho_ref), a.dist0, dzh, uzh);
+        for (int q = 0; q < Q; ++q) {
+            h[q] = ld(a.fin32, off[q]);
+            f0[q] = ld(a.fin32, off[q] + f0plane);
+            r0 += f0[q];
+            dr += h[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            h[q] = ld(a.fin32, off[q]);
+            f0[q] = ld(a.fin32, off[q] + f0plane);
+        }
+        if (film_links) {
+            // A phase that bounces back at a link into a film of the other
+            // phase takes its own outgoing population of the opposite
+            // direction; the other phase keeps the pulled one.  With
+            // fT = h + s and f1 = fT - f0:
+            //   phase 0 back: f0 = f0_loc, h = f0_loc + (h_up - f0_up);
+            //   phase 1 back: f0 = f0_up,  h = f0_up + (h_loc - f0_loc).
+            const uint64_t film = a.film[site];
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                const int oq = L::op[q];
+                const bool back0 = (film >> oq) & 1u, back1 = (film >> (32 + oq)) & 1u;
+                if (back0 || back1) {
+                    const O loc = O(oq) * P + O(slot);
+                    const float h_loc = ld(a.fin32, loc), f0_loc = ld(a.fin32, loc + f0plane);
+                    if (back0) {
+                        h[q] = f0_loc + (h[q] - f0[q]);
+                        f0[q] = f0_loc;
+                    } else {
+                        h[q] = f0[q] + (h_loc - f0_loc);
+                    }
+                }
+            }
+        }
+        if (lab.kind & KIND_FILM) {
+            // clearHalfFilm (colorfluid.cpp:752-764): phase-0 film: f1 = 0,
+            // so fT = f0; phase-1 film: f0 = 0, so fT = f1 = h + s - f0
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if (lab.phase == 0) {
+                    h[q] = f0[q] - float(a.dist0[q]);
+                } else {
+                    h[q] = h[q] - f0[q];
+                    f0[q] = 0.0f;
+                }
+            }
+        }
+        if (lab.kind & KIND_ZOUHE)
+            zouhe_color_f32<Q>(h, f0, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, float(a.rho_ref), a.dist0, dzh, uzh);
         r0 = 0.0f;
         dr = 0.0f;
 #pragma unroll

@@ -114,8 +114,11 @@ enum : int { T_VD = 0x1, T_GV = 0x2, T_PB = 0x4, T_VB = 0x8, T_AB = 0x10, T_BB =
              T_HF = 0x40, T_SD = 0x80 };
 
 // Per-site descriptor word: bits 1..Q-1 = bounce-back links (site in ibS[q]),
-// bits 24..31 = geometry label.
+// bit 19 = film links present, bits 24..31 = geometry label.
 constexpr unsigned DESC_LABEL_SHIFT = 24;
 constexpr unsigned DESC_LINK_MASK = 0x7ffffu;
+// Bit 19: the site has a link into a half-permeable film site of the other
+// phase for one of its phases (the per-site film word holds which).
+constexpr unsigned DESC_FILM = 1u << 19;
 
 }  // namespace lbf

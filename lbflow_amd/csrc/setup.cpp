@@ -157,6 +157,7 @@ std::string build_links(int D, int Q, const uint64_t* sz, const LabelInfo* label
             double norm[3] = {0.0, 0.0, 0.0}, ca = 0;
             int nCA = 0;
             uint32_t bits = 0;
+            uint64_t film = 0;
             for (int q = 1; q < Q; ++q) {
                 int e[3];
                 for (int d = 0; d < D; ++d) e[d] = Q == 9 ? Lattice<9>::e[q][d] : Lattice<19>::e[q][d];
@@ -167,7 +168,15 @@ std::string build_links(int D, int Q, const uint64_t* sz, const LabelInfo* label
                     for (int d = 0; d < D; ++d) norm[d] -= e[d];
                     ca += labels[nb].value;
                     nCA++;
+                } else if (type == T_HF) {
+                    // ibF[phase][q] (simulator.cpp:418-421): the other phase bounces back
+                    film |= uint64_t(1) << (labels[nb].phase == 1 ? q : 32 + q);
                 }
+            }
+            if (film) {
+                if (out.film.empty()) out.film.assign(N, 0);
+                out.film[i] = film;
+                out.desc[i] |= DESC_FILM;
             }
             if (bits) {
                 out.desc[i] |= bits;

@@ -35,6 +35,7 @@ struct Links {
     std::vector<double> sinca, cosca;
     std::vector<double> tib, tibinv;  // per wall entry: D*D
     std::vector<double> ginv;       // per wall entry: D*D, renormalized wall gradient
+    std::vector<uint64_t> film;     // per site: film links (ibF), empty without HF labels
     uint64_t nwall = 1;             // entries incl. the unused entry 0
 };
 

@@ -166,6 +166,59 @@ def zouhe_color_cases(nx=20, ny=10, nz=6, u0=1e-3, rho_in=1.0003):
     return cases
 
 
+def film_cases(nx=20, ny=8, nz=5, force=1e-5, transparent=False):
+    """Half-permeable films (HF) in the colour model, in a channel between
+    60-degree walls (y = 0, ny - 1), periodic along x:
+
+    - film-block: phase-0 film planes at x = 2 and x = 12 enclose a phase-1
+      slab (3 <= x <= 11) pushed along +x by a body force; a phase-1 film
+      plane at x = 7 sits inside the slab; phase 0 elsewhere.
+    - film-zh: a phase-0 velocity inlet (VB, +x) at x = 0 feeds through a
+      phase-0 film at x = 1 into a channel held by phase 1 that leaves
+      through a phase-1 film at x = nx - 2 and a phase-1 pressure outlet
+      (PB, -x) at x = nx - 1.
+
+    With transparent=True the film-block geometry holds phase 0 only (the
+    phase-1 labels become phase 0), for the check that a film of the only
+    phase present changes nothing."""
+    cases = []
+    for dims in (3, 2):
+        zdim = nz if dims == 3 else 1
+        for eq in ('SRT', 'MRT'):
+            def lab_block(x, y, z):
+                if y in (0, ny - 1):
+                    return 255
+                if x in (2, 12):
+                    return 30
+                if x == 7:
+                    return 31
+                return 101 if 3 <= x <= 11 else 0
+
+            ph1 = 0 if transparent else 1
+            cases.append(spec(f'film-block-{dims}d-{eq}' + ('-transparent' if transparent else ''),
+                              nx=nx, ny=ny, nz=zdim, dimensions=dims, model='color', eq=eq,
+                              tau=(1.0, 0.8), steps=300, write=100, sigma=0.005, label=lab_block,
+                              indices=['0 "VD" "N/A" "N/A" "0"', '30 "HF" "N/A" "N/A" "0"',
+                                       f'31 "HF" "N/A" "N/A" "{ph1}"',
+                                       f'101 "GV" "{force:.17g}" "1" "{ph1}"',
+                                       '255 "SD" "60" "N/A" "N/A"']))
+            if transparent:
+                continue
+
+            def lab_zh(x, y, z):
+                if y in (0, ny - 1):
+                    return 255
+                return {0: 50, 1: 30, nx - 2: 31, nx - 1: 200}.get(x, 0 if x < 5 else 100)
+
+            cases.append(spec(f'film-zh-{dims}d-{eq}', nx=nx, ny=ny, nz=zdim, dimensions=dims, model='color',
+                              eq=eq, tau=(1.0, 0.8), steps=300, write=100, sigma=0.005, label=lab_zh,
+                              indices=['50 "VB" "0.001" "1" "0"', '30 "HF" "N/A" "N/A" "0"',
+                                       '31 "HF" "N/A" "N/A" "1"', f'200 "PB" "1" "{3 if dims == 2 else 2}" "1"',
+                                       '0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"',
+                                       '255 "SD" "60" "N/A" "N/A"']))
+    return cases
+
+
 def entry_cases(scales=(1,), dimensions=(2,), walls=('legacy',)):
     """Capillary entry between Zou-He pressure boundaries in the colour model
     (scripts/porous_media.py:153-182): phase 0 enters a 60-degree slit (2D) or

@@ -191,6 +191,7 @@ struct orc_sim {
     uint64_t* bylabel; uint64_t start[257];
     uint64_t* adj;          /* N*Q periodic neighbours (Grid::calcAdjI) */
     uint64_t* ibs[MAXQ]; uint64_t nibs[MAXQ];
+    uint64_t* ibf[2][MAXQ]; uint64_t nibf[2][MAXQ];   /* ibF[phase][q]: q-neighbour is an HF site of that phase */
     int32_t* ibI;
     double* ibNorm; double* ibSin; double* ibCos; uint64_t nwall;
     double* tib; double* tibInv;   /* per wall index D*D */
@@ -288,6 +289,8 @@ static void build_ib(orc_sim* s)
     const int set = ORC_VD | ORC_PB | ORC_VB | ORC_GV | ORC_HF;
     uint64_t c = 1;
     for (int q = 0; q < s->Q; ++q) { s->ibs[q] = malloc(sizeof(uint64_t) * (s->N + 1)); s->nibs[q] = 0; }
+    for (int ph = 0; ph < 2; ++ph)
+        for (int q = 0; q < s->Q; ++q) { s->ibf[ph][q] = malloc(sizeof(uint64_t) * (s->N + 1)); s->nibf[ph][q] = 0; }
     s->ibNorm = calloc((s->N + 1) * 3, sizeof(double));
     s->ibSin = calloc(s->N + 1, sizeof(double));
     s->ibCos = calloc(s->N + 1, sizeof(double));
@@ -306,6 +309,9 @@ static void build_ib(orc_sim* s)
                     for (int d = 0; d < s->D; ++d) norm[d] -= s->qi[q][d];
                     ca += s->lab[s->raw[a]].value;
                     nCA++;
+                } else if (type == ORC_HF) {   /* half-permeable film, simulator.cpp:418-421 */
+                    const int phs = s->lab[s->raw[a]].phase;
+                    s->ibf[phs][q][s->nibf[phs][q]++] = i;
                 }
             }
             if (isIB) {
@@ -410,7 +416,7 @@ orc_sim* orc_new(const orc_desc* desc, const orc_label* labels, const uint8_t* g
         }
     for (int l = 0; l < 256; ++l) {
         int t = s->lab[l].type;
-        if (t == ORC_AB || t == ORC_HF) {
+        if (t == ORC_AB || (t == ORC_HF && desc->model == 0)) {
             snprintf(err, errlen, "oracle: label type %d is outside the restated scope", t);
             free(s->raw); free(s); return NULL;
         }
@@ -489,7 +495,7 @@ void orc_free(orc_sim* s)
 {
     if (!s) return;
     free(s->raw); free(s->bylabel); free(s->adj); free(s->ibI);
-    for (int q = 0; q < MAXQ; ++q) free(s->ibs[q]);
+    for (int q = 0; q < MAXQ; ++q) { free(s->ibs[q]); free(s->ibf[0][q]); free(s->ibf[1][q]); }
     free(s->ibNorm); free(s->ibSin); free(s->ibCos); free(s->tib); free(s->tibInv); free(s->ginv);
     for (int ph = 0; ph < 2; ++ph) { free(s->f[ph]); free(s->tmp[ph]); free(s->dens[ph]); }
     free(s->v); free(s->densT); free(s->densN); free(s->tau); free(s->adjV); free(s->adjChi);
@@ -1054,6 +1060,48 @@ static void stream_and_bounce(orc_sim* s, int ph)
     }
 }
 
+/* Half-permeable film (ColorFluid::Run, colorfluid.cpp:156-159): each phase
+ * bounces back at links into a film site of the other phase (bounceBack with
+ * ibF, simulator.cpp:103-111), then clearHalfFilm (colorfluid.cpp:752-764)
+ * empties the other phase at every film site. */
+static void film_bounce_and_clear(orc_sim* s)
+{
+    const uint64_t N = s->N;
+    for (int ph = 0; ph < 2; ++ph) {
+        const int other = 1 - ph;
+        for (int q = 1; q < s->Q; ++q) {
+            const int opq = s->op[q];
+            for (uint64_t k = 0; k < s->nibf[other][q]; ++k) {
+                const uint64_t i = s->ibf[other][q][k];
+                s->f[ph][(uint64_t)opq * N + i] = s->tmp[ph][(uint64_t)q * N + i];
+            }
+        }
+    }
+    for (int label = 0; label < 256; ++label) {
+        if (!label_in(s, label, ORC_HF)) continue;
+        const int clear = s->lab[label].phase == 0 ? 1 : 0;
+        for (uint64_t k = s->start[label]; k < s->start[label + 1]; ++k) {
+            const uint64_t i = s->bylabel[k];
+            for (int q = 0; q < s->Q; ++q) s->f[clear][(uint64_t)q * N + i] = 0.0;
+        }
+    }
+}
+
+/* The streaming half of a time step: streaming, bounce-back, the film and
+ * the Zou-He boundaries, leaving the pre-collision populations in place. */
+static void stream_half(orc_sim* s)
+{
+    for (int ph = 0; ph < s->NP; ++ph) stream_and_bounce(s, ph);
+    if (s->d.model == 0) {
+        scsp_zouhe(s);
+        return;
+    }
+    film_bounce_and_clear(s);
+    color_zouhe(s);
+}
+
+void orc_stream_only(orc_sim* s) { stream_half(s); }
+
 int orc_init(orc_sim* s)
 {
     s->t = 0;
@@ -1072,15 +1120,13 @@ int orc_init(orc_sim* s)
 int orc_step(orc_sim* s)
 {
     s->t++;
-    for (int ph = 0; ph < s->NP; ++ph) stream_and_bounce(s, ph);
+    stream_half(s);
     if (s->d.model == 0) {
-        scsp_zouhe(s);
         record_adjacent(s);
         if (scsp_macro(s)) return 1;
         scsp_body_force(s);
         return scsp_collision(s);
     }
-    color_zouhe(s);
     record_adjacent(s);
     if (color_macro(s)) return 1;
     color_body_force(s);

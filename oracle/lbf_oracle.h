@@ -72,6 +72,10 @@ void orc_free(orc_sim* s);
 int orc_init(orc_sim* s);
 /* One time step t: stream, bounce back, macro, body force, collision. */
 int orc_step(orc_sim* s);
+/* Test hook: the streaming half of a step (streaming, bounce-back, the
+ * half-permeable film and the Zou-He boundaries) applied to the current
+ * populations, which then hold the pre-collision state; time is unchanged. */
+void orc_stream_only(orc_sim* s);
 /* Run n steps; stops at the first error. */
 int orc_run(orc_sim* s, uint64_t n);
 const char* orc_error(const orc_sim* s);

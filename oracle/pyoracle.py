@@ -42,6 +42,8 @@ def lib():
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = C.c_int
         L.orc_run.argtypes = [vp, u64]
+        L.orc_stream_only.argtypes = [vp]
+        L.orc_stream_only.restype = None
         L.orc_run.restype = C.c_int
         L.orc_error.argtypes = [vp]
         L.orc_error.restype = C.c_char_p
@@ -155,6 +157,11 @@ class Oracle:
     def set_populations(self, pops, t):
         pops = np.ascontiguousarray(pops, dtype=np.float64)
         lib().orc_set_populations(self.h, _ptr(pops), t)
+
+    def stream_only(self):
+        """Streaming half of a step on the current populations (test hook)."""
+        lib().orc_stream_only(self.h)
+        return self
 
     def links(self):
         mask = np.zeros(self.N, np.uint32)

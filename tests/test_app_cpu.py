@@ -88,9 +88,18 @@ def test_missing_key(tmp_path):
 
 
 def test_unsupported_boundary_type(tmp_path):
-    spec = dict(base(), indices=['75 "HF" "N/A" "N/A" "1"'])
+    spec = dict(base(), indices=['75 "AB" "N/A" "N/A" "N/A"'])
     code, out = run_case(tmp_path, spec)
-    assert code != 0 and 'AB/HF' in out
+    assert code != 0 and 'Axisymmetric AB boundary handling is not implemented.' in out
+
+
+def test_film_needs_the_colour_model(tmp_path):
+    """HF labels run in the colour model only: the reference's single-phase
+    model never initialises or collides HF sites (scsp.cpp:116-175)."""
+    scsp = [s for s in H.force_cases() if s['model'] == 'scsp'][-1]
+    spec = dict(scsp, indices=['75 "HF" "N/A" "N/A" "0"'])
+    code, out = run_case(tmp_path, spec)
+    assert code != 0 and 'half-permeable film (HF)' in out
 
 
 def test_library_exports_every_declared_symbol():

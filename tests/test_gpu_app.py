@@ -114,6 +114,30 @@ def test_restart_is_bitwise_and_failure_contracts(tmp_path):
         assert timing is None
 
 
+@pytest.mark.parametrize('name', ['film-zh-3d-MRT', 'film-block-2d-SRT'])
+def test_film_case_through_process(name, tmp_path):
+    """Half-permeable films through the drop-in: the VTI fields equal the
+    oracle's bit for bit, and a restart from the step-200 checkpoint writes
+    the same step-300 files."""
+    spec = dict(next(s for s in H.film_cases() if s['name'] == name), restart=100)
+    cont = tmp_path / 'continuous'
+    code, out, timing = run(spec, cont)
+    assert code == 0, out[-2000:]
+    assert timing['completed_steps'] == spec['steps']
+    _, f = H.read_vti(cont / f'field.{spec["steps"]:010d}.vti')
+    o, case = oracle_run(spec, spec['steps'])
+    r0, r1, u = o.fields()
+    assert np.array_equal(f['r0'][..., 0].ravel(), r0)
+    assert np.array_equal(f['r1'][..., 0].ravel(), r1)
+    assert np.array_equal(f['v'], velocity_grid(u, case))
+    resumed = tmp_path / 'resumed'
+    code, out, timing = run(spec, resumed, 200, cont / 'field.0000000200')
+    assert code == 0, out[-2000:]
+    for suffix in ('.vti', '.rst'):
+        last = f'field.{spec["steps"]:010d}{suffix}'
+        assert (cont / last).read_bytes() == (resumed / last).read_bytes()
+
+
 def report_lines(out):
     return [l for l in out.splitlines() if l.startswith(' - ')]
 
