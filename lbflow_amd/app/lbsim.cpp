@@ -97,71 +97,146 @@ void append(const string& path, const void* data, size_t bytes)
     std::fclose(f);
 }
 
-// VTI writer, one piece (file/filevti.cpp:39-172): header, appended raw
-// blocks [uint64 byte length][Float64 ...], scalars in name order, then
-// vectors with interleaved (x, y, z) components.
+// Sub-box of one rank of an R-rank reference run (sizes in reference order,
+// slowest first): GridMc::initCoreDistribution and initLclGs
+// (grid/gridmc.cpp:55-95) with the rank's core coordinates from CalcI2P
+// (grid.h:153-161).  The prime factors of R, largest first (ALG::Prime,
+// algorithm/fn.cpp:14-27), each split the axis with the largest remaining
+// extent; along an axis split in c parts of a length L = n / c, the last
+// n % c parts hold one site more.
+struct RankBox {
+    size_t start[3] = {0, 0, 0}, size[3] = {1, 1, 1};
+};
+
+std::vector<RankBox> rank_boxes(size_t D, const std::vector<size_t>& sz, size_t R)
+{
+    std::vector<size_t> primes;
+    for (size_t v = R, i = 2; v > 1;) {
+        if (v % i == 0) {
+            primes.insert(primes.begin(), i);
+            v /= i;
+        } else {
+            ++i;
+        }
+    }
+    std::vector<double> rest(sz.begin(), sz.end());
+    std::vector<size_t> cores(D, 1);
+    for (size_t prime : primes) {
+        const size_t k = size_t(std::max_element(rest.begin(), rest.end()) - rest.begin());
+        if (rest[k] < double(prime)) throw std::invalid_argument("too many processors");
+        cores[k] *= prime;
+        rest[k] /= double(prime);
+    }
+    std::vector<RankBox> boxes(R);
+    for (size_t r = 0; r < R; ++r) {
+        size_t rem = r;
+        for (size_t d = D; d-- > 0;) {
+            const size_t c = rem % cores[d];
+            rem /= cores[d];
+            const size_t mode = cores[d] - sz[d] % cores[d], len = sz[d] / cores[d];
+            boxes[r].start[d] = c * len + (mode >= c ? 0 : c - mode);
+            boxes[r].size[d] = len + (mode > c ? 0 : 1);
+        }
+    }
+    return boxes;
+}
+
+// VTI writer (file/filevti.cpp:39-172): header, one Piece per rank of an
+// R-rank reference run (R = 1: the whole grid), appended raw blocks
+// [uint64 byte length][Float64 ...] per array and rank in rank order, scalars
+// in name order, then vectors with interleaved (x, y, z) components.  Piece
+// offsets follow FileVti::Init: the sites of the ranks before this one,
+// offRaw = sum_d start_d stride_d prod_{d' < d} size_d', plus one length word
+// per earlier rank.
 class Vti {
 public:
     struct Array {
         string name;
         std::vector<const double*> parts;   // scalar: 1 part; vector: components x, y(, z)
     };
-    Vti(string base, size_t D, const std::vector<size_t>& sz) : base_(std::move(base)), D_(D), sz_(sz)
+    Vti(string base, size_t D, const std::vector<size_t>& sz, size_t ranks = 1)
+        : base_(std::move(base)), D_(D), sz_(sz), boxes_(rank_boxes(D, sz, ranks))
     {
         N_ = 1;
         for (size_t n : sz) N_ *= n;
     }
     void save(size_t t, const std::vector<Array>& scalars, const std::vector<Array>& vectors) const
     {
-        std::ostringstream head, piece;
+        const size_t R = boxes_.size();
+        std::ostringstream head;
         head << "<VTKFile type=\"ImageData\" version=\"1.0\" byte_order=\"LittleEndian\" header_type=\"UInt64\">"
              << "<ImageData WholeExtent=\" ";
         for (size_t d = D_; d > 0; --d) head << "0 " << sz_[d - 1] - 1 << " ";
         for (size_t d = 3 - D_; d > 0; --d) head << "0 0 ";
         head << "\" Origin=\"0 0 0\" Spacing=\"1 1 1\">";
-        piece << "<Piece Extent=\"";
-        for (size_t d = D_; d > 0; --d) piece << 0 << " " << sz_[d - 1] - 1 << " ";
-        for (size_t d = 3 - D_; d > 0; --d) piece << "0 0 ";
-        piece << "\"><PointData Scalars=\"";
-        for (const auto& a : scalars) piece << a.name << " ";
-        piece << "\" Vectors=\"";
-        for (const auto& a : vectors) piece << a.name << " ";
-        piece << "\" >";
-        size_t offset = 0;
-        for (const auto& a : scalars) {
-            piece << "<DataArray type=\"Float64\" Name=\"" << a.name << "\" format=\"appended\" offset=\"" << offset
-                  << "\"/>";
-            offset += N_ * sizeof(double) + sizeof(uint64_t);
+        string text = head.str();
+        std::vector<size_t> stride(D_, 1);
+        for (size_t d = D_ - 1; d > 0; --d) stride[d - 1] = stride[d] * sz_[d];
+        for (size_t r = 0; r < R; ++r) {
+            const RankBox& b = boxes_[r];
+            std::ostringstream piece;
+            piece << "<Piece Extent=\"";
+            for (size_t d = D_; d > 0; --d) piece << b.start[d - 1] << " " << b.start[d - 1] + b.size[d - 1] - 1 << " ";
+            for (size_t d = 3 - D_; d > 0; --d) piece << "0 0 ";
+            piece << "\"><PointData Scalars=\"";
+            for (const auto& a : scalars) piece << a.name << " ";
+            piece << "\" Vectors=\"";
+            for (const auto& a : vectors) piece << a.name << " ";
+            piece << "\" >";
+            size_t offRaw = 0, lclN = 1;
+            for (size_t d = 0; d < D_; ++d) {
+                offRaw += b.start[d] * stride[d] * lclN;
+                lclN *= b.size[d];
+            }
+            size_t offset = offRaw * sizeof(double) + r * sizeof(uint64_t);
+            for (const auto& a : scalars) {
+                piece << "<DataArray type=\"Float64\" Name=\"" << a.name << "\" format=\"appended\" offset=\""
+                      << offset << "\"/>";
+                offset += N_ * sizeof(double) + R * sizeof(uint64_t);
+            }
+            if (!vectors.empty())
+                offset = offRaw * sizeof(double) * vectors.front().parts.size() + r * sizeof(uint64_t) +
+                         N_ * scalars.size() * sizeof(double) + R * scalars.size() * sizeof(uint64_t);
+            for (const auto& a : vectors) {
+                piece << "<DataArray type=\"Float64\" Name=\"" << a.name << "\" NumberOfComponents=\""
+                      << a.parts.size() << "\" format=\"appended\" offset=\"" << offset << "\"/>";
+                offset += N_ * sizeof(double) * 3 + R * sizeof(uint64_t);
+            }
+            piece << "</PointData><CellData></CellData></Piece>";
+            text += piece.str();
         }
-        offset = N_ * scalars.size() * sizeof(double) + scalars.size() * sizeof(uint64_t);
-        for (const auto& a : vectors) {
-            piece << "<DataArray type=\"Float64\" Name=\"" << a.name << "\" NumberOfComponents=\"" << a.parts.size()
-                  << "\" format=\"appended\" offset=\"" << offset << "\"/>";
-            offset += N_ * sizeof(double) * 3 + sizeof(uint64_t);
-        }
-        piece << "</PointData><CellData></CellData></Piece>";
+        text += "</ImageData><AppendedData encoding=\"raw\">_";
         const string path = numbered(base_, t, ".vti");
         std::remove(path.c_str());
-        string text = head.str() + piece.str() + "</ImageData><AppendedData encoding=\"raw\">_";
         std::vector<char> body(text.begin(), text.end());
         auto put = [&body](const void* p, size_t n) {
             const char* c = static_cast<const char*>(p);
             body.insert(body.end(), c, c + n);
         };
-        for (const auto& a : scalars) {
-            const uint64_t len = N_ * sizeof(double);
-            put(&len, sizeof len);
-            put(a.parts[0], len);
-        }
-        for (const auto& a : vectors) {
+        // one rank's sub-box of an array, local sites in z, y, x order, components interleaved
+        auto block = [&](const Array& a, const RankBox& b) {
             const size_t nc = a.parts.size();
-            const uint64_t len = nc * N_ * sizeof(double);
+            size_t lclN = 1;
+            for (size_t d = 0; d < D_; ++d) lclN *= b.size[d];
+            const uint64_t len = nc * lclN * sizeof(double);
             put(&len, sizeof len);
-            std::vector<double> inter(nc * N_);
-            for (size_t i = 0; i < N_; ++i)
-                for (size_t c = 0; c < nc; ++c) inter[i * nc + c] = a.parts[c][i];
-            put(inter.data(), len);
-        }
+            std::vector<double> out;
+            out.reserve(nc * lclN);
+            const size_t nz = D_ == 3 ? b.size[0] : 1, ny = b.size[D_ - 2], nx = b.size[D_ - 1];
+            const size_t z0 = D_ == 3 ? b.start[0] : 0, y0 = b.start[D_ - 2], x0 = b.start[D_ - 1];
+            const size_t sy = sz_[D_ - 1], sz = D_ == 3 ? sz_[1] * sz_[2] : 0;
+            for (size_t z = 0; z < nz; ++z)
+                for (size_t y = 0; y < ny; ++y)
+                    for (size_t x = 0; x < nx; ++x) {
+                        const size_t i = (z0 + z) * sz + (y0 + y) * sy + (x0 + x);
+                        for (size_t c = 0; c < nc; ++c) out.push_back(a.parts[c][i]);
+                    }
+            put(out.data(), len);
+        };
+        for (const auto& a : scalars)
+            for (const RankBox& b : boxes_) block(a, b);
+        for (const auto& a : vectors)
+            for (const RankBox& b : boxes_) block(a, b);
         const char* foot = "</AppendedData></VTKFile>";
         put(foot, std::strlen(foot));
         append(path, body.data(), body.size());
@@ -171,6 +246,7 @@ private:
     string base_;
     size_t D_, N_;
     std::vector<size_t> sz_;
+    std::vector<RankBox> boxes_;
 };
 
 class Simulation {
@@ -200,13 +276,28 @@ public:
         }
         unsupported();
 
-        string precision = p_.enginePrecision(), layout = p_.engineLayout();
+        string precision = p_.enginePrecision(), layout = p_.engineLayout(), ranks = p_.outputRanks();
+        const string ranksFlag = "--output-ranks=";
         for (const string& fl : flags) {
             if (fl == "--fp32") precision = "fp32";
             else if (fl == "--fp64") precision = "fp64";
             else if (fl == "--sparse") layout = "sparse";
             else if (fl == "--exact") layout = "exact";
+            else if (fl.compare(0, ranksFlag.size(), ranksFlag) == 0) ranks = fl.substr(ranksFlag.size());
             else throw std::invalid_argument("Unknown option " + fl);
+        }
+        {
+            size_t used = 0;
+            unsigned long r = 0;
+            try {
+                r = std::stoul(ranks, &used);
+            } catch (const std::exception&) {
+                used = 0;
+            }
+            if (used != ranks.size() || r == 0 || r > 65536)
+                throw std::invalid_argument("outputRanks must be a rank count of at least 1.");
+            outRanks_ = r;
+            rank_boxes(D_, sz_, outRanks_);   // an R-rank reference run fails here too (gridmc.cpp:67-68)
         }
         lbf_desc d{};
         d.D = int32_t(D_);
@@ -324,7 +415,7 @@ private:
             scalars.push_back({"density", {rho0_.data()}});
         }
         vectors.push_back(vel);
-        Vti(p_.outPath(), D_, sz_).save(t, scalars, vectors);
+        Vti(p_.outPath(), D_, sz_, outRanks_).save(t, scalars, vectors);
         fieldsAt_ = t;
     }
 
@@ -398,6 +489,7 @@ private:
     const Params& p_;
     bool color_ = false;
     size_t NP_ = 1, D_ = 0, Q_ = 0, N_ = 0;
+    size_t outRanks_ = 1;   // ranks of the reference run whose VTI layout is written
     std::vector<size_t> sz_;
     std::vector<uint8_t> geometry_;
     lbf_ctx* ctx_ = nullptr;

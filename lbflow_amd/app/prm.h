@@ -72,6 +72,8 @@ public:
     // engine selectors; the reference ignores unknown keys, so the same file runs on both
     std::string enginePrecision() const { return option("enginePrecision", "fp64"); }
     std::string engineLayout() const { return option("engineLayout", "exact"); }
+    // VTI files laid out as an R-rank reference run writes them (one piece per rank)
+    std::string outputRanks() const { return option("outputRanks", "1"); }
 
     const Label& label(std::size_t value) const;            // throws std::out_of_range "map::at"
     const std::map<std::size_t, Label>& labels() const { return labels_; }

@@ -93,6 +93,19 @@ def test_unsupported_boundary_type(tmp_path):
     assert code != 0 and 'Axisymmetric AB boundary handling is not implemented.' in out
 
 
+@pytest.mark.parametrize('ranks,message', [
+    ('"0"', 'outputRanks must be a rank count of at least 1.'),
+    ('"two"', 'outputRanks must be a rank count of at least 1.'),
+    # 8 x 6 x 4 grid (x, y, z): 11 ranks cannot split any axis
+    # (GridMc::initCoreDistribution, gridmc.cpp:55-73)
+    ('"11"', 'too many processors'),
+])
+def test_output_ranks_validation(tmp_path, ranks, message):
+    spec = dict(base(), overrides={'outputRanks': ranks})
+    code, out = run_case(tmp_path, spec)
+    assert code != 0 and message in out, out[-300:]
+
+
 def test_film_needs_the_colour_model(tmp_path):
     """HF labels run in the colour model only: the reference's single-phase
     model never initialises or collides HF sites (scsp.cpp:116-175)."""

@@ -138,6 +138,67 @@ def test_film_case_through_process(name, tmp_path):
         assert (cont / last).read_bytes() == (resumed / last).read_bytes()
 
 
+# Core grids (z, y, x) of an R-rank reference run on a 12^3 grid, by hand from
+# GridMc::initCoreDistribution (gridmc.cpp:55-73): the primes of R, largest
+# first, each divide the axis with the largest remaining extent (first of
+# equals in z, y, x order).
+CORE_GRIDS_12 = {2: (2, 1, 1), 3: (3, 1, 1), 4: (2, 2, 1), 5: (5, 1, 1), 6: (3, 2, 1),
+                 7: (7, 1, 1), 8: (2, 2, 2), 12: (3, 2, 2)}
+
+
+def test_output_ranks_vti_layout(tmp_path):
+    """VTI files laid out as R-rank reference runs write them
+    (filevti.cpp:39-172, gridmc.cpp:55-95): one piece per rank in rank order
+    (core coordinates with x fastest), the reference parser's rules hold
+    (every block where its offset says, pieces tile the grid once), the
+    fields equal the one-piece file's, pieces along a split axis differ by
+    at most one site with the larger ones last, and checkpoints do not
+    depend on R."""
+    spec = drop_spec()
+    base = tmp_path / 'r1'
+    code, out, _ = run(spec, base)
+    assert code == 0, out[-2000:]
+    _, ref = H.read_vti(base / 'field.0000000040.vti')
+    for R, cores in CORE_GRIDS_12.items():
+        folder = tmp_path / f'r{R}'
+        code, out, _ = run(spec, folder, extra=(f'--output-ranks={R}',))
+        assert code == 0, out[-2000:]
+        pieces = []
+        _, f = H.read_vti(folder / 'field.0000000040.vti', pieces)
+        assert f.keys() == ref.keys() and all(np.array_equal(f[k], ref[k]) for k in f)
+        assert len(pieces) == R
+        # rank r sits at core coordinates CalcI2P(r, cores) (grid.h:153-161)
+        starts = [sorted({p[2 * a] for p in pieces}) for a in (2, 1, 0)]
+        assert tuple(len(v) for v in starts) == cores
+        for r, p in enumerate(pieces):
+            cz, cy, cx = r // (cores[1] * cores[2]), (r // cores[2]) % cores[1], r % cores[2]
+            assert (p[4], p[2], p[0]) == (starts[0][cz], starts[1][cy], starts[2][cx])
+        for a, c in zip((2, 1, 0), cores):
+            sizes = [p[2 * a + 1] - p[2 * a] + 1 for p in pieces
+                     if all(p[2 * b] == 0 for b in (2, 1, 0) if b != a)]
+            assert len(sizes) == c and sum(sizes) == 12
+            assert sizes == sorted(sizes) and sizes[-1] - sizes[0] <= 1
+        # data blocks of each array appear in rank order
+        root = (folder / 'field.0000000040.vti').read_bytes()
+        offsets = [[int(m) for m in re.findall(rb'Name="' + n + rb'"[^>]*offset="(\d+)"', root)]
+                   for n in (b'r0', b'r1', b'v')]
+        assert all(o == sorted(o) and len(o) == R for o in offsets)
+        assert (folder / 'field.0000000040.rst').read_bytes() == (base / 'field.0000000040.rst').read_bytes()
+    # 2D: an 8 x 10 plane (x, y) in 3 ranks splits y as 3, 3, 4
+    spec2 = H.spec('plane-ranks', nx=8, ny=10, nz=1, dimensions=2, steps=20, write=20,
+                   label=lambda x, y, z: 100 if y < 4 else 0,
+                   indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"'])
+    code, out, _ = run(spec2, tmp_path / 'p1')
+    assert code == 0, out[-2000:]
+    code, out, _ = run(spec2, tmp_path / 'p3', extra=('--output-ranks=3',))
+    assert code == 0, out[-2000:]
+    pieces = []
+    _, a = H.read_vti(tmp_path / 'p1' / 'field.0000000020.vti')
+    _, b = H.read_vti(tmp_path / 'p3' / 'field.0000000020.vti', pieces)
+    assert all(np.array_equal(a[k], b[k]) for k in a)
+    assert pieces == [[0, 7, 0, 2, 0, 0], [0, 7, 3, 5, 0, 0], [0, 7, 6, 9, 0, 0]]
+
+
 def report_lines(out):
     return [l for l in out.splitlines() if l.startswith(' - ')]
 
