@@ -166,6 +166,39 @@ def zouhe_color_cases(nx=20, ny=10, nz=6, u0=1e-3, rho_in=1.0003):
     return cases
 
 
+def csf_cases():
+    """Experimental CSF capillary force (scripts/capillary_checks.py:23-45): a
+    bulk planar interface (phase 0 at x < 16) and a uniform body force, 32 x 4
+    (x 4) sites, D2Q9/D3Q19, SRT/MRT, 600 steps."""
+    cases = []
+    for dims in (2, 3):
+        for eq in ('SRT', 'MRT'):
+            common = dict(nx=32, ny=4, nz=1 if dims == 2 else 4, dimensions=dims, eq=eq,
+                          capillary_model='csf', steps=600, write=300)
+            cases.append(spec(f'csf-plane-d{dims}-{eq}', **common,
+                              label=lambda x, y, z: 0 if x < 16 else 100,
+                              indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"']))
+            cases.append(spec(f'csf-force-d{dims}-{eq}', **common, label=lambda x, y, z: 75,
+                              indices=['75 "GV" "0.000001" "1" "0"']))
+    return cases
+
+
+def csf_drop(wall_gradient='tanh', curvature='normal'):
+    """12^3 drop of radius 3 with CSF (scripts/capillary_checks.py:46-50)."""
+    return spec('csf-drop-continuous', nx=12, ny=12, nz=12, steps=40, write=20, restart=20,
+                capillary_model='csf', wall_gradient=wall_gradient, curvature_stencil=curvature,
+                label=lambda x, y, z: 100 if (x - 5.5) ** 2 + (y - 5.5) ** 2 + (z - 5.5) ** 2 <= 9 else 0,
+                indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"'])
+
+
+def csf_wall(wall_gradient='tanh'):
+    """2D contact line between 60-degree walls with CSF (scripts/capillary_checks.py:57-62)."""
+    return spec('csf-wall-continuous', nx=16, ny=10, nz=1, dimensions=2, steps=100, write=50,
+                capillary_model='csf', wall_gradient=wall_gradient,
+                label=lambda x, y, z: 255 if y in (0, 9) else (0 if 4 <= x < 12 else 100),
+                indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"', '255 "SD" "60" "N/A" "N/A"'])
+
+
 def film_cases(nx=20, ny=8, nz=5, force=1e-5, transparent=False):
     """Half-permeable films (HF) in the colour model, in a channel between
     60-degree walls (y = 0, ny - 1), periodic along x:
@@ -330,6 +363,8 @@ def lattice_case(s):
                 alpha=s['alpha'], sigma=s['sigma'], beta=s['beta'],
                 harmonic=s.get('viscosity_interpolation', 'arithmetic') == 'harmonic',
                 wall_gradient=s.get('wall_gradient', 'legacy'),
+                capillary=s.get('capillary_model', 'perturbation'),
+                curvature=s.get('curvature_stencil', 'normal'),
                 labels=dict(parse_index(line) for line in s['indices']))
 
 
@@ -352,6 +387,10 @@ def prm_text(s, folder, start=0):
                   surfaceTension=s['sigma'], interfaceThickness=s['beta'],
                   wallGradient=f'"{s.get("wall_gradient", "legacy")}"')
     params['viscosityInterpolation'] = f'"{s.get("viscosity_interpolation", "arithmetic")}"'
+    if 'capillary_model' in s:
+        params['capillaryModel'] = f'"{s["capillary_model"]}"'
+    if 'curvature_stencil' in s:
+        params['curvatureStencil'] = f'"{s["curvature_stencil"]}"'
     params.update(s.get('overrides', {}))
     text = '\n'.join(f'{key} {val};' for key, val in params.items())
     return text + '\n' + '\n'.join(s['indices']) + '\n'

@@ -196,6 +196,8 @@ struct orc_sim {
     double* ibNorm; double* ibSin; double* ibCos; uint64_t nwall;
     double* tib; double* tibInv;   /* per wall index D*D */
     double* ginv;                  /* per wall index D*D: renormalized wall gradient */
+    /* CSF (capillaryModel csf): colour gradient, curvature field and force, [d][site] */
+    double* capGrad; double* capField; double* capForce;
     double* f[2]; double* tmp[2];  /* [q][N] per phase */
     double* dens[2]; double* densT; double* densN; double* tau; double* v;
     double* adjV; double* adjChi;  /* adjacent report history of the last step */
@@ -208,6 +210,7 @@ struct orc_sim {
 };
 
 static int is_solid(int type) { return type == ORC_BB || type == ORC_SD; }
+static int calc_capillary_force(orc_sim* s);
 
 #define F(s, ph, q, i) ((s)->f[ph][(uint64_t)(q) * (s)->N + (i)])
 #define V(s, d, i) ((s)->v[(uint64_t)(d) * (s)->N + (i)])
@@ -482,7 +485,21 @@ orc_sim* orc_new(const orc_desc* desc, const orc_label* labels, const uint8_t* g
             for (int q = 0; q < 19; ++q) s->bi[q] = q == 0 ? b0 : (q <= 6 ? b1 : b7);
         }
         calc_tib(s);
-        if (s->d.wall_gradient == 1 && calc_wall_gradient_inverse(s)) {
+        if (s->d.capillary) {
+            for (int l = 0; l < 256; ++l)
+                if (s->lab[l].type && !label_in(s, l, ORC_VD | ORC_GV | ORC_SD | ORC_BB)) {
+                    snprintf(err, errlen, "oracle: CSF is restated for closed flow (VD/GV/SD/BB) only");
+                    orc_free(s);
+                    return NULL;
+                }
+            s->capGrad = calloc(s->N * s->D, sizeof(double));
+            s->capField = calloc(s->N * s->D, sizeof(double));
+            s->capForce = calloc(s->N * s->D, sizeof(double));
+        }
+        /* calcWallGradientInverse runs for renormalized and tanh walls and for CSF
+         * (colorfluid.cpp:917-920) */
+        if ((s->d.wall_gradient == 1 || s->d.wall_gradient == 2 || s->d.capillary) &&
+            calc_wall_gradient_inverse(s)) {
             snprintf(err, errlen, "%s", s->err);
             orc_free(s);
             return NULL;
@@ -497,6 +514,7 @@ void orc_free(orc_sim* s)
     free(s->raw); free(s->bylabel); free(s->adj); free(s->ibI);
     for (int q = 0; q < MAXQ; ++q) { free(s->ibs[q]); free(s->ibf[0][q]); free(s->ibf[1][q]); }
     free(s->ibNorm); free(s->ibSin); free(s->ibCos); free(s->tib); free(s->tibInv); free(s->ginv);
+    free(s->capGrad); free(s->capField); free(s->capForce);
     for (int ph = 0; ph < 2; ++ph) { free(s->f[ph]); free(s->tmp[ph]); free(s->dens[ph]); }
     free(s->v); free(s->densT); free(s->densN); free(s->tau); free(s->adjV); free(s->adjChi);
     free(s);
@@ -854,8 +872,18 @@ static int color_macro(orc_sim* s)
 }
 
 /* colorfluid.cpp:736-750 (no CSF) */
-static void color_body_force(orc_sim* s)
+static int color_body_force(orc_sim* s)
 {
+    if (s->d.capillary) {   /* colorfluid.cpp:738-743 */
+        if (calc_capillary_force(s)) return 1;
+        for (int l = 0; l < 256; ++l) {
+            if (!label_in(s, l, ORC_VD | ORC_GV)) continue;
+            for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+                const uint64_t i = s->bylabel[k];
+                for (int d = 0; d < s->D; ++d) V(s, d, i) += .5 * s->capForce[d * s->N + i] / s->densT[i];
+            }
+        }
+    }
     for (int l = 0; l < 256; ++l) {
         if (!label_in(s, l, ORC_GV)) continue;
         double force[3], fi[3];
@@ -866,9 +894,18 @@ static void color_body_force(orc_sim* s)
             for (int d = 0; d < s->D; ++d) V(s, d, i) += fi[d] / s->densT[i];
         }
     }
+    return 0;
 }
 
 /* ColorFluid::colorGradient, legacy wall branch, colorfluid.cpp:435-570 */
+/* atanh of the colour clamped to (-1, 1) by 1e-8 (colorfluid.cpp:497-499) */
+static double tanh_wall_transform(double color)
+{
+    const double hi = 1.0 - 1e-8, lo = -1.0 + 1e-8;
+    const double c = color < hi ? color : hi;   /* std::min(1 - 1e-8, color) */
+    return atanh(lo < c ? c : lo);              /* std::max(-1 + 1e-8, .) */
+}
+
 static void color_gradient(const orc_sim* s, uint64_t i, double* g, double* normOut)
 {
     const int D = s->D;
@@ -917,6 +954,35 @@ static void color_gradient(const orc_sim* s, uint64_t i, double* g, double* norm
             for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
             norm = sqrt(n2);
         }
+        if (s->d.wall_gradient == 2) {
+            /* colorfluid.cpp:494-518: fit atanh(chi) as affine over the fluid
+             * neighbours (slope = inverse moment), continue tanh over the full
+             * stencil and take the bulk gradient of that */
+            const double* inv = s->ginv + (uint64_t)c * D * D;
+            const double center = tanh_wall_transform(s->densN[i]);
+            double moment[3] = {0.0, 0.0, 0.0}, slope[3];
+            for (int q = 1; q < s->Q; ++q) {
+                const double nb = s->densN[s->adj[i * s->Q + q]];
+                if (nb == SOLID_DENS_N) continue;
+                const double difference = tanh_wall_transform(nb) - center;
+                for (int d = 0; d < D; ++d) moment[d] += s->wi[q] * s->qi[q][d] * difference;
+            }
+            for (int a = 0; a < D; ++a) {
+                double t = 0.0;
+                for (int b = 0; b < D; ++b) t = t + moment[b] * inv[a * D + b];
+                slope[a] = t;
+            }
+            for (int d = 0; d < D; ++d) g[d] = 0.0;
+            for (int q = 1; q < s->Q; ++q) {
+                double position = center;
+                for (int d = 0; d < D; ++d) position += s->qi[q][d] * slope[d];
+                const double color = tanh(position);
+                for (int d = 0; d < D; ++d) g[d] += 3.0 * s->wi[q] * s->qi[q][d] * color;
+            }
+            n2 = 0.0;
+            for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+            norm = sqrt(n2);
+        }
         if (D == 3) {
             double gn[3], txInv[9], tInv[9], h[3];
             mat_mul(s->tib + c * 9, g, 3, 3, 1, gn);
@@ -941,6 +1007,94 @@ void orc_color_gradient(const orc_sim* s, uint64_t site, double* g, double* norm
 }
 
 /* ColorFluid::collision, colorfluid.cpp:292-433 (perturbation, cosine recoloring) */
+/* ColorFluid::calcCapillaryForce, colorfluid.cpp:572-617: on VD/GV sites the
+ * colour gradient (with the wall treatment) and the curvature field (unit
+ * normal, or the gradient itself for curvatureStencil "gradient"); then
+ * kappa = sum_ab (n_a n_b - delta_ab) d_b field_a from the fluid neighbours
+ * (inverse moment at wall sites), and F = sigma/2 kappa direction. */
+static int calc_capillary_force(orc_sim* s)
+{
+    const int D = s->D, Q = s->Q;
+    const uint64_t N = s->N;
+    const int gc = s->d.curvature == 1;
+    for (int l = 0; l < 256; ++l) {
+        if (!label_in(s, l, ORC_VD | ORC_GV)) continue;
+        for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+            const uint64_t i = s->bylabel[k];
+            double g[3], norm;
+            color_gradient(s, i, g, &norm);
+            for (int d = 0; d < D; ++d) {
+                s->capGrad[d * N + i] = g[d];
+                s->capField[d * N + i] = gc ? g[d] : (norm > 1e-9 ? g[d] / norm : 0.0);
+            }
+        }
+    }
+    for (int l = 0; l < 256; ++l) {
+        if (!label_in(s, l, ORC_VD | ORC_GV)) continue;
+        for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+            const uint64_t i = s->bylabel[k];
+            double curvature = 0.0, norm2 = 0.0;
+            for (int d = 0; d < D; ++d) norm2 += s->capGrad[d * N + i] * s->capGrad[d * N + i];
+            if (norm2 > 1e-18) {
+                for (int a = 0; a < D; ++a) {
+                    double moment[3] = {0.0, 0.0, 0.0}, derivative[3];
+                    const double center = s->capField[a * N + i];
+                    for (int q = 1; q < Q; ++q) {
+                        const uint64_t j = s->adj[i * Q + q];
+                        if (s->densN[j] == SOLID_DENS_N) continue;
+                        const double difference = s->capField[a * N + j] - center;
+                        for (int b = 0; b < D; ++b) moment[b] += s->wi[q] * s->qi[q][b] * difference;
+                    }
+                    if (s->ibI[i]) {
+                        const double* inv = s->ginv + (uint64_t)s->ibI[i] * D * D;
+                        for (int r = 0; r < D; ++r) {
+                            double t = 0.0;
+                            for (int b = 0; b < D; ++b) t = t + moment[b] * inv[r * D + b];
+                            derivative[r] = t;
+                        }
+                    } else {
+                        for (int b = 0; b < D; ++b) derivative[b] = moment[b] * 3.0;
+                    }
+                    for (int b = 0; b < D; ++b) {
+                        const double na = gc ? s->capGrad[a * N + i] / sqrt(norm2) : center;
+                        const double nb = gc ? s->capGrad[b * N + i] / sqrt(norm2) : s->capField[b * N + i];
+                        curvature += (na * nb - (a == b ? 1.0 : 0.0)) * derivative[b];
+                    }
+                }
+            }
+            for (int d = 0; d < D; ++d) {
+                const double direction = gc && norm2 > 1e-18 ? s->capGrad[d * N + i] / sqrt(norm2)
+                                                             : s->capGrad[d * N + i];
+                const double force = .5 * s->d.sigma * curvature * direction;
+                s->capForce[d * N + i] = force;
+                if (!isfinite(force)) return fail(s, "Non-finite capillary force.");
+            }
+        }
+    }
+    return 0;
+}
+
+/* ColorFluid::initializeCapillaryForce, colorfluid.cpp:619-634 */
+static int initialize_capillary_force(orc_sim* s)
+{
+    const int D = s->D, Q = s->Q;
+    const uint64_t N = s->N;
+    if (calc_capillary_force(s)) return 1;
+    for (int l = 0; l < 256; ++l) {
+        if (!label_in(s, l, ORC_VD | ORC_GV)) continue;
+        for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
+            const uint64_t i = s->bylabel[k];
+            for (int q = 0; q < Q; ++q) {
+                double ef = 0.0;
+                for (int d = 0; d < D; ++d) ef += s->qi[q][d] * s->capForce[d * N + i];
+                for (int ph = 0; ph < s->NP; ++ph)
+                    F(s, ph, q, i) -= 1.5 * s->wi[q] * ef * s->dens[ph][i] / s->densT[i];
+            }
+        }
+    }
+    return color_macro(s);
+}
+
 static int color_collision(orc_sim* s)
 {
     const int Q = s->Q, D = s->D;
@@ -950,13 +1104,24 @@ static int color_collision(orc_sim* s)
     const int mrt = s->d.equil == 1;
     for (int l = 0; l < 256; ++l) {
         if (!label_in(s, l, ORC_VD | ORC_GV | ORC_VB | ORC_PB | ORC_HF)) continue;
-        double force[3];
-        force_for_label(s, l, force);
+        const int csf = s->d.capillary;
+        double external[3];
+        force_for_label(s, l, external);
         double ff2 = 0.0;
-        for (int d = 0; d < D; ++d) ff2 = ff2 + force[d] * force[d];
-        const int forced = ff2 != 0.0;
+        for (int d = 0; d < D; ++d) ff2 = ff2 + external[d] * external[d];
+        const int externally_forced = ff2 != 0.0;
         for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
             const uint64_t i = s->bylabel[k];
+            /* CSF: the site's force is the label's plus the capillary force
+             * (colorfluid.cpp:326-333) */
+            double force[3] = {external[0], external[1], external[2]};
+            int forced = externally_forced;
+            if (csf) {
+                for (int d = 0; d < D; ++d) force[d] += s->capForce[d * s->N + i];
+                double f2 = 0.0;
+                for (int d = 0; d < D; ++d) f2 = f2 + force[d] * force[d];
+                forced = f2 != 0.0;
+            }
             double usq = 0;
             for (int d = 0; d < D; ++d) usq += V(s, d, i) * V(s, d, i);
             double feq_u0[MAXQ], fT[MAXQ], src[MAXQ];
@@ -1007,7 +1172,16 @@ static int color_collision(orc_sim* s)
                 for (int q = 0; q < Q; ++q) fT[q] += corr[q];
             }
             double g[3], norm;
-            color_gradient(s, i, g, &norm);
+            if (csf) {   /* the stored capillary gradient, no perturbation (colorfluid.cpp:389-411) */
+                double n2 = 0.0;
+                for (int d = 0; d < D; ++d) {
+                    g[d] = s->capGrad[d * s->N + i];
+                    n2 = n2 + g[d] * g[d];
+                }
+                norm = sqrt(n2);
+            } else {
+                color_gradient(s, i, g, &norm);
+            }
             const double dens0T = s->dens[0][i] / s->densT[i];
             if (norm > 1e-9) {
                 const double b01 = beta * dens0T * s->dens[1][i] / s->densT[i];
@@ -1019,7 +1193,7 @@ static int color_collision(orc_sim* s)
                         else if (s->qi[q][d] == -1) angle -= g[d];
                     }
                     angle /= norm;
-                    fT[q] += akn * (s->wi[q] * angle * angle - s->bi[q]);
+                    if (!csf) fT[q] += akn * (s->wi[q] * angle * angle - s->bi[q]);
                     if (q != 0) angle /= s->qnorm[q];
                     F(s, 0, q, i) = dens0T * fT[q] + b01 * angle * feq_u0[q];
                     F(s, 1, q, i) = fT[q] - F(s, 0, q, i);
@@ -1102,6 +1276,8 @@ static void stream_half(orc_sim* s)
 
 void orc_stream_only(orc_sim* s) { stream_half(s); }
 
+int orc_macro_only(orc_sim* s) { return s->d.model == 0 ? scsp_macro(s) : color_macro(s); }
+
 int orc_init(orc_sim* s)
 {
     s->t = 0;
@@ -1113,7 +1289,8 @@ int orc_init(orc_sim* s)
     }
     color_initialize(s);
     if (color_macro(s)) return 1;
-    color_body_force(s);
+    if (s->d.capillary && initialize_capillary_force(s)) return 1;
+    if (color_body_force(s)) return 1;
     return color_collision(s);
 }
 
@@ -1129,7 +1306,7 @@ int orc_step(orc_sim* s)
     }
     record_adjacent(s);
     if (color_macro(s)) return 1;
-    color_body_force(s);
+    if (color_body_force(s)) return 1;
     return color_collision(s);
 }
 

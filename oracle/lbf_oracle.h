@@ -52,13 +52,15 @@ typedef struct orc_desc {
     int32_t model;         /* 0 = scsp, 1 = color */
     int32_t equil;         /* 0 = SRT, 1 = MRT */
     int32_t harmonic;      /* color viscosity interpolation: 0 arithmetic, 1 harmonic */
-    int32_t wall_gradient; /* color wallGradient: 0 legacy, 1 renormalized */
+    int32_t wall_gradient; /* color wallGradient: 0 legacy, 1 renormalized, 2 tanh */
     uint64_t sz[3];        /* grid size in reference order: slowest first (z,y,x / y,x) */
     double tau[2];         /* relaxation time per phase */
     double density[2];     /* reference density per phase */
     double alpha0;         /* color phase weight */
     double sigma;          /* surface tension */
     double beta;           /* recoloring beta */
+    int32_t capillary;     /* color capillaryModel: 0 perturbation, 1 csf */
+    int32_t curvature;     /* CSF curvatureStencil: 0 normal, 1 gradient */
 } orc_desc;
 
 typedef struct orc_sim orc_sim;
@@ -76,6 +78,9 @@ int orc_step(orc_sim* s);
  * half-permeable film and the Zou-He boundaries) applied to the current
  * populations, which then hold the pre-collision state; time is unchanged. */
 void orc_stream_only(orc_sim* s);
+/* Test hook: the macro variables (densities, colour indicator, velocity) of
+ * the current populations, without streaming.  Returns 0 or an error code. */
+int orc_macro_only(orc_sim* s);
 /* Run n steps; stops at the first error. */
 int orc_run(orc_sim* s, uint64_t n);
 const char* orc_error(const orc_sim* s);

@@ -24,7 +24,8 @@ class Desc(C.Structure):
     _fields_ = [('D', C.c_int32), ('Q', C.c_int32), ('model', C.c_int32), ('equil', C.c_int32),
                 ('harmonic', C.c_int32), ('wall_gradient', C.c_int32), ('sz', C.c_uint64 * 3),
                 ('tau', C.c_double * 2), ('density', C.c_double * 2), ('alpha0', C.c_double),
-                ('sigma', C.c_double), ('beta', C.c_double)]
+                ('sigma', C.c_double), ('beta', C.c_double), ('capillary', C.c_int32),
+                ('curvature', C.c_int32)]
 
 
 def lib():
@@ -44,6 +45,7 @@ def lib():
         L.orc_run.argtypes = [vp, u64]
         L.orc_stream_only.argtypes = [vp]
         L.orc_stream_only.restype = None
+        L.orc_macro_only.argtypes = [vp]
         L.orc_run.restype = C.c_int
         L.orc_error.argtypes = [vp]
         L.orc_error.restype = C.c_char_p
@@ -88,7 +90,9 @@ class Oracle:
         d.model = 1 if case['model'] == 'color' else 0
         d.equil = 1 if case['eq'] == 'MRT' else 0
         d.harmonic = 1 if case.get('harmonic') else 0
-        d.wall_gradient = 1 if case.get('wall_gradient') == 'renormalized' else 0
+        d.wall_gradient = {'legacy': 0, 'renormalized': 1, 'tanh': 2}[case.get('wall_gradient', 'legacy')]
+        d.capillary = 1 if case.get('capillary') == 'csf' else 0
+        d.curvature = 1 if case.get('curvature') == 'gradient' else 0
         for k, n in enumerate(case['size']):
             d.sz[k] = n
         tau = list(case['tau']) + [0.0] * (2 - len(case['tau']))
@@ -157,6 +161,11 @@ class Oracle:
     def set_populations(self, pops, t):
         pops = np.ascontiguousarray(pops, dtype=np.float64)
         lib().orc_set_populations(self.h, _ptr(pops), t)
+
+    def macro_only(self):
+        """Macro variables of the current populations (test hook)."""
+        self._check(lib().orc_macro_only(self.h))
+        return self
 
     def stream_only(self):
         """Streaming half of a step on the current populations (test hook)."""
