@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define LBF_ABI_VERSION 1
+#define LBF_ABI_VERSION 2
 
 /* Label types: same bit values as FilePrm::IDXTYPE (solver/inc/file/fileprm.h:44-53).
  * lbf_create accepts VD, GV, PB, VB, BB, SD, and HF in the colour model
@@ -71,7 +71,14 @@ enum lbf_create_flags {
  * colorfluid.cpp:458-492): the one-sided stencil as is, or renormalized by
  * the inverse of its fluid-link second moment (calcWallGradientInverse,
  * colorfluid.cpp:917-957); both followed by the contact-angle rotation. */
-enum lbf_wall_gradient { LBF_WALL_LEGACY = 0, LBF_WALL_RENORMALIZED = 1 };
+enum lbf_wall_gradient { LBF_WALL_LEGACY = 0, LBF_WALL_RENORMALIZED = 1, LBF_WALL_TANH = 2 };
+/* color capillaryModel (colorfluid.cpp:292-433, 572-634): the perturbation
+ * operator, or the experimental continuum-surface force (FP64, closed flow:
+ * VD/GV/SD/BB labels only, as the reference requires without pressure
+ * reservoirs). */
+enum lbf_capillary { LBF_CAPILLARY_PERTURBATION = 0, LBF_CAPILLARY_CSF = 1 };
+/* CSF curvatureStencil: curvature from the unit normal field, or from the gradient */
+enum lbf_curvature { LBF_CURVATURE_NORMAL = 0, LBF_CURVATURE_GRADIENT = 1 };
 
 /* Simulation description; the fields of input.prm that the step needs
  * (solver/src/file/fileprm.cpp:364-456). */
@@ -89,6 +96,8 @@ typedef struct lbf_desc {
     double   alpha0;           /* color alpha0 */
     double   sigma;            /* color surfaceTension */
     double   beta;             /* color interfaceThickness (recoloring beta) */
+    int32_t  capillary;        /* color capillaryModel: lbf_capillary */
+    int32_t  curvature;        /* CSF curvatureStencil: lbf_curvature */
 } lbf_desc;
 
 typedef struct lbf_ctx lbf_ctx;
@@ -108,7 +117,8 @@ enum lbf_status_bits {
     LBF_FAIL_NEGATIVE   = 0x2,  /* "Negative component density outside roundoff tolerance." */
     LBF_FAIL_TAU        = 0x4,  /* "Invalid local relaxation time." */
     LBF_FAIL_VELOCITY   = 0x8,  /* "Non-finite fluid velocity." */
-    LBF_FAIL_POPULATION = 0x10  /* "Non-finite post-collision population; no restart will be written." */
+    LBF_FAIL_POPULATION = 0x10, /* "Non-finite post-collision population; no restart will be written." */
+    LBF_FAIL_CAPILLARY  = 0x20  /* "Non-finite capillary force." (colorfluid.cpp:616) */
 };
 
 /* Report request: the reduction behind SCSP::report / ColorFluid::report and
@@ -157,8 +167,9 @@ uint64_t lbf_time(const lbf_ctx* ctx);
 int lbf_synchronize(lbf_ctx* ctx);
 
 /* Failure bits raised by the device since create/init/set_populations, and
- * the first step at which each bit was raised (5 entries, UINT64_MAX = never). */
-int lbf_status(lbf_ctx* ctx, uint32_t* bits, uint64_t* first_step /* 5 */);
+ * the first step at which each bit was raised (6 entries, one per bit in bit
+ * order, UINT64_MAX = never). */
+int lbf_status(lbf_ctx* ctx, uint32_t* bits, uint64_t* first_step /* 6 */);
 const char* lbf_status_message(uint32_t bit);
 
 /* --- data exchange -----------------------------------------------------

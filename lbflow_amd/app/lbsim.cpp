@@ -307,7 +307,11 @@ public:
         d.precision = precision == "fp32" ? LBF_FP32 : LBF_FP64;
         d.flags = layout == "sparse" ? 0 : LBF_STREAM_SOLIDS;
         d.harmonic = color_ && p_.viscosityInterpolation() == "harmonic";
-        d.wall_gradient = color_ && p_.wallGradient() == "renormalized" ? LBF_WALL_RENORMALIZED : LBF_WALL_LEGACY;
+        d.wall_gradient = !color_ ? LBF_WALL_LEGACY
+                          : p_.wallGradient() == "renormalized" ? LBF_WALL_RENORMALIZED
+                          : p_.wallGradient() == "tanh" ? LBF_WALL_TANH : LBF_WALL_LEGACY;
+        d.capillary = color_ && p_.capillaryModel() == "csf" ? LBF_CAPILLARY_CSF : LBF_CAPILLARY_PERTURBATION;
+        d.curvature = color_ && p_.curvatureStencil() == "gradient" ? LBF_CURVATURE_GRADIENT : LBF_CURVATURE_NORMAL;
         for (size_t k = 0; k < D_; ++k) d.sz[k] = sz_[k];
         for (size_t k = 0; k < NP_; ++k) {
             d.tau[k] = p_.relaxationTime()[k];
@@ -345,21 +349,22 @@ private:
 
     void unsupported() const
     {
-        // Operators outside this engine: the experimental research options of
-        // ColorFluid and the Zou-He / film / axisymmetric boundaries.
+        // Operators outside this engine: the remaining experimental options of
+        // ColorFluid (tangent/halfway/ghost walls, wall recolouring, phase
+        // flux, projection recolouring, the liu2015 spectrum, wall slip).
         auto need = [](const string& key, const string& value, const string& want) {
             if (value != want)
                 throw std::invalid_argument(key + " \"" + value + "\" is not implemented by this engine (only \"" +
                                             want + "\").");
         };
         if (color_) {
-            if (p_.wallGradient() != "legacy" && p_.wallGradient() != "renormalized")
-                throw std::invalid_argument("wallGradient \"" + p_.wallGradient() +
-                                            "\" is not implemented by this engine (only \"legacy\", \"renormalized\").");
-            need("capillaryModel", p_.capillaryModel(), "perturbation");
+            const string wg = p_.wallGradient();
+            if (wg != "legacy" && wg != "renormalized" && wg != "tanh")
+                throw std::invalid_argument("wallGradient \"" + wg +
+                                            "\" is not implemented by this engine (only \"legacy\", \"renormalized\", "
+                                            "\"tanh\").");
             need("wallRecoloring", p_.wallRecoloring(), "legacy");
             need("phaseBoundary", p_.phaseBoundary(), "bounce");
-            need("curvatureStencil", p_.curvatureStencil(), "normal");
             need("recoloringModel", p_.recoloringModel(), "cosine");
         }
         need("mrtSpectrum", p_.mrtSpectrum(), "legacy");
@@ -372,20 +377,24 @@ private:
     void failures(size_t t, bool afterCollision)
     {
         uint32_t bits = 0;
-        uint64_t first[5];
+        uint64_t first[6];
         if (lbf_status(ctx_, &bits, first)) throw std::runtime_error(lbf_last_error(ctx_));
         if (!bits) return;
         uint64_t tmin = UINT64_MAX;
-        for (int k = 0; k < 5; ++k)
+        for (int k = 0; k < 6; ++k)
             if (bits & (1u << k)) tmin = std::min(tmin, first[k]);
+        // bits in the order the reference's step throws them: calcMacroVar
+        // (0-3), bodyForce's capillary force (5), before the output; then the
+        // collision's population check (4)
+        static const int order[6] = {0, 1, 2, 3, 5, 4};
         const bool macroAt = [&] {
-            for (int k = 0; k < 4; ++k)
-                if ((bits & (1u << k)) && first[k] == tmin) return true;
+            for (int j = 0; j < 5; ++j)
+                if ((bits & (1u << order[j])) && first[order[j]] == tmin) return true;
             return false;
         }();
         if (tmin > t) return;
         if (!afterCollision && tmin == t && !macroAt) return;   // collision of step t still ahead
-        for (int k = 0; k < 5; ++k)
+        for (int k : order)
             if ((bits & (1u << k)) && first[k] == tmin) {
                 if (k == 0)
                     throw std::runtime_error(color_ ? "Non-finite or non-positive total fluid density."

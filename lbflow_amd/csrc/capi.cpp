@@ -59,6 +59,8 @@ struct lbf_ctx {
     int32_t* d_wall = nullptr;
     DevWall* d_walls = nullptr;
     uint64_t* d_film = nullptr;      // per-site film links, only with HF labels
+    double *d_capgrad = nullptr, *d_capfield = nullptr;   // CSF gradient and curvature field
+    double* d_tanhg = nullptr;       // tanh wall gradient per wall entry
     double* d_densN = nullptr;
     double *d_rho0 = nullptr, *d_rho1 = nullptr, *d_u = nullptr, *d_uadj = nullptr, *d_uprev = nullptr;
     double *d_chiadj = nullptr, *d_chiprev = nullptr;
@@ -162,6 +164,7 @@ const char* lbf_status_message(uint32_t bit)
     case LBF_FAIL_TAU: return "Invalid local relaxation time.";
     case LBF_FAIL_VELOCITY: return "Non-finite fluid velocity.";
     case LBF_FAIL_POPULATION: return "Non-finite post-collision population; no restart will be written.";
+    case LBF_FAIL_CAPILLARY: return "Non-finite capillary force.";
     default: return "";
     }
 }
@@ -246,15 +249,43 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     }
     std::string msg = build_links(d.D, d.Q, d.sz, c->labels.data(), geometry, d.model == LBF_MODEL_COLOR,
                                   c->links);
-    if (msg.empty() && d.model == LBF_MODEL_COLOR && d.wall_gradient == LBF_WALL_RENORMALIZED)
+    // the wall moment inverse serves renormalized and tanh walls and CSF (colorfluid.cpp:917-920)
+    if (msg.empty() && d.model == LBF_MODEL_COLOR &&
+        (d.wall_gradient == LBF_WALL_RENORMALIZED || d.wall_gradient == LBF_WALL_TANH ||
+         d.capillary == LBF_CAPILLARY_CSF))
         msg = wall_gradient_inverse(d.D, d.Q, d.sz, c->labels.data(), geometry, c->links);
     if (!msg.empty()) {
         lbf_destroy(c);
         return fail(nullptr, msg);
     }
-    if (d.wall_gradient != LBF_WALL_LEGACY && d.wall_gradient != LBF_WALL_RENORMALIZED) {
+    if (d.wall_gradient != LBF_WALL_LEGACY && d.wall_gradient != LBF_WALL_RENORMALIZED &&
+        d.wall_gradient != LBF_WALL_TANH) {
         lbf_destroy(c);
-        return fail(nullptr, "wallGradient must be legacy or renormalized in this engine.");
+        return fail(nullptr, "wallGradient must be legacy, renormalized or tanh in this engine.");
+    }
+    if (d.capillary == LBF_CAPILLARY_CSF) {
+        // closed flow only, as the reference requires without pressure
+        // reservoirs (fileprm.cpp validation of capillaryModel csf)
+        bool closed = d.model == LBF_MODEL_COLOR;
+        for (uint64_t i = 0; closed && i < N; ++i) {
+            const int t = labels[geometry[i]].type;
+            closed = t == LBF_VD || t == LBF_GV || t == LBF_SD || t == LBF_BB;
+        }
+        if (!closed) {
+            lbf_destroy(c);
+            return fail(nullptr, "CSF requires closed flow or guarded flat ghost/flux pressure reservoirs.");
+        }
+        if (d.precision == LBF_FP32) {
+            lbf_destroy(c);
+            return fail(nullptr, "lbf_create: capillaryModel csf runs in FP64 in this engine.");
+        }
+    } else if (d.capillary != LBF_CAPILLARY_PERTURBATION) {
+        lbf_destroy(c);
+        return fail(nullptr, "capillaryModel must be perturbation or csf.");
+    }
+    if (d.wall_gradient == LBF_WALL_TANH && d.model == LBF_MODEL_COLOR && d.precision == LBF_FP32) {
+        lbf_destroy(c);
+        return fail(nullptr, "lbf_create: wallGradient tanh runs in FP64 in this engine.");
     }
 
     // label table (simulator.cpp:218-228 forceForIndex)
@@ -369,6 +400,11 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         rc |= dalloc(c, &c->d_wall, N * 4);
         rc |= dalloc(c, &c->d_walls, c->links.nwall * sizeof(DevWall));
         if (!c->links.film.empty()) rc |= dalloc(c, &c->d_film, N * sizeof(uint64_t));
+        if (d.wall_gradient == LBF_WALL_TANH) rc |= dalloc(c, &c->d_tanhg, c->links.nwall * 3 * 8);
+        if (d.capillary == LBF_CAPILLARY_CSF) {
+            rc |= dalloc(c, &c->d_capgrad, size_t(d.D) * N * 8);
+            rc |= dalloc(c, &c->d_capfield, size_t(d.D) * N * 8);
+        }
     }
     if (rc) {
         std::string e = c->err;
@@ -482,6 +518,9 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         (void)hipMemset(c->d_rho1, 0, N * 8);
         (void)hipMemset(c->d_chiadj, 0, 2 * N * 8);
         (void)hipMemset(c->d_chiprev, 0, N * 8);
+        // only VD/GV sites are written; the others stay 0 (colorfluid.cpp:73-75)
+        if (c->d_capgrad) (void)hipMemset(c->d_capgrad, 0, size_t(d.D) * N * 8);
+        if (c->d_capfield) (void)hipMemset(c->d_capfield, 0, size_t(d.D) * N * 8);
     }
     DevStatus st{};
     for (auto& v : st.first) v = ~0ull;
@@ -558,6 +597,13 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
     a.wall = c->d_wall;
     a.walls = c->d_walls;
     a.film = c->d_film;
+    a.tanh_wall = c->d.wall_gradient == LBF_WALL_TANH;
+    a.tanhg = c->d_tanhg;
+    a.csf = c->d.capillary == LBF_CAPILLARY_CSF;
+    a.curv_gradient = c->d.curvature == LBF_CURVATURE_GRADIENT;
+    a.capgrad = c->d_capgrad;
+    a.capfield = c->d_capfield;
+    a.sigma = c->d.sigma;
     a.renormalized = c->d.wall_gradient == LBF_WALL_RENORMALIZED;
     a.densN = c->d_densN;
     a.A = c->d_A;
@@ -700,7 +746,7 @@ int lbf_status(lbf_ctx* c, uint32_t* bits, uint64_t* first)
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     if (bits) *bits = st.bits;
     if (first)
-        for (int k = 0; k < 5; ++k) first[k] = st.first[k];
+        for (int k = 0; k < 6; ++k) first[k] = st.first[k];
     return 0;
 }
 

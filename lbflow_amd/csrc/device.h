@@ -54,7 +54,7 @@ struct DevWall {
 struct DevStatus {
     uint32_t bits;
     uint32_t pad;
-    unsigned long long first[5];   // first step per bit (LBF_FAIL_*), ~0 = never
+    unsigned long long first[6];   // first step per bit (LBF_FAIL_*), ~0 = never
 };
 
 struct DevFastDiv {
@@ -225,6 +225,16 @@ struct ColorArgs {
     // film of phase 1), bit 32 + q = phase 1 bounces back (film of phase 0);
     // read only where the descriptor has DESC_FILM.
     const uint64_t* film;
+    // wallGradient tanh (colorfluid.cpp:494-518): 1 = tanh reconstruction at
+    // walls, computed per wall entry into tanhg [wall][3] by its own pass
+    int32_t tanh_wall;
+    double* tanhg;
+    // capillaryModel csf (colorfluid.cpp:572-634): per site [d][site] the colour
+    // gradient and the curvature field of the VD/GV sites (0 elsewhere)
+    int32_t csf, curv_gradient;
+    double* capgrad;
+    double* capfield;
+    double sigma;
 };
 
 struct ReportArgs {

@@ -36,8 +36,9 @@ namespace {
 // Pull (or initialise) both phases; returns the total fT_q = f0_q + f1_q and
 // the phase densities summed in direction order.  PB/VB sites apply the
 // Zou-He boundary to the pulled populations first and return its density and
-// velocity in rzh / uzh.
-template <int Q, bool INIT>
+// velocity in rzh / uzh.  OWN (with INIT): the site's own populations in fin,
+// not streamed (the capillary initialisation's second macro pass).
+template <int Q, bool INIT, bool OWN = false>
 __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint32_t desc,
                                            const DevLabel& lab, double* fT, double& r0, double& r1,
                                            double& rzh, double* uzh)
@@ -45,6 +46,17 @@ __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint3
 #pragma clang fp contract(off)
     using L = Lattice<Q>;
     const uint64_t N = a.g.N;
+    if (INIT && OWN) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double f0 = a.fin[uint64_t(q) * N + i];
+            const double f1 = a.fin[(uint64_t(Q) + q) * N + i];
+            if (q == 0) { r0 = f0; r1 = f1; }
+            else { r0 += f0; r1 += f1; }
+            fT[q] = f0 + f1;
+        }
+        return;
+    }
     if (INIT) {
         const int ph = lab.phase;
         const double* dist = ph == 0 ? a.dist0 : a.dist1;
@@ -113,9 +125,225 @@ __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint3
     }
 }
 
+// atanh of the colour clamped to (-1, 1) by 1e-8 (colorfluid.cpp:497-499)
+__device__ __forceinline__ double tanh_wall_transform(double color)
+{
+    const double hi = 1.0 - 1e-8, lo = -1.0 + 1e-8;
+    const double c = color < hi ? color : hi;
+    return atanh(lo < c ? c : lo);
+}
+
+// Colour gradient of a fluid site from the rhoN stencil (ColorFluid::
+// colorGradient, colorfluid.cpp:435-570): bulk stencil, or at wall sites the
+// one-sided stencil over fluid neighbours, the renormalized or tanh
+// reconstruction, and the geometric contact-angle rotation.  Returns |g|.
+template <int Q>
+__device__ __forceinline__ double color_gradient_f64(const ColorArgs& a, uint32_t i, int x, int y, int z,
+                                                     double* g)
+{
+#pragma clang fp contract(off)
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    g[0] = g[1] = g[2] = 0.0;
+    double norm;
+    const int32_t w = a.wall[i];
+    if (!w) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int k = 0; k < L::NREL; ++k) {
+                const int q = L::reld[d][k];
+                const double dnq = a.densN[push_target<Q>(a.g, x, y, z, q)];
+                if (L::e[q][d] == 1) g[d] += L::w[q] * dnq;
+                else g[d] -= L::w[q] * dnq;
+            }
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] *= 3.0;
+        double n2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+        norm = sqrt(n2);
+    } else {
+        const double dn0 = a.densN[i];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int k = 0; k < L::NREL; ++k) {
+                const int q = L::reld[d][k];
+                const double dnq = a.densN[push_target<Q>(a.g, x, y, z, q)];
+                if (dnq == SOLID_DENS_N) continue;
+                if (L::e[q][d] == 1) g[d] += L::w[q] * (dnq - dn0);
+                else g[d] -= L::w[q] * (dnq - dn0);
+            }
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] *= 3.0;
+        double n2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+        norm = sqrt(n2);
+        const DevWall& wl = a.walls[w];
+        if (a.renormalized) {
+            // wallGradient renormalized (colorfluid.cpp:487-492): g = inverse g / 3
+            double r[3];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < D; ++b) t = t + g[b] * wl.ginv[c * D + b];
+                r[c] = t;
+            }
+#pragma unroll
+            for (int c = 0; c < D; ++c) g[c] = r[c] / 3.0;
+            n2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+            norm = sqrt(n2);
+        }
+        if (a.tanh_wall) {
+            // wallGradient tanh: the reconstruction of color_tanh_wall
+#pragma unroll
+            for (int d = 0; d < D; ++d) g[d] = a.tanhg[uint64_t(w) * 3 + d];
+            n2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+            norm = sqrt(n2);
+        }
+        const double cA = wl.cosca, sA = wl.sinca;
+        if constexpr (D == 3) {
+            const double* T = wl.tib;
+            const double* Ti = wl.tibinv;
+            double gn[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) s = s + T[r * 3 + k] * g[k];
+                gn[r] = s;
+            }
+            // RotateTx3D(-gn0, gn1)
+            double c = 1, sn = 0;
+            const double so = -gn[0], sa = gn[1];
+            const double hyp = sqrt(so * so + sa * sa);
+            if (hyp != 0) { c = sa / hyp; sn = so / hyp; }
+            const double tx[9] = {c, -sn, 0, sn, c, 0, 0, 0, 1};
+            double ti[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) s = s + Ti[r * 3 + k] * tx[k * 3 + cc];
+                    ti[r * 3 + cc] = s;
+                }
+            const double h[3] = {0, norm * sA, norm * cA};
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) s = s + ti[r * 3 + k] * h[k];
+                g[r] = s;
+            }
+        } else {
+            const double* T = wl.tib;
+            const double* Ti = wl.tibinv;
+            double gn0 = 0.0;
+            gn0 = gn0 + T[0] * g[0];
+            gn0 = gn0 + T[1] * g[1];
+            double h[2];
+            if (gn0 >= 0) { h[0] = norm * sA; h[1] = norm * cA; }
+            else { h[0] = -norm * sA; h[1] = norm * cA; }
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                double s = 0.0;
+                s = s + Ti[r * 2 + 0] * h[0];
+                s = s + Ti[r * 2 + 1] * h[1];
+                g[r] = s;
+            }
+        }
+    }
+
+    return norm;
+}
+
+// Capillary force of a VD/GV site (ColorFluid::calcCapillaryForce,
+// colorfluid.cpp:591-617) from the site's stored gradient and the curvature
+// field of its fluid neighbours: kappa = sum_ab (n_a n_b - delta_ab) d_b c_a,
+// with d_b c_a from the inverse moment at wall sites, else 3 sum w e dc;
+// F = sigma/2 kappa direction.  Returns false if F is not finite.
+template <int Q>
+__device__ __forceinline__ bool capillary_force(const ColorArgs& a, uint32_t i, int x, int y, int z, double* F)
+{
+#pragma clang fp contract(off)
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    const uint64_t N = a.g.N;
+    const bool gc = a.curv_gradient;
+    double grad[3] = {0.0, 0.0, 0.0}, field[3] = {0.0, 0.0, 0.0};
+    double curvature = 0.0, norm2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        grad[d] = a.capgrad[uint64_t(d) * N + i];
+        field[d] = a.capfield[uint64_t(d) * N + i];
+        norm2 += grad[d] * grad[d];
+    }
+    if (norm2 > 1e-18) {
+        const int32_t w = a.wall[i];
+        uint32_t nb[Q];
+        bool fluid[Q];
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            nb[q] = push_target<Q>(a.g, x, y, z, q);
+            fluid[q] = a.densN[nb[q]] != SOLID_DENS_N;
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double moment[3] = {0.0, 0.0, 0.0}, derivative[3];
+            const double center = field[c];
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                if (!fluid[q]) continue;
+                const double difference = a.capfield[uint64_t(c) * N + nb[q]] - center;
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    if (L::e[q][b] == 1) moment[b] += L::w[q] * difference;
+                    else if (L::e[q][b] == -1) moment[b] -= L::w[q] * difference;
+                }
+            }
+            if (w) {
+                const double* inv = a.walls[w].ginv;
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int b = 0; b < D; ++b) t = t + moment[b] * inv[r * D + b];
+                    derivative[r] = t;
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < D; ++b) derivative[b] = moment[b] * 3.0;
+            }
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                const double na = gc ? grad[c] / sqrt(norm2) : center;
+                const double nbv = gc ? grad[b] / sqrt(norm2) : field[b];
+                curvature += (na * nbv - (c == b ? 1.0 : 0.0)) * derivative[b];
+            }
+        }
+    }
+    bool finite = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double direction = gc && norm2 > 1e-18 ? grad[d] / sqrt(norm2) : grad[d];
+        F[d] = .5 * a.sigma * curvature * direction;
+        finite = finite && isfinite(F[d]);
+    }
+    return finite;
+}
+
 }  // namespace
 
-template <int Q, bool INIT>
+template <int Q, bool INIT, bool OWN = false>
 __global__ void __launch_bounds__(256) color_density(ColorArgs a)
 {
 #pragma clang fp contract(off)
@@ -125,12 +353,145 @@ __global__ void __launch_bounds__(256) color_density(ColorArgs a)
     const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
     if (!(lab.kind & KIND_FLUID)) return;
     double fT[Q], r0, r1, rzh, uzh[Lattice<Q>::D];
-    load_color<Q, INIT>(a, i, desc, lab, fT, r0, r1, rzh, uzh);
+    load_color<Q, INIT, OWN>(a, i, desc, lab, fT, r0, r1, rzh, uzh);
     const double rt = r0 + r1;
     a.densN[i] = (r0 - r1) / rt;
 }
 
-template <int Q, bool MRT, bool INIT, bool STREAM_SOLIDS>
+// CSF, first half of calcCapillaryForce (colorfluid.cpp:577-586): the colour
+// gradient of every VD/GV site and its curvature field (the unit normal, or
+// the gradient for curvatureStencil "gradient").
+template <int Q>
+__global__ void __launch_bounds__(256) color_capillary(ColorArgs a)
+{
+#pragma clang fp contract(off)
+    constexpr int D = Lattice<Q>::D;
+    const uint32_t i = xcd_block() * 256u + threadIdx.x;
+    if (i >= a.g.N) return;
+    const uint64_t N = a.g.N;
+    const uint32_t desc = a.desc[i];
+    if (!(a.lab[desc >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) return;
+    int x, y, z;
+    site_coords(a.g, i, x, y, z);
+    double g[3];
+    const double norm = color_gradient_f64<Q>(a, i, x, y, z, g);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        a.capgrad[uint64_t(d) * N + i] = g[d];
+        a.capfield[uint64_t(d) * N + i] = a.curv_gradient ? g[d] : (norm > 1e-9 ? g[d] / norm : 0.0);
+    }
+}
+
+// wallGradient tanh (colorfluid.cpp:494-518), at every wall site: atanh(chi)
+// fitted as affine over the fluid neighbours (slope = inverse moment), tanh
+// continued over the full stencil, and the bulk gradient of that, stored per
+// wall entry for the gradient of the capillary and step passes.  A pass of its
+// own keeps the transcendental loops out of the step kernel's registers.
+template <int Q>
+__global__ void __launch_bounds__(256) color_tanh_wall(ColorArgs a)
+{
+#pragma clang fp contract(off)
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    const uint32_t i = xcd_block() * 256u + threadIdx.x;
+    if (i >= a.g.N) return;
+    const int32_t w = a.wall[i];
+    if (!w || !(a.lab[a.desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) return;
+    int x, y, z;
+    site_coords(a.g, i, x, y, z);
+    const DevWall& wl = a.walls[w];
+    const double dn0 = a.densN[i];
+    double g[3];
+    const double center = tanh_wall_transform(dn0);
+    double moment[3] = {0.0, 0.0, 0.0}, slope[3];
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        const double nb = a.densN[push_target<Q>(a.g, x, y, z, q)];
+        if (nb == SOLID_DENS_N) continue;
+        const double difference = tanh_wall_transform(nb) - center;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) moment[d] += L::w[q] * difference;
+            else if (L::e[q][d] == -1) moment[d] -= L::w[q] * difference;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < D; ++b) t = t + moment[b] * wl.ginv[c * D + b];
+        slope[c] = t;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = 0.0;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        double position = center;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) position += slope[d];
+            else if (L::e[q][d] == -1) position += -slope[d];
+        }
+        const double color = tanh(position);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) g[d] += 3.0 * L::w[q] * color;
+            else if (L::e[q][d] == -1) g[d] -= 3.0 * L::w[q] * color;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) a.tanhg[uint64_t(w) * 3 + d] = g[d];
+}
+
+// CSF initialisation (ColorFluid::initializeCapillaryForce,
+// colorfluid.cpp:619-634): the t = 0 populations of initialize() shifted by
+// the capillary force, f_k -= 1.5 w (e . F) rho_k / rho, written to the
+// site's own slots of fout.
+template <int Q>
+__global__ void __launch_bounds__(256) color_csf_init(ColorArgs a)
+{
+#pragma clang fp contract(off)
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    const uint32_t i = xcd_block() * 256u + threadIdx.x;
+    if (i >= a.g.N) return;
+    const uint64_t N = a.g.N;
+    const uint32_t desc = a.desc[i];
+    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
+    if (!(lab.kind & KIND_FLUID)) return;
+    const int ph = lab.phase;
+    const double* dist = ph == 0 ? a.dist0 : a.dist1;
+    double f0[Q], f1[Q], r0, r1;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double f = dist[q];
+        f -= 1.5 * L::w[q] * lab.ef[q];
+        f0[q] = ph == 0 ? f : 0.0;
+        f1[q] = ph == 0 ? 0.0 : f;
+        if (q == 0) { r0 = f0[q]; r1 = f1[q]; }
+        else { r0 += f0[q]; r1 += f1[q]; }
+    }
+    const double rt = r0 + r1;
+    int x, y, z;
+    site_coords(a.g, i, x, y, z);
+    double F[3] = {0.0, 0.0, 0.0};
+    if (!capillary_force<Q>(a, i, x, y, z, F)) raise_status(a.status, 5, a.t);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double ef = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) ef += F[d];
+            else if (L::e[q][d] == -1) ef += -F[d];
+        }
+        f0[q] -= 1.5 * L::w[q] * ef * r0 / rt;
+        f1[q] -= 1.5 * L::w[q] * ef * r1 / rt;
+        a.fout[uint64_t(q) * N + i] = f0[q];
+        a.fout[(uint64_t(Q) + q) * N + i] = f1[q];
+    }
+}
+
+template <int Q, bool MRT, bool INIT, bool STREAM_SOLIDS, bool CSF = false, bool OWN = false>
 __global__ void __launch_bounds__(256) color_step(ColorArgs a)
 {
 #pragma clang fp contract(off)
@@ -162,7 +523,7 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
     }
 
     double fT[Q], r0, r1, rzh = 0.0, uzh[D];
-    load_color<Q, INIT>(a, i, desc, lab, fT, r0, r1, rzh, uzh);
+    load_color<Q, INIT, OWN>(a, i, desc, lab, fT, r0, r1, rzh, uzh);
 
     // calcMacroVar
     const double rt = r0 + r1;
@@ -188,7 +549,16 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
         u[d] = s;
         if (!isfinite(s)) raise_status(a.status, 3, a.t);
     }
-    // bodyForce: u += (F/2)/rho on GV sites
+    // bodyForce (colorfluid.cpp:736-750): under CSF first u += (F_cap/2)/rho
+    // on VD/GV sites, then u += (F/2)/rho on GV sites
+    double fcap[3] = {0.0, 0.0, 0.0};
+    if constexpr (CSF) {
+        int cx, cy, cz;
+        site_coords(a.g, i, cx, cy, cz);
+        if (!capillary_force<Q>(a, i, cx, cy, cz, fcap)) raise_status(a.status, 5, a.t);
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] += .5 * fcap[d] / rt;
+    }
     if (kind & KIND_GV) {
 #pragma unroll
         for (int d = 0; d < D; ++d) u[d] += lab.half[d] / rt;
@@ -215,8 +585,19 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
         a.chiadj_prev[i] = rzh > 0.0 ? num / rzh : 0.0;
     }
 
-    // collision of the total population
-    const bool forced = kind & KIND_FORCED;
+    // collision of the total population; under CSF the site's force is the
+    // label's plus the capillary force (colorfluid.cpp:326-333)
+    bool forced = kind & KIND_FORCED;
+    double force[3] = {lab.force[0], lab.force[1], D == 3 ? lab.force[2] : 0.0};
+    if constexpr (CSF) {
+        double f2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            force[d] += fcap[d];
+            f2 = f2 + force[d] * force[d];
+        }
+        forced = f2 != 0.0;
+    }
     double usq = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) usq += u[d] * u[d];
@@ -224,7 +605,7 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
     if (forced) {
         double uf = 0.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) uf += u[d] * lab.force[d];
+        for (int d = 0; d < D; ++d) uf += u[d] * force[d];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             double eu = 0.0;
@@ -233,7 +614,17 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
                 if (L::e[q][d] == 1) eu += u[d];
                 else if (L::e[q][d] == -1) eu += -u[d];
             }
-            const double ef = lab.ef[q];
+            double ef;
+            if constexpr (CSF) {
+                ef = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    if (L::e[q][d] == 1) ef += force[d];
+                    else if (L::e[q][d] == -1) ef += -force[d];
+                }
+            } else {
+                ef = lab.ef[q];
+            }
             src[q] = L::w[q] * (3.0 * (ef - uf) + 9.0 * eu * ef);
         }
     }
@@ -333,117 +724,22 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
         for (int q = 0; q < Q; ++q) fT[q] += corr[q];
     }
 
-    // colour gradient (legacy branch)
+    // colour gradient: the stored capillary gradient under CSF (no
+    // perturbation, colorfluid.cpp:389-411), else colorGradient
     int x, y, z;
     site_coords(a.g, i, x, y, z);
     double g[3] = {0.0, 0.0, 0.0};
     double norm;
-    const int32_t w = a.wall[i];
-    if (!w) {
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-#pragma unroll
-            for (int k = 0; k < L::NREL; ++k) {
-                const int q = L::reld[d][k];
-                const double dnq = a.densN[push_target<Q>(a.g, x, y, z, q)];
-                if (L::e[q][d] == 1) g[d] += L::w[q] * dnq;
-                else g[d] -= L::w[q] * dnq;
-            }
-#pragma unroll
-        for (int d = 0; d < D; ++d) g[d] *= 3.0;
+    if constexpr (CSF) {
         double n2 = 0.0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
+        for (int d = 0; d < D; ++d) {
+            g[d] = a.capgrad[uint64_t(d) * N + i];
+            n2 = n2 + g[d] * g[d];
+        }
         norm = sqrt(n2);
     } else {
-        const double dn0 = a.densN[i];
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-#pragma unroll
-            for (int k = 0; k < L::NREL; ++k) {
-                const int q = L::reld[d][k];
-                const double dnq = a.densN[push_target<Q>(a.g, x, y, z, q)];
-                if (dnq == SOLID_DENS_N) continue;
-                if (L::e[q][d] == 1) g[d] += L::w[q] * (dnq - dn0);
-                else g[d] -= L::w[q] * (dnq - dn0);
-            }
-#pragma unroll
-        for (int d = 0; d < D; ++d) g[d] *= 3.0;
-        double n2 = 0.0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
-        norm = sqrt(n2);
-        const DevWall& wl = a.walls[w];
-        if (a.renormalized) {
-            // wallGradient renormalized (colorfluid.cpp:487-492): g = inverse g / 3
-            double r[3];
-#pragma unroll
-            for (int c = 0; c < D; ++c) {
-                double t = 0.0;
-#pragma unroll
-                for (int b = 0; b < D; ++b) t = t + g[b] * wl.ginv[c * D + b];
-                r[c] = t;
-            }
-#pragma unroll
-            for (int c = 0; c < D; ++c) g[c] = r[c] / 3.0;
-            n2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
-            norm = sqrt(n2);
-        }
-        const double cA = wl.cosca, sA = wl.sinca;
-        if constexpr (D == 3) {
-            const double* T = wl.tib;
-            const double* Ti = wl.tibinv;
-            double gn[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) s = s + T[r * 3 + k] * g[k];
-                gn[r] = s;
-            }
-            // RotateTx3D(-gn0, gn1)
-            double c = 1, sn = 0;
-            const double so = -gn[0], sa = gn[1];
-            const double hyp = sqrt(so * so + sa * sa);
-            if (hyp != 0) { c = sa / hyp; sn = so / hyp; }
-            const double tx[9] = {c, -sn, 0, sn, c, 0, 0, 0, 1};
-            double ti[9];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) s = s + Ti[r * 3 + k] * tx[k * 3 + cc];
-                    ti[r * 3 + cc] = s;
-                }
-            const double h[3] = {0, norm * sA, norm * cA};
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                double s = 0.0;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) s = s + ti[r * 3 + k] * h[k];
-                g[r] = s;
-            }
-        } else {
-            const double* T = wl.tib;
-            const double* Ti = wl.tibinv;
-            double gn0 = 0.0;
-            gn0 = gn0 + T[0] * g[0];
-            gn0 = gn0 + T[1] * g[1];
-            double h[2];
-            if (gn0 >= 0) { h[0] = norm * sA; h[1] = norm * cA; }
-            else { h[0] = -norm * sA; h[1] = norm * cA; }
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                double s = 0.0;
-                s = s + Ti[r * 2 + 0] * h[0];
-                s = s + Ti[r * 2 + 1] * h[1];
-                g[r] = s;
-            }
-        }
+        norm = color_gradient_f64<Q>(a, i, x, y, z, g);
     }
 
     // perturbation + recolouring
@@ -461,7 +757,7 @@ __global__ void __launch_bounds__(256) color_step(ColorArgs a)
                 else if (L::e[q][d] == -1) angle -= g[d];
             }
             angle /= norm;
-            fT[q] += akn * (L::w[q] * angle * angle - a.bi[q]);
+            if constexpr (!CSF) fT[q] += akn * (L::w[q] * angle * angle - a.bi[q]);
             if (L::diagonal(q)) angle /= SQRT2;
             const double f0 = dens0T * fT[q] + b01 * angle * feq0[q];
             const double f1 = fT[q] - f0;
@@ -486,7 +782,33 @@ template <int Q, bool MRT, bool INIT>
 static hipError_t launch_color_q(const ColorArgs& a, bool stream_solids, hipStream_t s)
 {
     const dim3 grid(uint32_t((a.g.N + 255) / 256)), block(256);
+    auto tanh_pass = [&](const ColorArgs& x) {
+        if (x.tanh_wall) hipLaunchKernelGGL((color_tanh_wall<Q>), grid, block, 0, s, x);
+    };
+    if (a.csf) {
+        // CSF (colorfluid.cpp:124-137): a step is density, gradient/curvature
+        // field, step.  t = 0: initialize + calcMacroVar, the capillary shift
+        // of the populations, calcMacroVar again, bodyForce and collision on
+        // the shifted populations in place.
+        hipLaunchKernelGGL((color_density<Q, INIT>), grid, block, 0, s, a);
+        tanh_pass(a);
+        hipLaunchKernelGGL((color_capillary<Q>), grid, block, 0, s, a);
+        if (!INIT) {
+            if (stream_solids) hipLaunchKernelGGL((color_step<Q, MRT, false, true, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((color_step<Q, MRT, false, false, true>), grid, block, 0, s, a);
+            return hipGetLastError();
+        }
+        hipLaunchKernelGGL((color_csf_init<Q>), grid, block, 0, s, a);
+        ColorArgs b = a;
+        b.fin = a.fout;
+        hipLaunchKernelGGL((color_density<Q, true, true>), grid, block, 0, s, b);
+        tanh_pass(b);
+        hipLaunchKernelGGL((color_capillary<Q>), grid, block, 0, s, b);
+        hipLaunchKernelGGL((color_step<Q, MRT, true, false, true, true>), grid, block, 0, s, b);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((color_density<Q, INIT>), grid, block, 0, s, a);
+    tanh_pass(a);
     if (stream_solids && !INIT) hipLaunchKernelGGL((color_step<Q, MRT, INIT, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((color_step<Q, MRT, INIT, false>), grid, block, 0, s, a);
     return hipGetLastError();

@@ -17,7 +17,7 @@ FP64, FP32 = 0, 1
 STREAM_SOLIDS = 0x1
 LAYOUT_DENSE = 0x2
 STEP_FIELDS, STEP_ADJACENT = 0x1, 0x2
-STATUS_BITS = (0x1, 0x2, 0x4, 0x8, 0x10)
+STATUS_BITS = (0x1, 0x2, 0x4, 0x8, 0x10, 0x20)
 
 SYMBOLS = ('lbf_create', 'lbf_destroy', 'lbf_last_error', 'lbf_init', 'lbf_step', 'lbf_advance',
            'lbf_time', 'lbf_synchronize', 'lbf_status', 'lbf_status_message', 'lbf_fields',
@@ -38,7 +38,7 @@ class Desc(C.Structure):
                 ('precision', C.c_int32), ('flags', C.c_int32), ('harmonic', C.c_int32),
                 ('wall_gradient', C.c_int32), ('sz', C.c_uint64 * 3), ('tau', C.c_double * 2),
                 ('density', C.c_double * 2), ('alpha0', C.c_double), ('sigma', C.c_double),
-                ('beta', C.c_double)]
+                ('beta', C.c_double), ('capillary', C.c_int32), ('curvature', C.c_int32)]
 
 
 class ReportReq(C.Structure):
@@ -176,7 +176,9 @@ class Engine:
         d.precision = precision
         d.flags = (STREAM_SOLIDS if stream_solids else 0) | (LAYOUT_DENSE if dense else 0)
         d.harmonic = 1 if case.get('harmonic') else 0
-        d.wall_gradient = 1 if case.get('wall_gradient') == 'renormalized' else 0
+        d.wall_gradient = {'legacy': 0, 'renormalized': 1, 'tanh': 2}[case.get('wall_gradient', 'legacy')]
+        d.capillary = 1 if case.get('capillary') == 'csf' else 0
+        d.curvature = 1 if case.get('curvature') == 'gradient' else 0
         for k, n in enumerate(case['size']):
             d.sz[k] = n
         tau = list(case['tau']) + [0.0] * (2 - len(case['tau']))
@@ -242,7 +244,7 @@ class Engine:
 
     def status(self):
         bits = C.c_uint32()
-        first = np.zeros(5, np.uint64)
+        first = np.zeros(6, np.uint64)
         self._check(self.L.lbf_status(self.h, C.byref(bits), _ptr(first)))
         return int(bits.value), first
 

@@ -43,7 +43,8 @@ def base():
     ({'reportDir': 3, 'reportSlice': '1 6'}, 'Report slice is outside the selected axis.'),
     ({'fluidModel': '"shanchen"'}, 'Only scsp and color fluid models are implemented.'),
     ({'fluidModel': '"lattice-gas"'}, 'Main(): Unsupport fluid model'),
-    ({'wallGradient': '"tanh"'}, 'wallGradient "tanh" is not implemented by this engine'),
+    ({'capillaryModel': '"csf"', 'enginePrecision': '"fp32"'}, 'capillaryModel csf runs in FP64 in this engine.'),
+    ({'wallGradient': '"tanh"', 'enginePrecision': '"fp32"'}, 'wallGradient tanh runs in FP64 in this engine.'),
 ])
 def test_invalid_inputs_fail_with_reference_message(tmp_path, overrides, message):
     spec = dict(base(), overrides=overrides)
@@ -76,6 +77,18 @@ def test_short_geometry(tmp_path):
     raw.write_bytes(raw.read_bytes()[:-1])
     p = subprocess.run([str(LBFLOW), str(prm)], cwd=tmp_path, capture_output=True, text=True)
     assert p.returncode != 0 and 'FileRaw::Load() - Failed to read from raw file' in p.stdout
+
+
+@pytest.mark.parametrize('overrides,message', [
+    ({'wallGradient': '"tangent"'}, 'wallGradient "tangent" is not implemented by this engine'),
+    ({'recoloringModel': '"projection"'}, 'recoloringModel "projection" is not implemented by this engine'),
+])
+def test_unimplemented_d2q9_options(tmp_path, overrides, message):
+    """The reference's remaining experimental colour options run on D2Q9 only
+    and are rejected here with a message."""
+    spec = [s for s in H.force_cases() if s['model'] == 'color' and s['dimensions'] == 2][-1]
+    code, out = run_case(tmp_path, dict(spec, overrides=overrides))
+    assert code != 0 and message in out, out[-300:]
 
 
 def test_missing_key(tmp_path):

@@ -138,6 +138,36 @@ def test_film_case_through_process(name, tmp_path):
         assert (cont / last).read_bytes() == (resumed / last).read_bytes()
 
 
+@pytest.mark.parametrize('name', ['drop', 'wall'])
+def test_csf_through_process(name, tmp_path):
+    """The reference's CSF replay cases (scripts/capillary_checks.py:46-65):
+    a 12^3 drop and a 2D contact line between tanh walls run through the
+    drop-in; a restart from the middle checkpoint writes the same final
+    files; the drop's fields equal the oracle's bit for bit, the tanh-wall
+    case's within 1e-12 (atanh/tanh differ by an ulp between the host and
+    device libraries)."""
+    spec = H.csf_drop() if name == 'drop' else dict(H.csf_wall(), restart=50)
+    half = spec['steps'] // 2
+    cont = tmp_path / 'continuous'
+    code, out, timing = run(spec, cont)
+    assert code == 0, out[-2000:]
+    last = f'field.{spec["steps"]:010d}'
+    _, f = H.read_vti(cont / f'{last}.vti')
+    o, case = oracle_run(spec, spec['steps'])
+    r0, r1, u = o.fields()
+    got = np.concatenate([f['r0'].ravel(), f['r1'].ravel(), f['v'].ravel()])
+    want = np.concatenate([r0, r1, velocity_grid(u, case).ravel()])
+    if name == 'drop':
+        assert np.array_equal(got, want)
+    else:
+        assert np.abs(got - want).max() <= 1e-12
+    resumed = tmp_path / 'resumed'
+    code, out, timing = run(spec, resumed, half, cont / f'field.{half:010d}')
+    assert code == 0, out[-2000:]
+    for suffix in ('.vti', '.rst'):
+        assert (cont / (last + suffix)).read_bytes() == (resumed / (last + suffix)).read_bytes()
+
+
 # Core grids (z, y, x) of an R-rank reference run on a 12^3 grid, by hand from
 # GridMc::initCoreDistribution (gridmc.cpp:55-73): the primes of R, largest
 # first, each divide the axis with the largest remaining extent (first of
