@@ -9,17 +9,20 @@
 //     rhoN stencil with the geometric contact-angle rotation at wall sites,
 //     perturbation and recolouring, write both phases.
 // Streaming is the same pull with periodic wrap and halfway bounce-back as
-// the single-phase kernel.
+// the single-phase kernel.  tanh walls add a wall pass after A
+// (color_tanh_wall); the CSF capillary force adds a pass between A and B
+// (color_capillary) that writes the colour gradient and curvature field.
 //
-// Reference operators (zhou1800/lbflow, default "legacy"/"perturbation"/
-// "cosine" options):
+// Reference operators (zhou1800/lbflow):
 //   initialize              solver/src/lbm/colorfluid.cpp:214-236
 //   calcMacroVar            solver/src/lbm/colorfluid.cpp:238-290
 //   bodyForce               solver/src/lbm/colorfluid.cpp:736-750
 //   collision               solver/src/lbm/colorfluid.cpp:292-433
-//   colorGradient (legacy / renormalized walls)  solver/src/lbm/colorfluid.cpp:435-570
+//   colorGradient (legacy / renormalized / tanh walls)  solver/src/lbm/colorfluid.cpp:435-570
+//   calcCapillaryForce, initializeCapillaryForce (CSF)  solver/src/lbm/colorfluid.cpp:572-634
 //   RotateTx3D etc.         solver/src/algorithm/fn.cpp:118-164
 //   zouHeBoundary (PB/VB)   solver/src/lbm/colorfluid.cpp:638-734 (zouhe.h)
+//   clearHalfFilm (HF)      solver/src/lbm/colorfluid.cpp:752-764
 //
 // FP64, reference operation order (built with -ffp-contract=off); zero
 // lattice coefficients are skipped exactly as in kernels_scsp.hip.

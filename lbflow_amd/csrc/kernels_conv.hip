@@ -582,7 +582,9 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
     constexpr int KP = conv_kp(CIN, 27);
     constexpr int CC = 16, BM = 32, NTH = 192;
     constexpr int NCH = CIN / CC, NCO = COUT / BM;
-    constexpr int HY = 6, HX = 34, ROWS = 24, HP = 17, RW = 40;   // block 2 x 4 x 32
+    // halo of the 2 x 4 x 32 block: ROWS = 4 x HY rows of 34 voxels, loaded as
+    // HP = 17 voxel pairs and stored with a row pitch of RW = 40 elements
+    constexpr int HY = 6, ROWS = 24, HP = 17, RW = 40;
     constexpr int NXI = (ROWS * HP * 2 + NTH - 1) / NTH, NDZ = (512 + NTH - 1) / NTH;
     static_assert(CIN % 32 == 0 && COUT % BM == 0, "shape");
     __shared__ __attribute__((aligned(16))) __bf16 sdz[BM * WG_DZS];
