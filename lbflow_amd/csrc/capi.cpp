@@ -283,10 +283,6 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         lbf_destroy(c);
         return fail(nullptr, "capillaryModel must be perturbation or csf.");
     }
-    if (d.wall_gradient == LBF_WALL_TANH && d.model == LBF_MODEL_COLOR && d.precision == LBF_FP32) {
-        lbf_destroy(c);
-        return fail(nullptr, "lbf_create: wallGradient tanh runs in FP64 in this engine.");
-    }
 
     // label table (simulator.cpp:218-228 forceForIndex)
     c->hlab.assign(256, DevLabel{});

@@ -133,6 +133,16 @@ __device__ __forceinline__ uint32_t push_target(const GridDev& g, int x, int y, 
     return uint32_t(zs) * g.nxy + uint32_t(ys) * g.nx + uint32_t(xs);
 }
 
+// wallGradient tanh: atanh of the colour clamped to (-1, 1) by 1e-8
+// (colorfluid.cpp:497-499), in FP64 for both engines (in FP32 the clamp
+// would round to +-1 and atanh to infinity)
+__device__ __forceinline__ double tanh_wall_transform(double color)
+{
+    const double hi = 1.0 - 1e-8, lo = -1.0 + 1e-8;
+    const double c = color < hi ? color : hi;
+    return atanh(lo < c ? c : lo);
+}
+
 __device__ __forceinline__ void raise_status(DevStatus* st, int bit, unsigned long long t)
 {
     atomicOr(&st->bits, 1u << bit);

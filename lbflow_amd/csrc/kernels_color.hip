@@ -128,14 +128,6 @@ __device__ __forceinline__ void load_color(const ColorArgs& a, uint32_t i, uint3
     }
 }
 
-// atanh of the colour clamped to (-1, 1) by 1e-8 (colorfluid.cpp:497-499)
-__device__ __forceinline__ double tanh_wall_transform(double color)
-{
-    const double hi = 1.0 - 1e-8, lo = -1.0 + 1e-8;
-    const double c = color < hi ? color : hi;
-    return atanh(lo < c ? c : lo);
-}
-
 // Colour gradient of a fluid site from the rhoN stencil (ColorFluid::
 // colorGradient, colorfluid.cpp:435-570): bulk stencil, or at wall sites the
 // one-sided stencil over fluid neighbours, the renormalized or tanh

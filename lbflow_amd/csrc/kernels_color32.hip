@@ -380,6 +380,15 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
             for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
             norm = sqrtf(n2);
         }
+        if (a.tanh_wall) {
+            // wallGradient tanh: the reconstruction of color_tanh_wall_f32
+#pragma unroll
+            for (int d = 0; d < D; ++d) g[d] = float(a.tanhg[uint64_t(wi) * 3 + d]);
+            n2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
+            norm = sqrtf(n2);
+        }
         const float cA = float(wl.cosca), sA = float(wl.sinca);
         if constexpr (D == 3) {
             float T[9], Ti[9];
@@ -457,12 +466,68 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     if (!finite) raise_status(a.status, 4, a.t);
 }
 
+// wallGradient tanh at the wall sites (colorfluid.cpp:494-518), from the FP32
+// colour indicator but in FP64 (the clamp at 1 - 1e-8 needs it): atanh(chi)
+// fitted as affine over the fluid neighbours, tanh continued over the full
+// stencil, the bulk gradient of that stored per wall entry.
+template <int Q, int LAYOUT>
+__global__ void __launch_bounds__(256) color_tanh_wall_f32(ColorArgs a)
+{
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    uint32_t site, slot;
+    if (!resolve32<LAYOUT>(a, site, slot)) return;
+    const int32_t w = a.wall[site];
+    if (!w || !(a.lab[a.desc[slot] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) return;
+    int x, y, z;
+    site_coords(a.g, site, x, y, z);
+    const DevWall& wl = a.walls[w];
+    const double center = tanh_wall_transform(double(a.densN32[site]));
+    double moment[3] = {0.0, 0.0, 0.0}, slope[3], g[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        const float nb = a.densN32[push_target<Q>(a.g, x, y, z, q)];
+        if (nb == SOLID_N32) continue;
+        const double difference = tanh_wall_transform(double(nb)) - center;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) moment[d] += L::w[q] * difference;
+            else if (L::e[q][d] == -1) moment[d] -= L::w[q] * difference;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < D; ++b) t = t + moment[b] * wl.ginv[c * D + b];
+        slope[c] = t;
+    }
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        double position = center;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) position += slope[d];
+            else if (L::e[q][d] == -1) position += -slope[d];
+        }
+        const double color = tanh(position);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) g[d] += 3.0 * L::w[q] * color;
+            else if (L::e[q][d] == -1) g[d] -= 3.0 * L::w[q] * color;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) a.tanhg[uint64_t(w) * 3 + d] = g[d];
+}
+
 template <int Q, bool MRT, bool INIT, int LAYOUT, typename O>
 static hipError_t launch_color32_l(const ColorArgs& a, hipStream_t s)
 {
     const dim3 grid(uint32_t((a.P + 255) / 256)), block(256);
     if (a.P == 0) return hipSuccess;
     hipLaunchKernelGGL((color_density_f32<Q, INIT, LAYOUT, O>), grid, block, 0, s, a);
+    if (a.tanh_wall) hipLaunchKernelGGL((color_tanh_wall_f32<Q, LAYOUT>), grid, block, 0, s, a);
     hipLaunchKernelGGL((color_step_f32<Q, MRT, INIT, LAYOUT, O>), grid, block, 0, s, a);
     return hipGetLastError();
 }

@@ -44,7 +44,6 @@ def base():
     ({'fluidModel': '"shanchen"'}, 'Only scsp and color fluid models are implemented.'),
     ({'fluidModel': '"lattice-gas"'}, 'Main(): Unsupport fluid model'),
     ({'capillaryModel': '"csf"', 'enginePrecision': '"fp32"'}, 'capillaryModel csf runs in FP64 in this engine.'),
-    ({'wallGradient': '"tanh"', 'enginePrecision': '"fp32"'}, 'wallGradient tanh runs in FP64 in this engine.'),
 ])
 def test_invalid_inputs_fail_with_reference_message(tmp_path, overrides, message):
     spec = dict(base(), overrides=overrides)

@@ -1,4 +1,5 @@
-"""Experimental CSF capillary force and tanh wall gradient on the GPU (FP64).
+"""Experimental CSF capillary force and tanh wall gradient on the GPU (FP64;
+tanh walls also in the FP32 colour engine).
 
 CSF without tanh walls must equal the CPU oracle bit for bit: the reference's
 CSF check cases (planar interface, uniform force; D2Q9/D3Q19, SRT/MRT, both
@@ -18,7 +19,7 @@ import numpy as np
 import pytest
 
 from lbflow_amd import harness as H
-from lbflow_amd.engine import FP64, STEP_FIELDS, Engine
+from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Engine
 from tests.helpers import oracle_run, sphere_case, velocity_grid
 
 pytestmark = pytest.mark.gpu
@@ -30,9 +31,9 @@ def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def engine_run(spec, steps):
+def engine_run(spec, steps, precision=FP64, stream_solids=True, dense=False):
     case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), FP64, True, False).init()
+    e = Engine(case, H.geometry(spec), precision, stream_solids, dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
     e.synchronize()
@@ -114,3 +115,28 @@ def test_tanh_walls_track_oracle(spec, steps):
     # the tanh reconstruction is active: legacy walls give a different run
     legacy, _ = oracle_run(dict(spec, wall_gradient='legacy'), steps)
     assert np.abs(legacy.populations() - po).max() > 1e3 * np.abs(pe - po).max()
+
+
+@pytest.mark.parametrize('name', ['pack', 'zhc-x-3d-MRT'])
+def test_tanh_walls_fp32(name):
+    """FP32 colour engine with tanh walls: sparse == dense fluid-only bit for
+    bit; within the FP32 colour tolerances of the FP64 engine (rho0 5e-4
+    absolute, u 1e-2 relative L2); and different from legacy walls."""
+    if name == 'pack':
+        spec = sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, porosity=0.8)
+    else:
+        spec = next(s for s in H.zouhe_color_cases() if s['name'] == name)
+    spec = dict(spec, wall_gradient='tanh')
+    a, _ = engine_run(spec, 200)
+    b, _ = engine_run(spec, 200, FP32, stream_solids=False, dense=True)
+    c, _ = engine_run(spec, 200, FP32, stream_solids=False)
+    assert 'sparse' in c.step_kernel
+    assert np.array_equal(b.populations(), c.populations())
+    ra0, _, ua = a.fields()
+    rc0, _, uc = c.fields()
+    du = np.linalg.norm(ua - uc) / np.linalg.norm(ua)
+    say(name, 'max |rho0 diff|', np.max(np.abs(ra0 - rc0)), 'rel L2 u', du)
+    assert np.max(np.abs(ra0 - rc0)) < 5e-4
+    assert du < 1e-2
+    legacy, _ = engine_run(dict(spec, wall_gradient='legacy'), 200, FP32, stream_solids=False)
+    assert np.linalg.norm(legacy.fields()[2] - uc) > 10 * np.linalg.norm(ua - uc)
