@@ -73,9 +73,9 @@ enum lbf_create_flags {
  * colorfluid.cpp:917-957); both followed by the contact-angle rotation. */
 enum lbf_wall_gradient { LBF_WALL_LEGACY = 0, LBF_WALL_RENORMALIZED = 1, LBF_WALL_TANH = 2 };
 /* color capillaryModel (colorfluid.cpp:292-433, 572-634): the perturbation
- * operator, or the experimental continuum-surface force (FP64, closed flow:
+ * operator, or the experimental continuum-surface force (closed flow:
  * VD/GV/SD/BB labels only, as the reference requires without pressure
- * reservoirs). */
+ * reservoirs; in FP32 the normal field is cut at |g| <= 1e-5, not 1e-9). */
 enum lbf_capillary { LBF_CAPILLARY_PERTURBATION = 0, LBF_CAPILLARY_CSF = 1 };
 /* CSF curvatureStencil: curvature from the unit normal field, or from the gradient */
 enum lbf_curvature { LBF_CURVATURE_NORMAL = 0, LBF_CURVATURE_GRADIENT = 1 };

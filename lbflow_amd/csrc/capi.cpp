@@ -275,10 +275,6 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
             lbf_destroy(c);
             return fail(nullptr, "CSF requires closed flow or guarded flat ghost/flux pressure reservoirs.");
         }
-        if (d.precision == LBF_FP32) {
-            lbf_destroy(c);
-            return fail(nullptr, "lbf_create: capillaryModel csf runs in FP64 in this engine.");
-        }
     } else if (d.capillary != LBF_CAPILLARY_PERTURBATION) {
         lbf_destroy(c);
         return fail(nullptr, "capillaryModel must be perturbation or csf.");

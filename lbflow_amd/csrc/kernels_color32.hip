@@ -34,6 +34,14 @@ namespace {
 constexpr float SOLID_N32 = 86.0f;   // colorfluid.cpp:49
 constexpr float NEG_TOL32 = 1e-5f;
 constexpr float SQRT2_32 = 1.41421356f;
+// CSF in FP32: the reference cuts the normal field at |g| <= 1e-9 and the
+// curvature at |g|^2 <= 1e-18 (colorfluid.cpp:583, 592), cuts chosen for
+// FP64.  The FP32 populations leave the indicator about 1e-7 off +-1 after
+// the capillary shift, so pure-phase sites get |g| of order 1e-8 and, past
+// the FP64 cut, a random unit normal (or, for the gradient stencil, a
+// unit force direction) where FP64 has exactly 0.  The FP32 engine cuts at
+// |g| <= 1e-5 instead, ten times that noise floor.
+constexpr double CSF_GRAD_CUT32 = 1e-5;
 
 template <int LAYOUT>
 __device__ __forceinline__ bool resolve32(const ColorArgs& a, uint32_t& site, uint32_t& slot)
@@ -47,7 +55,7 @@ __device__ __forceinline__ bool resolve32(const ColorArgs& a, uint32_t& site, ui
 
 // PB/VB sites apply the Zou-He boundary to the pulled populations and return
 // its density deviation and velocity in dzh / uzh.
-template <int Q, bool INIT, int LAYOUT, typename O>
+template <int Q, bool INIT, int LAYOUT, typename O, bool OWN = false>
 __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, uint32_t slot, uint32_t desc,
                                              const DevLabel& lab, float* h, float* f0, float& r0, float& dr,
                                              float& dzh, float* uzh)
@@ -55,6 +63,18 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
     using L = Lattice<Q>;
     r0 = 0.0f;
     dr = 0.0f;
+    if (INIT && OWN) {
+        // the site's own slots of fin32, not streamed (CSF initialisation)
+        const O P = a.P;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            h[q] = ld(a.fin32, O(q) * P + O(slot));
+            f0[q] = ld(a.fin32, (O(Q) + q) * P + O(slot));
+            r0 += f0[q];
+            dr += h[q];
+        }
+        return;
+    }
     if (INIT) {
         // colorfluid.cpp:214-236: f[phase] = ai_phase rho_phase - 1.5 w (e.F), other phase 0
         const int ph = lab.phase;
@@ -139,193 +159,16 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
     }
 }
 
-}  // namespace
-
-template <int Q, bool INIT, int LAYOUT, typename O>
-__global__ void __launch_bounds__(256) color_density_f32(ColorArgs a)
-{
-    uint32_t site, slot;
-    if (!resolve32<LAYOUT>(a, site, slot)) return;
-    const uint32_t desc = a.desc[slot];
-    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
-    if (!(lab.kind & KIND_FLUID)) return;
-    float h[Q], f0[Q], r0, dr, dzh, uzh[Lattice<Q>::D];
-    load_color32<Q, INIT, LAYOUT, O>(a, site, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
-    const float rho = float(a.rho_ref) + dr;
-    a.densN32[site] = (2.0f * r0 - rho) / rho;   // (rho0 - rho1) / rho
-}
-
-template <int Q, bool MRT, bool INIT, int LAYOUT, typename O>
-__global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
+// Colour gradient of a fluid site from the FP32 indicator (ColorFluid::
+// colorGradient, colorfluid.cpp:435-570): bulk stencil, or at wall sites the
+// one-sided stencil, the renormalized or tanh reconstruction and the
+// contact-angle rotation.  Returns |g|.
+template <int Q>
+__device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t i, int x, int y, int z, float* g)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
-    constexpr int H = L::NHALF;
-    constexpr MomentTables<Q> MT = kMT<Q>;
-    uint32_t i, slot;
-    if (!resolve32<LAYOUT>(a, i, slot)) return;
-    const uint64_t N = a.g.N;
-    const uint32_t desc = a.desc[slot];
-    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
-    const int kind = lab.kind;
-    if (!(kind & KIND_FLUID)) return;
-
-    float h[Q], f0[Q], r0, dr, dzh = 0.0f, uzh[D];
-    load_color32<Q, INIT, LAYOUT, O>(a, i, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
-
-    // calcMacroVar (colorfluid.cpp:238-290)
-    const float rt = float(a.rho_ref) + dr;
-    const float r1 = rt - r0;
-    if (!isfinite(r0) || !isfinite(rt) || rt <= 0.0f) raise_status(a.status, 0, a.t);
-    else if (r0 < -NEG_TOL32 * rt || r1 < -NEG_TOL32 * rt) raise_status(a.status, 1, a.t);
-    const float inv_rt = 1.0f / rt;
-    const float dn = (r0 - r1) * inv_rt;
-    float tau = (1.0f + dn) * float(a.relax0) + (1.0f - dn) * float(a.relax1);
-    if (a.harmonic) {
-        const float phi = r0 * inv_rt;
-        tau = 0.5f + 3.0f / (phi / float(a.nu0) + (1.0f - phi) / float(a.nu1));
-    }
-    if (!isfinite(tau) || tau <= 0.5f) raise_status(a.status, 2, a.t);
-    float u[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        float s = 0.0f;
-#pragma unroll
-        for (int q = 1; q < Q; ++q) {
-            if (L::e[q][d] == 1) s += h[q];
-            else if (L::e[q][d] == -1) s -= h[q];
-        }
-        u[d] = s * inv_rt;
-        if (kind & KIND_GV) u[d] += float(lab.half[d]) * inv_rt;   // bodyForce (colorfluid.cpp:736-750)
-        if (!isfinite(u[d])) raise_status(a.status, 3, a.t);
-    }
-    if (a.flags & 1u) {
-        a.rho0[i] = double(r0);
-        a.rho1[i] = double(r1);
-#pragma unroll
-        for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = double(u[d]);
-    }
-    // adjacent report history; PB/VB sites as in kernels_color.hip
-    if (a.flags & 2u) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = double(u[d]);
-        a.chiadj[i] = (kind & KIND_ZOUHE) ? double(r0 - r1) : double(dn);
-    }
-    if (!INIT && (a.flags & 1u) && (kind & KIND_ZOUHE)) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) a.uadj_prev[uint64_t(d) * N + i] = double(uzh[d]);
-        const double rzh = a.rho_ref + double(dzh);
-        const double num = a.chiadj_prev[i];
-        a.chiadj_prev[i] = rzh > 0.0 ? num / rzh : 0.0;
-    }
-
-    // collision of the total population (colorfluid.cpp:292-386), on h
-    const bool forced = kind & KIND_FORCED;
-    float F[3] = {0.0f, 0.0f, 0.0f};
-    if (forced)
-#pragma unroll
-        for (int d = 0; d < D; ++d) F[d] = float(lab.force[d]);
-    float usq = 0.0f, uF = 0.0f;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        usq += u[d] * u[d];
-        uF += u[d] * F[d];
-    }
-    const float a0 = r0 - float(a.rho_ref);   // rest-state weights of the shifted equilibrium
-    if constexpr (!MRT) {
-        const float inv_tau = 1.0f / tau, cg = 1.0f - 0.5f * inv_tau;
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            float eu = 0.0f, ef = 0.0f;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                if (L::e[q][d] == 1) { eu += u[d]; ef += F[d]; }
-                else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
-            }
-            const float w = float(L::w[q]);
-            const float heq = a0 * float(a.ai0[q]) + r1 * float(a.ai1[q]) +
-                              rt * w * (eu * (3.0f + 4.5f * eu) - 1.5f * usq);
-            h[q] += (heq - h[q]) * inv_tau;
-            if (forced) h[q] += cg * w * (3.0f * (ef - uF) + 9.0f * eu * ef);
-        }
-    } else {
-        const float su = 1.0f / tau;
-        const float sq = 8.0f * (2.0f - su) / (8.0f - su);
-        float sp[H], sm[H];
-#pragma unroll
-        for (int j = 0; j < H; ++j) {
-            const int q = L::half[j];
-            sp[j] = h[q] + h[L::op[q]];
-            sm[j] = h[q] - h[L::op[q]];
-        }
-        float y[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            float mg = 0.0f;
-            if (forced) {
-#pragma unroll
-                for (int p = 0; p < D; ++p) {
-                    if (MT.T1[k][p] != 0.0) mg += float(3.0 * MT.T1[k][p]) * F[p];
-#pragma unroll
-                    for (int b = 0; b < D; ++b)
-                        if (MT.T2[k][p][b] != 0.0) mg += float(9.0 * MT.T2[k][p][b]) * u[p] * F[b];
-                }
-                if (MT.T0[k] != 0.0) mg -= float(3.0 * MT.T0[k]) * uF;
-            }
-            if (MT.rate[k] == 0) {
-                y[k] = mg * float(1.0 / MT.n2[k]);
-                continue;
-            }
-            float m = 0.0f;
-            if (MT.parity[k] > 0) {
-                if (L::M[k][0] != 0) m += float(L::M[k][0]) * h[0];
-#pragma unroll
-                for (int j = 0; j < H; ++j)
-                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sp[j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < H; ++j)
-                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sm[j];
-            }
-            float meq = 0.0f;
-#pragma unroll
-            for (int p = 0; p < D; ++p) {
-                if (MT.T1[k][p] != 0.0) meq += float(3.0 * MT.T1[k][p]) * u[p];
-#pragma unroll
-                for (int b = 0; b < D; ++b)
-                    if (MT.T2[k][p][b] != 0.0) meq += float(4.5 * MT.T2[k][p][b]) * u[p] * u[b];
-            }
-            if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
-            meq = meq * rt + a0 * float(a.mai0[k]) + r1 * float(a.mai1[k]);
-            const float s = MT.rate[k] == 1 ? su : sq;
-            y[k] = (s * (meq - m) + (1.0f - 0.5f * s) * mg) * float(1.0 / MT.n2[k]);
-        }
-        {
-            float e0 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < Q; ++k)
-                if (MT.parity[k] > 0 && L::M[k][0] != 0) e0 += float(L::M[k][0]) * y[k];
-            h[0] += e0;
-        }
-#pragma unroll
-        for (int j = 0; j < H; ++j) {
-            const int q = L::half[j];
-            float ev = 0.0f, od = 0.0f;
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                if (L::M[k][q] == 0) continue;
-                if (MT.parity[k] > 0) ev += float(L::M[k][q]) * y[k];
-                else od += float(L::M[k][q]) * y[k];
-            }
-            h[q] += ev + od;
-            h[L::op[q]] += ev - od;
-        }
-    }
-
-    // colour gradient (legacy branch, colorfluid.cpp:435-570)
-    int x, y, z;
-    site_coords(a.g, i, x, y, z);
-    float g[3] = {0.0f, 0.0f, 0.0f};
+    g[0] = g[1] = g[2] = 0.0f;
     float norm;
     const int32_t wi = a.wall[i];
     if (!wi) {
@@ -430,6 +273,300 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
         }
     }
 
+    return norm;
+}
+
+// Capillary force of a VD/GV site (ColorFluid::calcCapillaryForce,
+// colorfluid.cpp:591-617), in FP64 from the stored gradient and the
+// curvature field of the fluid neighbours (the FP32 indicator marks solids).
+template <int Q>
+__device__ __forceinline__ bool capillary_force_f32(const ColorArgs& a, uint32_t i, int x, int y, int z, float* F)
+{
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    const uint64_t N = a.g.N;
+    const bool gc = a.curv_gradient;
+    double grad[3] = {0.0, 0.0, 0.0}, field[3] = {0.0, 0.0, 0.0};
+    double curvature = 0.0, norm2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        grad[d] = a.capgrad[uint64_t(d) * N + i];
+        field[d] = a.capfield[uint64_t(d) * N + i];
+        norm2 += grad[d] * grad[d];
+    }
+    const bool curved = norm2 > CSF_GRAD_CUT32 * CSF_GRAD_CUT32;
+    if (curved) {
+        const int32_t w = a.wall[i];
+        uint32_t nb[Q];
+        bool fluid[Q];
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            nb[q] = push_target<Q>(a.g, x, y, z, q);
+            fluid[q] = a.densN32[nb[q]] != SOLID_N32;
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double moment[3] = {0.0, 0.0, 0.0}, derivative[3];
+            const double center = field[c];
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                if (!fluid[q]) continue;
+                const double difference = a.capfield[uint64_t(c) * N + nb[q]] - center;
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    if (L::e[q][b] == 1) moment[b] += L::w[q] * difference;
+                    else if (L::e[q][b] == -1) moment[b] -= L::w[q] * difference;
+                }
+            }
+            if (w) {
+                const double* inv = a.walls[w].ginv;
+#pragma unroll
+                for (int r = 0; r < D; ++r) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int b = 0; b < D; ++b) t = t + moment[b] * inv[r * D + b];
+                    derivative[r] = t;
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < D; ++b) derivative[b] = moment[b] * 3.0;
+            }
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                const double na = gc ? grad[c] / sqrt(norm2) : center;
+                const double nbv = gc ? grad[b] / sqrt(norm2) : field[b];
+                curvature += (na * nbv - (c == b ? 1.0 : 0.0)) * derivative[b];
+            }
+        }
+    }
+    bool finite = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const double direction = gc && curved ? grad[d] / sqrt(norm2) : grad[d];
+        F[d] = float(.5 * a.sigma * curvature * direction);
+        finite = finite && isfinite(F[d]);
+    }
+    return finite;
+}
+
+}  // namespace
+
+template <int Q, bool INIT, int LAYOUT, typename O, bool OWN = false>
+__global__ void __launch_bounds__(256) color_density_f32(ColorArgs a)
+{
+    uint32_t site, slot;
+    if (!resolve32<LAYOUT>(a, site, slot)) return;
+    const uint32_t desc = a.desc[slot];
+    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
+    if (!(lab.kind & KIND_FLUID)) return;
+    float h[Q], f0[Q], r0, dr, dzh, uzh[Lattice<Q>::D];
+    load_color32<Q, INIT, LAYOUT, O, OWN>(a, site, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
+    const float rho = float(a.rho_ref) + dr;
+    a.densN32[site] = (2.0f * r0 - rho) / rho;   // (rho0 - rho1) / rho
+}
+
+template <int Q, bool MRT, bool INIT, int LAYOUT, typename O, bool CSF = false, bool OWN = false>
+__global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
+{
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    constexpr int H = L::NHALF;
+    constexpr MomentTables<Q> MT = kMT<Q>;
+    uint32_t i, slot;
+    if (!resolve32<LAYOUT>(a, i, slot)) return;
+    const uint64_t N = a.g.N;
+    const uint32_t desc = a.desc[slot];
+    const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
+    const int kind = lab.kind;
+    if (!(kind & KIND_FLUID)) return;
+
+    float h[Q], f0[Q], r0, dr, dzh = 0.0f, uzh[D];
+    load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
+    int cx, cy, cz;
+    site_coords(a.g, i, cx, cy, cz);
+
+    // calcMacroVar (colorfluid.cpp:238-290)
+    const float rt = float(a.rho_ref) + dr;
+    const float r1 = rt - r0;
+    if (!isfinite(r0) || !isfinite(rt) || rt <= 0.0f) raise_status(a.status, 0, a.t);
+    else if (r0 < -NEG_TOL32 * rt || r1 < -NEG_TOL32 * rt) raise_status(a.status, 1, a.t);
+    const float inv_rt = 1.0f / rt;
+    const float dn = (r0 - r1) * inv_rt;
+    float tau = (1.0f + dn) * float(a.relax0) + (1.0f - dn) * float(a.relax1);
+    if (a.harmonic) {
+        const float phi = r0 * inv_rt;
+        tau = 0.5f + 3.0f / (phi / float(a.nu0) + (1.0f - phi) / float(a.nu1));
+    }
+    if (!isfinite(tau) || tau <= 0.5f) raise_status(a.status, 2, a.t);
+    // bodyForce (colorfluid.cpp:736-750): under CSF first (F_cap/2)/rho on
+    // VD/GV sites, then (F/2)/rho on GV sites
+    float fcap[3] = {0.0f, 0.0f, 0.0f};
+    if constexpr (CSF) {
+        if (!capillary_force_f32<Q>(a, i, cx, cy, cz, fcap)) raise_status(a.status, 5, a.t);
+    }
+    float u[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            if (L::e[q][d] == 1) s += h[q];
+            else if (L::e[q][d] == -1) s -= h[q];
+        }
+        u[d] = s * inv_rt;
+        if constexpr (CSF) u[d] += 0.5f * fcap[d] * inv_rt;
+        if (kind & KIND_GV) u[d] += float(lab.half[d]) * inv_rt;
+        if (!isfinite(u[d])) raise_status(a.status, 3, a.t);
+    }
+    if (a.flags & 1u) {
+        a.rho0[i] = double(r0);
+        a.rho1[i] = double(r1);
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = double(u[d]);
+    }
+    // adjacent report history; PB/VB sites as in kernels_color.hip
+    if (a.flags & 2u) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = double(u[d]);
+        a.chiadj[i] = (kind & KIND_ZOUHE) ? double(r0 - r1) : double(dn);
+    }
+    if (!INIT && (a.flags & 1u) && (kind & KIND_ZOUHE)) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.uadj_prev[uint64_t(d) * N + i] = double(uzh[d]);
+        const double rzh = a.rho_ref + double(dzh);
+        const double num = a.chiadj_prev[i];
+        a.chiadj_prev[i] = rzh > 0.0 ? num / rzh : 0.0;
+    }
+
+    // collision of the total population (colorfluid.cpp:292-386), on h;
+    // under CSF the force is the label's plus the capillary force
+    bool forced = kind & KIND_FORCED;
+    float F[3] = {0.0f, 0.0f, 0.0f};
+    if (forced)
+#pragma unroll
+        for (int d = 0; d < D; ++d) F[d] = float(lab.force[d]);
+    if constexpr (CSF) {
+        float f2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            F[d] += fcap[d];
+            f2 += F[d] * F[d];
+        }
+        forced = f2 != 0.0f;
+    }
+    float usq = 0.0f, uF = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        usq += u[d] * u[d];
+        uF += u[d] * F[d];
+    }
+    const float a0 = r0 - float(a.rho_ref);   // rest-state weights of the shifted equilibrium
+    if constexpr (!MRT) {
+        const float inv_tau = 1.0f / tau, cg = 1.0f - 0.5f * inv_tau;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            float eu = 0.0f, ef = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                if (L::e[q][d] == 1) { eu += u[d]; ef += F[d]; }
+                else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
+            }
+            const float w = float(L::w[q]);
+            const float heq = a0 * float(a.ai0[q]) + r1 * float(a.ai1[q]) +
+                              rt * w * (eu * (3.0f + 4.5f * eu) - 1.5f * usq);
+            h[q] += (heq - h[q]) * inv_tau;
+            if (forced) h[q] += cg * w * (3.0f * (ef - uF) + 9.0f * eu * ef);
+        }
+    } else {
+        const float su = 1.0f / tau;
+        const float sq = 8.0f * (2.0f - su) / (8.0f - su);
+        float sp[H], sm[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int q = L::half[j];
+            sp[j] = h[q] + h[L::op[q]];
+            sm[j] = h[q] - h[L::op[q]];
+        }
+        float y[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            float mg = 0.0f;
+            if (forced) {
+#pragma unroll
+                for (int p = 0; p < D; ++p) {
+                    if (MT.T1[k][p] != 0.0) mg += float(3.0 * MT.T1[k][p]) * F[p];
+#pragma unroll
+                    for (int b = 0; b < D; ++b)
+                        if (MT.T2[k][p][b] != 0.0) mg += float(9.0 * MT.T2[k][p][b]) * u[p] * F[b];
+                }
+                if (MT.T0[k] != 0.0) mg -= float(3.0 * MT.T0[k]) * uF;
+            }
+            if (MT.rate[k] == 0) {
+                y[k] = mg * float(1.0 / MT.n2[k]);
+                continue;
+            }
+            float m = 0.0f;
+            if (MT.parity[k] > 0) {
+                if (L::M[k][0] != 0) m += float(L::M[k][0]) * h[0];
+#pragma unroll
+                for (int j = 0; j < H; ++j)
+                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sp[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < H; ++j)
+                    if (L::M[k][L::half[j]] != 0) m += float(L::M[k][L::half[j]]) * sm[j];
+            }
+            float meq = 0.0f;
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                if (MT.T1[k][p] != 0.0) meq += float(3.0 * MT.T1[k][p]) * u[p];
+#pragma unroll
+                for (int b = 0; b < D; ++b)
+                    if (MT.T2[k][p][b] != 0.0) meq += float(4.5 * MT.T2[k][p][b]) * u[p] * u[b];
+            }
+            if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
+            meq = meq * rt + a0 * float(a.mai0[k]) + r1 * float(a.mai1[k]);
+            const float s = MT.rate[k] == 1 ? su : sq;
+            y[k] = (s * (meq - m) + (1.0f - 0.5f * s) * mg) * float(1.0 / MT.n2[k]);
+        }
+        {
+            float e0 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < Q; ++k)
+                if (MT.parity[k] > 0 && L::M[k][0] != 0) e0 += float(L::M[k][0]) * y[k];
+            h[0] += e0;
+        }
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int q = L::half[j];
+            float ev = 0.0f, od = 0.0f;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                if (L::M[k][q] == 0) continue;
+                if (MT.parity[k] > 0) ev += float(L::M[k][q]) * y[k];
+                else od += float(L::M[k][q]) * y[k];
+            }
+            h[q] += ev + od;
+            h[L::op[q]] += ev - od;
+        }
+    }
+
+    // colour gradient: the stored capillary gradient under CSF (no
+    // perturbation, colorfluid.cpp:389-411), else colorGradient
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    float norm;
+    if constexpr (CSF) {
+        float n2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            g[d] = float(a.capgrad[uint64_t(d) * N + i]);
+            n2 += g[d] * g[d];
+        }
+        norm = sqrtf(n2);
+    } else {
+        norm = color_gradient_f32<Q>(a, i, cx, cy, cz, g);
+    }
+
     // perturbation + recolouring (colorfluid.cpp:389-428)
     const float dens0T = r0 * inv_rt;
     bool finite = true;
@@ -446,7 +583,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
                 else if (L::e[q][d] == -1) angle -= g[d];
             }
             angle *= inv_norm;
-            h[q] += akn * (float(L::w[q]) * angle * angle - float(a.bi[q]));
+            if constexpr (!CSF) h[q] += akn * (float(L::w[q]) * angle * angle - float(a.bi[q]));
             if (L::diagonal(q)) angle *= (1.0f / SQRT2_32);
             const float feq0 = r0 * float(a.ai0[q]) + r1 * float(a.ai1[q]);
             const float fo = dens0T * (h[q] + float(a.dist0[q])) + b01 * angle * feq0;
@@ -521,11 +658,96 @@ __global__ void __launch_bounds__(256) color_tanh_wall_f32(ColorArgs a)
     for (int d = 0; d < D; ++d) a.tanhg[uint64_t(w) * 3 + d] = g[d];
 }
 
+// CSF, first half of calcCapillaryForce (colorfluid.cpp:577-586): colour
+// gradient and curvature field of every VD/GV site, stored in FP64 per site.
+template <int Q, int LAYOUT>
+__global__ void __launch_bounds__(256) color_capillary_f32(ColorArgs a)
+{
+    constexpr int D = Lattice<Q>::D;
+    uint32_t site, slot;
+    if (!resolve32<LAYOUT>(a, site, slot)) return;
+    if (!(a.lab[a.desc[slot] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) return;
+    const uint64_t N = a.g.N;
+    int x, y, z;
+    site_coords(a.g, site, x, y, z);
+    float g[3];
+    const float norm = color_gradient_f32<Q>(a, site, x, y, z, g);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        a.capgrad[uint64_t(d) * N + site] = double(g[d]);
+        a.capfield[uint64_t(d) * N + site] =
+            a.curv_gradient ? double(g[d]) : (norm > float(CSF_GRAD_CUT32) ? double(g[d] / norm) : 0.0);
+    }
+}
+
+// CSF initialisation (ColorFluid::initializeCapillaryForce,
+// colorfluid.cpp:619-634): the t = 0 populations of initialize() with
+// f_k -= 1.5 w (e . F) rho_k / rho, written to the site's own slots of fout32.
+template <int Q, int LAYOUT, typename O>
+__global__ void __launch_bounds__(256) color_csf_init_f32(ColorArgs a)
+{
+    using L = Lattice<Q>;
+    constexpr int D = L::D;
+    uint32_t site, slot;
+    if (!resolve32<LAYOUT>(a, site, slot)) return;
+    const DevLabel& lab = a.lab[a.desc[slot] >> DESC_LABEL_SHIFT];
+    if (!(lab.kind & KIND_FLUID)) return;
+    const int ph = lab.phase;
+    double f0[Q], f1[Q], r0 = 0.0, r1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const double f = (ph == 0 ? a.dist0[q] : a.dist1[q]) - 1.5 * L::w[q] * lab.ef[q];
+        f0[q] = ph == 0 ? f : 0.0;
+        f1[q] = ph == 0 ? 0.0 : f;
+        r0 += f0[q];
+        r1 += f1[q];
+    }
+    const double rt = r0 + r1;
+    int x, y, z;
+    site_coords(a.g, site, x, y, z);
+    float F[3] = {0.0f, 0.0f, 0.0f};
+    if (!capillary_force_f32<Q>(a, site, x, y, z, F)) raise_status(a.status, 5, a.t);
+    const O P = a.P;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double ef = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (L::e[q][d] == 1) ef += double(F[d]);
+            else if (L::e[q][d] == -1) ef -= double(F[d]);
+        }
+        const double g0 = f0[q] - 1.5 * L::w[q] * ef * r0 / rt;
+        const double g1 = f1[q] - 1.5 * L::w[q] * ef * r1 / rt;
+        st(a.fout32, O(q) * P + O(slot), float(g0 + g1 - a.dist0[q]));
+        st(a.fout32, (O(Q) + q) * P + O(slot), float(g0));
+    }
+}
+
 template <int Q, bool MRT, bool INIT, int LAYOUT, typename O>
 static hipError_t launch_color32_l(const ColorArgs& a, hipStream_t s)
 {
     const dim3 grid(uint32_t((a.P + 255) / 256)), block(256);
     if (a.P == 0) return hipSuccess;
+    if (a.csf) {
+        // CSF (colorfluid.cpp:124-137): density, [tanh walls,] gradient and
+        // curvature field, step; t = 0: the capillary shift of the initial
+        // populations, then density, field and collision in place on them
+        hipLaunchKernelGGL((color_density_f32<Q, INIT, LAYOUT, O>), grid, block, 0, s, a);
+        if (a.tanh_wall) hipLaunchKernelGGL((color_tanh_wall_f32<Q, LAYOUT>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((color_capillary_f32<Q, LAYOUT>), grid, block, 0, s, a);
+        if (!INIT) {
+            hipLaunchKernelGGL((color_step_f32<Q, MRT, false, LAYOUT, O, true>), grid, block, 0, s, a);
+            return hipGetLastError();
+        }
+        hipLaunchKernelGGL((color_csf_init_f32<Q, LAYOUT, O>), grid, block, 0, s, a);
+        ColorArgs b = a;
+        b.fin32 = a.fout32;
+        hipLaunchKernelGGL((color_density_f32<Q, true, LAYOUT, O, true>), grid, block, 0, s, b);
+        if (b.tanh_wall) hipLaunchKernelGGL((color_tanh_wall_f32<Q, LAYOUT>), grid, block, 0, s, b);
+        hipLaunchKernelGGL((color_capillary_f32<Q, LAYOUT>), grid, block, 0, s, b);
+        hipLaunchKernelGGL((color_step_f32<Q, MRT, true, LAYOUT, O, true, true>), grid, block, 0, s, b);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((color_density_f32<Q, INIT, LAYOUT, O>), grid, block, 0, s, a);
     if (a.tanh_wall) hipLaunchKernelGGL((color_tanh_wall_f32<Q, LAYOUT>), grid, block, 0, s, a);
     hipLaunchKernelGGL((color_step_f32<Q, MRT, INIT, LAYOUT, O>), grid, block, 0, s, a);

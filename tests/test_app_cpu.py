@@ -43,7 +43,6 @@ def base():
     ({'reportDir': 3, 'reportSlice': '1 6'}, 'Report slice is outside the selected axis.'),
     ({'fluidModel': '"shanchen"'}, 'Only scsp and color fluid models are implemented.'),
     ({'fluidModel': '"lattice-gas"'}, 'Main(): Unsupport fluid model'),
-    ({'capillaryModel': '"csf"', 'enginePrecision': '"fp32"'}, 'capillaryModel csf runs in FP64 in this engine.'),
 ])
 def test_invalid_inputs_fail_with_reference_message(tmp_path, overrides, message):
     spec = dict(base(), overrides=overrides)
