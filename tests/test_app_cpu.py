@@ -117,6 +117,30 @@ def test_output_ranks_validation(tmp_path, ranks, message):
     assert code != 0 and message in out, out[-300:]
 
 
+CSF_GUARD = H.spec('guard', nx=4, ny=4, nz=1, dimensions=2, steps=2, write=1, capillary_model='csf')
+
+
+@pytest.mark.parametrize('overrides,indices,message', [
+    ({'capillaryModel': '"missing"'}, None, 'capillaryModel must be perturbation or csf.'),
+    ({'wallGradient': '"missing"'}, None, 'wallGradient must be legacy, renormalized, tanh, tangent, halfway or ghost.'),
+    ({'density': '1 2'}, None, 'Experimental CSF requires equal reference densities and standard pressure weights.'),
+    ({'latticeSpeed': 2}, None, 'This research implementation requires latticeSpeed=1.'),
+    ({'alpha0': .5}, None, 'Experimental CSF requires equal reference densities and standard pressure weights.'),
+    ({}, ['0 "PB" "1" "1" "0"'], 'Pressure CSF requires flat D2Q9 ghost walls and phaseBoundary flux.'),
+    ({}, ['0 "VB" "0" "1" "0"'], 'CSF requires closed flow or guarded flat ghost/flux pressure reservoirs.'),
+    ({}, ['0 "HF" "N/A" "N/A" "0"'], 'CSF requires closed flow or guarded flat ghost/flux pressure reservoirs.'),
+], ids=['invalid-capillary', 'invalid-wall', 'unsupported-density', 'unsupported-speed',
+        'unsupported-pressure-weights', 'pressure-boundary', 'velocity-boundary', 'selective-film'])
+def test_csf_failure_contracts(tmp_path, overrides, indices, message):
+    """The reference's CSF failure cases (scripts/capillary_checks.py:66-82):
+    each run must fail with the reference's message."""
+    spec = dict(CSF_GUARD, overrides=overrides)
+    if indices:
+        spec['indices'] = indices
+    code, out = run_case(tmp_path, spec)
+    assert code != 0 and '0 Failed to excute: ' + message in out, out[-300:]
+
+
 def test_film_needs_the_colour_model(tmp_path):
     """HF labels run in the colour model only: the reference's single-phase
     model never initialises or collides HF sites (scsp.cpp:116-175)."""
