@@ -235,7 +235,7 @@ def main():
     ap.add_argument('--batch-max-steps', type=int, default=60000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported '
+        print(f'bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported '
               '(sharding across GPUs is not built)', file=sys.stderr)
         sys.exit(2)
 

@@ -5,8 +5,8 @@ FP32 accumulation) maps a crop's fluid mask to its steady velocity and
 density.  Its prediction initialises the populations at equilibrium
 (lbf_batch_set_fields), and the plain lattice Boltzmann iteration then runs
 to the unchanged stop rule (Simulator::updateConvergence,
-solver/src/lbm/simulator.cpp:248-325), which stays the judge: a poor
-prediction costs iterations, never correctness.
+solver/src/lbm/simulator.cpp:248-325), which alone decides convergence: a
+poor prediction costs iterations, never correctness.
 
 Pipeline: generate sphere-pack crops -> solve them to steady state with the
 batched engine -> train the network on (mask, fields) -> evaluate cold vs
