@@ -150,6 +150,23 @@ def test_film_needs_the_colour_model(tmp_path):
     assert code != 0 and 'half-permeable film (HF)' in out
 
 
+def test_create_rejects_lattices_above_the_index_range():
+    """More than 2^32 - 1 sites on one device is refused before any device
+    call (site indices are 32-bit)."""
+    import ctypes as C
+    from lbflow_amd.engine import Desc, Label, load_library
+    L = load_library()
+    d = Desc()
+    d.D, d.Q, d.model, d.equil, d.precision = 3, 19, 0, 1, 1
+    d.sz[0], d.sz[1], d.sz[2] = 1024, 2048, 2048
+    d.tau[0], d.density[0] = 0.8, 1.0
+    labels = (Label * 256)()
+    h = C.c_void_p()
+    one_byte = C.create_string_buffer(1)   # never read: the size check comes first
+    assert L.lbf_create(C.byref(d), labels, one_byte, C.byref(h)) != 0
+    assert 'more than 2^32-1 sites' in L.lbf_last_error(None).decode()
+
+
 def test_library_exports_every_declared_symbol():
     """Every function declared in include/lbflow.h is exported by liblbflow.so."""
     import re
