@@ -177,6 +177,30 @@ def ttp_warm_vs_cold(labels, interval, max_steps):
     return out
 
 
+def slab_bench(labels, ways, steps, warmup):
+    """The FP32 sparse step on one GPU's share of an R-way z-slab sharding of
+    the lattice: the first nz / R planes (periodic in z here; in a sharded run
+    the two boundary planes pull from the neighbouring slabs instead).  Per
+    step each slab would exchange the 5 populations crossing each z-face."""
+    from lbflow_amd import harness as H
+    from lbflow_amd.engine import FP32
+    out = {}
+    nz, ny, nx = labels.shape
+    for r in ways:
+        sub = np.ascontiguousarray(labels[:nz // r])
+        spec = H.spec(f'slab-{r}', nx=nx, ny=ny, nz=nz // r, model='scsp', eq='MRT', tau=(0.8,),
+                      geometry=sub.ravel(), indices=['75 "GV" "1e-06" "1" "0"', '255 "SD" "90" "N/A" "N/A"'])
+        m = measure(spec, sub, FP32, False, steps, warmup)
+        fluid_face = int((sub[0] == 75).sum())
+        out[f'{r}-way'] = dict(shape=f'{nx}x{ny}x{nz // r}', kernel=m['kernel'],
+                               ms_per_step=round(m['ms_per_step'], 5),
+                               site_mlups_per_gpu=round(sub.size / m['ms_per_step'] / 1e3, 1),
+                               porosity=round(float((sub == 75).mean()), 4),
+                               exchange_bytes_per_face=5 * 4 * fluid_face,
+                               achieved_gbs=round(m['bytes_per_launch'] / m['ms_per_step'] / 1e6, 1))
+    return out
+
+
 def batch_crops(ncrops):
     """The batched workload's crops (before the GPU is touched: a process pool forks)."""
     from lbflow_amd import geometry as G
@@ -231,6 +255,8 @@ def main():
                     help='also run config 2 to convergence from rest and from the warm-start network')
     ap.add_argument('--colour', action='store_true', help='also time the two-phase colour model (128x128x256)')
     ap.add_argument('--batch', type=int, default=0, help='also run N batched crops to convergence (512)')
+    ap.add_argument('--slabs', default='', help='also time one GPU\'s z-slab of the lattice for R-way '
+                    'sharding, e.g. 2,4,8 (the step at the shape each GPU would hold)')
     ap.add_argument('--batch-interval', type=int, default=200)
     ap.add_argument('--batch-max-steps', type=int, default=60000)
     args = ap.parse_args()
@@ -273,6 +299,8 @@ def main():
         extra['colour'] = colour_bench(max(10, args.steps // 2), 5, args.fp64)
     if args.batch:
         extra['batch'] = batch_bench(crops, args.batch_interval, args.batch_max_steps)
+    if args.slabs:
+        extra['slabs'] = slab_bench(labels, [int(r) for r in args.slabs.split(',')], args.steps, args.warmup)
     for k, v in extra.items():
         log(k, v)
 
