@@ -31,10 +31,14 @@ def sphere_pack(shape_zyx, porosity, rmin, rmax, seed, pore=75, solid=255):
         dy = (iy + 0.5 - c[1])[None, :, None]
         dx = (ix + 0.5 - c[2])[None, None, :]
         inside = dz * dz + dy * dy + dx * dx <= r * r
-        sub = np.ix_(iz % nz, iy % ny, ix % nx)
-        block = mask[sub]
-        nsolid += int(np.count_nonzero(inside & ~block))
-        mask[sub] = block | inside
+        # periodic images: where the sphere is wider than an axis (a single
+        # z-plane in 2D) several of its indices wrap onto one site, so the
+        # covered sites are collected as a set before marking them
+        flat = ((iz % nz)[:, None, None] * ny + (iy % ny)[None, :, None]) * nx + (ix % nx)[None, None, :]
+        covered = np.unique(flat[inside])
+        flat_mask = mask.reshape(-1)
+        nsolid += int(np.count_nonzero(~flat_mask[covered]))
+        flat_mask[covered] = True
     labels = np.full((nz, ny, nx), pore, dtype=np.uint8)
     labels[mask] = solid
     return labels

@@ -108,3 +108,36 @@ def test_fp32_colour_sparse_matches_dense(kw):
     assert np.array_equal(a.populations(), b.populations())
     for x, y in zip(a.fields(), b.fields()):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize('kw', [dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, porosity=0.8),
+                                dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, dims=2, angle=45,
+                                     porosity=0.6, seed=4)],
+                         ids=['mrt-3d', 'mrt-2d-45'])
+def test_fp32_renormalized_walls_track_fp64(kw):
+    """wallGradient renormalized in FP32: within the FP32 colour tolerances of
+    the FP64 engine (which equals the oracle bit for bit, test_gpu_zouhe)."""
+    spec = dict(sphere_case(**kw), wall_gradient='renormalized')
+    a = run(spec, 200, FP64)
+    b = run(spec, 200, FP32)
+    ra0, _, ua = a.fields()
+    rb0, _, ub = b.fields()
+    d0 = np.max(np.abs(ra0 - rb0))
+    du = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
+    say('renormalized', kw, 'max |rho0 diff|', d0, 'rel L2 u', du)
+    assert d0 < 5e-4
+    assert du < 1e-2
+
+
+@pytest.mark.parametrize('spec', [s for s in H.layered_cases('harmonic') if s['height'] == 16],
+                         ids=lambda s: s['name'])
+def test_fp32_harmonic_viscosity_tracks_fp64(spec):
+    """viscosityInterpolation harmonic (colorfluid.cpp:263-268) in FP32: the
+    layered channel's velocity profile after 3000 steps within 1e-3 relative
+    of the FP64 engine's (which reproduces the reference's recorded profile)."""
+    a = run(spec, 3000, FP64)
+    b = run(spec, 3000, FP32)
+    ua, ub = a.fields()[2], b.fields()[2]
+    du = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
+    say(spec['name'], 'rel L2 u', du)
+    assert du < 1e-3

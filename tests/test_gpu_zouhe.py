@@ -252,7 +252,8 @@ def test_zouhe_color_adjacent_report(precision, name):
 @pytest.mark.parametrize('kw', [dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, porosity=0.8),
                                 dict(model='color', eq='SRT', tau=(1.0, 0.8), two_phase=True, angle=120,
                                      porosity=0.8, seed=4),
-                                dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, dims=2, angle=45)],
+                                dict(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, dims=2, angle=45,
+                                     porosity=0.6, seed=4)],
                          ids=['mrt-3d', 'srt-3d-120', 'mrt-2d-45'])
 def test_renormalized_wall_bitwise(kw):
     """wallGradient renormalized (colorfluid.cpp:487-492, 917-957) on curved walls:
