@@ -3,8 +3,8 @@
 Every expected value below was produced by the reference solver and is
 stored in the reference tree (research/drlab-publication/paper/data/*.json,
 extracted into tests/golden/reference_records.json by make_golden.py), except
-the 8x10x4 checkpoint digest, which was recorded with the reference binary in
-the survey of the reference.  Comparisons are exact (==): the oracle keeps
+the 8x10x4 checkpoint digest: the SHA-256 of the checkpoint that the
+reference, built from its own sources, wrote after 20 000 steps.  Comparisons are exact (==): the oracle keeps
 the reference's floating-point operation order.
 """
 import hashlib
