@@ -129,7 +129,7 @@ def colour_bench(steps, warmup, fp64=False):
 
 
 def time_to_permeability(spec, labels, interval, max_steps):
-    """Config-2 permeability in FP64 and FP32: wall clock from init to the
+    """Permeability of the 256^3 pack in FP64 and FP32: wall clock from init to the
     reference's stop rule (residual 1e-6, three stable reports) and the
     Darcy K of each, with |K32 - K64| / K64."""
     from lbflow_amd import harness as H
@@ -153,7 +153,7 @@ def time_to_permeability(spec, labels, interval, max_steps):
 
 
 def ttp_warm_vs_cold(labels, interval, max_steps):
-    """Config-2 time to permeability from rest and from the warm-start
+    """Time to permeability of the 256^3 pack from rest and from the warm-start
     network (trained on 64^3 crops; fully convolutional and periodic, so it
     runs on the whole 256^3 lattice): the batched FP32 engine with one crop,
     same stop rule (residual 1e-6, reports every `interval` steps), wall
@@ -210,17 +210,23 @@ def batch_crops(ncrops):
     return crops
 
 
-def batch_bench(crops, interval, max_steps):
+def batch_bench(crops, interval, max_steps, tune_only=False):
     """Batched permeability: independent cubic sphere-pack crops (side 32-96,
     porosity 0.15-0.35, seeds 1000+i), D3Q19 MRT FP32, tau 0.8, F 1e-6 along x,
     every crop run to residual 1e-5 in one persistent hipGraph-captured launch
-    sequence."""
+    sequence.  With tune_only, only the autotuner's eager steps of all crops
+    run (no captured graph, no convergence run)."""
     from lbflow_amd.engine import Batch
     ncrops = len(crops)
     labels = {75: ('GV', 1e-6, 1, 0), 255: ('SD', 90.0, 0, 0)}
     b = Batch(3, 19, 'MRT', 0.8, 1.0, labels, crops)
     per_cu, tune_ms = b.tune(steps=10)
     log(f'batch: {b.sites} sites, {b.fluid} fluid, tuned {per_cu} workgroups/CU, {tune_ms:.3f} ms/step')
+    if tune_only:
+        out = dict(crops=ncrops, sites=b.sites, fluid_sites=b.fluid, tuned_workgroups_per_cu=per_cu,
+                   all_crops_ms_per_step=round(tune_ms, 4))
+        b.close()
+        return out
     t0 = time.perf_counter()
     b.init()
     res, sec = b.run(axis=2, tol=1e-5, interval=interval, max_steps=max_steps)
@@ -252,13 +258,15 @@ def main():
     ap.add_argument('--ttp-interval', type=int, default=500)
     ap.add_argument('--ttp-max-steps', type=int, default=100000)
     ap.add_argument('--ttp-warm', action='store_true',
-                    help='also run config 2 to convergence from rest and from the warm-start network')
+                    help='also run the 256^3 pack to convergence from rest and from the warm-start network')
     ap.add_argument('--colour', action='store_true', help='also time the two-phase colour model (128x128x256)')
     ap.add_argument('--batch', type=int, default=0, help='also run N batched crops to convergence (512)')
     ap.add_argument('--slabs', default='', help='also time one GPU\'s z-slab of the lattice for R-way '
                     'sharding, e.g. 2,4,8 (the step at the shape each GPU would hold)')
     ap.add_argument('--batch-interval', type=int, default=200)
     ap.add_argument('--batch-max-steps', type=int, default=60000)
+    ap.add_argument('--batch-tune-only', action='store_true',
+                    help='with --batch: only tune and time one eager step of all crops')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported '
@@ -298,7 +306,7 @@ def main():
     if args.colour:
         extra['colour'] = colour_bench(max(10, args.steps // 2), 5, args.fp64)
     if args.batch:
-        extra['batch'] = batch_bench(crops, args.batch_interval, args.batch_max_steps)
+        extra['batch'] = batch_bench(crops, args.batch_interval, args.batch_max_steps, args.batch_tune_only)
     if args.slabs:
         extra['slabs'] = slab_bench(labels, [int(r) for r in args.slabs.split(',')], args.steps, args.warmup)
     for k, v in extra.items():

@@ -252,7 +252,7 @@ void lbf_batch_destroy(lbf_batch* batch);
 const char* lbf_batch_error(const lbf_batch* batch);  /* batch may be NULL: last create error */
 /* Total voxels of all crops; *fluid (if not NULL) = fluid sites. */
 uint64_t lbf_batch_sites(const lbf_batch* batch, uint64_t* fluid);
-/* Times `steps` steps for each persistent grid size (1..32 workgroups per
+/* Times `steps` steps for each persistent grid size (1..128 workgroups per
  * CU) and keeps the fastest.  Steps the current state: call before
  * lbf_batch_init. */
 int lbf_batch_tune(lbf_batch* batch, uint32_t steps, uint32_t* per_cu, double* ms_per_step);

@@ -477,7 +477,7 @@ int lbf_batch_advance(lbf_batch* b, uint64_t n)
 int lbf_batch_tune(lbf_batch* b, uint32_t steps, uint32_t* best_per_cu, double* best_ms)
 {
     if (!b) return -1;
-    static const uint32_t candidates[] = {1, 2, 4, 8, 16, 32};
+    static const uint32_t candidates[] = {1, 2, 4, 8, 16, 32, 64, 128};
     double best = 1e300;
     uint32_t pick = 8;
     hipEvent_t e0, e1;
