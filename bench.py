@@ -38,12 +38,16 @@ def log(*a):
 
 def workload(size):
     from lbflow_amd import geometry as G
-    from lbflow_amd import harness as H
     labels = G.sphere_pack((size, size, size), 0.20, 4.0, 10.0, seed=2)
-    spec = H.spec(f'c2-{size}', nx=size, ny=size, nz=size, model='scsp', eq='MRT', tau=(0.8,),
-                  geometry=labels.ravel(),
-                  indices=['75 "GV" "1e-06" "1" "0"', '255 "SD" "90" "N/A" "N/A"'])
-    return spec, labels
+    return pack_case(f'c2-{size}', labels), labels
+
+
+def pack_case(name, labels):
+    """Single-phase D3Q19 MRT, tau 0.8, F 1e-6 along x on pore label 75,
+    halfway bounce-back on label 255."""
+    from lbflow_amd import cases as CS
+    return CS.make(name, labels, [CS.label_line(75, 'GV', 1e-6, 1, 0), CS.label_line(255, 'SD', 90)],
+                   model='scsp', eq='MRT', tau=(0.8,))
 
 
 PMC_PROFILE = ROOT / 'profiles' / 'scsp_step_f32_sparse_256cubed_pmc.md'
@@ -62,9 +66,9 @@ def measured_traffic(kernel, size):
 
 
 def measure(spec, labels, precision, stream_solids, steps, warmup, dense=False):
-    from lbflow_amd import harness as H
+    from lbflow_amd import cases as CS
     from lbflow_amd.engine import Engine
-    case = H.lattice_case(spec)
+    case = CS.lattice(spec)
     eng = Engine(case, labels.ravel(), precision, stream_solids, dense).init()
     eng.advance(warmup)
     eng.synchronize()
@@ -82,9 +86,9 @@ def measure(spec, labels, precision, stream_solids, steps, warmup, dense=False):
 def cpu_baseline(spec, labels, budget_s):
     """The CPU oracle (single-threaded C restatement of the reference) timed on
     the same 256^3 lattice for as many whole steps as fit the budget."""
-    from lbflow_amd import harness as H
+    from lbflow_amd import cases as CS
     from oracle.pyoracle import Oracle
-    case = H.lattice_case(spec)
+    case = CS.lattice(spec)
     o = Oracle(case, labels.ravel()).init()
     t0 = time.perf_counter()
     o.step(1)
@@ -105,13 +109,13 @@ def colour_bench(steps, warmup, fp64=False):
     of the two phases (50 %), tau 1/1, densities 1/1, alpha0 1/3, sigma 1e-3,
     beta 0.7, contact angle 60 deg, force 1e-6 along z (direction 5)."""
     from lbflow_amd import geometry as G
-    from lbflow_amd import harness as H
+    from lbflow_amd import cases as CS
     from lbflow_amd.engine import FP32, FP64
     labels = G.sphere_pack((256, 128, 128), 0.30, 6.0, 12.0, seed=3)
     labels = G.random_blocks(labels, 8, 0.5, seed=103, phase0=75, phase1=76, pore=75)
-    spec = H.spec('c3', nx=128, ny=128, nz=256, model='color', eq='MRT', tau=(1.0, 1.0),
-                  geometry=labels.ravel(), sigma=1e-3, beta=0.7, alpha=1 / 3,
-                  indices=['75 "GV" "1e-06" "5" "0"', '76 "GV" "1e-06" "5" "1"', '255 "SD" "60" "N/A" "N/A"'])
+    spec = CS.make('c3', labels, [CS.label_line(75, 'GV', 1e-6, 5, 0), CS.label_line(76, 'GV', 1e-6, 5, 1),
+                                  CS.label_line(255, 'SD', 60)],
+                   model='color', eq='MRT', tau=(1.0, 1.0), sigma=1e-3, beta=0.7, alpha=1 / 3)
     out = {'workload': 'colour-gradient D3Q19 MRT, 128x128x256 sphere pack (r 6-12, porosity '
                        f'{float(np.isin(labels, (75, 76)).mean()):.4f}, seed 3), 8^3 phase blocks 50 %, '
                        'tau 1/1, sigma 1e-3, beta 0.7, 60 deg, F 1e-6 z'}
@@ -132,10 +136,10 @@ def time_to_permeability(spec, labels, interval, max_steps):
     """Permeability of the 256^3 pack in FP64 and FP32: wall clock from init to the
     reference's stop rule (residual 1e-6, three stable reports) and the
     Darcy K of each, with |K32 - K64| / K64."""
-    from lbflow_amd import harness as H
+    from lbflow_amd import cases as CS
     from lbflow_amd.engine import FP32, FP64, Engine
     from lbflow_amd.permeability import converge
-    case = H.lattice_case(spec)
+    case = CS.lattice(spec)
     n = labels.shape[2]
     out = {}
     for name, prec in (('fp64', FP64), ('fp32', FP32)):
@@ -182,14 +186,13 @@ def slab_bench(labels, ways, steps, warmup):
     the lattice: the first nz / R planes (periodic in z here; in a sharded run
     the two boundary planes pull from the neighbouring slabs instead).  Per
     step each slab would exchange the 5 populations crossing each z-face."""
-    from lbflow_amd import harness as H
+    from lbflow_amd import cases as CS
     from lbflow_amd.engine import FP32
     out = {}
     nz, ny, nx = labels.shape
     for r in ways:
         sub = np.ascontiguousarray(labels[:nz // r])
-        spec = H.spec(f'slab-{r}', nx=nx, ny=ny, nz=nz // r, model='scsp', eq='MRT', tau=(0.8,),
-                      geometry=sub.ravel(), indices=['75 "GV" "1e-06" "1" "0"', '255 "SD" "90" "N/A" "N/A"'])
+        spec = pack_case(f'slab-{r}', sub)
         m = measure(spec, sub, FP32, False, steps, warmup)
         fluid_face = int((sub[0] == 75).sum())
         out[f'{r}-way'] = dict(shape=f'{nx}x{ny}x{nz // r}', kernel=m['kernel'],

@@ -1,7 +1,8 @@
 """MI355X-native lattice Boltzmann engine with LBFlow's file and process contract.
 
 ``lbflow_amd.engine`` binds the C ABI of the HIP engine (include/lbflow.h),
-``lbflow_amd.harness`` writes LBFlow study cases and reads VTI output,
+``lbflow_amd.cases`` describes lattices and writes input.prm + geometry.raw,
+``lbflow_amd.vti`` reads VTI output,
 ``lbflow_amd.geometry`` generates the deterministic porous media of the
 benchmarks.
 """

@@ -162,7 +162,7 @@ def copy_bandwidth(nbytes=1 << 30, iters=20):
 class Engine:
     """One lattice on the GPU.
 
-    ``case`` is the plain lattice description of ``harness.lattice_case``:
+    ``case`` is the plain lattice description of ``cases.lattice``:
     D, Q, size (slowest first), model, eq, tau, density, alpha, sigma, beta,
     harmonic, labels {value: (TYPE, value, dir, phase)}.
     """

@@ -10,7 +10,7 @@ scalars per report:
   (or their RMS change <= 1e-12), together with the pressure-based K of
   SCSP::report (scsp.cpp:343-368) when a pressure drop exists, for 3
   consecutive reports;
-* the permeability is the harness's Darcy estimate k = nu <u_x> / F with the
+* the permeability is the reference study scripts' Darcy estimate k = nu <u_x> / F with the
   mean over all voxels, solids counted as zero (scripts/rock_workflow.py:63-69,
   scripts/porous_media.py:20-44).
 """

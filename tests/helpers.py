@@ -1,17 +1,15 @@
-"""Shared helpers of the test suite: study cases, oracle runs and VTI-like views."""
+"""Shared helpers of the test suite: oracle runs and VTI-like views of fields."""
 import numpy as np
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 
 C1_SHA256 = '03d845deefdaf322bbffd2db52e60aa5aba43938e20aa83d3461237df07ed664'
 
 
 def c1_spec():
-    """8x10x4 SRT channel: examples/generate.py:15-53 with equilModel SRT, tau 1, 20000 steps."""
-    return H.spec('c1', nx=8, ny=10, nz=4, model='scsp', eq='SRT', tau=(1.0,), steps=20000,
-                  write=250, restart=500, alpha=1 / 3,
-                  label=lambda x, y, z: 255 if y in (0, 9) else 75,
-                  indices=['75 "GV" "0.000001" "1" "0"', '255 "SD" "90" "N/A" "N/A"'])
+    """The 8x10x4 SRT channel, 20 000 steps."""
+    return TC.c1()
 
 
 def velocity_grid(u, case):
@@ -34,27 +32,11 @@ def darcy(u, case):
 
 def oracle_run(spec, steps, geometry=None):
     from oracle.pyoracle import Oracle
-    case = H.lattice_case(spec)
-    o = Oracle(case, H.geometry(spec) if geometry is None else geometry).init()
+    case = CS.lattice(spec)
+    o = Oracle(case, CS.geometry(spec) if geometry is None else geometry).init()
     if steps:
         o.step(steps)
     return o, case
 
 
-def sphere_case(n=(12, 14, 16), model='scsp', eq='MRT', tau=(0.8,), porosity=0.35, seed=3,
-                angle=60, two_phase=False, dims=3):
-    """Small periodic sphere pack with curved walls (3D) or discs (2D)."""
-    from lbflow_amd import geometry as G
-    if dims == 3:
-        labels = G.sphere_pack(n, porosity, 2.0, 4.0, seed)
-    else:
-        labels = G.sphere_pack((1,) + tuple(n[-2:]), porosity, 2.0, 4.0, seed)
-    if two_phase:
-        labels = G.random_blocks(labels, 3, 0.5, seed + 100)
-    nz, ny, nx = labels.shape
-    indices = ['75 "GV" "1e-05" "1" "0"', f'255 "SD" "{angle}" "N/A" "N/A"']
-    if two_phase:
-        indices.append('76 "GV" "1e-05" "1" "1"')
-    s = H.spec(f'pack-{model}-{eq}-{dims}d', nx=nx, ny=ny, nz=nz if dims == 3 else 1, dimensions=dims,
-               model=model, eq=eq, tau=tau, geometry=labels.ravel(), indices=indices, sigma=0.01)
-    return s
+sphere_case = TC.sphere_case

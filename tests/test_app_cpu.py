@@ -7,22 +7,24 @@ scripts/ci_smoke.py:58-63, scripts/benchmarks.py:257-260).
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from lbflow_amd import BIN_DIR
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 
 LBFLOW = BIN_DIR / 'LBFlow'
 
 
 def run_case(tmp_path, spec, start=0):
-    prm = H.write_case(spec, tmp_path, start)
+    prm = CS.write_input(spec, tmp_path, start)
     p = subprocess.run([str(LBFLOW), str(prm)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     return p.returncode, p.stdout + p.stderr
 
 
 def base():
-    return H.force_cases()[-1]
+    return TC.reference('force')[-1]
 
 
 @pytest.mark.parametrize('overrides,message', [
@@ -53,7 +55,10 @@ def test_invalid_inputs_fail_with_reference_message(tmp_path, overrides, message
 
 
 def test_undefined_label(tmp_path):
-    spec = dict(base(), label=lambda x, y, z: 75 if x else 9)
+    spec = base()
+    g = CS.grid(spec).copy()
+    g[..., 0] = 9
+    spec = dict(spec, geometry=g.ravel())
     code, out = run_case(tmp_path, spec)
     assert code != 0 and '0 Failed to excute: map::at' in out
 
@@ -70,7 +75,7 @@ def test_missing_argument():
 
 def test_short_geometry(tmp_path):
     spec = base()
-    prm = H.write_case(spec, tmp_path)
+    prm = CS.write_input(spec, tmp_path)
     raw = tmp_path / 'geometry.raw'
     raw.write_bytes(raw.read_bytes()[:-1])
     p = subprocess.run([str(LBFLOW), str(prm)], cwd=tmp_path, capture_output=True, text=True)
@@ -84,14 +89,14 @@ def test_short_geometry(tmp_path):
 def test_unimplemented_d2q9_options(tmp_path, overrides, message):
     """The reference's remaining experimental colour options run on D2Q9 only
     and are rejected here with a message."""
-    spec = [s for s in H.force_cases() if s['model'] == 'color' and s['dimensions'] == 2][-1]
+    spec = [s for s in TC.reference('force') if s['model'] == 'color' and s['dimensions'] == 2][-1]
     code, out = run_case(tmp_path, dict(spec, overrides=overrides))
     assert code != 0 and message in out, out[-300:]
 
 
 def test_missing_key(tmp_path):
     spec = base()
-    prm = H.write_case(spec, tmp_path)
+    prm = CS.write_input(spec, tmp_path)
     text = '\n'.join(l for l in prm.read_text().splitlines() if not l.startswith('endTime'))
     prm.write_text(text + '\n')
     p = subprocess.run([str(LBFLOW), str(prm)], cwd=tmp_path, capture_output=True, text=True)
@@ -117,7 +122,8 @@ def test_output_ranks_validation(tmp_path, ranks, message):
     assert code != 0 and message in out, out[-300:]
 
 
-CSF_GUARD = H.spec('guard', nx=4, ny=4, nz=1, dimensions=2, steps=2, write=1, capillary_model='csf')
+CSF_GUARD = CS.make('guard', np.zeros((1, 4, 4)), [CS.label_line(0, 'VD', phase=0)], model='color', steps=2, write=1,
+                    capillary_model='csf')
 
 
 @pytest.mark.parametrize('overrides,indices,message', [
@@ -144,7 +150,7 @@ def test_csf_failure_contracts(tmp_path, overrides, indices, message):
 def test_film_needs_the_colour_model(tmp_path):
     """HF labels run in the colour model only: the reference's single-phase
     model never initialises or collides HF sites (scsp.cpp:116-175)."""
-    scsp = [s for s in H.force_cases() if s['model'] == 'scsp'][-1]
+    scsp = [s for s in TC.reference('force') if s['model'] == 'scsp'][-1]
     spec = dict(scsp, indices=['75 "HF" "N/A" "N/A" "0"'])
     code, out = run_case(tmp_path, spec)
     assert code != 0 and 'half-permeable film (HF)' in out

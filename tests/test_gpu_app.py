@@ -18,7 +18,9 @@ import numpy as np
 import pytest
 
 from lbflow_amd import BIN_DIR
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from lbflow_amd.vti import read_vti
+from tests import cases as TC
 from tests.helpers import C1_SHA256, c1_spec, oracle_run, velocity_grid
 
 pytestmark = pytest.mark.gpu
@@ -30,7 +32,7 @@ def say(*a):
 
 
 def run(spec, folder, start=0, checkpoint=None, extra=()):
-    prm = H.write_case(spec, folder, start)
+    prm = CS.write_input(spec, folder, start)
     if checkpoint:
         for suffix in ('.rst', '.rst.meta'):
             Path(folder / f'field.{start:010d}{suffix}').write_bytes(Path(str(checkpoint) + suffix).read_bytes())
@@ -56,7 +58,7 @@ def test_c1_process_and_checkpoint_digest(tmp_path):
                     'INITIAL_COLLISION_INCLUDED 20001\nCONVERGENCE_HISTORY_RESET_ON_LOAD\n')
     assert len(sorted(tmp_path.glob('field.*.vti'))) == 20000 // 250 + 1
     # VTI fields equal the oracle's macro fields
-    (nx, ny, nz), f = H.read_vti(tmp_path / 'field.0000020000.vti')
+    (nx, ny, nz), f = read_vti(tmp_path / 'field.0000020000.vti')
     o, case = oracle_run(spec, 20000)
     r0, _, u = o.fields()
     assert np.array_equal(f['density'][..., 0].ravel(), r0)
@@ -64,12 +66,12 @@ def test_c1_process_and_checkpoint_digest(tmp_path):
     assert (tmp_path / 'field.0000020000.vti').stat().st_size == 10770
 
 
-@pytest.mark.parametrize('spec', [s for s in H.force_cases() if s['dimensions'] == 3], ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', [s for s in TC.reference('force') if s['dimensions'] == 3], ids=lambda s: s['name'])
 def test_force_cases_through_process(spec, tmp_path, records):
     code, out, timing = run(spec, tmp_path)
     assert code == 0, out[-2000:]
     assert timing['completed_steps'] == spec['steps']
-    _, f = H.read_vti(tmp_path / f'field.{spec["steps"]:010d}.vti')
+    _, f = read_vti(tmp_path / f'field.{spec["steps"]:010d}.vti')
     v = f['v' if spec['model'] == 'color' else 'velocity']
     rec = records['force'][spec['name'] + '-1rank']
     assert float(np.max(np.abs(v[..., 0] - 1e-4))) == rec['absolute_error']
@@ -77,9 +79,10 @@ def test_force_cases_through_process(spec, tmp_path, records):
 
 def drop_spec():
     """12^3 two-phase drop (scripts/ci_smoke.py:42-56)."""
-    return H.spec('drop-continuous', nx=12, ny=12, nz=12, steps=40, write=20, restart=20,
-                  label=lambda x, y, z: 100 if (x - 5.5) ** 2 + (y - 5.5) ** 2 + (z - 5.5) ** 2 <= 9 else 0,
-                  indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"'])
+    z, y, x = np.indices((12, 12, 12))
+    inside = (x - 5.5) ** 2 + (y - 5.5) ** 2 + (z - 5.5) ** 2 <= 9
+    return CS.make('drop-continuous', np.where(inside, 100, 0), [CS.label_line(0, 'VD', phase=0), CS.label_line(100, 'VD', phase=1)],
+                   model='color', steps=40, write=20, restart=20)
 
 
 def test_restart_is_bitwise_and_failure_contracts(tmp_path):
@@ -119,12 +122,12 @@ def test_film_case_through_process(name, tmp_path):
     """Half-permeable films through the drop-in: the VTI fields equal the
     oracle's bit for bit, and a restart from the step-200 checkpoint writes
     the same step-300 files."""
-    spec = dict(next(s for s in H.film_cases() if s['name'] == name), restart=100)
+    spec = dict(next(s for s in TC.film_cases() if s['name'] == name), restart=100)
     cont = tmp_path / 'continuous'
     code, out, timing = run(spec, cont)
     assert code == 0, out[-2000:]
     assert timing['completed_steps'] == spec['steps']
-    _, f = H.read_vti(cont / f'field.{spec["steps"]:010d}.vti')
+    _, f = read_vti(cont / f'field.{spec["steps"]:010d}.vti')
     o, case = oracle_run(spec, spec['steps'])
     r0, r1, u = o.fields()
     assert np.array_equal(f['r0'][..., 0].ravel(), r0)
@@ -146,13 +149,13 @@ def test_csf_through_process(name, tmp_path):
     files; the drop's fields equal the oracle's bit for bit, the tanh-wall
     case's within 1e-12 (atanh/tanh differ by an ulp between the host and
     device libraries)."""
-    spec = H.csf_drop() if name == 'drop' else dict(H.csf_wall(), restart=50)
+    spec = TC.csf_drop() if name == 'drop' else dict(TC.csf_wall(), restart=50)
     half = spec['steps'] // 2
     cont = tmp_path / 'continuous'
     code, out, timing = run(spec, cont)
     assert code == 0, out[-2000:]
     last = f'field.{spec["steps"]:010d}'
-    _, f = H.read_vti(cont / f'{last}.vti')
+    _, f = read_vti(cont / f'{last}.vti')
     o, case = oracle_run(spec, spec['steps'])
     r0, r1, u = o.fields()
     got = np.concatenate([f['r0'].ravel(), f['r1'].ravel(), f['v'].ravel()])
@@ -188,13 +191,13 @@ def test_output_ranks_vti_layout(tmp_path):
     base = tmp_path / 'r1'
     code, out, _ = run(spec, base)
     assert code == 0, out[-2000:]
-    _, ref = H.read_vti(base / 'field.0000000040.vti')
+    _, ref = read_vti(base / 'field.0000000040.vti')
     for R, cores in CORE_GRIDS_12.items():
         folder = tmp_path / f'r{R}'
         code, out, _ = run(spec, folder, extra=(f'--output-ranks={R}',))
         assert code == 0, out[-2000:]
         pieces = []
-        _, f = H.read_vti(folder / 'field.0000000040.vti', pieces)
+        _, f = read_vti(folder / 'field.0000000040.vti', pieces)
         assert f.keys() == ref.keys() and all(np.array_equal(f[k], ref[k]) for k in f)
         assert len(pieces) == R
         # rank r sits at core coordinates CalcI2P(r, cores) (grid.h:153-161)
@@ -215,16 +218,15 @@ def test_output_ranks_vti_layout(tmp_path):
         assert all(o == sorted(o) and len(o) == R for o in offsets)
         assert (folder / 'field.0000000040.rst').read_bytes() == (base / 'field.0000000040.rst').read_bytes()
     # 2D: an 8 x 10 plane (x, y) in 3 ranks splits y as 3, 3, 4
-    spec2 = H.spec('plane-ranks', nx=8, ny=10, nz=1, dimensions=2, steps=20, write=20,
-                   label=lambda x, y, z: 100 if y < 4 else 0,
-                   indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"'])
+    y = np.indices((1, 10, 8))[1]
+    spec2 = CS.make('plane-ranks', np.where(y < 4, 100, 0), [CS.label_line(0, 'VD', phase=0), CS.label_line(100, 'VD', phase=1)], model='color', steps=20, write=20)
     code, out, _ = run(spec2, tmp_path / 'p1')
     assert code == 0, out[-2000:]
     code, out, _ = run(spec2, tmp_path / 'p3', extra=('--output-ranks=3',))
     assert code == 0, out[-2000:]
     pieces = []
-    _, a = H.read_vti(tmp_path / 'p1' / 'field.0000000020.vti')
-    _, b = H.read_vti(tmp_path / 'p3' / 'field.0000000020.vti', pieces)
+    _, a = read_vti(tmp_path / 'p1' / 'field.0000000020.vti')
+    _, b = read_vti(tmp_path / 'p3' / 'field.0000000020.vti', pieces)
     assert all(np.array_equal(a[k], b[k]) for k in a)
     assert pieces == [[0, 7, 0, 2, 0, 0], [0, 7, 3, 5, 0, 0], [0, 7, 6, 9, 0, 0]]
 
@@ -236,7 +238,7 @@ def report_lines(out):
 def test_reports_and_convergence_match_oracle(tmp_path):
     """Report values and the stop step follow SCSP::report / Simulator::updateConvergence,
     evaluated here with the oracle's fields (sums to rounding, decisions exactly)."""
-    spec = [s for s in H.channel_cases() if s['name'] == 'channel-scsp-d3-MRT-tau1.4-H8'][0]
+    spec = TC.reference_case('channel', 'channel-scsp-d3-MRT-tau1.4-H8')
     spec = dict(spec, steps=3000, write=3000, report_interval=50, report_dir=1, stop_error=1e-4, restart=3000,
                 name='converge')
     code, out, timing = run(spec, tmp_path)
@@ -292,10 +294,10 @@ def test_fp32_sparse_engine_option(tmp_path):
     """`--fp32 --sparse` runs the FP32 shifted sparse engine; the Poiseuille profile stays
     within 1e-5 relative of the reference's recorded FP64 profile (the FP32 tolerance;
     3e-6 measured at H8, tau 0.6)."""
-    spec = [s for s in H.channel_cases() if s['name'] == 'channel-scsp-d3-MRT-tau0.6-H8'][0]
+    spec = TC.reference_case('channel', 'channel-scsp-d3-MRT-tau0.6-H8')
     code, out, timing = run(spec, tmp_path, extra=('--fp32', '--sparse'))
     assert code == 0, out[-2000:]
-    _, f = H.read_vti(tmp_path / f'field.{spec["steps"]:010d}.vti')
+    _, f = read_vti(tmp_path / f'field.{spec["steps"]:010d}.vti')
     prof = f['velocity'][..., 0].mean(axis=(0, 2))[1:-1]
     import json as _j
     want = np.array(_j.loads((Path(__file__).parent / 'golden' / 'reference_records.json').read_text())
@@ -306,7 +308,7 @@ def test_fp32_sparse_engine_option(tmp_path):
 def test_pressure_slit_report_line(tmp_path, records):
     """scripts/porous_media.py:257-279 through the drop-in executable: the last
     ' - dP = ' and ' - K = ' report lines carry the values the reference printed."""
-    spec = H.pressure_case()
+    spec = TC.reference('pressure')[0]
     code, out, timing = run(spec, tmp_path)
     assert code == 0, out[-2000:]
     dP = re.findall(r' - dP = ([\deE+.-]+)', out)[-1]

@@ -18,13 +18,14 @@ import sys
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Engine
 from tests.helpers import oracle_run, sphere_case, velocity_grid
 
 pytestmark = pytest.mark.gpu
 
-CASES = H.csf_cases()
+CASES = TC.csf_cases()
 
 
 def say(*a):
@@ -32,8 +33,8 @@ def say(*a):
 
 
 def engine_run(spec, steps, precision=FP64, stream_solids=True, dense=False):
-    case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), precision, stream_solids, dense).init()
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec), precision, stream_solids, dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
     e.synchronize()
@@ -63,8 +64,8 @@ def test_csf_bitwise(spec, curvature):
 @pytest.mark.parametrize('spec', CASES, ids=lambda s: s['name'])
 def test_csf_reference_checks_gpu(spec):
     say('case', spec['name'])
-    case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), FP64, True, False).init()
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec), FP64, True, False).init()
     initial = np.linalg.norm(velocity_grid(e.fields()[2], case), axis=-1).max()
     for k in range(spec['steps']):
         e.step(STEP_FIELDS if k == spec['steps'] - 1 else 0)
@@ -79,7 +80,7 @@ def test_csf_reference_checks_gpu(spec):
 
 @pytest.mark.parametrize('curvature', ['normal', 'gradient'])
 def test_csf_drop_bitwise(curvature):
-    spec = H.csf_drop(wall_gradient='legacy', curvature=curvature)
+    spec = TC.csf_drop(wall_gradient='legacy', curvature=curvature)
     e, _ = engine_run(spec, 40)
     o, _ = oracle_run(spec, 40)
     assert_bitwise(e, o)
@@ -98,9 +99,9 @@ def test_csf_walls_bitwise(eq):
 
 def tanh_cases():
     pack = sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, porosity=0.8)
-    zh = next(s for s in H.zouhe_color_cases() if s['name'] == 'zhc-x-3d-MRT')
+    zh = next(s for s in TC.zouhe_color_cases() if s['name'] == 'zhc-x-3d-MRT')
     return [(dict(pack, wall_gradient='tanh'), 100), (dict(zh, wall_gradient='tanh'), 200),
-            (H.csf_wall('tanh'), 100), (dict(pack, wall_gradient='tanh', capillary_model='csf'), 60)]
+            (TC.csf_wall('tanh'), 100), (dict(pack, wall_gradient='tanh', capillary_model='csf'), 60)]
 
 
 @pytest.mark.parametrize('spec,steps', tanh_cases(), ids=lambda v: v['name'] if isinstance(v, dict) else str(v))
@@ -125,7 +126,7 @@ def test_tanh_walls_fp32(name):
     if name == 'pack':
         spec = sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, porosity=0.8)
     else:
-        spec = next(s for s in H.zouhe_color_cases() if s['name'] == name)
+        spec = next(s for s in TC.zouhe_color_cases() if s['name'] == name)
     spec = dict(spec, wall_gradient='tanh')
     a, _ = engine_run(spec, 200)
     b, _ = engine_run(spec, 200, FP32, stream_solids=False, dense=True)
@@ -144,8 +145,8 @@ def test_tanh_walls_fp32(name):
 
 def csf_fp32_cases():
     pack = sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True, porosity=0.8)
-    return [(H.csf_drop(wall_gradient='legacy'), 40), (dict(pack, capillary_model='csf'), 100),
-            (dict(H.csf_drop(wall_gradient='legacy'), curvature_stencil='gradient'), 40),
+    return [(TC.csf_drop(wall_gradient='legacy'), 40), (dict(pack, capillary_model='csf'), 100),
+            (dict(TC.csf_drop(wall_gradient='legacy'), curvature_stencil='gradient'), 40),
             (dict(pack, capillary_model='csf', wall_gradient='tanh'), 100)]
 
 

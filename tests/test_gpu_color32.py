@@ -14,7 +14,8 @@ import sys
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Engine
 from tests.helpers import sphere_case
 
@@ -33,8 +34,8 @@ def say(*a):
 
 
 def run(spec, steps, precision, dense=False):
-    case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), precision, stream_solids=precision == FP64, dense=dense).init()
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec), precision, stream_solids=precision == FP64, dense=dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
     e.synchronize()
@@ -86,7 +87,7 @@ def test_fp32_colour_restart_is_exact():
     spec = sphere_case(model='color', eq='MRT', tau=(0.7, 1.2), two_phase=True)
     a = run(spec, 50, FP32)
     pops = a.populations()
-    b = Engine(H.lattice_case(spec), H.geometry(spec), FP32, stream_solids=False)
+    b = Engine(CS.lattice(spec), CS.geometry(spec), FP32, stream_solids=False)
     b.set_populations(pops, a.t)
     for e in (a, b):
         for _ in range(30):
@@ -129,7 +130,7 @@ def test_fp32_renormalized_walls_track_fp64(kw):
     assert du < 1e-2
 
 
-@pytest.mark.parametrize('spec', [s for s in H.layered_cases('harmonic') if s['height'] == 16],
+@pytest.mark.parametrize('spec', TC.reference('layered', viscosity_interpolation='harmonic', height=16),
                          ids=lambda s: s['name'])
 def test_fp32_harmonic_viscosity_tracks_fp64(spec):
     """viscosityInterpolation harmonic (colorfluid.cpp:263-268) in FP32: the

@@ -10,7 +10,7 @@ import sys
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
 from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Engine
 from tests.helpers import oracle_run
 
@@ -24,12 +24,12 @@ def say(*a):
 
 
 def lattice(model, fluid_site):
-    phase = '0'
-    indices = ['255 "SD" "90" "N/A" "N/A"', f'0 "VD" "N/A" "N/A" "{phase}"']
-    label = (lambda x, y, z: 0 if (x, y, z) == (1, 1, 1) else 255) if fluid_site else (lambda x, y, z: 255)
-    return H.spec(f'edge-{model}-{"one" if fluid_site else "none"}', nx=3, ny=3, nz=3, dimensions=3,
-                  model=model, eq='MRT', tau=(0.8, 0.8) if model == 'color' else (0.8,),
-                  label=label, indices=indices)
+    indices = [CS.label_line(255, 'SD', 90), CS.label_line(0, 'VD', phase=0)]
+    labels = np.full((3, 3, 3), 255, np.uint8)
+    if fluid_site:
+        labels[1, 1, 1] = 0
+    return CS.make(f'edge-{model}-{"one" if fluid_site else "none"}', labels, indices,
+                   model=model, eq='MRT', tau=(0.8, 0.8) if model == 'color' else (0.8,))
 
 
 @pytest.mark.parametrize('precision,stream_solids,dense', LAYOUTS, ids=['fp64-dense', 'fp64-fluid', 'fp32-sparse',
@@ -38,15 +38,15 @@ def lattice(model, fluid_site):
 @pytest.mark.parametrize('fluid_site', [False, True], ids=['no-fluid', 'one-fluid-site'])
 def test_degenerate_lattices(model, fluid_site, precision, stream_solids, dense):
     spec = lattice(model, fluid_site)
-    case = H.lattice_case(spec)
+    case = CS.lattice(spec)
     say('case', spec['name'], precision, stream_solids, dense)
-    e = Engine(case, H.geometry(spec), precision, stream_solids, dense).init()
+    e = Engine(case, CS.geometry(spec), precision, stream_solids, dense).init()
     for k in range(20):
         e.step(STEP_FIELDS if k == 19 else 0)
     e.synchronize()
     e.check_status()
     r0, r1, u = e.fields()
-    solid = H.geometry(spec) == 255
+    solid = CS.geometry(spec) == 255
     assert (r0[solid] == 0).all() and (u[:, solid] == 0).all()
     if fluid_site:
         site = 13

@@ -13,13 +13,14 @@ import sys
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Engine
 from tests.helpers import oracle_run
 
 pytestmark = pytest.mark.gpu
 
-CASES = H.film_cases()
+CASES = TC.film_cases()
 
 
 def say(*a):
@@ -27,8 +28,8 @@ def say(*a):
 
 
 def engine_run(spec, steps, precision=FP64, stream_solids=True, dense=False):
-    case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), precision, stream_solids, dense).init()
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec), precision, stream_solids, dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
     e.synchronize()
@@ -57,7 +58,7 @@ def test_film_bitwise(spec):
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-@pytest.mark.parametrize('spec', H.film_cases(transparent=True), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.film_cases(transparent=True), ids=lambda s: s['name'])
 def test_film_transparent_gpu(spec):
     """A film of the only phase present changes nothing: bit for bit in FP64.
     In FP32 the planes carry f1 = h + s - f0 at rounding level, which the film
@@ -94,7 +95,7 @@ def test_film_fp32_layouts_tracking_and_barrier(spec):
     say('max |rho0 diff|', np.max(np.abs(ra0 - rc0)), 'rel L2 u', du)
     assert np.max(np.abs(ra0 - rc0)) < 5e-4
     assert du < 1e-2
-    g = H.geometry(spec)
+    g = CS.geometry(spec)
     fluid = np.isin(g, [k for k, v in case['labels'].items() if v[0] not in ('SD', 'BB')])
     no1, no0 = barrier_masks(spec, case)
     say('max rho1 beyond phase-0 films', np.abs(rc1[no1 & fluid]).max(),

@@ -10,7 +10,8 @@ import sys
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from lbflow_amd.engine import FP32, FP64, STEP_ADJACENT, STEP_FIELDS, Engine
 from tests.helpers import C1_SHA256, c1_spec, darcy, oracle_run, profile, sphere_case
 
@@ -22,8 +23,8 @@ def say(*a):
 
 
 def engine_run(spec, steps, precision=FP64, stream_solids=True, geometry=None, dense=False):
-    case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec) if geometry is None else geometry, precision, stream_solids,
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec) if geometry is None else geometry, precision, stream_solids,
                dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
@@ -43,7 +44,7 @@ def assert_same(eng, orc):
             assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-@pytest.mark.parametrize('spec', H.force_cases(), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('force'), ids=lambda s: s['name'])
 def test_force_cases_bitwise(spec):
     say('case', spec['name'])
     e, case = engine_run(spec, spec['steps'])
@@ -52,7 +53,7 @@ def test_force_cases_bitwise(spec):
     e.check_status()
 
 
-@pytest.mark.parametrize('spec', [s for s in H.channel_cases() if s['height'] == 8], ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('channel', height=8), ids=lambda s: s['name'])
 def test_channel_bitwise(spec, records):
     say('case', spec['name'])
     e, case = engine_run(spec, spec['steps'])
@@ -63,7 +64,7 @@ def test_channel_bitwise(spec, records):
     assert_same(e, o)
 
 
-@pytest.mark.parametrize('spec', [s for s in H.layered_cases() if s['height'] == 16], ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('layered', viscosity_interpolation='arithmetic', height=16), ids=lambda s: s['name'])
 def test_layered_bitwise(spec, records):
     say('case', spec['name'])
     e, case = engine_run(spec, spec['steps'])
@@ -72,7 +73,7 @@ def test_layered_bitwise(spec, records):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize('spec', H.bundle_cases(scales=(1,)), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('bundle', scale=1), ids=lambda s: s['name'])
 def test_bundle_bitwise(spec, records):
     say('case', spec['name'])
     rec = records['bundle'][spec['name'] + '-1rank']['history'][1]
@@ -118,7 +119,7 @@ def test_fluid_only_layouts_match(precision, eq, dims):
     b, _ = engine_run(spec, 60, precision, stream_solids=False, dense=True)
     c, _ = engine_run(spec, 60, precision, stream_solids=False)
     assert 'sparse' in c.step_kernel and 'dense-fluid' in b.step_kernel
-    g = H.geometry(spec)
+    g = CS.geometry(spec)
     fluid = np.tile(g == 75, case['Q'])
     pa, pb, pc = a.populations(), b.populations(), c.populations()
     assert np.array_equal(pa[fluid], pb[fluid]) and np.array_equal(pa[fluid], pc[fluid])
@@ -134,7 +135,7 @@ def test_restart_equivalence():
     cont, case = engine_run(spec, 80)
     half, _ = engine_run(spec, 40)
     pops = half.populations()
-    res = Engine(case, H.geometry(spec), FP64, True)
+    res = Engine(case, CS.geometry(spec), FP64, True)
     res.set_populations(pops, 40)
     for k in range(40):
         res.step(STEP_FIELDS if k == 39 else 0)

@@ -9,7 +9,8 @@ import sys
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from lbflow_amd.engine import FP32, FP64, Engine
 from lbflow_amd.permeability import converge
 from tests.helpers import sphere_case
@@ -23,9 +24,9 @@ def say(*a):
 
 @pytest.mark.parametrize('precision', [FP64, FP32], ids=['fp64', 'fp32'])
 def test_bundle_permeability_closed_form(precision):
-    spec = [s for s in H.bundle_cases(scales=(2,), dimensions=(3,)) if s['model'] == 'scsp'][0]
-    case = H.lattice_case(spec)
-    eng = Engine(case, H.geometry(spec), precision, stream_solids=False).init()
+    spec = TC.reference('bundle', scale=2, dimensions=3, model='scsp')[0]
+    case = CS.lattice(spec)
+    eng = Engine(case, CS.geometry(spec), precision, stream_solids=False).init()
     nx = spec['size'][0]
     # FP32 residuals level off near 1e-7, so its stop tolerance is the FP32 one
     tol = 1e-9 if precision == FP64 else 1e-6
@@ -41,10 +42,10 @@ def test_bundle_permeability_closed_form(precision):
 def test_fp32_permeability_within_1e5_of_fp64():
     spec = sphere_case(n=(32, 32, 32), model='scsp', eq='MRT', tau=(0.8,), porosity=0.3, seed=5)
     spec = dict(spec, indices=['75 "GV" "1e-06" "1" "0"', '255 "SD" "90" "N/A" "N/A"'])
-    case = H.lattice_case(spec)
+    case = CS.lattice(spec)
     out = {}
     for prec in (FP64, FP32):
-        eng = Engine(case, H.geometry(spec), prec, stream_solids=False).init()
+        eng = Engine(case, CS.geometry(spec), prec, stream_solids=False).init()
         out[prec] = converge(eng, axis=2, first=1, last=30, tol=1e-6, interval=200, max_steps=60000,
                              force=1e-6, tau=0.8)
         say(prec, out[prec]['steps'], out[prec]['K'])

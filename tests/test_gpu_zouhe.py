@@ -12,13 +12,14 @@ import sys
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from lbflow_amd.engine import FP32, FP64, STEP_ADJACENT, STEP_FIELDS, Engine
 from tests.helpers import oracle_run
 
 pytestmark = pytest.mark.gpu
 
-CASES = H.zouhe_cases()
+CASES = TC.zouhe_cases()
 
 
 def say(*a):
@@ -26,8 +27,8 @@ def say(*a):
 
 
 def engine_run(spec, steps, precision=FP64, stream_solids=True, dense=False):
-    case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), precision, stream_solids, dense).init()
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec), precision, stream_solids, dense).init()
     for k in range(steps):
         e.step(STEP_FIELDS if k == steps - 1 else 0)
     e.synchronize()
@@ -56,7 +57,7 @@ def test_zouhe_layouts_match(spec, precision):
     b, _ = engine_run(spec, 120, precision, stream_solids=False, dense=True)
     c, _ = engine_run(spec, 120, precision, stream_solids=False)
     assert 'sparse' in c.step_kernel and 'dense-fluid' in b.step_kernel
-    g = H.geometry(spec)
+    g = CS.geometry(spec)
     solid = np.isin(g, [k for k, v in case['labels'].items() if v[0] in ('SD', 'BB')])
     fluid = np.tile(~solid, case['Q'])
     pa, pb, pc = a.populations(), b.populations(), c.populations()
@@ -77,7 +78,7 @@ def slit_report(e, spec, case, first, last, tau=1.0, lattice_len=1e-6):
 
 
 def test_pressure_slit_matches_reference_report(records):
-    spec = H.pressure_case()
+    spec = TC.reference('pressure')[0]
     nx = spec['size'][0]
     e, case = engine_run(spec, spec['steps'])
     o, _ = oracle_run(spec, spec['steps'])
@@ -93,7 +94,7 @@ def test_pressure_slit_fp32():
     """FP32 shifted populations on the slit: K within 2e-5 and dP within 2e-5 (relative)
     of the FP64 engine (dP is 1e-5 of the density, so FP32 rounding of rho is ~1e-3
     of dP per site; the slab averages bring it down)."""
-    spec = H.pressure_case()
+    spec = TC.reference('pressure')[0]
     nx = spec['size'][0]
     a, case = engine_run(spec, spec['steps'])
     b, _ = engine_run(spec, spec['steps'], FP32, stream_solids=False)
@@ -118,7 +119,7 @@ def test_zouhe_adjacent_report_matches_oracle():
     adj = o.fields()[2].ravel().copy()
     e.step(STEP_FIELDS)
     o.step(1)
-    g = np.tile(H.geometry(spec), case['D'])
+    g = np.tile(CS.geometry(spec), case['D'])
     zh = np.isin(g, [50, 200])
     adj[zh] = o.fields()[2].ravel()[zh]
     r = e.report(axis, 1, case['size'][axis] - 2, convergence=True, adjacent=True)
@@ -127,7 +128,7 @@ def test_zouhe_adjacent_report_matches_oracle():
 
 
 # ---------------------------------------------------------------- colour model
-COLOR_CASES = H.zouhe_color_cases()
+COLOR_CASES = TC.zouhe_color_cases()
 
 
 @pytest.mark.parametrize('spec', COLOR_CASES, ids=lambda s: s['name'])
@@ -173,7 +174,7 @@ def entry_measure(r0, r1, u, case, height):
             [float(r0.sum()), float(r1.sum())])
 
 
-@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2, 3)), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('entry', wall_gradient='legacy'), ids=lambda s: s['name'])
 def test_entry_history_gpu(spec, records):
     """Capillary entry (porous_media.py:153-253) on the GPU in FP64: the reference's
     recorded history at every output step (3D has no oracle run here; the recorded
@@ -181,8 +182,8 @@ def test_entry_history_gpu(spec, records):
     non-contiguous view whose numpy summation order differs between numpy versions,
     so the sums are compared to 4 ulp."""
     say('case', spec['name'])
-    case = H.lattice_case(spec)
-    e = Engine(case, H.geometry(spec), FP64, True).init()
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec), FP64, True).init()
     rec = records['entry'][spec['name'] + '-1rank']
     for h in rec['history'][1:]:
         n = h['step'] - e.t
@@ -199,12 +200,12 @@ def test_entry_history_gpu(spec, records):
             np.testing.assert_allclose(np.hstack(got), np.hstack(want), rtol=4 * np.finfo(float).eps, atol=0)
 
 
-@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2,)), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('entry', dimensions=2, wall_gradient='legacy'), ids=lambda s: s['name'])
 def test_entry_fp32(spec, records):
     """FP32 colour on the entry cases: the phase-0 length after 8000 steps within 5e-3
     (absolute, lattice units; 1.9e-3 measured) and the pore velocity within 1e-3
     relative (1.8e-4 measured) of the record."""
-    case = H.lattice_case(spec)
+    case = CS.lattice(spec)
     e, _ = engine_run(spec, spec['steps'], FP32, stream_solids=False)
     rec = records['entry'][spec['name'] + '-1rank']['history'][-1]
     L, ux, _, _ = entry_measure(*e.fields(), case, spec['height'])
@@ -271,7 +272,7 @@ def test_renormalized_wall_bitwise(kw):
     assert np.array_equal(b.populations(), c.populations())
 
 
-@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2, 3), walls=('renormalized',)), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('entry', wall_gradient='renormalized'), ids=lambda s: s['name'])
 def test_entry_history_renormalized_gpu(spec, records):
     test_entry_history_gpu(spec, records)
 
@@ -285,6 +286,6 @@ def test_renormalized_wall_rejects_rank_deficient_stencils():
     spec = dict(sphere_case(model='color', eq='MRT', tau=(1.0, 1.0), two_phase=True), wall_gradient='renormalized')
     msg = 'Rank-deficient fluid wall stencil; reconstructed derivatives are unsupported here.'
     with pytest.raises(Exception, match=msg):
-        Oracle(H.lattice_case(spec), H.geometry(spec))
+        Oracle(CS.lattice(spec), CS.geometry(spec))
     with pytest.raises(EngineError, match=msg):
-        Engine(H.lattice_case(spec), H.geometry(spec), FP64, True)
+        Engine(CS.lattice(spec), CS.geometry(spec), FP64, True)

@@ -20,11 +20,12 @@ The reference records no CSF or tanh-wall output.  What pins them:
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from tests.helpers import oracle_run, velocity_grid
 from tests.test_oracle_film import lattice_vectors
 
-CASES = H.csf_cases()
+CASES = TC.csf_cases()
 
 
 def speeds(o, case):
@@ -51,10 +52,9 @@ def test_csf_laplace_law():
     """Pressure jump of a 2D drop (radius 8, sigma 0.01) within 10 % of
     sigma / R, inside higher than outside."""
     n, R, sigma = 40, 8.0, 0.01
-    spec = H.spec('csf-laplace', nx=n, ny=n, nz=1, dimensions=2, eq='MRT', capillary_model='csf',
-                  sigma=sigma, steps=4000, write=4000,
-                  label=lambda x, y, z: 100 if (x - 19.5) ** 2 + (y - 19.5) ** 2 <= R * R else 0,
-                  indices=['0 "VD" "N/A" "N/A" "0"', '100 "VD" "N/A" "N/A" "1"'])
+    _, y, x = np.indices((1, n, n))
+    spec = CS.make('csf-laplace', np.where((x - 19.5) ** 2 + (y - 19.5) ** 2 <= R * R, 100, 0), [CS.label_line(0, 'VD', phase=0), CS.label_line(100, 'VD', phase=1)],
+                   model='color', capillary_model='csf', sigma=sigma, steps=4000, write=4000)
     o, case = oracle_run(spec, 4000)
     r0, r1, u = o.fields()
     rho = (r0 + r1).reshape(n, n)
@@ -68,16 +68,16 @@ def test_csf_laplace_law():
 def test_tanh_wall_reconstructs_a_tanh_profile():
     from oracle.pyoracle import Oracle
     nx, ny, nz = 10, 6, 5
-    spec = H.spec('tanh-flat-wall', nx=nx, ny=ny, nz=nz, dimensions=3, steps=1, write=1,
-                  label=lambda x, y, z: 255 if y in (0, ny - 1) else 0,
-                  indices=['0 "VD" "N/A" "N/A" "0"', '255 "SD" "90" "N/A" "N/A"'])
+    _, y, _ = np.indices((nz, ny, nx))
+    spec = CS.make('tanh-flat-wall', np.where((y == 0) | (y == ny - 1), 255, 0),
+                   [CS.label_line(0, 'VD', phase=0), CS.label_line(255, 'SD', 90)], model='color', steps=1, write=1)
     e = lattice_vectors(3)
     w = np.array([1 / 3] + [1 / 18 if (v ** 2).sum() == 1 else 1 / 36 for v in e[1:]])
     c0, k = 0.2, np.array([0.0, 0.0, 0.35])          # chi = tanh(c0 + k . (z, y, x - 4))
     grids = {}
     for wall in ('tanh', 'legacy'):
-        case = H.lattice_case(dict(spec, wall_gradient=wall))
-        o = Oracle(case, H.geometry(spec)).init()
+        case = CS.lattice(dict(spec, wall_gradient=wall))
+        o = Oracle(case, CS.geometry(spec)).init()
         f = np.zeros((2, 19, nz, ny, nx))
         chi = np.tanh(c0 + k[2] * (np.arange(nx) - 4.0))
         f[0, 0] = (1 + chi) / 2

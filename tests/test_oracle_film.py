@@ -21,10 +21,11 @@ colorfluid.cpp:752-764) fixes, independently of how the oracle states it:
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from tests.helpers import oracle_run
 
-CASES = H.film_cases()
+CASES = TC.film_cases()
 SOLID = ('SD', 'BB')
 FLUID = ('VD', 'GV', 'PB', 'VB', 'HF')
 
@@ -34,15 +35,15 @@ def lattice_vectors(D):
     population through an all-fluid periodic lattice with the oracle."""
     from oracle.pyoracle import Oracle
     n = 5
-    spec = H.spec('probe', nx=n, ny=n, nz=n if D == 3 else 1, dimensions=D, model='color',
-                  steps=1, write=1)
-    case = H.lattice_case(spec)
+    spec = CS.make('probe', np.zeros((n if D == 3 else 1, n, n)), [CS.label_line(0, 'VD', phase=0)], model='color', dimensions=D,
+                   steps=1, write=1)
+    case = CS.lattice(spec)
     Q = case['Q']
     size = case['size']
     centre = tuple(k // 2 for k in size)
     e = np.zeros((Q, D), int)
     for q in range(1, Q):
-        o = Oracle(case, H.geometry(spec)).init()
+        o = Oracle(case, CS.geometry(spec)).init()
         f = np.zeros((2, Q) + tuple(size))
         f[(0, q) + centre] = 1.0
         o.set_populations(f.ravel(), 0)
@@ -87,7 +88,7 @@ def stream_half_numpy(F, g, case, e):
 def test_film_streaming_matches_destination_form(spec):
     o, case = oracle_run(spec, 0)
     Q, size = case['Q'], tuple(case['size'])
-    g = H.geometry(spec).reshape(size)
+    g = CS.geometry(spec).reshape(size)
     e = lattice_vectors(case['D'])
     rng = np.random.default_rng(7)
     F = rng.uniform(0.01, 0.1, (2, Q) + size)
@@ -100,7 +101,7 @@ def test_film_streaming_matches_destination_form(spec):
     assert (want[1][:, g == 30] == 0).all() and (want[0][:, g == 31] == 0).all()
 
 
-@pytest.mark.parametrize('spec', H.film_cases(transparent=True), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.film_cases(transparent=True), ids=lambda s: s['name'])
 def test_film_of_the_only_phase_is_transparent(spec):
     plain = dict(spec, indices=[i.replace('"HF"', '"VD"') for i in spec['indices']])
     a, _ = oracle_run(spec, 150)
@@ -121,7 +122,7 @@ def barrier_masks(spec, case):
 @pytest.mark.parametrize('spec', CASES, ids=lambda s: s['name'])
 def test_film_is_a_barrier_and_conserves_each_phase(spec):
     o, case = oracle_run(spec, 0)
-    g = H.geometry(spec).reshape(case['size'])
+    g = CS.geometry(spec).reshape(case['size'])
     fluid = np.isin(g, [k for k, v in case['labels'].items() if v[0] in FLUID])
     no1, no0 = barrier_masks(spec, case)
     block = spec['name'].startswith('film-block')

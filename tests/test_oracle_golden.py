@@ -12,11 +12,12 @@ import hashlib
 import numpy as np
 import pytest
 
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
+from tests import cases as TC
 from tests.helpers import C1_SHA256, c1_spec, darcy, oracle_run, profile, velocity_grid
 
 
-@pytest.mark.parametrize('spec', H.force_cases(), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('force'), ids=lambda s: s['name'])
 def test_uniform_force(spec, records):
     """u(t) = t F after 100 steps; recorded max errors reproduced exactly (benchmarks.py:176-188)."""
     o, case = oracle_run(spec, spec['steps'])
@@ -29,7 +30,7 @@ def test_uniform_force(spec, records):
     assert err <= 1e-12
 
 
-CHANNELS = [s for s in H.channel_cases() if s['height'] in (8, 16)]
+CHANNELS = TC.reference('channel', height=(8, 16))
 
 
 @pytest.mark.parametrize('spec', CHANNELS, ids=lambda s: s['name'])
@@ -42,7 +43,7 @@ def test_channel_profile(spec, records):
 
 
 def _layered(interp):
-    return [s for s in H.layered_cases(interp) if s['height'] == 16]
+    return TC.reference('layered', viscosity_interpolation=interp, height=16)
 
 
 @pytest.mark.parametrize('spec', _layered('arithmetic') + _layered('harmonic'), ids=lambda s: s['name'])
@@ -56,7 +57,7 @@ def test_layered_two_phase(spec, records):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize('spec', H.bundle_cases(scales=(1,)), ids=lambda s: s['name'])
+@pytest.mark.parametrize('spec', TC.reference('bundle', scale=1), ids=lambda s: s['name'])
 def test_bundle_history(spec, records):
     """Slit bundles (porous_media.py:67-151): Darcy velocity and total mass at every output step."""
     rec = records['bundle'][spec['name'] + '-1rank']['history']
@@ -85,7 +86,7 @@ def test_pressure_boundaries_report(records):
     """Zou-He pressure inlet/outlet slit (porous_media.py:257-279): the report line the
     reference recorded (dP and K printed with 7 significant digits) is reproduced, and
     both lie within 2e-4 of the analytic slit."""
-    spec = H.pressure_case()
+    spec = TC.reference('pressure')[0]
     o, case = oracle_run(spec, spec['steps'])
     nx = spec['size'][0]
     dP, K = pressure_report(o.report_sums(2, 1, nx - 2), case, 1.0, 1, nx - 2)
@@ -112,15 +113,15 @@ def entry_measure(r0, r1, u, case, height):
             [float(r0.sum()), float(r1.sum())])
 
 
-@pytest.mark.parametrize('spec', H.entry_cases(dimensions=(2,), walls=('legacy', 'renormalized')),
+@pytest.mark.parametrize('spec', TC.reference('entry', dimensions=2),
                          ids=lambda s: s['name'])
 def test_entry_history(spec, records):
     """Colour-model capillary entry between Zou-He pressure boundaries
     (porous_media.py:153-253), legacy and renormalized wall gradients: the recorded
     displacement history is reproduced exactly."""
     from oracle.pyoracle import Oracle
-    case = H.lattice_case(spec)
-    o = Oracle(case, H.geometry(spec)).init()
+    case = CS.lattice(spec)
+    o = Oracle(case, CS.geometry(spec)).init()
     rec = records['entry'][spec['name'] + '-1rank']
     for h in rec['history'][1:]:
         o.step(h['step'] - o.t)
@@ -143,12 +144,9 @@ def test_c1_checkpoint_digest():
 @pytest.mark.slow
 def test_bentheimer48_history(records):
     """Bentheimer48 crop, D3Q19 MRT, tau 1, F 1e-6 (rock_workflow.py:19-39): step-1000 record."""
-    from tests.conftest import GOLDEN
-    g = np.fromfile(GOLDEN / 'bentheimer48_single_phase.raw', dtype=np.uint8)
+    s = TC.reference_case('rock', 'rock-scsp-F1e-06')
     rec = records['rock_scsp']['rock-scsp-F1e-06-2rank']
-    assert hashlib.sha256(g.tobytes()).hexdigest() == rec['geometry_sha256']
-    s = H.spec('rock', nx=48, ny=48, nz=48, model='scsp', tau=(1.0,), geometry=g,
-               indices=['75 "GV" "1e-06" "1" "0"', '255 "SD" "60" "N/A" "N/A"'])
+    assert hashlib.sha256(CS.geometry(s).tobytes()).hexdigest() == rec['geometry_sha256']
     o, case = oracle_run(s, 1000)
     r0, _, u = o.fields()
     h = rec['history'][1]
@@ -176,7 +174,7 @@ def test_mrt_operator_identities():
 
 def test_planted_error_is_caught():
     """A plausible mistake (wrong relaxation time) in the oracle input changes the recorded profile."""
-    s = [c for c in H.channel_cases() if c['name'] == 'channel-scsp-d3-MRT-tau0.6-H8'][0]
+    s = TC.reference_case('channel', 'channel-scsp-d3-MRT-tau0.6-H8')
     bad = dict(s, tau=(0.6000001,))
     o, case = oracle_run(bad, s['steps'])
     got = profile(o.fields()[2], case)

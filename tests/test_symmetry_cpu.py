@@ -13,7 +13,7 @@ import pytest
 import torch
 
 from lbflow_amd import geometry as G
-from lbflow_amd import harness as H
+from lbflow_amd import cases as CS
 from lbflow_amd.surrogate import symmetry
 from oracle.pyoracle import Oracle
 
@@ -21,9 +21,8 @@ N, STEPS = 12, 120
 
 
 def run(labels):
-    s = H.spec('sym', nx=N, ny=N, nz=N, model='scsp', eq='MRT', tau=(0.8,), geometry=labels.ravel(),
-               indices=['75 "GV" "1e-05" "1" "0"', '255 "SD" "90" "N/A" "N/A"'])
-    o = Oracle(H.lattice_case(s), H.geometry(s)).init()
+    s = CS.make('sym', labels, [CS.label_line(75, 'GV', 1e-5, 1, 0), CS.label_line(255, 'SD', 90)], tau=(0.8,))
+    o = Oracle(CS.lattice(s), CS.geometry(s)).init()
     o.step(STEPS)
     r, _, u = o.fields()
     # (u_z, u_y, u_x, rho - 1) per site, unnormalised (rms 1 per channel),
