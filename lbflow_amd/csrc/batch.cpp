@@ -61,6 +61,7 @@ struct lbf_batch {
     float *d_ucur = nullptr, *d_uprev = nullptr, *d_uadj = nullptr, *d_upres = nullptr;
     DevStatus* d_status = nullptr;
     double *d_partial = nullptr, *d_out = nullptr;
+    double* h_out = nullptr;         // pinned host copy of d_out, read after each graph replay
     std::vector<void*> allocations;
     uint32_t grid = 0, cus = 0;
     std::string err;
@@ -70,6 +71,17 @@ struct lbf_batch {
 namespace {
 
 thread_local std::string g_batch_error;
+
+// A captured graph and its executable, released on every exit path.
+struct CapturedGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~CapturedGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
 
 int bfail(lbf_batch* b, const std::string& m)
 {
@@ -255,6 +267,7 @@ void lbf_batch_destroy(lbf_batch* b)
 {
     if (!b) return;
     for (void* p : b->allocations) (void)hipFree(p);
+    if (b->h_out) (void)hipHostFree(b->h_out);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -416,6 +429,8 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     rc |= balloc(b, &b->d_status, sizeof(DevStatus));
     rc |= balloc(b, &b->d_partial, b->T * 4 * 8);
     rc |= balloc(b, &b->d_out, size_t(b->ncrops) * 4 * 8);
+    if (!rc && hipHostMalloc(reinterpret_cast<void**>(&b->h_out), size_t(b->ncrops) * 4 * 8) != hipSuccess)
+        rc = bfail(b, "HIP error: hipHostMalloc failed");
     if (rc) {
         std::string e = b->err;
         lbf_batch_destroy(b);
@@ -480,26 +495,29 @@ int lbf_batch_tune(lbf_batch* b, uint32_t steps, uint32_t* best_per_cu, double* 
     static const uint32_t candidates[] = {1, 2, 4, 8, 16, 32, 64, 128};
     double best = 1e300;
     uint32_t pick = 8;
-    hipEvent_t e0, e1;
-    BHIP(b, hipEventCreate(&e0));
-    BHIP(b, hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = bfail(b, "HIP error: hipEventCreate failed");
     for (uint32_t per : candidates) {
+        if (rc) break;
         b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * per)));
-        if (step(b, 0u, false) || step(b, 0u, false)) return -1;   // warm-up, keeps buffer parity
-        BHIP(b, hipEventRecord(e0, b->stream));
-        for (uint32_t k = 0; k < steps; ++k)
-            if (step(b, 0u, false)) return -1;
-        BHIP(b, hipEventRecord(e1, b->stream));
-        BHIP(b, hipEventSynchronize(e1));
+        rc = step(b, 0u, false) || step(b, 0u, false) ? -1 : 0;   // warm-up, keeps buffer parity
+        if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = bfail(b, "HIP error: hipEventRecord failed");
+        for (uint32_t k = 0; k < steps && !rc; ++k)
+            if (step(b, 0u, false)) rc = -1;
         float ms = 0.0f;
-        BHIP(b, hipEventElapsedTime(&ms, e0, e1));
-        if (ms / steps < best) {
+        if (!rc && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = bfail(b, "HIP error: timing events failed");
+        if (!rc && ms / steps < best) {
             best = ms / steps;
             pick = per;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return -1;
     b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * pick)));
     if (best_per_cu) *best_per_cu = pick;
     if (best_ms) *best_ms = best;
@@ -515,39 +533,34 @@ int lbf_batch_run(lbf_batch* b, int axis, double tol, uint32_t interval, uint64_
     if (interval % 2) ++interval;   // keeps the population buffers in the same role per replay
     const auto start = std::chrono::steady_clock::now();
     // capture: interval steps (the last two keep adjacent and current velocity) + reduction
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    CapturedGraph cg;
     BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const uint64_t t0 = b->t;
     for (uint32_t k = 0; k < interval; ++k) {
         const uint32_t flags = k == interval - 2 ? 2u : (k == interval - 1 ? 1u : 0u);
         if (step(b, flags, false)) {
-            (void)hipStreamEndCapture(b->stream, &graph);
+            (void)hipStreamEndCapture(b->stream, &cg.graph);
             return -1;
         }
     }
     hipError_t e = launch_batch_report(b->grid, b->S, b->D, axis, buffers(b, false), b->ncrops, b->d_partial,
                                        b->d_out, b->stream);
-    BHIP(b, hipStreamEndCapture(b->stream, &graph));
+    BHIP(b, hipStreamEndCapture(b->stream, &cg.graph));
     BHIP(b, e);
-    BHIP(b, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    BHIP(b, hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0));
     b->t = t0;   // capture advanced the host counters without running anything
     std::vector<char> active(b->ncrops, 1);
-    if (set_active(b, active)) {
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        return -1;
-    }
+    if (set_active(b, active)) return -1;
     std::vector<int> stable(b->ncrops, 0);
     std::vector<char> has_prev(b->ncrops, 0);
-    std::vector<double> out(size_t(b->ncrops) * 4);
+    const double* out = b->h_out;
     for (uint32_t c = 0; c < b->ncrops; ++c) std::memset(&results[c], 0, sizeof(lbf_crop_result));
     uint32_t remaining = b->ncrops;
     const double nu = (b->d.tau - 0.5) / 3.0;
     while (remaining && b->t < max_steps) {
-        BHIP(b, hipGraphLaunch(exec, b->stream));
+        BHIP(b, hipGraphLaunch(cg.exec, b->stream));
         b->t += interval;
-        BHIP(b, hipMemcpyAsync(out.data(), b->d_out, out.size() * 8, hipMemcpyDeviceToHost, b->stream));
+        BHIP(b, hipMemcpyAsync(b->h_out, b->d_out, size_t(b->ncrops) * 4 * 8, hipMemcpyDeviceToHost, b->stream));
         BHIP(b, hipStreamSynchronize(b->stream));
         bool changed = false;
         for (uint32_t c = 0; c < b->ncrops; ++c) {
@@ -562,11 +575,8 @@ int lbf_batch_run(lbf_batch* b, int axis, double tol, uint32_t interval, uint64_
             r.adjacent_residual = ares;
             const double F = b->crop_force[size_t(c) * 3 + axis];
             r.k_darcy = F != 0.0 ? nu * (ux / double(b->crop_sites[c])) / F : 0.0;
-            if (!std::isfinite(du2) || !std::isfinite(u2)) {
-                (void)hipGraphExecDestroy(exec);
-                (void)hipGraphDestroy(graph);
+            if (!std::isfinite(du2) || !std::isfinite(u2) || !std::isfinite(da2))
                 return bfail(b, "Non-finite velocity in convergence report.");
-            }
             if (has_prev[c]) {
                 const bool ok = (rms <= 1e-12 || res < tol) && (arms <= 1e-12 || ares < tol);
                 stable[c] = ok ? stable[c] + 1 : 0;
@@ -579,14 +589,10 @@ int lbf_batch_run(lbf_batch* b, int axis, double tol, uint32_t interval, uint64_
                 changed = true;
             }
         }
-        if (changed && remaining && set_active(b, active)) {
-            (void)hipGraphExecDestroy(exec);
-            (void)hipGraphDestroy(graph);
-            return -1;
-        }
+        if (changed && remaining && set_active(b, active)) return -1;
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    // Only the failure bits are reported: steps replayed from the captured graph
+    // carry their capture-time step numbers, so DevStatus.first is not a step here.
     DevStatus st{};
     BHIP(b, hipMemcpy(&st, b->d_status, sizeof st, hipMemcpyDeviceToHost));
     if (st.bits) return bfail(b, "Non-finite post-collision population in the batch.");

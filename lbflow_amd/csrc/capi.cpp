@@ -650,6 +650,8 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
         a.mai1[k] = m1;
     }
     a.rho_ref = c->d.density[0];
+    a.rest32 = 0.0;
+    for (int q = 0; q < c->Q; ++q) a.rest32 += double(float(a.dist0[q]));
     return a;
 }
 
@@ -796,10 +798,13 @@ int lbf_get_populations(lbf_ctx* c, double* out)
 int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
 {
     if (!c || !in) return -1;
+    // Steps queued on the context's (non-blocking) stream may still be writing
+    // the buffers replaced here: wait for them, then upload on the same stream.
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
     const size_t planes = size_t(c->NP) * c->Q;
     const size_t n = planes * c->P;
     if (c->elem == 8 && c->layout == 0) {
-        HIP_CHECK(c, hipMemcpy(c->f[c->cur], in, n * 8, hipMemcpyHostToDevice));
+        HIP_CHECK(c, hipMemcpyAsync(c->f[c->cur], in, n * 8, hipMemcpyHostToDevice, c->stream));
     } else {
         std::vector<double> h64(c->elem == 8 ? n : 0);
         std::vector<float> h32(c->elem == 4 ? n : 0);
@@ -822,27 +827,34 @@ int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
                 if (c->elem == 8) h64[plane * c->P + s] = v;
                 else h32[plane * c->P + s] = float(v - c->shift[plane]);
             }
-        if (c->elem == 8) HIP_CHECK(c, hipMemcpy(c->f[c->cur], h64.data(), n * 8, hipMemcpyHostToDevice));
-        else HIP_CHECK(c, hipMemcpy(c->f[c->cur], h32.data(), n * 4, hipMemcpyHostToDevice));
+        if (c->elem == 8)
+            HIP_CHECK(c, hipMemcpyAsync(c->f[c->cur], h64.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+        else
+            HIP_CHECK(c, hipMemcpyAsync(c->f[c->cur], h32.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));   // the staging vectors end here
     }
     // A restarted run holds no macro history (Simulator::validateRestart resets it,
     // simulator.cpp:382-383; the macro arrays are fresh zeros until the first step).
-    HIP_CHECK(c, hipMemset(c->d_rho0, 0, c->N * 8));
-    HIP_CHECK(c, hipMemset(c->d_u, 0, size_t(c->D) * c->N * 8));
-    HIP_CHECK(c, hipMemset(c->d_uadj, 0, 2 * size_t(c->D) * c->N * 8));
+    HIP_CHECK(c, hipMemsetAsync(c->d_rho0, 0, c->N * 8, c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->d_u, 0, size_t(c->D) * c->N * 8, c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->d_uadj, 0, 2 * size_t(c->D) * c->N * 8, c->stream));
     if (c->NP == 2) {
-        HIP_CHECK(c, hipMemset(c->d_rho1, 0, c->N * 8));
-        HIP_CHECK(c, hipMemset(c->d_chiadj, 0, 2 * c->N * 8));
+        HIP_CHECK(c, hipMemsetAsync(c->d_rho1, 0, c->N * 8, c->stream));
+        HIP_CHECK(c, hipMemsetAsync(c->d_chiadj, 0, 2 * c->N * 8, c->stream));
     }
-    reset_status(c);
+    reset_status(c);   // synchronises the stream
     c->t = t;
     c->has_prev = false;
     return 0;
 }
 
-int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
+}  // extern "C"
+
+// One report's reductions with the block partials in `partial` and the sums
+// in `sums` (device memory of at least report_blocks(N) * NREP and NREP
+// doubles).
+static int report_into(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out, double* partial, double* sums)
 {
-    if (!c || !req || !out) return -1;
     if (req->axis < 0 || req->axis >= c->D) return fail(c, "lbf_report: axis outside the lattice");
     ReportArgs a{};
     a.g = c->g;
@@ -857,7 +869,7 @@ int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
     a.uadj = c->d_uadj + adjacent_buffer * uint64_t(c->D) * c->N;
     a.chiprev = c->d_chiprev;
     a.chiadj = c->NP == 2 ? c->d_chiadj + adjacent_buffer * c->N : nullptr;
-    a.partial = c->d_partial;
+    a.partial = partial;
     a.D = c->D;
     a.axis = req->axis;
     a.color = c->NP == 2;
@@ -865,9 +877,9 @@ int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
     a.adjacent = req->adjacent ? 1 : 0;
     a.first = uint32_t(req->first);
     a.last = uint32_t(req->last);
-    HIP_CHECK(c, launch_report(a, c->d_out, c->stream));
+    HIP_CHECK(c, launch_report(a, sums, c->stream));
     double r[NREP];
-    HIP_CHECK(c, hipMemcpyAsync(r, c->d_out, sizeof r, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(c, hipMemcpyAsync(r, sums, sizeof r, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     std::memset(out, 0, sizeof *out);
     out->vel_range = r[0];
@@ -886,6 +898,14 @@ int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
     out->has_previous = req->convergence ? (c->has_prev ? 1 : 0) : 0;
     if (req->convergence) c->has_prev = true;
     return 0;
+}
+
+extern "C" {
+
+int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
+{
+    if (!c || !req || !out) return -1;
+    return report_into(c, req, out, c->d_partial, c->d_out);
 }
 
 uint64_t lbf_links(lbf_ctx* c, uint32_t* linkmask, int32_t* wall, double* normal, double* sinca, double* cosca)

@@ -226,6 +226,10 @@ struct ColorArgs {
     float* densN32;
     double mai0[19], mai1[19];     // M ai_k: rest-state moments of each phase
     double rho_ref;                // rho_0, the density of the shift s_q = ai0_q rho_0
+    // sum over q of float(s_q): the density of the FP32 shift as the kernels
+    // add it back (fT_q = h_q + float(s_q)); it differs from rho_0 by about
+    // 1.5e-8, which recolouring must not turn into a mass transfer
+    double rest32;
     // FP32 layouts: DENSE_FLUID (slot = site) or SPARSE (slot = fluid index)
     const uint32_t* list;          // SPARSE: fluid site of each slot
     const uint2* rank;             // SPARSE: rank index (see stream.h)

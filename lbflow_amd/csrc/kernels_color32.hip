@@ -567,8 +567,13 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
         norm = color_gradient_f32<Q>(a, i, cx, cy, cz, g);
     }
 
-    // perturbation + recolouring (colorfluid.cpp:389-428)
-    const float dens0T = r0 * inv_rt;
+    // perturbation + recolouring (colorfluid.cpp:389-428).  Phase 0 takes the
+    // share r0 / rho of the total fT_q = h_q + float(s_q) of each direction;
+    // rho is taken as the sum of exactly those totals, dr + sum float(s_q),
+    // formed in FP64 (near 1 an FP32 rho has only 6e-8 resolution), so that
+    // sum_q f0_q = r0 up to unbiased rounding.  With rt (built on rho_0) the
+    // phase-0 mass would grow by a factor 1 + 1.5e-8 per step.
+    const float dens0T = float(double(r0) / (double(dr) + a.rest32));
     bool finite = true;
     if (norm > 1e-9f) {
         const float b01 = float(a.beta) * dens0T * r1 * inv_rt;

@@ -330,28 +330,76 @@ static void build_ib(orc_sim* s)
     s->nwall = c;
 }
 
-/* ColorFluid::calcTibNorm, colorfluid.cpp:1111-1129 */
+/* One wall site's frame rotation and its inverse from its integer normal
+ * (ColorFluid::calcTibNorm, colorfluid.cpp:1111-1129). */
+static void wall_frame(int D, const double* n, double* tib, double* tibInv)
+{
+    if (D == 3) {
+        double tx[9], txInv[9], tz[9], tzInv[9];
+        rotate_tx3d(n[0], n[1], tx);
+        rotate_tx3d(-n[0], n[1], txInv);
+        double val = sqrt(n[0] * n[0] + n[1] * n[1]);
+        rotate_tz3d(val, n[2], tz);
+        rotate_tz3d(-val, n[2], tzInv);
+        mat_mul(tz, tx, 3, 3, 3, tib);
+        mat_mul(txInv, tzInv, 3, 3, 3, tibInv);
+    } else {
+        rotate_t2d(n[0], n[1], tib);
+        rotate_t2d(-n[0], n[1], tibInv);
+    }
+}
+
 static void calc_tib(orc_sim* s)
 {
     const int D = s->D;
     s->tib = calloc(s->nwall * D * D, sizeof(double));
     s->tibInv = calloc(s->nwall * D * D, sizeof(double));
-    for (uint64_t c = 1; c < s->nwall; ++c) {
-        const double* n = s->ibNorm + c * 3;
-        if (D == 3) {
-            double tx[9], txInv[9], tz[9], tzInv[9];
-            rotate_tx3d(n[0], n[1], tx);
-            rotate_tx3d(-n[0], n[1], txInv);
-            double val = sqrt(n[0] * n[0] + n[1] * n[1]);
-            rotate_tz3d(val, n[2], tz);
-            rotate_tz3d(-val, n[2], tzInv);
-            mat_mul(tz, tx, 3, 3, 3, s->tib + c * 9);
-            mat_mul(txInv, tzInv, 3, 3, 3, s->tibInv + c * 9);
-        } else {
-            rotate_t2d(n[0], n[1], s->tib + c * 4);
-            rotate_t2d(-n[0], n[1], s->tibInv + c * 4);
-        }
+    for (uint64_t c = 1; c < s->nwall; ++c)
+        wall_frame(D, s->ibNorm + c * 3, s->tib + c * D * D, s->tibInv + c * D * D);
+}
+
+/* The geometric contact angle at a wall site (colorfluid.cpp:550-567): the
+ * gradient g, of length norm, is replaced by the vector of the same length
+ * that makes the contact angle (sin sA, cos cA) with the wall normal in the
+ * plane of the normal and g.  3D: rotate g into the wall frame (normal along
+ * +z), turn its tangential part onto +y (RotateTx3D), set
+ * (0, norm sA, norm cA) and rotate back.  2D: the tangential sign of g in
+ * the wall frame is kept. */
+static void contact_angle(int D, const double* tib, const double* tibInv, double sA, double cA, double norm,
+                          double* g)
+{
+    if (D == 3) {
+        double gn[3], txInv[9], tInv[9], h[3];
+        mat_mul(tib, g, 3, 3, 1, gn);
+        rotate_tx3d(-gn[0], gn[1], txInv);
+        mat_mul(tibInv, txInv, 3, 3, 3, tInv);
+        h[0] = 0; h[1] = norm * sA; h[2] = norm * cA;
+        mat_mul(tInv, h, 3, 3, 1, g);
+    } else {
+        double gn[2], h[2];
+        mat_mul(tib, g, 2, 2, 1, gn);
+        if (gn[0] >= 0) { h[0] = norm * sA; h[1] = norm * cA; }
+        else { h[0] = -norm * sA; h[1] = norm * cA; }
+        mat_mul(tibInv, h, 2, 2, 1, g);
     }
+}
+
+int orc_contact_angle(int D, const double* n, const double* g, double sA, double cA, double* out, int plant)
+{
+    if (D != 2 && D != 3) return -1;
+    double tib[9], tibInv[9], norm2 = 0.0;
+    wall_frame(D, n, tib, tibInv);
+    for (int d = 0; d < D; ++d) { out[d] = g[d]; norm2 += g[d] * g[d]; }
+    switch (plant) {   /* planted mistakes, for the test that must catch them */
+    case 0: break;
+    case 1: sA = -sA; break;                                   /* sign of the tangential part */
+    case 2: memcpy(tibInv, tib, sizeof tib); break;            /* frame not inverted */
+    case 3: { double t = sA; sA = cA; cA = t; } break;         /* sin and cos swapped */
+    case 4: for (int d = 0; d < D; ++d) out[d] = -g[d]; break;  /* gradient reversed before the rotation */
+    default: return -1;
+    }
+    contact_angle(D, tib, tibInv, sA, cA, sqrt(norm2), out);
+    return 0;
 }
 
 /* ColorFluid::calcWallGradientInverse (colorfluid.cpp:917-957): per wall
@@ -983,20 +1031,7 @@ static void color_gradient(const orc_sim* s, uint64_t i, double* g, double* norm
             for (int d = 0; d < D; ++d) n2 = n2 + g[d] * g[d];
             norm = sqrt(n2);
         }
-        if (D == 3) {
-            double gn[3], txInv[9], tInv[9], h[3];
-            mat_mul(s->tib + c * 9, g, 3, 3, 1, gn);
-            rotate_tx3d(-gn[0], gn[1], txInv);
-            mat_mul(s->tibInv + c * 9, txInv, 3, 3, 3, tInv);
-            h[0] = 0; h[1] = norm * sA; h[2] = norm * cA;
-            mat_mul(tInv, h, 3, 3, 1, g);
-        } else {
-            double gn[2], h[2];
-            mat_mul(s->tib + c * 4, g, 2, 2, 1, gn);
-            if (gn[0] >= 0) { h[0] = norm * sA; h[1] = norm * cA; }
-            else { h[0] = -norm * sA; h[1] = norm * cA; }
-            mat_mul(s->tibInv + c * 4, h, 2, 2, 1, g);
-        }
+        contact_angle(D, s->tib + c * D * D, s->tibInv + c * D * D, sA, cA, norm, g);
     }
     *normOut = norm;
 }

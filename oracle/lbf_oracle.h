@@ -110,6 +110,12 @@ void orc_mrt_matrices(int Q, double tau, double* A, double* B, double* M, double
 
 /* Colour gradient of one site after the wall treatment (diagnostic). */
 void orc_color_gradient(const orc_sim* s, uint64_t site, double* g, double* norm);
+/* The geometric contact-angle rotation of one wall gradient g (D entries,
+ * reference component order) for the integer wall normal n and the contact
+ * angle (sA, cA) = (sin, cos); out receives the rotated gradient.  plant > 0
+ * plants a known mistake (1 sign of sin, 2 frame not inverted, 3 sin/cos
+ * swapped, 4 gradient reversed) for the test that must catch it. */
+int orc_contact_angle(int D, const double* n, const double* g, double sA, double cA, double* out, int plant);
 
 /* Report sums (scsp and color) over the report range/slices, as in
  * SCSP::report / ColorFluid::report.  out: [velR, rhoR0, rhoR1, rhoS0_0, rhoS0_1,

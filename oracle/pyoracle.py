@@ -61,6 +61,8 @@ def lib():
         L.orc_links.restype = u64
         L.orc_mrt_matrices.argtypes = [C.c_int, C.c_double, vp, vp, vp, vp]
         L.orc_color_gradient.argtypes = [vp, u64, vp, vp]
+        L.orc_contact_angle.argtypes = [C.c_int, vp, vp, C.c_double, C.c_double, vp, C.c_int]
+        L.orc_contact_angle.restype = C.c_int
         L.orc_report_sums.argtypes = [vp, C.c_int, u64, u64, vp]
         L.orc_convergence_sums.argtypes = [vp, vp, vp, vp]
         _LIB = L
@@ -208,3 +210,17 @@ def mrt_matrices(Q, tau):
     MInv = np.zeros(Q * Q)
     lib().orc_mrt_matrices(Q, tau, _ptr(A), _ptr(B), _ptr(M), _ptr(MInv))
     return A.reshape(Q, Q), B.reshape(Q, Q), M.reshape(Q, Q), MInv.reshape(Q, Q)
+
+
+def contact_angle(n, g, angle_deg, plant=0):
+    """The oracle's geometric contact-angle rotation of one wall gradient
+    (reference component order) for integer wall normal n; plant > 0 plants a
+    known mistake (see orc_contact_angle)."""
+    D = len(n)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    out = np.zeros(D)
+    a = angle_deg / 180.0 * np.pi
+    if lib().orc_contact_angle(D, _ptr(n), _ptr(g), float(np.sin(a)), float(np.cos(a)), _ptr(out), plant):
+        raise ValueError('orc_contact_angle: bad arguments')
+    return out

@@ -12,6 +12,17 @@ def c1_spec():
     return TC.c1()
 
 
+def c1_profile_error(got, tau=1.0, force=1e-6):
+    """Largest deviation of the 8x10x4 channel's u_x(y), y = 1..8, from the
+    closed form of body-force Poiseuille flow between halfway bounce-back
+    walls at y = 1/2 and 17/2 with the BGK slip, relative to the peak:
+    u(y) = F (y - 1/2)(17/2 - y) / (2 nu) + F (16 L - 3) / (24 nu), L = (tau - 1/2)^2."""
+    nu, lam = (tau - 0.5) / 3, (tau - 0.5) ** 2
+    y = np.arange(1, 9, dtype=float)
+    want = force * (y - 0.5) * (8.5 - y) / (2 * nu) + force * (16 * lam - 3) / (24 * nu)
+    return float(np.max(np.abs(np.asarray(got) - want)) / np.max(want))
+
+
 def velocity_grid(u, case):
     """u[d][site] (reference component order) -> array (nz, ny, nx, D) with components x, y(, z),
     the VTI 'velocity'/'v' layout (filevti.cpp:153-167)."""

@@ -1,0 +1,145 @@
+"""The benchmark workloads at their full sizes on the GPU.
+
+- 8x10x4 SRT channel (20 000 steps): the velocity profile equals the closed
+  form of body-force Poiseuille flow between halfway bounce-back walls,
+  u(y) = F (y - 1/2)(8.5 - y) / (2 nu) + F (16 L - 3) / (24 nu), L = (tau - 1/2)^2.
+- 256^3 single-phase sphere pack (r 4-10, porosity 0.20, seed 2; D3Q19 MRT,
+  tau 0.8, F 1e-6 along x): FP64 and FP32 run to the reference's stop rule at
+  residual 1e-6; both converge, K agrees within 1e-5 relative, mass is kept.
+- 128x128x256 two-phase colour sphere pack (r 6-12, porosity 0.30, seed 3,
+  8^3 phase blocks, 60 degrees, sigma 1e-3, F 1e-6 along z): the FP32 engine
+  runs 50 000 steps without a failure and keeps both component masses; over
+  the first 200 steps it tracks the FP64 engine within the tolerances below.
+- 512 batched crops (side 32-96, porosity 0.15-0.35, seeds 1000+i; MRT, tau
+  0.8, F 1e-6, residual 1e-5): every crop converges, and the smallest and the
+  largest crop stop at the step of their single-domain runs with bitwise the
+  same populations.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from lbflow_amd import cases as CS
+from lbflow_amd import geometry as G
+from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Batch, Engine
+from lbflow_amd.permeability import converge
+from tests import cases as TC
+from tests.helpers import c1_profile_error, profile
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pack_case(labels, name):
+    return CS.make(name, labels, [CS.label_line(75, 'GV', 1e-6, 1, 0), CS.label_line(255, 'SD', 90)], tau=(0.8,))
+
+
+def test_c1_profile_closed_form():
+    spec = TC.c1()
+    case = CS.lattice(spec)
+    e = Engine(case, CS.geometry(spec), FP64).init()
+    e.advance(spec['steps'] - 1)
+    e.step(STEP_FIELDS)
+    err = c1_profile_error(profile(e.fields()[2], case))
+    say('c1 profile rel error', err)
+    assert err < 1e-10
+
+
+def test_c2_fp64_and_fp32_to_permeability():
+    labels = G.sphere_pack((256, 256, 256), 0.20, 4.0, 10.0, seed=2)
+    spec = pack_case(labels, 'c2-256')
+    case = CS.lattice(spec)
+    out = {}
+    for name, prec in (('fp64', FP64), ('fp32', FP32)):
+        eng = Engine(case, labels.ravel(), prec, stream_solids=False).init()
+        m0 = float(eng.fields()[0].sum())
+        r = converge(eng, axis=2, first=1, last=254, tol=1e-6, interval=500, max_steps=60000,
+                     force=1e-6, tau=0.8)
+        m1 = float(eng.fields()[0].sum())
+        eng.close()
+        out[name] = r
+        say(name, 'steps', r['steps'], 'K', r['K'], 'seconds', round(r['seconds'], 2), 'mass drift', m1 / m0 - 1)
+        assert r['converged']
+        assert abs(m1 / m0 - 1) < (5e-9 if prec == FP64 else 1e-6)
+    rel = abs(out['fp32']['K'] - out['fp64']['K']) / out['fp64']['K']
+    say('|K32 - K64| / K64', rel)
+    assert rel <= 1e-5
+
+
+def c3_case():
+    labels = G.sphere_pack((256, 128, 128), 0.30, 6.0, 12.0, seed=3)
+    labels = G.random_blocks(labels, 8, 0.5, seed=103, phase0=75, phase1=76, pore=75)
+    return CS.make('c3', labels, [CS.label_line(75, 'GV', 1e-6, 5, 0), CS.label_line(76, 'GV', 1e-6, 5, 1),
+                                  CS.label_line(255, 'SD', 60)],
+                   model='color', tau=(1.0, 1.0), sigma=1e-3, beta=0.7, alpha=1 / 3)
+
+
+def test_c3_fp32_50000_steps_conserves_components():
+    spec = c3_case()
+    e = Engine(CS.lattice(spec), CS.geometry(spec), FP32, stream_solids=False).init()
+    r0, r1, _ = e.fields()
+    m0 = np.array([r0.sum(), r1.sum()])
+    for k in range(10):
+        e.advance(4999)
+        e.step(STEP_FIELDS)
+        e.check_status()
+        r0, r1, u = e.fields()
+        m = np.array([r0.sum(), r1.sum()])
+        say('c3 fp32 step', e.t, 'component mass drift', (m / m0 - 1).tolist(), 'max |u|', float(np.abs(u).max()))
+    assert e.t == 50000
+    assert np.all(np.isfinite(u))
+    assert np.max(np.abs(m / m0 - 1)) < 1e-5
+
+
+# FP32 colour engine against the FP64 one after 200 steps of the 128x128x256 pack
+C3_FP32_RHO0 = 5e-4        # max |rho0_32 - rho0_64|
+C3_FP32_U = 1e-2           # relative L2 of the velocity field
+
+
+def test_c3_fp32_tracks_fp64():
+    spec = c3_case()
+    case = CS.lattice(spec)
+    runs = []
+    for prec in (FP64, FP32):
+        e = Engine(case, CS.geometry(spec), prec, stream_solids=prec == FP64).init()
+        e.advance(199)
+        e.step(STEP_FIELDS)
+        e.check_status()
+        runs.append(e.fields())
+        e.close()
+    (a0, a1, ua), (b0, b1, ub) = runs
+    drho = float(np.max(np.abs(a0 - b0)))
+    urel = float(np.linalg.norm(ua - ub) / np.linalg.norm(ua))
+    say('c3 fp32 vs fp64 after 200 steps: max |d rho0|', drho, 'u rel L2', urel)
+    assert drho < C3_FP32_RHO0 and urel < C3_FP32_U
+
+
+def test_c4_all_crops_converge():
+    crops, _ = G.batch_crops(512)
+    labels = {75: ('GV', 1e-6, 1, 0), 255: ('SD', 90.0, 0, 0)}
+    b = Batch(3, 19, 'MRT', 0.8, 1.0, labels, crops)
+    per, ms = b.tune(steps=5)
+    b.init()
+    res, sec = b.run(axis=2, tol=1e-5, interval=200, max_steps=60000)
+    steps = [r['steps'] for r in res]
+    say('c4', len(crops), 'crops in', round(sec, 2), 's, tuned', per, 'wg/CU; steps', min(steps), max(steps))
+    assert all(r['converged'] for r in res)
+    sizes = [c.size for c in crops]
+    for c in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        g = crops[c]
+        case = dict(D=3, Q=19, size=g.shape, model='scsp', eq='MRT', tau=(0.8,), density=(1.0,), labels=labels)
+        e = Engine(case, g, FP32, stream_solids=False).init()
+        one = converge(e, axis=2, first=0, last=g.shape[2] - 1, tol=1e-5, interval=200, max_steps=60000,
+                       force=1e-6, tau=0.8)
+        say('c4 crop', c, g.shape, 'batch', res[c]['steps'], 'single', one['steps'])
+        assert one['converged'] and one['steps'] == res[c]['steps']
+        e2 = Engine(case, g, FP32, stream_solids=False).init()
+        e2.advance(one['steps'])
+        assert np.array_equal(b.populations(c), e2.populations())
+        e.close()
+        e2.close()
+    b.close()

@@ -14,7 +14,7 @@ import pytest
 
 from lbflow_amd import cases as CS
 from tests import cases as TC
-from tests.helpers import C1_SHA256, c1_spec, darcy, oracle_run, profile, velocity_grid
+from tests.helpers import C1_SHA256, c1_profile_error, c1_spec, darcy, oracle_run, profile, velocity_grid
 
 
 @pytest.mark.parametrize('spec', TC.reference('force'), ids=lambda s: s['name'])
@@ -131,6 +131,14 @@ def test_entry_history(spec, records):
         assert ux == h['pore_mean_ux']
         assert mass == h['total_mass']
         assert phases == h['phase_masses']
+
+
+def test_c1_profile_closed_form():
+    """8x10x4 SRT channel after 20000 steps: u_x(y) is the halfway bounce-back
+    Poiseuille profile with the BGK slip F (16 L - 3) / (24 nu) to roundoff;
+    without the slip term it would be off by 5e-3."""
+    o, case = oracle_run(c1_spec(), 20000)
+    assert c1_profile_error(profile(o.fields()[2], case)) < 1e-10
 
 
 def test_c1_checkpoint_digest():
