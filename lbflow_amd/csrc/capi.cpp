@@ -651,7 +651,21 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
     }
     a.rho_ref = c->d.density[0];
     a.rest32 = 0.0;
-    for (int q = 0; q < c->Q; ++q) a.rest32 += double(float(a.dist0[q]));
+    for (int q = 0; q < c->Q; ++q) {
+        a.rest32 += double(float(a.dist0[q]));
+        a.dai[q] = a.ai1[q] - a.ai0[q];
+        a.rest_err[q] = a.dist0[q] - double(float(a.dist0[q]));
+    }
+    for (int k = 0; k < c->Q; ++k) {
+        double m = 0.0, e = 0.0;
+        for (int q = 0; q < c->Q; ++q) {
+            const double mk = c->Q == 9 ? Lattice<9>::M[k][q] : Lattice<19>::M[k][q];
+            m += mk * a.dai[q];
+            e += mk * a.rest_err[q];
+        }
+        a.mdai[k] = m;
+        a.mrest_err[k] = e;
+    }
     return a;
 }
 
@@ -771,7 +785,8 @@ int lbf_get_populations(lbf_ctx* c, double* out)
     std::memset(out, 0, planes * c->N * sizeof(double));
     const std::vector<uint32_t>& desc = c->links.desc;
     if (c->NP == 2 && c->elem == 4) {
-        // device planes: h = fT - s (Q planes), then f0 (Q planes); checkpoint: f0, f1
+        // device planes: h = fT - float(s) (Q planes), then f0 (Q planes);
+        // checkpoint: f0, f1 (the kernels add the shift back as float(s))
         const ColorArgs a = color_args(c, 0u);
         const int Q = c->Q;
         for (int q = 0; q < Q; ++q)
@@ -780,7 +795,7 @@ int lbf_get_populations(lbf_ctx* c, double* out)
                 if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
                 const double h = h32[size_t(q) * c->P + s], f0 = h32[size_t(Q + q) * c->P + s];
                 out[size_t(q) * c->N + i] = f0;
-                out[size_t(Q + q) * c->N + i] = h + a.dist0[q] - f0;
+                out[size_t(Q + q) * c->N + i] = h + double(float(a.dist0[q])) - f0;
             }
         return 0;
     }
@@ -817,7 +832,7 @@ int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
                     const uint64_t i = c->layout == 2 ? c->list[s] : s;
                     if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
                     const double f0 = in[size_t(q) * c->N + i], f1 = in[size_t(Q + q) * c->N + i];
-                    h32[size_t(q) * c->P + s] = float(f0 + f1 - a.dist0[q]);
+                    h32[size_t(q) * c->P + s] = float(f0 + f1 - double(float(a.dist0[q])));
                     h32[size_t(Q + q) * c->P + s] = float(f0);
                 }
         } else

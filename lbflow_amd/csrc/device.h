@@ -230,6 +230,8 @@ struct ColorArgs {
     // add it back (fT_q = h_q + float(s_q)); it differs from rho_0 by about
     // 1.5e-8, which recolouring must not turn into a mass transfer
     double rest32;
+    double dai[19], mdai[19];         // ai1 - ai0 and M (ai1 - ai0)
+    double rest_err[19], mrest_err[19];   // s_q - float(s_q) and M of it
     // FP32 layouts: DENSE_FLUID (slot = site) or SPARSE (slot = fluid index)
     const uint32_t* list;          // SPARSE: fluid site of each slot
     const uint2* rank;             // SPARSE: rank index (see stream.h)

@@ -20,8 +20,11 @@
 // and reports are written in FP64, so the component-mass and flux reductions
 // run in FP64.
 //
+// Per-site sums (rho_0 = sum f0, rho = sum h + sum float(s_q)) and the
+// recolouring share are formed in FP64 from the FP32 populations; the
+// equilibrium's rest part is written without cancelling order-one terms.
 // FP32 tolerance: the negative-component check uses 1e-5 rho (the FP64
-// kernel's 1e-10 rho is below the FP32 rounding of rho1 = rho - rho0).
+// kernel's 1e-10 rho is below the rounding of the stored FP32 populations).
 #include "collide_f32.h"
 #include "device.h"
 #include "stream.h"
@@ -57,12 +60,12 @@ __device__ __forceinline__ bool resolve32(const ColorArgs& a, uint32_t& site, ui
 // its density deviation and velocity in dzh / uzh.
 template <int Q, bool INIT, int LAYOUT, typename O, bool OWN = false>
 __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, uint32_t slot, uint32_t desc,
-                                             const DevLabel& lab, float* h, float* f0, float& r0, float& dr,
+                                             const DevLabel& lab, float* h, float* f0, double& r0, double& dr,
                                              float& dzh, float* uzh)
 {
     using L = Lattice<Q>;
-    r0 = 0.0f;
-    dr = 0.0f;
+    r0 = 0.0;
+    dr = 0.0;
     if (INIT && OWN) {
         // the site's own slots of fin32, not streamed (CSF initialisation)
         const O P = a.P;
@@ -70,8 +73,8 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
         for (int q = 0; q < Q; ++q) {
             h[q] = ld(a.fin32, O(q) * P + O(slot));
             f0[q] = ld(a.fin32, (O(Q) + q) * P + O(slot));
-            r0 += f0[q];
-            dr += h[q];
+            r0 += double(f0[q]);
+            dr += double(h[q]);
         }
         return;
     }
@@ -81,10 +84,10 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const double f = (ph == 0 ? a.dist0[q] : a.dist1[q]) - 1.5 * L::w[q] * lab.ef[q];
-            h[q] = float(f - a.dist0[q]);
+            h[q] = float(f - double(float(a.dist0[q])));
             f0[q] = ph == 0 ? float(f) : 0.0f;
-            r0 += f0[q];
-            dr += h[q];
+            r0 += double(f0[q]);
+            dr += double(h[q]);
         }
         return;
     }
@@ -101,8 +104,8 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
         for (int q = 0; q < Q; ++q) {
             h[q] = ld(a.fin32, off[q]);
             f0[q] = ld(a.fin32, off[q] + f0plane);
-            r0 += f0[q];
-            dr += h[q];
+            r0 += double(f0[q]);
+            dr += double(h[q]);
         }
     } else {
 #pragma unroll
@@ -149,12 +152,12 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
         }
         if (lab.kind & KIND_ZOUHE)
             zouhe_color_f32<Q>(h, f0, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, float(a.rho_ref), a.dist0, dzh, uzh);
-        r0 = 0.0f;
-        dr = 0.0f;
+        r0 = 0.0;
+        dr = 0.0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            r0 += f0[q];
-            dr += h[q];
+            r0 += double(f0[q]);
+            dr += double(h[q]);
         }
     }
 }
@@ -359,10 +362,11 @@ __global__ void __launch_bounds__(256) color_density_f32(ColorArgs a)
     const uint32_t desc = a.desc[slot];
     const DevLabel& lab = a.lab[desc >> DESC_LABEL_SHIFT];
     if (!(lab.kind & KIND_FLUID)) return;
-    float h[Q], f0[Q], r0, dr, dzh, uzh[Lattice<Q>::D];
+    float h[Q], f0[Q], dzh, uzh[Lattice<Q>::D];
+    double r0, dr;
     load_color32<Q, INIT, LAYOUT, O, OWN>(a, site, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
-    const float rho = float(a.rho_ref) + dr;
-    a.densN32[site] = (2.0f * r0 - rho) / rho;   // (rho0 - rho1) / rho
+    const double rho = dr + a.rest32;
+    a.densN32[site] = float((2.0 * r0 - rho) / rho);   // (rho0 - rho1) / rho
 }
 
 template <int Q, bool MRT, bool INIT, int LAYOUT, typename O, bool CSF = false, bool OWN = false>
@@ -380,16 +384,19 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     const int kind = lab.kind;
     if (!(kind & KIND_FLUID)) return;
 
-    float h[Q], f0[Q], r0, dr, dzh = 0.0f, uzh[D];
-    load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
+    float h[Q], f0[Q], dzh = 0.0f, uzh[D];
+    double r0d, drd;
+    load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0d, drd, dzh, uzh);
     int cx, cy, cz;
     site_coords(a.g, i, cx, cy, cz);
 
-    // calcMacroVar (colorfluid.cpp:238-290)
-    const float rt = float(a.rho_ref) + dr;
-    const float r1 = rt - r0;
-    if (!isfinite(r0) || !isfinite(rt) || rt <= 0.0f) raise_status(a.status, 0, a.t);
-    else if (r0 < -NEG_TOL32 * rt || r1 < -NEG_TOL32 * rt) raise_status(a.status, 1, a.t);
+    // calcMacroVar (colorfluid.cpp:238-290).  The site's density is the sum
+    // of its totals fT_q = h_q + float(s_q), formed in FP64 from the FP32
+    // populations (rho = sum h + sum float(s_q)); rho_1 = rho - rho_0.
+    const double rtd = drd + a.rest32, r1d = rtd - r0d;
+    const float r0 = float(r0d), rt = float(rtd), r1 = float(r1d);
+    if (!isfinite(r0d) || !isfinite(rtd) || rtd <= 0.0) raise_status(a.status, 0, a.t);
+    else if (r0d < -NEG_TOL32 * rtd || r1d < -NEG_TOL32 * rtd) raise_status(a.status, 1, a.t);
     const float inv_rt = 1.0f / rt;
     const float dn = (r0 - r1) * inv_rt;
     float tau = (1.0f + dn) * float(a.relax0) + (1.0f - dn) * float(a.relax1);
@@ -419,8 +426,8 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
         if (!isfinite(u[d])) raise_status(a.status, 3, a.t);
     }
     if (a.flags & 1u) {
-        a.rho0[i] = double(r0);
-        a.rho1[i] = double(r1);
+        a.rho0[i] = r0d;
+        a.rho1[i] = r1d;
 #pragma unroll
         for (int d = 0; d < D; ++d) a.u[uint64_t(d) * N + i] = double(u[d]);
     }
@@ -428,7 +435,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     if (a.flags & 2u) {
 #pragma unroll
         for (int d = 0; d < D; ++d) a.uadj[uint64_t(d) * N + i] = double(u[d]);
-        a.chiadj[i] = (kind & KIND_ZOUHE) ? double(r0 - r1) : double(dn);
+        a.chiadj[i] = (kind & KIND_ZOUHE) ? r0d - r1d : double(dn);
     }
     if (!INIT && (a.flags & 1u) && (kind & KIND_ZOUHE)) {
 #pragma unroll
@@ -460,7 +467,11 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
         usq += u[d] * u[d];
         uF += u[d] * F[d];
     }
-    const float a0 = r0 - float(a.rho_ref);   // rest-state weights of the shifted equilibrium
+    // Rest part of the shifted equilibrium, r0 ai0 + r1 ai1 - float(s_q), as
+    // (rho - rho_0) ai0 + r1 (ai1 - ai0) + (s_q - float(s_q)): no term of
+    // order one that cancels, so FP32 keeps its digits, and its sum over q is
+    // exactly sum h (the SRT collision adds no mass).
+    const float drest = float(rtd - a.rho_ref);
     if constexpr (!MRT) {
         const float inv_tau = 1.0f / tau, cg = 1.0f - 0.5f * inv_tau;
 #pragma unroll
@@ -472,7 +483,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
                 else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
             }
             const float w = float(L::w[q]);
-            const float heq = a0 * float(a.ai0[q]) + r1 * float(a.ai1[q]) +
+            const float heq = drest * float(a.ai0[q]) + r1 * float(a.dai[q]) + float(a.rest_err[q]) +
                               rt * w * (eu * (3.0f + 4.5f * eu) - 1.5f * usq);
             h[q] += (heq - h[q]) * inv_tau;
             if (forced) h[q] += cg * w * (3.0f * (ef - uF) + 9.0f * eu * ef);
@@ -525,7 +536,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
                     if (MT.T2[k][p][b] != 0.0) meq += float(4.5 * MT.T2[k][p][b]) * u[p] * u[b];
             }
             if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
-            meq = meq * rt + a0 * float(a.mai0[k]) + r1 * float(a.mai1[k]);
+            meq = meq * rt + drest * float(a.mai0[k]) + r1 * float(a.mdai[k]) + float(a.mrest_err[k]);
             const float s = MT.rate[k] == 1 ? su : sq;
             y[k] = (s * (meq - m) + (1.0f - 0.5f * s) * mg) * float(1.0 / MT.n2[k]);
         }
@@ -568,12 +579,13 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     }
 
     // perturbation + recolouring (colorfluid.cpp:389-428).  Phase 0 takes the
-    // share r0 / rho of the total fT_q = h_q + float(s_q) of each direction;
-    // rho is taken as the sum of exactly those totals, dr + sum float(s_q),
-    // formed in FP64 (near 1 an FP32 rho has only 6e-8 resolution), so that
-    // sum_q f0_q = r0 up to unbiased rounding.  With rt (built on rho_0) the
-    // phase-0 mass would grow by a factor 1 + 1.5e-8 per step.
-    const float dens0T = float(double(r0) / (double(dr) + a.rest32));
+    // share r0 / rho of the total fT_q = h_q + float(s_q) of each direction,
+    // formed in FP64: rho is the sum of exactly those totals, so that
+    // sum_q f0_q = r0 up to unbiased rounding of the stored values (near 1 an
+    // FP32 rho has only 6e-8 resolution, and with rho_0 + sum h in place of
+    // that sum the phase-0 mass grew by a factor 1 + 1.5e-8 per step).
+    const double dens0T64 = r0d / rtd;
+    const float dens0T = float(dens0T64);
     bool finite = true;
     if (norm > 1e-9f) {
         const float b01 = float(a.beta) * dens0T * r1 * inv_rt;
@@ -591,7 +603,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
             if constexpr (!CSF) h[q] += akn * (float(L::w[q]) * angle * angle - float(a.bi[q]));
             if (L::diagonal(q)) angle *= (1.0f / SQRT2_32);
             const float feq0 = r0 * float(a.ai0[q]) + r1 * float(a.ai1[q]);
-            const float fo = dens0T * (h[q] + float(a.dist0[q])) + b01 * angle * feq0;
+            const float fo = float(fma(dens0T64, double(h[q]) + double(float(a.dist0[q])), double(b01 * angle * feq0)));
             finite = finite && isfinite(h[q]) && isfinite(fo);
             st(a.fout32, O(q) * O(a.P) + slot, h[q]);
             st(a.fout32, (O(Q) + q) * O(a.P) + slot, fo);
@@ -599,7 +611,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const float fo = dens0T * (h[q] + float(a.dist0[q]));
+            const float fo = float(dens0T64 * (double(h[q]) + double(float(a.dist0[q]))));
             finite = finite && isfinite(h[q]) && isfinite(fo);
             st(a.fout32, O(q) * O(a.P) + slot, h[q]);
             st(a.fout32, (O(Q) + q) * O(a.P) + slot, fo);
@@ -723,7 +735,7 @@ __global__ void __launch_bounds__(256) color_csf_init_f32(ColorArgs a)
         }
         const double g0 = f0[q] - 1.5 * L::w[q] * ef * r0 / rt;
         const double g1 = f1[q] - 1.5 * L::w[q] * ef * r1 / rt;
-        st(a.fout32, O(q) * P + O(slot), float(g0 + g1 - a.dist0[q]));
+        st(a.fout32, O(q) * P + O(slot), float(g0 + g1 - double(float(a.dist0[q]))));
         st(a.fout32, (O(Q) + q) * P + O(slot), float(g0));
     }
 }

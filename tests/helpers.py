@@ -51,3 +51,33 @@ def oracle_run(spec, steps, geometry=None):
 
 
 sphere_case = TC.sphere_case
+
+
+def lattice_shifts(D):
+    """The neighbour vectors of D3Q19 (D2Q9) in (z, y, x) order: components in
+    {-1, 0, 1}, one or two of them non-zero."""
+    import itertools
+    if D == 2:
+        return [(0,) + v for v in itertools.product((-1, 0, 1), repeat=2) if any(v)]
+    return [v for v in itertools.product((-1, 0, 1), repeat=3) if 0 < sum(map(abs, v)) <= 2]
+
+
+def solid_shift_map(before, after, geometry, case, solid=255):
+    """Streaming identity of solid sites (simulator.cpp:73-101: every voxel is
+    streamed, bounce-back only writes fluid sites): after one step the rest
+    population of a solid site is unchanged and its population q is the
+    previous population q of its upstream site.  Returns, for q = 1..Q-1, the
+    neighbour vectors under which plane q of `after` equals plane q of
+    `before` shifted, on all solid sites."""
+    Q, D = case['Q'], case['D']
+    shape = (1,) + tuple(case['size']) if D == 2 else tuple(case['size'])
+    mask = (np.asarray(geometry) == solid).reshape(shape)
+    a = np.asarray(before).reshape((Q,) + shape)
+    b = np.asarray(after).reshape((Q,) + shape)
+    assert np.array_equal(a[0][mask].view(np.uint64), b[0][mask].view(np.uint64))
+    out = []
+    for q in range(1, Q):
+        out.append([v for v in lattice_shifts(D)
+                    if np.array_equal(np.roll(a[q], v, axis=(0, 1, 2))[mask].view(np.uint64),
+                                      b[q][mask].view(np.uint64))])
+    return out

@@ -12,6 +12,7 @@ import pytest
 
 from lbflow_amd import cases as CS
 from lbflow_amd.engine import FP32, FP64, STEP_FIELDS, Engine
+from tests import cases as TC
 from tests.helpers import oracle_run
 
 pytestmark = pytest.mark.gpu
@@ -60,3 +61,47 @@ def test_degenerate_lattices(model, fluid_site, precision, stream_solids, dense)
             for a, b in zip(e.fields(), o.fields()):
                 if a is not None:
                     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def overflow_case():
+    """An 8^3 MRT sphere pack with one fluid site s whose +x and -x
+    neighbours are fluid, and the populations that make s overflow in the
+    collision while its density and velocity stay finite: after streaming,
+    s holds f_0 = -2X, f_1 = f_2 = X (X = 8e307), so rho and u are those of
+    its other populations, and the moment transform of the collision
+    overflows (simulator.cpp:327-333 then throws)."""
+    spec = TC.sphere_case(n=(8, 8, 8), model='scsp', eq='MRT', tau=(0.8,))
+    case = CS.lattice(spec)
+    g = CS.geometry(spec)
+    nz, ny, nx = case['size']
+    for s in np.flatnonzero(g == 75):
+        z, y, x = s // (ny * nx), (s // nx) % ny, s % nx
+        left, right = (z * ny + y) * nx + (x - 1) % nx, (z * ny + y) * nx + (x + 1) % nx
+        if g[left] == 75 and g[right] == 75:
+            return spec, case, g, s, left, right
+    raise AssertionError('no fluid site with fluid x-neighbours')
+
+
+@pytest.mark.parametrize('stream_solids', [True, False], ids=['dense', 'fluid-only'])
+def test_post_collision_population_failure(stream_solids):
+    """Only the post-collision population check fires, at the step after the
+    restart, as in the oracle."""
+    from oracle.pyoracle import Oracle
+    spec, case, g, s, left, right = overflow_case()
+    o = Oracle(case, g).init()
+    o.step(5)
+    P = o.populations().reshape(case['Q'], -1).copy()
+    X = 8e307
+    P[0, s], P[1, left], P[2, right] = -2 * X, X, X
+    e = Engine(case, g, FP64, stream_solids).init()
+    e.set_populations(P.ravel(), 5)
+    e.step(STEP_FIELDS)
+    bits, first = e.status()
+    assert bits == 0x10, hex(bits)
+    assert int(first[4]) == 6
+    with pytest.raises(Exception, match='Non-finite post-collision population'):
+        e.check_status()
+    ref = Oracle(case, g)
+    ref.set_populations(P.ravel(), 5)
+    with pytest.raises(Exception, match='Non-finite post-collision population'):
+        ref.step(1)

@@ -13,7 +13,7 @@ import pytest
 from lbflow_amd import cases as CS
 from tests import cases as TC
 from lbflow_amd.engine import FP32, FP64, STEP_ADJACENT, STEP_FIELDS, Engine
-from tests.helpers import C1_SHA256, c1_spec, darcy, oracle_run, profile, sphere_case
+from tests.helpers import C1_SHA256, c1_spec, darcy, oracle_run, profile, solid_shift_map, sphere_case
 
 pytestmark = pytest.mark.gpu
 
@@ -141,6 +141,19 @@ def test_restart_equivalence():
         res.step(STEP_FIELDS if k == 39 else 0)
     assert res.t == 80
     assert np.array_equal(res.populations(), cont.populations())
+
+
+def test_solid_entries_are_upstream_copies_gpu():
+    """The engine streams solid sites like the reference: every solid entry
+    of plane q after a step is the previous entry of the upstream site, one
+    distinct lattice vector per plane."""
+    spec = sphere_case(n=(12, 14, 16), model='scsp', eq='MRT', tau=(0.8,))
+    e, case = engine_run(spec, 50)
+    before = e.populations()
+    e.step(0)
+    shifts = solid_shift_map(before, e.populations(), CS.geometry(spec), case)
+    assert all(len(m) == 1 for m in shifts), shifts
+    assert len({m[0] for m in shifts}) == case['Q'] - 1
 
 
 def test_c1_checkpoint_digest_gpu():

@@ -152,9 +152,13 @@ def test_pure_phase_colour_reduces_to_single_phase(phase, tau, records):
 # FP32 engine against the recorded FP64 values: the largest error of each
 # quantity over the recorded steps, relative to its largest recorded
 # magnitude (vectors: over all components).
-FP32_TOLERANCE = {'single': dict(darcy_velocity=2e-5, total_mass=2e-7, density_range=2e-7, max_speed=5e-5),
-                  'two': dict(darcy_velocity=0.25, total_mass=2e-7, density_range=1e-3, max_speed=0.25,
-                              phase_masses=1e-5, phase_saturations=1e-5, diffuse_pore_fraction=0.05)}
+# Single-phase and pure-phase runs track FP64 closely; the two-phase runs
+# are transient interface motion, whose Darcy velocity and peak speed follow
+# FP64 to a few per cent over 16 000 steps while the phase masses and
+# saturations stay within a few 1e-6.
+FP32_TOLERANCE = {'single': dict(darcy_velocity=5e-6, total_mass=1e-9, density_range=1e-9, max_speed=5e-6),
+                  'two': dict(darcy_velocity=0.1, total_mass=2e-7, density_range=2e-4, max_speed=0.1,
+                              phase_masses=5e-6, phase_saturations=5e-6, diffuse_pore_fraction=1e-2)}
 
 
 @pytest.mark.parametrize('spec', ROCK, ids=TC.names(ROCK))

@@ -14,7 +14,8 @@ import pytest
 
 from lbflow_amd import cases as CS
 from tests import cases as TC
-from tests.helpers import C1_SHA256, c1_profile_error, c1_spec, darcy, oracle_run, profile, velocity_grid
+from tests.helpers import (C1_SHA256, c1_profile_error, c1_spec, darcy, oracle_run, profile, solid_shift_map,
+                           sphere_case, velocity_grid)
 
 
 @pytest.mark.parametrize('spec', TC.reference('force'), ids=lambda s: s['name'])
@@ -139,6 +140,27 @@ def test_c1_profile_closed_form():
     without the slip term it would be off by 5e-3."""
     o, case = oracle_run(c1_spec(), 20000)
     assert c1_profile_error(profile(o.fields()[2], case)) < 1e-10
+
+
+def test_solid_entries_are_upstream_copies():
+    """Solid-site checkpoint bytes follow from streaming alone: on a sphere
+    pack every solid entry of plane q after a step is the previous entry of
+    the upstream site along one lattice vector, a different vector per plane;
+    with those vectors the same holds for the 8x10x4 channel's solid rows at
+    step 20000, whose checkpoint digest the reference recorded."""
+    pack = sphere_case(n=(12, 14, 16), model='scsp', eq='MRT', tau=(0.8,))
+    o, case = oracle_run(pack, 50)
+    before = o.populations()
+    o.step(1)
+    shifts = solid_shift_map(before, o.populations(), CS.geometry(pack), case)
+    assert all(len(m) == 1 for m in shifts), shifts
+    vectors = [m[0] for m in shifts]
+    assert len(set(vectors)) == case['Q'] - 1
+    c1, ccase = oracle_run(c1_spec(), 19999)
+    before = c1.populations()
+    c1.step(1)
+    on_c1 = solid_shift_map(before, c1.populations(), CS.geometry(c1_spec()), ccase)
+    assert all(v in m for v, m in zip(vectors, on_c1))
 
 
 def test_c1_checkpoint_digest():
