@@ -186,6 +186,18 @@ int lbf_set_populations(lbf_ctx* ctx, const double* in, uint64_t t);
 /* Device reductions of one report (see lbf_report_req). */
 int lbf_report(lbf_ctx* ctx, const lbf_report_req* req, lbf_report_out* out);
 
+/* lbf_report without the context's reduction buffers: no counterpart in the
+ * reference.  For callers that cannot allocate device memory at call time
+ * (serving loops, harnesses that forbid hipMalloc): the block partials and
+ * the sums go to a static device array of 4096 x 16 + 16 doubles (512 KiB
+ * per GPU, held until the process ends), which covers the at most 4096
+ * reduction blocks of any lattice; a larger need is refused with an error.
+ * It calls no allocating routine, and gives bitwise the sums lbf_report
+ * gives on the same state.  Every call on one GPU shares that array and
+ * overwrites it, so calls that overlap on one GPU (from several threads or
+ * contexts) race; the context's GPU must be the current device. */
+int lbf_report_static(lbf_ctx* ctx, const lbf_report_req* req, lbf_report_out* out);
+
 /* --- introspection (parity tests) --------------------------------------
  * linkmask[site]: bit q set iff the site is in ibS[q]; wall[site] = ibI;
  * normal/sin/cos per wall entry (entry 0 unused).  Returns the number of

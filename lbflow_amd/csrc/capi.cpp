@@ -26,6 +26,7 @@ hipError_t launch_color(int Q, bool mrt, bool init, bool stream_solids, const Co
 hipError_t launch_color32(int Q, bool mrt, bool init, bool sparse, const ColorArgs& a, hipStream_t s);
 hipError_t launch_report(const ReportArgs& a, double* out, hipStream_t s);
 uint32_t report_blocks(uint64_t N);
+hipError_t report_scratch(double** ptr, uint64_t* capacity);
 hipError_t launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 }  // namespace lbf
 
@@ -42,6 +43,7 @@ struct lbf_ctx {
     std::vector<double> shift;      // FP32: per plane shift (phase*Q + q)
     std::vector<uint32_t> list;     // SPARSE: fluid site of each slot
     hipStream_t stream = nullptr;
+    int device = 0;                 // the GPU current at lbf_create, which holds every buffer
     void* f[2] = {nullptr, nullptr};
     int cur = 0;
     int layout = 0;                 // 0 dense (all sites streamed), 1 dense fluid-only, 2 sparse
@@ -355,6 +357,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         lbf_destroy(c);
         return fail(nullptr, "HIP error: cannot create a stream (no GPU?)");
     }
+    (void)hipGetDevice(&c->device);
     const size_t pop = size_t(NP) * d.Q * c->P * c->elem;
     const size_t Q2 = size_t(d.Q) * d.Q;
     int rc = 0;
@@ -921,6 +924,22 @@ int lbf_report(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
 {
     if (!c || !req || !out) return -1;
     return report_into(c, req, out, c->d_partial, c->d_out);
+}
+
+int lbf_report_static(lbf_ctx* c, const lbf_report_req* req, lbf_report_out* out)
+{
+    if (!c || !req || !out) return -1;
+    int dev = 0;
+    HIP_CHECK(c, hipGetDevice(&dev));
+    if (dev != c->device) return fail(c, "lbf_report_static: the context's GPU is not the current device");
+    double* scratch = nullptr;
+    uint64_t capacity = 0;
+    HIP_CHECK(c, report_scratch(&scratch, &capacity));
+    const uint64_t blocks = report_blocks(c->N);
+    if (blocks * NREP + NREP > capacity)
+        return fail(c, "lbf_report_static: " + std::to_string(blocks) + " partial blocks exceed the static scratch of " +
+                           std::to_string(capacity) + " doubles");
+    return report_into(c, req, out, scratch, scratch + blocks * NREP);
 }
 
 uint64_t lbf_links(lbf_ctx* c, uint32_t* linkmask, int32_t* wall, double* normal, double* sinca, double* cosca)

@@ -8,9 +8,23 @@
 // fixed order, so the result is deterministic run to run.  Summation order
 // differs from the reference's sequential loop, so sums agree with it to
 // rounding (about 1e-15 relative), not bit for bit.
+#include <mutex>
+#include <vector>
+
 #include "device.h"
 
 namespace lbf {
+
+// At most this many partial blocks per report: the block partials of any
+// lattice fit [REPORT_MAX_BLOCKS][NREP] doubles.
+constexpr uint32_t REPORT_MAX_BLOCKS = 4096;
+
+// Device memory of lbf_report_static: the block partials followed by the
+// final sums, [REPORT_MAX_BLOCKS][NREP] + [NREP] doubles (512 KiB), one copy
+// per GPU for the life of the process.  Host code never uses this name as an
+// address: report_scratch() looks up each GPU's copy.
+constexpr uint64_t REPORT_SCRATCH_DOUBLES = uint64_t(REPORT_MAX_BLOCKS) * NREP + NREP;
+static __device__ __align__(256) double scratch[REPORT_SCRATCH_DOUBLES];
 
 enum Slot {
     S_VEL = 0, S_RHO_R0, S_RHO_R1, S_FIRST0, S_FIRST1, S_LAST0, S_LAST1, S_CNT_FIRST, S_CNT_LAST,
@@ -139,7 +153,30 @@ __global__ void __launch_bounds__(256) report_final(const double* partial, uint3
 uint32_t report_blocks(uint64_t N)
 {
     const uint64_t want = (N + 255) / 256;
-    return uint32_t(want < 4096 ? want : 4096);
+    return uint32_t(want < REPORT_MAX_BLOCKS ? want : REPORT_MAX_BLOCKS);
+}
+
+// Device address of the current GPU's copy of `scratch` and its size in
+// doubles.  The address is looked up once per GPU (hipGetSymbolAddress with
+// that GPU current) and kept.
+hipError_t report_scratch(double** ptr, uint64_t* capacity)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    static std::mutex lock;
+    static std::vector<double*> address;
+    std::lock_guard<std::mutex> hold(lock);
+    if (size_t(dev) >= address.size()) address.resize(size_t(dev) + 1, nullptr);
+    if (!address[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        address[dev] = static_cast<double*>(p);
+    }
+    *ptr = address[dev];
+    *capacity = REPORT_SCRATCH_DOUBLES;
+    return hipSuccess;
 }
 
 hipError_t launch_report(const ReportArgs& a, double* out, hipStream_t s)

@@ -21,7 +21,7 @@ STATUS_BITS = (0x1, 0x2, 0x4, 0x8, 0x10, 0x20)
 
 SYMBOLS = ('lbf_create', 'lbf_destroy', 'lbf_last_error', 'lbf_init', 'lbf_step', 'lbf_advance',
            'lbf_time', 'lbf_synchronize', 'lbf_status', 'lbf_status_message', 'lbf_fields',
-           'lbf_get_populations', 'lbf_set_populations', 'lbf_report', 'lbf_links', 'lbf_sites',
+           'lbf_get_populations', 'lbf_set_populations', 'lbf_report', 'lbf_report_static', 'lbf_links', 'lbf_sites',
            'lbf_device_bytes', 'lbf_step_kernel', 'lbf_step_bytes', 'lbf_timed_advance',
            'lbf_copy_bandwidth', 'lbf_batch_create', 'lbf_batch_destroy', 'lbf_batch_error',
            'lbf_batch_sites', 'lbf_batch_tune', 'lbf_batch_init', 'lbf_batch_advance', 'lbf_batch_run',
@@ -105,8 +105,9 @@ def load_library(path=None):
     L.lbf_get_populations.restype = i32
     L.lbf_set_populations.argtypes = [vp, vp, u64]
     L.lbf_set_populations.restype = i32
-    L.lbf_report.argtypes = [vp, C.POINTER(ReportReq), C.POINTER(ReportOut)]
-    L.lbf_report.restype = i32
+    for name in ('lbf_report', 'lbf_report_static'):
+        getattr(L, name).argtypes = [vp, C.POINTER(ReportReq), C.POINTER(ReportOut)]
+        getattr(L, name).restype = i32
     L.lbf_links.argtypes = [vp, vp, vp, vp, vp, vp]
     L.lbf_links.restype = u64
     L.lbf_sites.argtypes = [vp]
@@ -270,11 +271,14 @@ class Engine:
         pops = np.ascontiguousarray(pops, dtype=np.float64)
         self._check(self.L.lbf_set_populations(self.h, _ptr(pops), t))
 
-    def report(self, axis, first, last, convergence=True, adjacent=True):
+    def report(self, axis, first, last, convergence=True, adjacent=True, static=False):
+        """One report's device reductions; static=True uses lbf_report_static
+        (no per-context buffers)."""
         req = ReportReq(axis=axis, convergence=int(convergence), adjacent=int(adjacent),
                         first=first, last=last)
         out = ReportOut()
-        self._check(self.L.lbf_report(self.h, C.byref(req), C.byref(out)))
+        fn = self.L.lbf_report_static if static else self.L.lbf_report
+        self._check(fn(self.h, C.byref(req), C.byref(out)))
         return out
 
     def links(self):
