@@ -285,6 +285,9 @@ uint64_t lbf_batch_time(const lbf_batch* batch);
  * (reference axis order), fields[site][D] = rho, over all crop voxels
  * (solids 0).  `fields` is DEVICE memory of prod(sz) * (D+1) floats. */
 int lbf_batch_crop_fields(lbf_batch* batch, uint32_t c, float* fields);
+/* Sizes of crop c (slowest axis first) into sz[0..D-1]; returns D, or 0 for
+ * a crop that does not exist. */
+int lbf_batch_crop_size(const lbf_batch* batch, uint32_t c, uint64_t* sz);
 /* Warm start: set crop c to the equilibrium of the given fields (same layout,
  * DEVICE memory), shifted by half the body force so that the first macro
  * step returns u; the FP32 counterpart of SCSP::initialize (scsp.cpp:116-134)
@@ -325,6 +328,15 @@ uint64_t lbf_net_param_count(const lbf_net* net);
 int lbf_net_get_params(lbf_net* net, float* host);
 int lbf_net_set_params(lbf_net* net, const float* host);
 int lbf_net_forward(lbf_net* net, const uint8_t* mask, uint32_t count, float* out);
+/* Warm start of the first `count` crops of a batch from the network (no
+ * reference counterpart).  masks: HOST memory, count x side^3 bytes, crop
+ * order, z,y,x with x fastest, 1 = pore; every one of those crops must be a
+ * side^3 D3Q19 crop.  `capacity` crops at a time go through the network;
+ * each prediction p becomes fields f_c = p_c scale[c] + offset[c] (the
+ * velocity channels 0 on solid voxels), loaded with lbf_batch_set_fields,
+ * then `relax` sweeps of lbf_batch_relax.  Call after lbf_batch_init. */
+int lbf_batch_warm_start(lbf_batch* batch, lbf_net* net, const uint8_t* masks, uint32_t count,
+                         const float* scale /* 4 */, const float* offset /* 4 */, uint32_t relax);
 /* Loss (mean over sites of sum_c w_c (out_c - target_c)^2) and its gradient
  * in the parameter layout (host memory, may be NULL); no update. */
 int lbf_net_gradients(lbf_net* net, const uint8_t* mask, const float* target, uint32_t count,

@@ -600,6 +600,15 @@ int lbf_batch_run(lbf_batch* b, int axis, double tol, uint32_t interval, uint64_
     return 0;
 }
 
+int lbf_batch_crop_size(const lbf_batch* b, uint32_t c, uint64_t* sz)
+{
+    if (!b || !sz || c >= b->ncrops) return 0;
+    const GridDev& g = b->crops[c].g;
+    const uint64_t n[3] = {g.nz, g.ny, g.nx};
+    for (int k = 0; k < b->D; ++k) sz[k] = n[3 - b->D + k];
+    return b->D;
+}
+
 int lbf_batch_crop_fields(lbf_batch* b, uint32_t c, float* fields)
 {
     if (!b || !fields || c >= b->ncrops) return -1;

@@ -268,6 +268,23 @@ __global__ void net_head4(const float* __restrict__ out16, uint32_t nvox, float*
     out4[i] = out16[size_t(i >> 2) * 16 + (i & 3u)];
 }
 
+// Physical fields of a prediction: out4[v][c] = out16[v][c] scale[c] +
+// offset[c], the velocity channels (c < 3) zero where mask[v] is 0 (solid).
+__global__ void net_head_fields(const float* __restrict__ out16, const uint8_t* __restrict__ mask, uint32_t nvox,
+                                float4 scale, float4 offset, float* __restrict__ out4)
+{
+    const uint32_t v = blockIdx.x * 256u + threadIdx.x;
+    if (v >= nvox) return;
+    const float* p = out16 + size_t(v) * 16;
+    const float m = mask[v] ? 1.0f : 0.0f;
+    float4 r;
+    r.x = m * (p[0] * scale.x + offset.x);
+    r.y = m * (p[1] * scale.y + offset.y);
+    r.z = m * (p[2] * scale.z + offset.z);
+    r.w = p[3] * scale.w + offset.w;
+    reinterpret_cast<float4*>(out4)[v] = r;
+}
+
 // FP32 -> BF16 copy.
 __global__ void net_to_bf16(const float* __restrict__ p, uint32_t n, __bf16* __restrict__ pb)
 {
@@ -356,6 +373,14 @@ hipError_t launch_net_adam(float* p, float* m, float* v, const float* g, uint32_
 hipError_t launch_net_head4(const float* out16, uint32_t nvox, float* out4, hipStream_t s)
 {
     hipLaunchKernelGGL(net_head4, grid_for(uint64_t(nvox) * 4), dim3(256), 0, s, out16, nvox, out4);
+    return hipGetLastError();
+}
+hipError_t launch_net_head_fields(const float* out16, const uint8_t* mask, uint32_t nvox, const float* scale,
+                                  const float* offset, float* out4, hipStream_t s)
+{
+    const float4 sc = make_float4(scale[0], scale[1], scale[2], scale[3]);
+    const float4 of = make_float4(offset[0], offset[1], offset[2], offset[3]);
+    hipLaunchKernelGGL(net_head_fields, grid_for(nvox), dim3(256), 0, s, out16, mask, nvox, sc, of, out4);
     return hipGetLastError();
 }
 hipError_t launch_net_to_bf16(const float* p, uint32_t n, __bf16* pb, hipStream_t s)

@@ -46,6 +46,8 @@ hipError_t launch_net_bias_grad(Chan dz, uint32_t C, uint32_t nvox, float* parti
 hipError_t launch_net_adam(float* p, float* m, float* v, const float* g, uint32_t n, float lr, float b1, float b2,
                            float eps, float c1, float c2, __bf16* pb, hipStream_t s);
 hipError_t launch_net_head4(const float* out16, uint32_t nvox, float* out4, hipStream_t s);
+hipError_t launch_net_head_fields(const float* out16, const uint8_t* mask, uint32_t nvox, const float* scale,
+                                  const float* offset, float* out4, hipStream_t s);
 hipError_t launch_net_to_bf16(const float* p, uint32_t n, __bf16* pb, hipStream_t s);
 hipError_t launch_net_transpose_w(const __bf16* w, uint32_t cin, uint32_t cout, uint32_t taps, uint32_t kp,
                                   uint32_t kpt, __bf16* wt, hipStream_t s);
@@ -91,6 +93,9 @@ struct lbf_net {
     float *wpart, *bpart;
     double *lpart, *loss;
     size_t wpart_floats = 0;
+    // warm starts: capacity pore masks and their physical fields, allocated on first use
+    uint8_t* warm_mask = nullptr;
+    float* warm_fields = nullptr;
 };
 
 namespace {
@@ -429,6 +434,36 @@ int lbf_net_forward(lbf_net* n, const uint8_t* mask, uint32_t count, float* out)
     if (forward(n, mask, count)) return -1;
     NHIP(n, launch_net_head4(n->out16, nvox_at(n, 0, count), out, n->stream));
     NHIP(n, hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+int lbf_batch_warm_start(lbf_batch* b, lbf_net* n, const uint8_t* masks, uint32_t count, const float* scale,
+                         const float* offset, uint32_t relax)
+{
+    if (!b || !n || !masks || !scale || !offset) return -1;
+    const uint64_t s3 = uint64_t(n->side) * n->side * n->side;
+    for (uint32_t c = 0; c < count; ++c) {
+        uint64_t sz[3] = {0, 0, 0};
+        if (lbf_batch_crop_size(b, c, sz) != 3 || sz[0] != n->side || sz[1] != n->side || sz[2] != n->side)
+            return nfail(n, "lbf_batch_warm_start: crop " + std::to_string(c) + " is not a " +
+                                std::to_string(n->side) + "^3 D3Q19 crop of the batch");
+    }
+    if (!n->warm_mask && (nalloc(n, &n->warm_mask, size_t(n->cap) * s3) ||
+                          nalloc(n, &n->warm_fields, size_t(n->cap) * s3 * 4)))
+        return -1;
+    for (uint32_t c0 = 0; c0 < count; c0 += n->cap) {
+        const uint32_t k = std::min(n->cap, count - c0);
+        NHIP(n, hipMemcpyAsync(n->warm_mask, masks + size_t(c0) * s3, size_t(k) * s3, hipMemcpyHostToDevice,
+                               n->stream));
+        if (forward(n, n->warm_mask, k)) return -1;
+        NHIP(n, launch_net_head_fields(n->out16, n->warm_mask, uint32_t(k * s3), scale, offset, n->warm_fields,
+                                       n->stream));
+        NHIP(n, hipStreamSynchronize(n->stream));
+        for (uint32_t j = 0; j < k; ++j)
+            if (lbf_batch_set_fields(b, c0 + j, n->warm_fields + size_t(j) * s3 * 4))
+                return nfail(n, std::string("lbf_batch_warm_start: ") + lbf_batch_error(b));
+    }
+    if (relax && lbf_batch_relax(b, relax)) return nfail(n, std::string("lbf_batch_warm_start: ") + lbf_batch_error(b));
     return 0;
 }
 

@@ -10,8 +10,11 @@ poor prediction costs iterations, never correctness.
 
 Pipeline: generate sphere-pack crops -> solve them to steady state with the
 batched engine -> train the network on (mask, fields) -> evaluate cold vs
-warm starts on unseen crops.  Device memory is held in torch tensors
-(plumbing only); all arithmetic runs in the engine's HIP kernels.
+warm starts on unseen crops.  A warm start (run_crops) runs entirely in
+the engine's library: masks go in from host memory and the prediction stays
+in the network's device buffers (lbf_batch_warm_start).  Training keeps its
+data set in torch tensors (device memory only); all arithmetic of the
+network runs in the engine's HIP kernels.
 """
 import ctypes as C
 import json
@@ -65,6 +68,8 @@ def _bind(L):
     L.lbf_batch_set_fields.restype = i32
     L.lbf_batch_relax.argtypes = [vp, u32]
     L.lbf_batch_relax.restype = i32
+    L.lbf_batch_warm_start.argtypes = [vp, vp, vp, u32, vp, vp, u32]
+    L.lbf_batch_warm_start.restype = i32
     L._net_bound = True
     return L
 
@@ -87,10 +92,10 @@ def _torch():
 
 
 class Net:
-    """The warm-start network on the GPU (device buffers are torch CUDA tensors)."""
+    """The warm-start network on the GPU.  forward/gradients/train_step take
+    torch CUDA tensors; warm starts (run_crops) need no torch."""
 
     def __init__(self, side=64, capacity=4, seed=1):
-        _torch()
         self.L = lib()
         h = C.c_void_p()
         if self.L.lbf_net_create(side, capacity, seed, C.byref(h)):
@@ -255,6 +260,13 @@ class Scaling:
         out[..., :3] *= m
         return out
 
+    def affine(self, tau, force):
+        """(scale, offset) of Scaling.fields as per-channel factors, for
+        lbf_batch_warm_start: f_c = p_c scale_c + offset_c."""
+        nu = (tau - 0.5) / 3.0
+        scale = [self.rms[c] * force / nu for c in range(3)] + [self.rms[3] * force]
+        return np.asarray(scale, np.float32), np.asarray([0.0, 0.0, 0.0, 1.0], np.float32)
+
     def to_json(self):
         return dict(rms=self.rms.tolist(), tau=(self.nu * 3.0 + 0.5), force=self.force)
 
@@ -356,32 +368,21 @@ def load(path, capacity=4, side=None):
 # ---------------------------------------------------------------------------
 def run_crops(crops, tau, force, tol, interval, max_steps, net=None, scaling=None, relax=0):
     """Runs all crops to the stop rule in one batch, cold (rest state) or warm
-    (network prediction).  Returns (results, seconds from init to the stop,
-    inference seconds)."""
-    torch = _torch()
+    (network prediction, lbf_batch_warm_start).  Returns (results, seconds
+    from init to the stop, inference seconds)."""
     b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), crops)
     b.tune(steps=4)
     L = lib()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     b.init()
     t_inf = 0.0
     if net is not None:
         t1 = time.perf_counter()
-        side = crops[0].shape[0]
-        masks = torch.as_tensor(np.stack([(c == PORE) for c in crops]).astype(np.uint8), device='cuda')
-        pred = torch.empty((net.capacity, side ** 3, 4), dtype=torch.float32, device='cuda')
-        for i in range(0, len(crops), net.capacity):
-            m = masks[i:i + net.capacity].contiguous()
-            p = pred[:m.shape[0]]
-            net.forward(m, p)
-            f = scaling.fields(p, tau, force, m).contiguous()
-            torch.cuda.synchronize()
-            for j in range(m.shape[0]):
-                if L.lbf_batch_set_fields(b.h, i + j, C.c_void_p(f[j].data_ptr())):
-                    raise EngineError(L.lbf_batch_error(b.h).decode())
-        if relax and L.lbf_batch_relax(b.h, relax):
-            raise EngineError(L.lbf_batch_error(b.h).decode())
+        masks = np.ascontiguousarray(np.stack([(c == PORE) for c in crops]).astype(np.uint8))
+        scale, offset = scaling.affine(tau, force)
+        if L.lbf_batch_warm_start(b.h, net.h, masks.ctypes.data_as(C.c_void_p), len(crops),
+                                  scale.ctypes.data_as(C.c_void_p), offset.ctypes.data_as(C.c_void_p), relax):
+            raise EngineError(L.lbf_net_error(net.h).decode())
         t_inf = time.perf_counter() - t1
     res, _ = b.run(axis=2, tol=tol, interval=interval, max_steps=max_steps)
     sec = time.perf_counter() - t0

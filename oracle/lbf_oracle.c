@@ -596,14 +596,18 @@ static void scsp_initialize(orc_sim* s)
 static int scsp_macro(orc_sim* s)
 {
     const double dens0 = s->d.density[0];
+    int bad = 0;
     for (int l = 0; l < 256; ++l) {
         if (!label_in(s, l, ORC_VD | ORC_GV)) continue;
+        /* sites are independent: threads change no result, only which sites
+         * are updated before a failure is reported */
+#pragma omp parallel for schedule(static) reduction(| : bad)
         for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
             uint64_t i = s->bylabel[k];
             double r = F(s, 0, 0, i);
             for (int q = 1; q < s->Q; ++q) r += F(s, 0, q, i);
             s->dens[0][i] = r;
-            if (!isfinite(r) || r <= 0.0) return fail(s, "Non-finite or non-positive fluid density.");
+            if (!isfinite(r) || r <= 0.0) bad = 1;
             for (int d = 0; d < s->D; ++d) {
                 double u = 0;
                 for (int q = 1; q < s->Q; ++q) {
@@ -614,6 +618,7 @@ static int scsp_macro(orc_sim* s)
                 V(s, d, i) = u;
             }
         }
+        if (bad) return fail(s, "Non-finite or non-positive fluid density.");
     }
     return 0;
 }
@@ -628,6 +633,7 @@ static void scsp_body_force(orc_sim* s)
         force_for_label(s, l, force);
         for (int d = 0; d < s->D; ++d) fi[d] = force[d] * 0.5;
         for (int d = 0; d < s->D; ++d)
+#pragma omp parallel for schedule(static)
             for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k)
                 V(s, d, s->bylabel[k]) += fi[d] / rhoRef;
     }
@@ -706,9 +712,11 @@ static void scsp_zouhe(orc_sim* s)
 static int validate(orc_sim* s, int ph)
 {
     const uint64_t n = s->N * s->Q;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
     for (uint64_t k = 0; k < n; ++k)
-        if (!isfinite(s->f[ph][k]))
-            return fail(s, "Non-finite post-collision population; no restart will be written.");
+        if (!isfinite(s->f[ph][k])) bad = 1;
+    if (bad) return fail(s, "Non-finite post-collision population; no restart will be written.");
     return 0;
 }
 
@@ -727,6 +735,7 @@ static int scsp_collision(orc_sim* s)
         double ff2 = 0.0;
         for (int d = 0; d < s->D; ++d) ff2 = ff2 + force[d] * force[d];
         const int forced = ff2 != 0.0;
+#pragma omp parallel for schedule(static)
         for (uint64_t k = s->start[l]; k < s->start[l + 1]; ++k) {
             uint64_t i = s->bylabel[k];
             double usq = 0, src[MAXQ];
@@ -1253,15 +1262,18 @@ static void stream_and_bounce(orc_sim* s, int ph)
     for (int q = 1; q < s->Q; ++q) {
         double* src = s->f[ph] + (uint64_t)q * N;
         double* dst = s->tmp[ph] + (uint64_t)q * N;
+#pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < N; ++i) dst[s->adj[i * s->Q + q]] = src[i];
     }
     for (int q = 1; q < s->Q; ++q) {   /* swap f[q] <-> fOld[q] */
         double* a = s->f[ph] + (uint64_t)q * N;
         double* b = s->tmp[ph] + (uint64_t)q * N;
+#pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < N; ++i) { double t = a[i]; a[i] = b[i]; b[i] = t; }
     }
     for (int q = 1; q < s->Q; ++q) {
         const int opq = s->op[q];
+#pragma omp parallel for schedule(static)
         for (uint64_t k = 0; k < s->nibs[q]; ++k) {
             const uint64_t i = s->ibs[q][k];
             s->f[ph][(uint64_t)opq * N + i] = s->tmp[ph][(uint64_t)q * N + i];

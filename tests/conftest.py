@@ -1,6 +1,12 @@
 import json
+import os
 import sys
 from pathlib import Path
+
+# the CPU oracle runs its single-phase step loops on OpenMP threads; parallel
+# test workers each keep to one
+if 'PYTEST_XDIST_WORKER' in os.environ:
+    os.environ.setdefault('OMP_NUM_THREADS', '1')
 
 import pytest
 

@@ -5,7 +5,8 @@ Uses the network trained on the GPU by `python -m lbflow_amd.surrogate`
 lbflow_amd/weights/warm64.npz.  On unseen crops (seeds 20000+i) the same
 batched solver runs to the same stop rule (Simulator::updateConvergence,
 simulator.cpp:248-325, tolerance 1e-3) from rest and from the network's
-prediction; the warm runs must need fewer iterations, and their
+prediction; the warm runs must need fewer iterations and less wall clock
+(inference included), by the floors below, and their
 permeability must be as close to the tightly converged one (tolerance 1e-5)
 as the cold runs' is: a stop at residual 1e-3 leaves K uncertain by about
 1e-3 either way, so the warm K is checked against the tight K with the cold
@@ -46,6 +47,9 @@ def test_warm_start_beats_cold_start():
     say('K error vs tight: cold', ec, 'warm', ew)
     say('cold s', t_cold, 'warm s', t_warm, 'inference', t_inf)
     assert all(r['converged'] for r in cold + warm + tight)
-    assert sw.sum() < sc.sum()
+    # measured on these 8 crops: 1.98x fewer iterations, 1.29x less wall
+    # clock with the inference and the relaxation sweeps counted
+    assert sc.sum() / sw.sum() >= 1.6
+    assert t_cold / t_warm >= 1.1
     assert ew.mean() <= 1.5 * ec.mean() + 1e-4
     assert ew.max() < 2e-2
