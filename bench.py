@@ -170,12 +170,15 @@ def batch_bench(crops, interval, max_steps):
     """Batched permeability: independent cubic sphere-pack crops (side 32-96,
     porosity 0.15-0.35, seeds 1000+i), D3Q19 MRT FP32, tau 0.8, F 1e-6 along x,
     every crop run to residual 1e-5 in one persistent hipGraph-captured launch
-    sequence; the autotuner picks the persistent grid first."""
+    sequence; the autotuner first picks the launch shape (threads per
+    workgroup, slots per work unit) and the persistent grid."""
     from lbflow_amd.engine import Batch
     labels = {75: ('GV', 1e-6, 1, 0), 255: ('SD', 90.0, 0, 0)}
     b = Batch(3, 19, 'MRT', 0.8, 1.0, labels, crops)
     per_cu, tune_ms = b.tune(steps=10)
-    log(f'batch: {b.sites} sites, {b.fluid} fluid, tuned {per_cu} workgroups/CU, {tune_ms:.3f} ms/step')
+    threads, unit, grid = b.shape()
+    log(f'batch: {b.sites} sites, {b.fluid} fluid, tuned {threads} threads x {unit}-slot units, '
+        f'{per_cu} workgroups/CU, {tune_ms:.3f} ms/step')
     t0 = time.perf_counter()
     b.init()
     res, _ = b.run(axis=2, tol=1e-5, interval=interval, max_steps=max_steps)
@@ -187,7 +190,8 @@ def batch_bench(crops, interval, max_steps):
                crops_per_s=round(len(crops) / wall, 3), fluid_mlups=round(fluid_updates / wall / 1e6, 1),
                site_mlups=round(site_updates / wall / 1e6, 1), sites=b.sites, fluid_sites=b.fluid,
                steps_min=min(steps), steps_max=max(steps), steps_mean=round(sum(steps) / len(steps), 1),
-               report_interval=interval, tuned_workgroups_per_cu=per_cu, all_crops_ms_per_step=round(tune_ms, 4),
+               report_interval=interval, tuned_workgroups_per_cu=per_cu, tuned_threads_per_workgroup=threads,
+               tuned_slots_per_unit=unit, all_crops_ms_per_step=round(tune_ms, 4),
                workload=f'{len(crops)} cubic sphere-pack crops, side U{{32..96}}, porosity U[0.15,0.35], '
                         'seeds 1000+i, D3Q19 MRT FP32, tau 0.8, F 1e-6 x, residual 1e-5')
     b.close()

@@ -264,10 +264,15 @@ void lbf_batch_destroy(lbf_batch* batch);
 const char* lbf_batch_error(const lbf_batch* batch);  /* batch may be NULL: last create error */
 /* Total voxels of all crops; *fluid (if not NULL) = fluid sites. */
 uint64_t lbf_batch_sites(const lbf_batch* batch, uint64_t* fluid);
-/* Times `steps` steps for each persistent grid size (1..128 workgroups per
- * CU) and keeps the fastest.  Steps the current state: call before
- * lbf_batch_init. */
+/* Times `steps` steps for each launch shape of the persistent step (64, 128
+ * or 256 threads per workgroup, work units of 64, 128 or 256 slots) and each
+ * persistent grid size (1..128 workgroups per CU), and keeps the fastest.
+ * Steps the current state: call before lbf_batch_init.  Every shape gives
+ * bitwise the same trajectory. */
 int lbf_batch_tune(lbf_batch* batch, uint32_t steps, uint32_t* per_cu, double* ms_per_step);
+/* The step's launch shape in use: threads per workgroup, slots per work unit
+ * and workgroups of the persistent grid (any pointer may be NULL). */
+int lbf_batch_shape(const lbf_batch* batch, uint32_t* threads, uint32_t* unit, uint32_t* workgroups);
 /* t = 0 state of every crop (SCSP::Run initial branch, scsp.cpp:60-72). */
 int lbf_batch_init(lbf_batch* batch);
 /* n steps of every crop, no convergence bookkeeping. */

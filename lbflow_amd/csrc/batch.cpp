@@ -7,6 +7,7 @@
 // and adjacent residual below the tolerance for three consecutive reports,
 // Simulator::updateConvergence, simulator.cpp:248-325) and drops converged
 // crops from the device-resident active-tile list.  The autotuner times the
+// step's launch shape (threads per workgroup, slots per work unit) and
 // persistent grid size (workgroups per CU) on the actual batch.
 #include <hip/hip_runtime.h>
 
@@ -23,7 +24,7 @@
 #include "setup.h"
 
 namespace lbf {
-hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, const BatchParams& p,
+hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, BatchShape sh, const BatchParams& p,
                              const BatchBuffers& b, hipStream_t s);
 hipError_t launch_batch_fields(int Q, uint64_t S, float rho0, uint32_t tile_begin, uint32_t ntiles,
                                const uint32_t* site, const float* f, const float* ucur, float* fields, hipStream_t s);
@@ -55,6 +56,7 @@ struct lbf_batch {
     CropDev* d_crops = nullptr;
     uint32_t* d_site = nullptr;
     uint8_t* d_lab8 = nullptr;
+    uint32_t* d_desc = nullptr;      // label and bounce-back links per slot (kernels_batch.hip)
     uint8_t* d_flow8 = nullptr;      // 1 on fluid sites of clusters that carry the steady flow
     uint2* d_rank = nullptr;
     LabelF* d_lab = nullptr;
@@ -64,6 +66,8 @@ struct lbf_batch {
     double* h_out = nullptr;         // pinned host copy of d_out, read after each graph replay
     std::vector<void*> allocations;
     uint32_t grid = 0, cus = 0;
+    uint32_t report_grid = 0;        // persistent grid of the report reduction (256-thread tiles)
+    BatchShape shape;                // launch shape of the step (autotuned)
     std::string err;
     uint64_t t = 0;
 };
@@ -123,6 +127,7 @@ BatchBuffers buffers(lbf_batch* b, bool swap)
     x.crops = b->d_crops;
     x.site = b->d_site;
     x.lab8 = b->d_lab8;
+    x.desc = b->d_desc;
     x.rank = b->d_rank;
     x.lab = b->d_lab;
     x.ucur = b->d_ucur;
@@ -159,7 +164,8 @@ int set_active(lbf_batch* b, const std::vector<char>& active)
 int step(lbf_batch* b, uint32_t flags, bool init)
 {
     const BatchParams p = params(b, flags);
-    hipError_t e = launch_batch_step(b->Q, b->d.equil == LBF_MRT, init, b->grid, p, buffers(b, !init), b->stream);
+    hipError_t e = launch_batch_step(b->Q, b->d.equil == LBF_MRT, init, init ? b->report_grid : b->grid, b->shape, p,
+                                     buffers(b, !init), b->stream);
     if (e != hipSuccess) return bfail(b, std::string("HIP error: ") + hipGetErrorString(e));
     if (!init) {
         b->cur ^= 1;
@@ -311,6 +317,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     // slot space
     std::vector<uint32_t> site;
     std::vector<uint8_t> lab8, flow8;
+    std::vector<uint32_t> descs;
     std::vector<uint2> rank;
     b->crops.resize(d.ncrops);
     b->crop_sites.resize(d.ncrops);
@@ -364,10 +371,32 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
             rank[w].x = before;
             before += uint32_t(__builtin_popcount(rank[w].y));
         }
+        // per fluid slot: its label and bit op(q) where the upstream site
+        // x - e_q (periodic in the crop) is solid
+        for (uint64_t k = cd.slot_base; k < site.size(); ++k) {
+            const uint32_t i = site[k];
+            const int64_t x = i % nx, y = (i / nx) % ny, z = i / (uint64_t(nx) * ny);
+            uint32_t w = uint32_t(lab8[k]) << DESC_LABEL_SHIFT;
+            for (int q = 1; q < d.Q; ++q) {
+                int64_t ex, ey, ez;
+                if (d.Q == 9) {
+                    ez = 0; ey = Lattice<9>::e[q][0]; ex = Lattice<9>::e[q][1];
+                } else {
+                    ez = Lattice<19>::e[q][0]; ey = Lattice<19>::e[q][1]; ex = Lattice<19>::e[q][2];
+                }
+                const uint64_t ux = uint64_t((x - ex + nx) % nx), uy = uint64_t((y - ey + ny) % ny),
+                               uz = uint64_t((z - ez + nz) % nz);
+                const uint64_t up = (uz * ny + uy) * nx + ux;
+                if (!((rank[cd.rank_base + up / 32].y >> (up % 32)) & 1u))
+                    w |= 1u << (d.Q == 9 ? Lattice<9>::op[q] : Lattice<19>::op[q]);
+            }
+            descs.push_back(w);
+        }
         while (site.size() % TILE) {
             site.push_back(NO_SITE);
             lab8.push_back(0);
             flow8.push_back(0);
+            descs.push_back(0);
         }
         cd.nfluid = nf;
         cd.tile_end = uint32_t(site.size() / TILE);
@@ -409,6 +438,8 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     (void)hipGetDeviceProperties(&prop, dev);
     b->cus = uint32_t(prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256);
     b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * 8)));
+    b->report_grid = b->grid;
+    // populations blocked per 64 slots, [slot / 64][q][slot % 64]; S is whole tiles
     const size_t pop = size_t(b->Q) * b->S * 4;
     int rc = 0;
     rc |= balloc(b, &b->f[0], pop);
@@ -419,6 +450,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     rc |= balloc(b, &b->d_crops, b->crops.size() * sizeof(CropDev));
     rc |= balloc(b, &b->d_site, b->S * 4);
     rc |= balloc(b, &b->d_lab8, b->S);
+    rc |= balloc(b, &b->d_desc, b->S * 4);
     rc |= balloc(b, &b->d_flow8, b->S);
     rc |= balloc(b, &b->d_rank, rank.size() * sizeof(uint2));
     rc |= balloc(b, &b->d_lab, 256 * sizeof(LabelF));
@@ -440,6 +472,7 @@ int lbf_batch_create(const lbf_batch_desc* desc, const lbf_label* labels, const 
     bool ok = up(b->d_tile_crop, b->tile_crop.data(), b->T * 4) == hipSuccess &&
               up(b->d_crops, b->crops.data(), b->crops.size() * sizeof(CropDev)) == hipSuccess &&
               up(b->d_site, site.data(), b->S * 4) == hipSuccess && up(b->d_lab8, lab8.data(), b->S) == hipSuccess &&
+              up(b->d_desc, descs.data(), b->S * 4) == hipSuccess &&
               up(b->d_flow8, flow8.data(), b->S) == hipSuccess &&
               up(b->d_rank, rank.data(), rank.size() * sizeof(uint2)) == hipSuccess &&
               up(b->d_lab, lab.data(), 256 * sizeof(LabelF)) == hipSuccess;
@@ -492,35 +525,55 @@ int lbf_batch_advance(lbf_batch* b, uint64_t n)
 int lbf_batch_tune(lbf_batch* b, uint32_t steps, uint32_t* best_per_cu, double* best_ms)
 {
     if (!b) return -1;
-    static const uint32_t candidates[] = {1, 2, 4, 8, 16, 32, 64, 128};
+    if (steps == 0) steps = 1;
+    static const uint32_t per_cu[] = {1, 2, 4, 8, 16, 32, 64, 128};
+    static const BatchShape shapes[] = {{256, 256}, {128, 256}, {128, 128}, {64, 256}, {64, 64}};
     double best = 1e300;
     uint32_t pick = 8;
+    BatchShape pick_shape{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = 0;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         rc = bfail(b, "HIP error: hipEventCreate failed");
-    for (uint32_t per : candidates) {
-        if (rc) break;
-        b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * per)));
-        rc = step(b, 0u, false) || step(b, 0u, false) ? -1 : 0;   // warm-up, keeps buffer parity
-        if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = bfail(b, "HIP error: hipEventRecord failed");
-        for (uint32_t k = 0; k < steps && !rc; ++k)
-            if (step(b, 0u, false)) rc = -1;
-        float ms = 0.0f;
-        if (!rc && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = bfail(b, "HIP error: timing events failed");
-        if (!rc && ms / steps < best) {
-            best = ms / steps;
-            pick = per;
+    for (const BatchShape& sh : shapes) {
+        // work units of the whole batch: the largest useful grid
+        const uint64_t units = b->T * (TILE / sh.unit);
+        for (uint32_t per : per_cu) {
+            if (rc) break;
+            b->shape = sh;
+            b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(units, uint64_t(b->cus) * per)));
+            rc = step(b, 0u, false) || step(b, 0u, false) ? -1 : 0;   // warm-up, keeps buffer parity
+            if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = bfail(b, "HIP error: hipEventRecord failed");
+            for (uint32_t k = 0; k < steps && !rc; ++k)
+                if (step(b, 0u, false)) rc = -1;
+            float ms = 0.0f;
+            if (!rc && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = bfail(b, "HIP error: timing events failed");
+            if (!rc && ms / steps < best) {
+                best = ms / steps;
+                pick = per;
+                pick_shape = sh;
+            }
         }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (rc) return -1;
-    b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T, uint64_t(b->cus) * pick)));
+    b->shape = pick_shape;
+    b->grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(b->T * (TILE / pick_shape.unit),
+                                                                 uint64_t(b->cus) * pick)));
     if (best_per_cu) *best_per_cu = pick;
     if (best_ms) *best_ms = best;
+    return 0;
+}
+
+int lbf_batch_shape(const lbf_batch* b, uint32_t* threads, uint32_t* unit, uint32_t* workgroups)
+{
+    if (!b) return -1;
+    if (threads) *threads = b->shape.block;
+    if (unit) *unit = b->shape.unit;
+    if (workgroups) *workgroups = b->grid;
     return 0;
 }
 
@@ -543,7 +596,7 @@ int lbf_batch_run(lbf_batch* b, int axis, double tol, uint32_t interval, uint64_
             return -1;
         }
     }
-    hipError_t e = launch_batch_report(b->grid, b->S, b->D, axis, buffers(b, false), b->ncrops, b->d_partial,
+    hipError_t e = launch_batch_report(b->report_grid, b->S, b->D, axis, buffers(b, false), b->ncrops, b->d_partial,
                                        b->d_out, b->stream);
     BHIP(b, hipStreamEndCapture(b->stream, &cg.graph));
     BHIP(b, e);
@@ -659,16 +712,18 @@ int lbf_batch_crop_populations(lbf_batch* b, uint32_t c, double* out)
     const CropDev& cd = b->crops[c];
     const uint64_t N = b->crop_sites[c];
     const uint64_t n = uint64_t(cd.tile_end - cd.tile_begin) * TILE;
-    std::vector<float> h(n);
     std::vector<uint32_t> site(n);
     BHIP(b, hipMemcpy(site.data(), b->d_site + cd.slot_base, n * 4, hipMemcpyDeviceToHost));
     std::memset(out, 0, size_t(b->Q) * N * sizeof(double));
+    // the crop's slots are whole blocks of 64: one contiguous range of Q n values
+    std::vector<float> hq(size_t(b->Q) * n);
+    BHIP(b, hipMemcpy(hq.data(), static_cast<const float*>(b->f[b->cur]) + uint64_t(cd.slot_base) * b->Q,
+                      hq.size() * 4, hipMemcpyDeviceToHost));
     for (int q = 0; q < b->Q; ++q) {
-        BHIP(b, hipMemcpy(h.data(), static_cast<const float*>(b->f[b->cur]) + uint64_t(q) * b->S + cd.slot_base,
-                          n * 4, hipMemcpyDeviceToHost));
         const double shift = (b->Q == 9 ? Lattice<9>::w[q] : Lattice<19>::w[q]) * b->d.density;
         for (uint64_t k = 0; k < n; ++k)
-            if (site[k] != NO_SITE) out[uint64_t(q) * N + site[k]] = double(h[k]) + shift;
+            if (site[k] != NO_SITE)
+                out[uint64_t(q) * N + site[k]] = double(hq[(k >> 6) * (uint64_t(b->Q) * 64) + uint64_t(q) * 64 + (k & 63)]) + shift;
     }
     return 0;
 }

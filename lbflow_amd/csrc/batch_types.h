@@ -27,6 +27,12 @@ struct BatchParams {
     RelaxF32 relax;
 };
 
+// Launch shape of the persistent step: BS threads per workgroup walk work
+// units of UNIT slots (inside one tile); the autotuner picks it with the grid.
+struct BatchShape {
+    uint32_t block = 256, unit = 256;
+};
+
 struct BatchBuffers {
     const void* fin;
     void* fout;
@@ -36,6 +42,7 @@ struct BatchBuffers {
     const CropDev* crops;
     const uint32_t* site;       // crop-local site of every slot, NO_SITE on padding
     const uint8_t* lab8;        // geometry label of every slot
+    const uint32_t* desc;       // per slot: label << DESC_LABEL_SHIFT | bit op(q) where x - e_q is solid
     const uint2* rank;          // per crop, per 32 sites: {fluid sites before, fluid bit mask}
     const LabelF* lab;
     float* ucur;                // velocity of the last fields step [d][slot]

@@ -48,6 +48,10 @@ struct lbf_ctx {
     int cur = 0;
     int layout = 0;                 // 0 dense (all sites streamed), 1 dense fluid-only, 2 sparse
     uint64_t P = 0;                 // slots per population plane
+    // sparse layout: populations in blocks of 64 slots, [s / 64][plane][s % 64]
+    // (stream.h pop_index); the dense layouts store plane after plane, q P + s
+    bool blocked = false;
+    uint64_t pop_elems = 0;         // elements of one population buffer
     size_t elem = 8;
     uint32_t* d_desc = nullptr;
     uint32_t* d_list = nullptr;
@@ -334,6 +338,8 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     else if (d.flags & LBF_STREAM_SOLIDS) c->layout = 0;
     else c->layout = (d.flags & LBF_LAYOUT_DENSE) ? 1 : 2;
     c->P = c->layout == 2 ? c->nfluid : N;
+    c->blocked = c->layout == 2;
+    c->pop_elems = uint64_t(NP) * d.Q * (c->blocked ? (c->P + 63) / 64 * 64 : c->P);
 
     build_mrt(d.Q, d.tau[0], c->ops);
 
@@ -358,7 +364,7 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         return fail(nullptr, "HIP error: cannot create a stream (no GPU?)");
     }
     (void)hipGetDevice(&c->device);
-    const size_t pop = size_t(NP) * d.Q * c->P * c->elem;
+    const size_t pop = c->pop_elems * c->elem;
     const size_t Q2 = size_t(d.Q) * d.Q;
     int rc = 0;
     rc |= dalloc(c, &c->f[0], pop);
@@ -530,6 +536,13 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
     c->kernel_name += std::string(" (") + layout_names[c->layout] + ")";
     *out = c;
     return 0;
+}
+
+// Element index of plane q, slot s in a population buffer (stream.h pop_index).
+static size_t pop_elem(const lbf_ctx* c, size_t q, uint64_t s)
+{
+    if (c->blocked) return size_t(s >> 6) * (size_t(c->NP) * c->Q * 64) + q * 64 + size_t(s & 63);
+    return q * c->P + s;
 }
 
 static void reset_status(lbf_ctx* c)
@@ -718,7 +731,7 @@ int lbf_init(lbf_ctx* c)
     reset_status(c);
     c->t = 0;
     c->has_prev = false;
-    const size_t pop = size_t(c->NP) * c->Q * c->P * c->elem;
+    const size_t pop = c->pop_elems * c->elem;
     HIP_CHECK(c, hipMemsetAsync(c->f[c->cur], 0, pop, c->stream));
     if (run_step(c, true, 3u)) return -1;   // fields + adjacent history of a report at t = 1
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
@@ -776,7 +789,7 @@ int lbf_get_populations(lbf_ctx* c, double* out)
     if (!c || !out) return -1;
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     const size_t planes = size_t(c->NP) * c->Q;
-    const size_t n = planes * c->P;
+    const size_t n = c->pop_elems;
     if (c->elem == 8 && c->layout == 0) {
         HIP_CHECK(c, hipMemcpy(out, c->f[c->cur], n * 8, hipMemcpyDeviceToHost));
         return 0;
@@ -796,7 +809,7 @@ int lbf_get_populations(lbf_ctx* c, double* out)
             for (uint64_t s = 0; s < c->P; ++s) {
                 const uint64_t i = c->layout == 2 ? c->list[s] : s;
                 if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
-                const double h = h32[size_t(q) * c->P + s], f0 = h32[size_t(Q + q) * c->P + s];
+                const double h = h32[pop_elem(c, q, s)], f0 = h32[pop_elem(c, Q + q, s)];
                 out[size_t(q) * c->N + i] = f0;
                 out[size_t(Q + q) * c->N + i] = h + double(float(a.dist0[q])) - f0;
             }
@@ -807,8 +820,8 @@ int lbf_get_populations(lbf_ctx* c, double* out)
             const uint64_t i = c->layout == 2 ? c->list[s] : s;
             // solid sites that are never streamed keep the value 0
             if (c->layout == 1 && !(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
-            out[plane * c->N + i] = c->elem == 8 ? h64[plane * c->P + s]
-                                                 : double(h32[plane * c->P + s]) + c->shift[plane];
+            const size_t k = pop_elem(c, plane, s);
+            out[plane * c->N + i] = c->elem == 8 ? h64[k] : double(h32[k]) + c->shift[plane];
         }
     return 0;
 }
@@ -820,7 +833,7 @@ int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
     // the buffers replaced here: wait for them, then upload on the same stream.
     HIP_CHECK(c, hipStreamSynchronize(c->stream));
     const size_t planes = size_t(c->NP) * c->Q;
-    const size_t n = planes * c->P;
+    const size_t n = c->pop_elems;
     if (c->elem == 8 && c->layout == 0) {
         HIP_CHECK(c, hipMemcpyAsync(c->f[c->cur], in, n * 8, hipMemcpyHostToDevice, c->stream));
     } else {
@@ -835,15 +848,16 @@ int lbf_set_populations(lbf_ctx* c, const double* in, uint64_t t)
                     const uint64_t i = c->layout == 2 ? c->list[s] : s;
                     if (!(c->hlab[desc[i] >> DESC_LABEL_SHIFT].kind & KIND_FLUID)) continue;
                     const double f0 = in[size_t(q) * c->N + i], f1 = in[size_t(Q + q) * c->N + i];
-                    h32[size_t(q) * c->P + s] = float(f0 + f1 - double(float(a.dist0[q])));
-                    h32[size_t(Q + q) * c->P + s] = float(f0);
+                    h32[pop_elem(c, q, s)] = float(f0 + f1 - double(float(a.dist0[q])));
+                    h32[pop_elem(c, Q + q, s)] = float(f0);
                 }
         } else
         for (size_t plane = 0; plane < planes; ++plane)
             for (uint64_t s = 0; s < c->P; ++s) {
                 const double v = in[plane * c->N + (c->layout == 2 ? c->list[s] : s)];
-                if (c->elem == 8) h64[plane * c->P + s] = v;
-                else h32[plane * c->P + s] = float(v - c->shift[plane]);
+                const size_t k = pop_elem(c, plane, s);
+                if (c->elem == 8) h64[k] = v;
+                else h32[k] = float(v - c->shift[plane]);
             }
         if (c->elem == 8)
             HIP_CHECK(c, hipMemcpyAsync(c->f[c->cur], h64.data(), n * 8, hipMemcpyHostToDevice, c->stream));

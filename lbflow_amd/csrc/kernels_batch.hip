@@ -13,111 +13,120 @@
 #include <type_traits>
 
 #include "batch_types.h"
+#include "stream.h"
 
 namespace lbf {
 
-template <int Q, bool MRT, bool INIT, bool WIDE>
-__global__ void __launch_bounds__(TILE)
-batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fout,
-           const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ntiles,
-           const uint32_t* __restrict__ tile_crop, const CropDev* __restrict__ crops,
-           const uint32_t* __restrict__ site, const uint8_t* __restrict__ lab8, const uint2* __restrict__ rank,
-           const LabelF* __restrict__ lab, float* __restrict__ ucur, float* __restrict__ uadj,
-           const float* __restrict__ upres, DevStatus* __restrict__ status)
+// One fluid slot of one crop: pull (bounce-back where the upstream site is
+// solid), macro, optional prescribed-velocity sweep, collision, store.
+// Populations are blocked per 64 slots ([slot / 64][q][slot % 64], stream.h
+// pop_index); a crop starts on a whole tile, so its slots form whole blocks
+// and a crop-local offset plus the crop's block base addresses the batch.
+template <int Q, bool MRT, bool INIT, typename O>
+__device__ __forceinline__ void batch_site(const BatchParams& p, const float* __restrict__ fin,
+                                           float* __restrict__ fout, const CropDev& cr, uint32_t slot,
+                                           uint32_t s, uint32_t dsc, const uint2* __restrict__ rank,
+                                           const LabelF* __restrict__ lab,
+                                           float* __restrict__ ucur, float* __restrict__ uadj,
+                                           const float* __restrict__ upres, DevStatus* __restrict__ status)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
-    using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
-    const uint32_t n = *ntiles;
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        const uint32_t tile = tiles[k];
-        const CropDev cr = crops[tile_crop[tile]];
-        const uint32_t slot = tile * TILE + threadIdx.x;
-        const uint32_t s = site[slot];
-        if (s == NO_SITE) continue;
-        const LabelF lb = lab[lab8[slot]];
-        const float F[3] = {lb.force[0], lb.force[1], D == 3 ? lb.force[2] : 0.0f};
-        float h[Q];
-        if (INIT) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                float ef = 0.0f;
-#pragma unroll
-                for (int d = 0; d < D; ++d) ef += float(L::e[q][d]) * F[d];
-                h[q] = float(-1.5 * L::w[q]) * ef;
-            }
-        } else {
-            int x, y, z;
-            site_coords(cr.g, s, x, y, z);
-            const uint32_t xs[3] = {x == 0 ? cr.g.nx - 1 : uint32_t(x - 1), uint32_t(x),
-                                    uint32_t(x) + 1 == cr.g.nx ? 0u : uint32_t(x + 1)};
-            const uint32_t ys[3] = {(y == 0 ? cr.g.ny - 1 : uint32_t(y - 1)) * cr.g.nx, uint32_t(y) * cr.g.nx,
-                                    (uint32_t(y) + 1 == cr.g.ny ? 0u : uint32_t(y + 1)) * cr.g.nx};
-            const uint32_t zs[3] = {(z == 0 ? cr.g.nz - 1 : uint32_t(z - 1)) * cr.g.nxy, uint32_t(z) * cr.g.nxy,
-                                    (uint32_t(z) + 1 == cr.g.nz ? 0u : uint32_t(z + 1)) * cr.g.nxy};
-            h[0] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(fin) + O(slot) * O(4));
-#pragma unroll
-            for (int q = 1; q < Q; ++q) {
-                int ex, ey, ez;
-                if constexpr (Q == 9) {
-                    ez = 0; ey = L::e[q][0]; ex = L::e[q][1];
-                } else {
-                    ez = L::e[q][0]; ey = L::e[q][1]; ex = L::e[q][2];
-                }
-                // the upstream site's slot is its rank among the crop's fluid
-                // sites; a solid upstream site bounces back into op(q)
-                const uint32_t src = zs[1 - ez] + ys[1 - ey] + xs[1 - ex];
-                const uint2 r = rank[cr.rank_base + (src >> 5)];
-                const uint32_t bit = src & 31u;
-                const O off = ((r.y >> bit) & 1u)
-                                  ? O(q) * O(p.S) + (cr.slot_base + r.x + uint32_t(__popc(r.y & ((1u << bit) - 1u))))
-                                  : O(L::op[q]) * O(p.S) + slot;
-                h[q] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(fin) + off * O(4));
-            }
-        }
-        float u[D];
-        const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
-        if (!isfinite(drho)) raise_status(status, 0, p.t);
-        if (p.flags & 4u) {
-            // Consistent initialisation in the spirit of Mei, Luo, Lallemand &
-            // d'Humieres (Computers & Fluids 35 (2006) 855-862): the streamed
-            // populations get the momentum of the prescribed velocity (added as
-            // 3 w_q e_q . dj, which carries momentum dj and no mass) before the
-            // ordinary collision, so the non-equilibrium parts and the density
-            // settle to the given flow.  With MRT the momentum moments do not
-            // relax, so the velocity is imposed on the populations themselves
-            // rather than only through the equilibrium.
-            float dj[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const float up = upres[uint64_t(d) * p.S + slot];
-                dj[d] = p.relax.rho0 * (up - u[d]);
-                u[d] = up;
-            }
-#pragma unroll
-            for (int q = 1; q < Q; ++q) {
-                float ej = 0.0f;
-#pragma unroll
-                for (int d = 0; d < D; ++d) ej += float(L::e[q][d]) * dj[d];
-                h[q] += float(3.0 * L::w[q]) * ej;
-            }
-        }
-        if (p.flags & 1u) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) ucur[uint64_t(d) * p.S + slot] = u[d];
-        }
-        if (p.flags & 2u) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) uadj[uint64_t(d) * p.S + slot] = u[d];
-        }
-        collide_f32<Q, MRT>(h, drho, u, F, p.relax);
-        bool finite = true;
+    const LabelF lb = lab[dsc >> DESC_LABEL_SHIFT];
+    const float F[3] = {lb.force[0], lb.force[1], D == 3 ? lb.force[2] : 0.0f};
+    const uint32_t local = slot - cr.slot_base;
+    const O base = O(cr.slot_base >> 6) * O(Q * 64);
+    float h[Q];
+    if (INIT) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            finite = finite && isfinite(h[q]);
-            *reinterpret_cast<float*>(reinterpret_cast<char*>(fout) + (O(q) * O(p.S) + slot) * O(4)) = h[q];
+            float ef = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) ef += float(L::e[q][d]) * F[d];
+            h[q] = float(-1.5 * L::w[q]) * ef;
         }
-        if (!finite) raise_status(status, 4, p.t);
+    } else {
+        // bit op(q) of the descriptor: x - e_q is solid and bounces back into
+        // op(q); otherwise the crop's rank index gives the upstream slot
+        const Stencil stn = make_stencil(cr.g, s);
+        O off[Q];
+        h[0] = ld(fin, base + pop_index<Q, true, O>(0u, 0, local));
+        pull_offsets<Q, SPARSE, O, true>(rank + cr.rank_base, dsc & DESC_LINK_MASK, local, stn, cr.g.nx, 0u, off);
+#pragma unroll
+        for (int q = 1; q < Q; ++q) h[q] = ld(fin, base + off[q]);
+    }
+    float u[D];
+    const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
+    if (!isfinite(drho)) raise_status(status, 0, p.t);
+    if (p.flags & 4u) {
+        // Consistent initialisation in the spirit of Mei, Luo, Lallemand &
+        // d'Humieres (Computers & Fluids 35 (2006) 855-862): the streamed
+        // populations get the momentum of the prescribed velocity (added as
+        // 3 w_q e_q . dj, which carries momentum dj and no mass) before the
+        // ordinary collision, so the non-equilibrium parts and the density
+        // settle to the given flow.  With MRT the momentum moments do not
+        // relax, so the velocity is imposed on the populations themselves
+        // rather than only through the equilibrium.
+        float dj[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const float up = upres[uint64_t(d) * p.S + slot];
+            dj[d] = p.relax.rho0 * (up - u[d]);
+            u[d] = up;
+        }
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            float ej = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) ej += float(L::e[q][d]) * dj[d];
+            h[q] += float(3.0 * L::w[q]) * ej;
+        }
+    }
+    if (p.flags & 1u) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) ucur[uint64_t(d) * p.S + slot] = u[d];
+    }
+    if (p.flags & 2u) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) uadj[uint64_t(d) * p.S + slot] = u[d];
+    }
+    collide_f32<Q, MRT>(h, drho, u, F, p.relax);
+    bool finite = true;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        finite = finite && isfinite(h[q]);
+        st(fout, base + pop_index<Q, true, O>(0u, q, local), h[q]);
+    }
+    if (!finite) raise_status(status, 4, p.t);
+}
+
+// Persistent step over the active tiles.  The work unit is UNIT slots of one
+// tile (TILE / UNIT units per tile) and a workgroup of BS threads walks a
+// unit in UNIT / BS passes; the autotuner picks BS, UNIT and the grid.
+template <int Q, bool MRT, bool INIT, bool WIDE, uint32_t BS, uint32_t UNIT>
+__global__ void __launch_bounds__(BS)
+batch_step(BatchParams p, const float* __restrict__ fin, float* __restrict__ fout,
+           const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ntiles,
+           const uint32_t* __restrict__ tile_crop, const CropDev* __restrict__ crops,
+           const uint32_t* __restrict__ site, const uint32_t* __restrict__ desc, const uint2* __restrict__ rank,
+           const LabelF* __restrict__ lab, float* __restrict__ ucur, float* __restrict__ uadj,
+           const float* __restrict__ upres, DevStatus* __restrict__ status)
+{
+    static_assert(TILE % UNIT == 0 && UNIT % BS == 0, "a unit is whole passes of the block inside one tile");
+    using O = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
+    constexpr uint32_t PER_TILE = TILE / UNIT;
+    const uint32_t n = *ntiles * PER_TILE;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint32_t tile = tiles[k / PER_TILE];
+        const CropDev cr = crops[tile_crop[tile]];
+#pragma unroll
+        for (uint32_t j = 0; j < UNIT; j += BS) {
+            const uint32_t slot = tile * TILE + (k % PER_TILE) * UNIT + j + threadIdx.x;
+            const uint32_t s = site[slot];
+            if (s != NO_SITE)
+                batch_site<Q, MRT, INIT, O>(p, fin, fout, cr, slot, s, desc[slot], rank, lab, ucur, uadj, upres,
+                                            status);
+        }
     }
 }
 
@@ -197,7 +206,7 @@ batch_fields(uint64_t S, float rho0, uint32_t tile_begin, const uint32_t* __rest
     if (s == NO_SITE) return;
     float drho = 0.0f;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) drho += f[uint64_t(q) * S + slot];
+    for (int q = 0; q < Q; ++q) drho += f[pop_index<Q, true, uint64_t>(0u, q, slot)];
 #pragma unroll
     for (int d = 0; d < D; ++d) fields[uint64_t(s) * (D + 1) + d] = ucur[uint64_t(d) * S + slot];
     fields[uint64_t(s) * (D + 1) + D] = rho0 + drho;
@@ -246,7 +255,7 @@ batch_warm(BatchParams p, uint32_t tile_begin, const uint32_t* __restrict__ site
     const float drho = macro_f32<Q>(h, p.inv_rho0, lb.half_rho, u);
     collide_f32<Q, MRT>(h, drho, u, F, p.relax);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[uint64_t(q) * p.S + slot] = h[q];
+    for (int q = 0; q < Q; ++q) f[pop_index<Q, true, uint64_t>(0u, q, slot)] = h[q];
 }
 
 // a = (a + b) / 2 over n floats: the average of two consecutive states, which
@@ -292,37 +301,58 @@ hipError_t launch_batch_warm(int Q, bool mrt, const BatchParams& p, uint32_t til
     return hipGetLastError();
 }
 
-template <int Q, bool MRT, bool INIT>
-static void launch_batch_q(uint32_t grid, const BatchParams& p, const BatchBuffers& b, bool wide, hipStream_t s)
+template <int Q, bool MRT, bool INIT, bool WIDE, uint32_t BS, uint32_t UNIT>
+static void launch_batch_s(uint32_t grid, const BatchParams& p, const BatchBuffers& b, hipStream_t s)
 {
-    if (wide)
-        hipLaunchKernelGGL((batch_step<Q, MRT, INIT, true>), dim3(grid), dim3(TILE), 0, s, p,
-                           static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
-                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.rank, b.lab, b.ucur,
-                           b.uadj, b.upres, b.status);
-    else
-        hipLaunchKernelGGL((batch_step<Q, MRT, INIT, false>), dim3(grid), dim3(TILE), 0, s, p,
-                           static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles,
-                           b.tile_crop, static_cast<const CropDev*>(b.crops), b.site, b.lab8, b.rank, b.lab, b.ucur,
-                           b.uadj, b.upres, b.status);
+    hipLaunchKernelGGL((batch_step<Q, MRT, INIT, WIDE, BS, UNIT>), dim3(grid), dim3(BS), 0, s, p,
+                       static_cast<const float*>(b.fin), static_cast<float*>(b.fout), b.tiles, b.ntiles, b.tile_crop,
+                       static_cast<const CropDev*>(b.crops), b.site, b.desc, b.rank, b.lab, b.ucur, b.uadj, b.upres,
+                       b.status);
 }
 
-hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, const BatchParams& p,
+template <int Q, bool MRT, bool INIT, bool WIDE>
+static hipError_t launch_batch_w(uint32_t grid, const BatchParams& p, const BatchBuffers& b, BatchShape sh,
+                                 hipStream_t s)
+{
+    // the initial step runs once, in the default shape
+    if constexpr (INIT) {
+        launch_batch_s<Q, MRT, INIT, WIDE, 256, 256>(grid, p, b, s);
+        return hipSuccess;
+    }
+    if (sh.block == 256 && sh.unit == 256) launch_batch_s<Q, MRT, INIT, WIDE, 256, 256>(grid, p, b, s);
+    else if (sh.block == 128 && sh.unit == 256) launch_batch_s<Q, MRT, INIT, WIDE, 128, 256>(grid, p, b, s);
+    else if (sh.block == 128 && sh.unit == 128) launch_batch_s<Q, MRT, INIT, WIDE, 128, 128>(grid, p, b, s);
+    else if (sh.block == 64 && sh.unit == 256) launch_batch_s<Q, MRT, INIT, WIDE, 64, 256>(grid, p, b, s);
+    else if (sh.block == 64 && sh.unit == 64) launch_batch_s<Q, MRT, INIT, WIDE, 64, 64>(grid, p, b, s);
+    else return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+template <int Q, bool MRT, bool INIT>
+static hipError_t launch_batch_q(uint32_t grid, const BatchParams& p, const BatchBuffers& b, BatchShape sh, bool wide,
+                                 hipStream_t s)
+{
+    return wide ? launch_batch_w<Q, MRT, INIT, true>(grid, p, b, sh, s)
+                : launch_batch_w<Q, MRT, INIT, false>(grid, p, b, sh, s);
+}
+
+hipError_t launch_batch_step(int Q, bool mrt, bool init, uint32_t grid, BatchShape sh, const BatchParams& p,
                              const BatchBuffers& b, hipStream_t s)
 {
     const bool wide = p.S * uint64_t(Q) * 4u >= (uint64_t(1) << 32);
+    hipError_t e;
     if (Q == 19) {
-        if (mrt) init ? launch_batch_q<19, true, true>(grid, p, b, wide, s)
-                      : launch_batch_q<19, true, false>(grid, p, b, wide, s);
-        else init ? launch_batch_q<19, false, true>(grid, p, b, wide, s)
-                  : launch_batch_q<19, false, false>(grid, p, b, wide, s);
+        if (mrt) e = init ? launch_batch_q<19, true, true>(grid, p, b, sh, wide, s)
+                          : launch_batch_q<19, true, false>(grid, p, b, sh, wide, s);
+        else e = init ? launch_batch_q<19, false, true>(grid, p, b, sh, wide, s)
+                      : launch_batch_q<19, false, false>(grid, p, b, sh, wide, s);
     } else {
-        if (mrt) init ? launch_batch_q<9, true, true>(grid, p, b, wide, s)
-                      : launch_batch_q<9, true, false>(grid, p, b, wide, s);
-        else init ? launch_batch_q<9, false, true>(grid, p, b, wide, s)
-                  : launch_batch_q<9, false, false>(grid, p, b, wide, s);
+        if (mrt) e = init ? launch_batch_q<9, true, true>(grid, p, b, sh, wide, s)
+                          : launch_batch_q<9, true, false>(grid, p, b, sh, wide, s);
+        else e = init ? launch_batch_q<9, false, true>(grid, p, b, sh, wide, s)
+                      : launch_batch_q<9, false, false>(grid, p, b, sh, wide, s);
     }
-    return hipGetLastError();
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_batch_report(uint32_t grid, uint64_t S, int D, int axis, const BatchBuffers& b, uint32_t ncrops,

@@ -68,11 +68,11 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
     dr = 0.0;
     if (INIT && OWN) {
         // the site's own slots of fin32, not streamed (CSF initialisation)
-        const O P = a.P;
+        constexpr bool BLK = LAYOUT == SPARSE;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            h[q] = ld(a.fin32, O(q) * P + O(slot));
-            f0[q] = ld(a.fin32, (O(Q) + q) * P + O(slot));
+            h[q] = ld(a.fin32, pop_index<2 * Q, BLK, O>(a.P, q, slot));
+            f0[q] = ld(a.fin32, pop_index<2 * Q, BLK, O>(a.P, Q + q, slot));
             r0 += double(f0[q]);
             dr += double(h[q]);
         }
@@ -91,11 +91,13 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
         }
         return;
     }
-    const O P = a.P, f0plane = O(Q) * P;
+    // sparse: blocks of 64 slots holding the 2Q planes (stream.h pop_index)
+    constexpr bool BLK = LAYOUT == SPARSE;
+    const O P = a.P, f0plane = BLK ? O(Q * 64) : O(Q) * P;
     const Stencil stn = make_stencil(a.g, site);
     O off[Q];
-    off[0] = O(slot);
-    pull_offsets<Q, LAYOUT, O>(a.rank, desc & DESC_LINK_MASK, slot, stn, a.g.nx, uint32_t(P), off);
+    off[0] = pop_index<2 * Q, BLK, O>(a.P, 0, slot);
+    pull_offsets<Q, LAYOUT, O, BLK, 2 * Q>(a.rank, desc & DESC_LINK_MASK, slot, stn, a.g.nx, uint32_t(P), off);
     // The boundary path is chosen before the loads, so that plain sites can
     // fold their populations into the sums as they arrive.
     const bool film_links = desc & DESC_FILM;
@@ -126,7 +128,7 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
                 const int oq = L::op[q];
                 const bool back0 = (film >> oq) & 1u, back1 = (film >> (32 + oq)) & 1u;
                 if (back0 || back1) {
-                    const O loc = O(oq) * P + O(slot);
+                    const O loc = pop_index<2 * Q, BLK, O>(a.P, oq, slot);
                     const float h_loc = ld(a.fin32, loc), f0_loc = ld(a.fin32, loc + f0plane);
                     if (back0) {
                         h[q] = f0_loc + (h[q] - f0[q]);
@@ -605,16 +607,16 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
             const float feq0 = r0 * float(a.ai0[q]) + r1 * float(a.ai1[q]);
             const float fo = float(fma(dens0T64, double(h[q]) + double(float(a.dist0[q])), double(b01 * angle * feq0)));
             finite = finite && isfinite(h[q]) && isfinite(fo);
-            st(a.fout32, O(q) * O(a.P) + slot, h[q]);
-            st(a.fout32, (O(Q) + q) * O(a.P) + slot, fo);
+            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
+            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
         }
     } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const float fo = float(dens0T64 * (double(h[q]) + double(float(a.dist0[q]))));
             finite = finite && isfinite(h[q]) && isfinite(fo);
-            st(a.fout32, O(q) * O(a.P) + slot, h[q]);
-            st(a.fout32, (O(Q) + q) * O(a.P) + slot, fo);
+            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
+            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
         }
     }
     if (!finite) raise_status(a.status, 4, a.t);
@@ -724,7 +726,6 @@ __global__ void __launch_bounds__(256) color_csf_init_f32(ColorArgs a)
     site_coords(a.g, site, x, y, z);
     float F[3] = {0.0f, 0.0f, 0.0f};
     if (!capillary_force_f32<Q>(a, site, x, y, z, F)) raise_status(a.status, 5, a.t);
-    const O P = a.P;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         double ef = 0.0;
@@ -735,8 +736,8 @@ __global__ void __launch_bounds__(256) color_csf_init_f32(ColorArgs a)
         }
         const double g0 = f0[q] - 1.5 * L::w[q] * ef * r0 / rt;
         const double g1 = f1[q] - 1.5 * L::w[q] * ef * r1 / rt;
-        st(a.fout32, O(q) * P + O(slot), float(g0 + g1 - double(float(a.dist0[q]))));
-        st(a.fout32, (O(Q) + q) * P + O(slot), float(g0));
+        st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), float(g0 + g1 - double(float(a.dist0[q]))));
+        st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), float(g0));
     }
 }
 
@@ -774,8 +775,10 @@ static hipError_t launch_color32_l(const ColorArgs& a, hipStream_t s)
 template <int Q, bool MRT, bool INIT>
 static hipError_t launch_color32_q(const ColorArgs& a, bool sparse, hipStream_t s)
 {
-    // 32-bit element offsets while both planes sets fit below 4 GiB
-    const bool wide = uint64_t(2) * Q * a.P * 4u >= (uint64_t(1) << 32);
+    // 32-bit element offsets while both plane sets fit below 4 GiB (sparse:
+    // the last block of 64 slots is allocated whole)
+    const uint64_t P = sparse ? (uint64_t(a.P) + 63) / 64 * 64 : a.P;
+    const bool wide = uint64_t(2) * Q * P * 4u >= (uint64_t(1) << 32);
     if (sparse)
         return wide ? launch_color32_l<Q, MRT, INIT, SPARSE, uint64_t>(a, s)
                     : launch_color32_l<Q, MRT, INIT, SPARSE, uint32_t>(a, s);

@@ -99,6 +99,7 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
     uint32_t i, slot;
     if (!resolve<LAYOUT>(p, list, i, slot)) return;
     const uint32_t P = LAYOUT == SPARSE ? p.nfluid : uint32_t(p.g.N);   // plane length
+    constexpr bool BLK = LAYOUT == SPARSE;   // sparse: blocks of 64 slots
     const uint32_t dsc = desc[LAYOUT == SPARSE ? slot : i];
     const DevLabel* lb = lab + (dsc >> DESC_LABEL_SHIFT);
     const int kind = lb->kind;
@@ -137,9 +138,9 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
             f[q] -= 1.5 * L::w[q] * ef;
         }
     } else {
-        f[0] = ld(fin, O(slot));
+        f[0] = ld(fin, pop_index<Q, BLK, O>(P, 0, slot));
         O off[Q];
-        pull_offsets<Q, LAYOUT, O>(rank, dsc, slot, stn, p.g.nx, P, off);
+        pull_offsets<Q, LAYOUT, O, BLK>(rank, dsc, slot, stn, p.g.nx, P, off);
 #pragma unroll
         for (int q = 1; q < Q; ++q) f[q] = ld(fin, off[q]);
     }
@@ -260,7 +261,7 @@ scsp_step_f64(ScspParams p, const double* __restrict__ fin, double* __restrict__
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         finite = finite && isfinite(f[q]);
-        st(fout, O(q) * P + slot, f[q]);
+        st(fout, pop_index<Q, BLK, O>(P, q, slot), f[q]);
     }
     if (!finite) raise_status(status, 4, p.t);
 }
@@ -282,6 +283,7 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
     uint32_t i, slot;
     if (!resolve<LAYOUT>(p, list, i, slot)) return;
     const uint32_t P = LAYOUT == SPARSE ? p.nfluid : uint32_t(p.g.N);
+    constexpr bool BLK = LAYOUT == SPARSE;   // sparse: blocks of 64 slots
     const uint32_t dsc = desc[LAYOUT == SPARSE ? slot : i];
     const LabelF lb = lab[dsc >> DESC_LABEL_SHIFT];
     float h[Q];
@@ -305,9 +307,9 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
             h[q] = float(-1.5 * L::w[q]) * ef;
         }
     } else {
-        h[0] = ld(fin, O(slot));
+        h[0] = ld(fin, pop_index<Q, BLK, O>(P, 0, slot));
         O off[Q];
-        pull_offsets<Q, LAYOUT, O>(rank, dsc, slot, stn, p.g.nx, P, off);
+        pull_offsets<Q, LAYOUT, O, BLK>(rank, dsc, slot, stn, p.g.nx, P, off);
 #pragma unroll
         for (int q = 1; q < Q; ++q) h[q] = ld(fin, off[q]);
     }
@@ -336,7 +338,7 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         finite = finite && isfinite(h[q]);
-        st(fout, O(q) * P + slot, h[q]);
+        st(fout, pop_index<Q, BLK, O>(P, q, slot), h[q]);
     }
     if (!finite) raise_status(status, 4, p.t);
 }
@@ -363,7 +365,8 @@ static void launch_scsp_w(const ScspParams& p, const ScspBuffers& b, bool fp32, 
 template <int Q, bool MRT, bool INIT, int LAYOUT>
 static void launch_scsp_l(const ScspParams& p, const ScspBuffers& b, bool fp32, hipStream_t s)
 {
-    const uint64_t P = LAYOUT == SPARSE ? p.nfluid : p.g.N;
+    // sparse: the last block of 64 slots is allocated whole
+    const uint64_t P = LAYOUT == SPARSE ? (uint64_t(p.nfluid) + 63) / 64 * 64 : p.g.N;
     const uint64_t bytes = P * uint64_t(Q) * (fp32 ? 4u : 8u);
     if (bytes >= (uint64_t(1) << 32)) launch_scsp_w<Q, MRT, INIT, LAYOUT, true>(p, b, fp32, s);
     else launch_scsp_w<Q, MRT, INIT, LAYOUT, false>(p, b, fp32, s);

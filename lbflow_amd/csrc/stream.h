@@ -73,6 +73,20 @@ __device__ __forceinline__ O pull_offset(const uint2* __restrict__ rank, uint32_
     }
 }
 
+// Element index of plane q of slot s in an array of NPL planes.  The dense
+// layouts store plane after plane (q P + s).  The sparse layout (BLK) stores
+// blocks of 64 slots, each holding its NPL planes one after another
+// ([s / 64][q][s % 64]): a wave's 64 lanes still read and write 64
+// consecutive values per plane, while the values one site pulls, and the
+// ones it writes, lie within a few nearby blocks instead of NPL planes P
+// values apart (fewer DRAM pages and TLB entries per wave).
+template <int NPL, bool BLK, typename O>
+__device__ __forceinline__ O pop_index(uint32_t P, int q, uint32_t s)
+{
+    if constexpr (BLK) return O(s >> 6) * O(NPL * 64) + O(q * 64) + O(s & 63u);
+    else return O(q) * P + s;
+}
+
 // Offsets of the values streaming into directions 1..Q-1 (off[0] unused).
 // Sparse layout: the pull sources of one site lie in 9 rows (dz, dy) at
 // columns x-1, x, x+1.  A fluid site's slot is its rank among the fluid
@@ -81,7 +95,7 @@ __device__ __forceinline__ O pull_offset(const uint2* __restrict__ rank, uint32_
 // boundaries).  One rank lookup per row serves the whole row: 8 lookups for
 // D3Q19 instead of 18 (the site's own row needs none).  Columns that wrap
 // around the row end are looked up on their own.
-template <int Q, int LAYOUT, typename O>
+template <int Q, int LAYOUT, typename O, bool BLK = false, int NPL = Q>
 __device__ __forceinline__ void pull_offsets(const uint2* __restrict__ rank, uint32_t links, uint32_t slot,
                                              const Stencil& st, uint32_t nx, uint32_t P, O* off)
 {
@@ -137,7 +151,8 @@ __device__ __forceinline__ void pull_offsets(const uint2* __restrict__ rank, uin
                 m = x + 1 == nx ? rank_of(st.z[a] + st.y[b] + st.x[2], fl) : rr[a][b] + rf[a][b];
             }
             // site x - e_q solid <=> site in ibS[op(q)] (simulator.cpp:103-111)
-            off[q] = ((links >> L::op[q]) & 1u) ? O(L::op[q]) * P + slot : O(q) * P + m;
+            const bool bounce = (links >> L::op[q]) & 1u;
+            off[q] = pop_index<NPL, BLK, O>(P, bounce ? L::op[q] : q, bounce ? slot : m);
         }
     }
 }

@@ -25,7 +25,7 @@ SYMBOLS = ('lbf_create', 'lbf_destroy', 'lbf_last_error', 'lbf_init', 'lbf_step'
            'lbf_device_bytes', 'lbf_step_kernel', 'lbf_step_bytes', 'lbf_timed_advance',
            'lbf_copy_bandwidth', 'lbf_batch_create', 'lbf_batch_destroy', 'lbf_batch_error',
            'lbf_batch_sites', 'lbf_batch_tune', 'lbf_batch_init', 'lbf_batch_advance', 'lbf_batch_run',
-           'lbf_batch_crop_populations', 'lbf_batch_time')
+           'lbf_batch_crop_populations', 'lbf_batch_time', 'lbf_batch_shape')
 
 
 class Label(C.Structure):
@@ -131,6 +131,8 @@ def load_library(path=None):
     L.lbf_batch_sites.restype = u64
     L.lbf_batch_tune.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     L.lbf_batch_tune.restype = i32
+    L.lbf_batch_shape.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.lbf_batch_shape.restype = i32
     L.lbf_batch_init.argtypes = [vp]
     L.lbf_batch_init.restype = i32
     L.lbf_batch_advance.argtypes = [vp, u64]
@@ -356,6 +358,13 @@ class Batch:
         ms = C.c_double()
         self._check(self.L.lbf_batch_tune(self.h, steps, C.byref(per), C.byref(ms)))
         return int(per.value), ms.value
+
+    def shape(self):
+        """Launch shape of the step: (threads per workgroup, slots per work
+        unit, workgroups of the persistent grid)."""
+        t, u, w = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self.L.lbf_batch_shape(self.h, C.byref(t), C.byref(u), C.byref(w)))
+        return int(t.value), int(u.value), int(w.value)
 
     def init(self):
         self._check(self.L.lbf_batch_init(self.h))
