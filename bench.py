@@ -19,7 +19,7 @@ FP32 (residual 1e-6) and from the learned warm start, the two-phase colour
 model on a 128x128x256 pack, 512 batched crops run to convergence, and the
 warm start against the cold start on 256 unseen 64^3 crops.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--legs dense,fp64,...]
 """
 import argparse
 import json
@@ -199,6 +199,46 @@ def batch_bench(crops, interval, max_steps):
     return out
 
 
+def slab_bench(labels, steps, warmup, ways=(2, 4, 8)):
+    """One GPU's share of a z-slab sharding of the pack: the FP32 sparse step
+    on the first 1/k of the z-layers (periodic in z for the timing), with
+    the fluid sites on a z-face (what a slab would exchange: 5 FP32
+    populations per face site and direction)."""
+    from lbflow_amd.engine import FP32
+    out = {}
+    for k in ways:
+        sub = np.ascontiguousarray(labels[:labels.shape[0] // k])
+        r = measure(pack_case(f'c2-slab{k}', sub), sub, FP32, False, steps, warmup)
+        face = int((sub[0] == 75).sum())
+        out[f'{k}way'] = dict(slab=list(sub.shape), ms_per_step=round(r['ms_per_step'], 5),
+                              site_mlups=round(sub.size / r['ms_per_step'] / 1e3, 1),
+                              fluid_mlups=round(int((sub == 75).sum()) / r['ms_per_step'] / 1e3, 1),
+                              face_fluid_sites=face, exchange_bytes_per_face_and_direction=face * 5 * 4)
+        log('slab', k, out[f'{k}way'])
+    return out
+
+
+def batch_tune_bench(crops, steps=20):
+    """The batched step alone: the autotuner's sweep of launch shapes and
+    grids, then `steps` eager launches of the picked shape (no graph, no
+    convergence bookkeeping), for kernel traces of the step itself."""
+    from lbflow_amd.engine import Batch
+    labels = {75: ('GV', 1e-6, 1, 0), 255: ('SD', 90.0, 0, 0)}
+    b = Batch(3, 19, 'MRT', 0.8, 1.0, labels, crops)
+    per_cu, tune_ms = b.tune(steps=10)
+    threads, unit, grid = b.shape()
+    b.init()
+    t0 = time.perf_counter()
+    b.advance(steps)
+    wall = time.perf_counter() - t0
+    out = dict(crops=len(crops), fluid_sites=b.fluid, tuned_threads_per_workgroup=threads, tuned_slots_per_unit=unit,
+               tuned_workgroups_per_cu=per_cu, tuned_ms_per_step=round(tune_ms, 4),
+               eager_ms_per_step=round(wall / steps * 1e3, 4))
+    b.close()
+    log('batch tune', out)
+    return out
+
+
 def warm_start_bench(crops, interval=50, max_steps=60000):
     """Learned warm start on unseen 64^3 crops (seeds 20000+i, porosity
     0.15-0.35): the batched FP32 engine from rest and from the network's
@@ -272,6 +312,11 @@ def cpu_baseline(size, cores, steps=2, reps=3):
                        f'{reps} runs of {steps} steps, median {t:.2f} s per run')
 
 
+# legs of the default run after the headline FP32 sparse step
+LEGS = ('dense', 'fp64', 'ttp', 'ttpwarm', 'colour', 'batch', 'warm')
+OPTIONAL_LEGS = ('batchtune', 'slabs')   # run only when named in --legs
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument('--gpus', type=int, default=1)
@@ -285,21 +330,28 @@ def main():
     ap.add_argument('--max-steps', type=int, default=100000)
     ap.add_argument('--batch', type=int, default=512, help='crops of the batched leg (0: skip)')
     ap.add_argument('--warm-crops', type=int, default=256, help='unseen crops of the warm-start leg (0: skip)')
+    ap.add_argument('--legs', default=','.join(LEGS),
+                    help='comma-separated legs after the headline step (default: all of ' + ','.join(LEGS) +
+                    '; also ' + ','.join(OPTIONAL_LEGS) + ')')
     args = ap.parse_args()
+    legs = set(x for x in args.legs.split(',') if x)
+    if legs - set(LEGS) - set(OPTIONAL_LEGS):
+        ap.error(f'unknown legs {sorted(legs - set(LEGS) - set(OPTIONAL_LEGS))}')
     if args.gpus != 1:
         print(f'bench.py: the engine runs on one GPU; --gpus {args.gpus} is not supported '
               '(sharding across GPUs is not built)', file=sys.stderr)
         sys.exit(2)
-    full = not args.quick
+    if args.quick:
+        legs = set()
 
     # CPU-side generation first: the process pools fork before the GPU is used
     crops = warm_crops = None
-    if full and args.batch:
+    if legs & {'batch', 'batchtune'} and args.batch:
         from lbflow_amd import geometry as G
         t0 = time.perf_counter()
         crops, _ = G.batch_crops(args.batch, workers=16)
         log(f'batch: {args.batch} crops generated in {time.perf_counter() - t0:.1f} s')
-    if full and args.warm_crops and WEIGHTS.exists():
+    if 'warm' in legs and args.warm_crops and WEIGHTS.exists():
         from lbflow_amd import surrogate as S
         warm_crops = S.make_crops(args.warm_crops, 20000, 64)
     spec, labels = workload(args.size)
@@ -313,24 +365,30 @@ def main():
     main_run = measure(spec, labels, FP32, False, args.steps, args.warmup)
     log('fp32 sparse', main_run)
     extra = {}
-    if full:
+    if 'dense' in legs:
         extra['c2_fp32_dense_all_sites'] = summary(measure(spec, labels, FP32, True, args.steps, args.warmup),
                                                    labels, fluid)
         extra['c2_fp32_dense_fluid_sites'] = summary(
             measure(spec, labels, FP32, False, args.steps, args.warmup, dense=True), labels, fluid)
+    if 'fp64' in legs:
         extra['c2_fp64_sparse'] = summary(measure(spec, labels, FP64, False, max(10, args.steps // 2), 5),
                                           labels, fluid)
         extra['c2_fp64_dense_all_sites'] = summary(measure(spec, labels, FP64, True, max(10, args.steps // 4), 5),
                                                    labels, fluid)
+    if 'ttp' in legs:
         extra['c2_time_to_permeability'] = time_to_permeability(spec, labels, args.ttp_interval, args.max_steps)
-        if WEIGHTS.exists():
-            extra['c2_time_to_permeability_warm_vs_cold'] = ttp_warm_vs_cold(labels, args.ttp_interval,
-                                                                             args.max_steps)
+    if 'ttpwarm' in legs and WEIGHTS.exists():
+        extra['c2_time_to_permeability_warm_vs_cold'] = ttp_warm_vs_cold(labels, args.ttp_interval, args.max_steps)
+    if 'colour' in legs:
         extra['c3_colour'] = colour_bench(max(10, args.steps // 2), 5)
-        if crops:
-            extra['c4_batch'] = batch_bench(crops, 200, 60000)
-        if warm_crops:
-            extra['c5_warm_vs_cold'] = warm_start_bench(warm_crops)
+    if crops and 'batch' in legs:
+        extra['c4_batch'] = batch_bench(crops, 200, 60000)
+    if 'slabs' in legs:
+        extra['c2_slabs'] = slab_bench(labels, args.steps, args.warmup)
+    if crops and 'batchtune' in legs:
+        extra['c4_batch_tune'] = batch_tune_bench(crops)
+    if warm_crops:
+        extra['c5_warm_vs_cold'] = warm_start_bench(warm_crops)
     cpu = None
     cores = cpu_cores() if args.cpu_cores == 0 else args.cpu_cores
     if cores > 0:

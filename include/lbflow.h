@@ -360,6 +360,57 @@ int lbf_conv3d_forward(int cin, int cout, int taps, uint32_t side, uint32_t samp
                        int relu, int out_f32);
 int lbf_conv3d_wgrad(int cin, int cout, int taps, uint32_t side, uint32_t samples, const void* in,
                      uint32_t in_stride, const void* dz, uint32_t dz_stride, uint32_t splits, float* dw);
+/* Cubic side the network was built for. */
+uint32_t lbf_net_side(const lbf_net* net);
+
+/* --- device buffers (no reference counterpart) ----------------------------
+ * Plain device memory on the library's own HIP runtime, for callers that
+ * have no GPU runtime of their own and must hand DEVICE pointers to the
+ * entry points above (lbf_net_forward, lbf_conv3d_*, lbf_batch_crop_fields).
+ * lbf_buffer_alloc zero-fills; all calls are synchronous. */
+int  lbf_buffer_alloc(uint64_t bytes, void** ptr);
+void lbf_buffer_free(void* ptr);
+int  lbf_buffer_upload(void* dst, const void* host, uint64_t bytes);
+int  lbf_buffer_download(void* host, const void* src, uint64_t bytes);
+
+/* --- training set of the warm-start network (no reference counterpart) ----
+ * count cubic samples of `side` voxels (a power of two) held on the device:
+ * per sample a pore mask byte and (u_z, u_y, u_x, rho) floats per voxel.
+ * The reference workflow has no learned warm start; the training data are
+ * steady fields of the batched engine (lbf_batch_run), taken over without
+ * leaving the device, scaled as targets u nu / F and (rho - 1) / F over
+ * their RMS on pore voxels (0 on solid voxels), and fed to the network in
+ * minibatches mapped by one of 16 lattice symmetries: bit 0 flips z, bit 1
+ * flips y, bit 3 flips x (u_z, u_y, u_x as vectors; u_y, u_z and the
+ * density deviation change sign under the x flip, the force staying +x),
+ * bit 2 swaps y and z after the flips (u_y and u_z swapped, carried across
+ * their RMS).  Flips of y, z and the swap map steady fields of the periodic
+ * D3Q19 lattice exactly; the x flip holds in the Stokes limit. */
+typedef struct lbf_dataset lbf_dataset;
+int  lbf_dataset_create(uint32_t side, uint32_t count, lbf_dataset** out);
+void lbf_dataset_destroy(lbf_dataset* data);
+const char* lbf_dataset_error(const lbf_dataset* data);   /* data may be NULL: last create error */
+/* Masks of samples first..first+n-1 from host memory, n * side^3 bytes. */
+int lbf_dataset_set_masks(lbf_dataset* data, uint32_t first, uint32_t n, const uint8_t* masks);
+/* Targets (or raw fields) of samples first..first+n-1 from host memory. */
+int lbf_dataset_set_targets(lbf_dataset* data, uint32_t first, uint32_t n, const float* targets);
+/* Steady fields (lbf_batch_crop_fields) of crops crop0..crop0+n-1 of a batch
+ * of side^3 crops into samples first..first+n-1. */
+int lbf_dataset_take_fields(lbf_dataset* data, uint32_t first, lbf_batch* batch, uint32_t crop0, uint32_t n);
+/* Raw fields -> targets, once, in place.  fit = 1: rms[4] receives the RMS
+ * of each raw target channel over all pore voxels (FP64, fixed order);
+ * fit = 0: rms[4] is given. */
+int lbf_dataset_normalise(lbf_dataset* data, double tau, double force, int fit, double* rms);
+/* Sample i mapped by symmetry sym (0..15) into host memory (mask side^3
+ * bytes, target side^3 * 4 floats; either may be NULL). */
+int lbf_dataset_sample(lbf_dataset* data, uint32_t i, uint32_t sym, uint8_t* mask, float* target);
+/* Device addresses of the masks and targets of sample 0 (samples follow
+ * each other), for lbf_net_forward / lbf_net_timed_train. */
+int lbf_dataset_device(const lbf_dataset* data, uint8_t** masks, float** targets);
+/* One Adam step (lbf_net_train_step) on samples idx[0..count) mapped by
+ * symmetry sym; idx is host memory; count <= the network's capacity. */
+int lbf_net_train_samples(lbf_net* net, lbf_dataset* data, const uint32_t* idx, uint32_t count, uint32_t sym,
+                          const lbf_adam* opt, double* loss);
 
 #ifdef __cplusplus
 }

@@ -427,6 +427,8 @@ int lbf_net_set_params(lbf_net* n, const float* host)
     return 0;
 }
 
+uint32_t lbf_net_side(const lbf_net* n) { return n ? n->side : 0; }
+
 int lbf_net_forward(lbf_net* n, const uint8_t* mask, uint32_t count, float* out)
 {
     if (!n || !mask || !out) return -1;

@@ -25,7 +25,8 @@ SYMBOLS = ('lbf_create', 'lbf_destroy', 'lbf_last_error', 'lbf_init', 'lbf_step'
            'lbf_device_bytes', 'lbf_step_kernel', 'lbf_step_bytes', 'lbf_timed_advance',
            'lbf_copy_bandwidth', 'lbf_batch_create', 'lbf_batch_destroy', 'lbf_batch_error',
            'lbf_batch_sites', 'lbf_batch_tune', 'lbf_batch_init', 'lbf_batch_advance', 'lbf_batch_run',
-           'lbf_batch_crop_populations', 'lbf_batch_time', 'lbf_batch_shape')
+           'lbf_batch_crop_populations', 'lbf_batch_time', 'lbf_batch_shape', 'lbf_buffer_alloc', 'lbf_buffer_free',
+           'lbf_buffer_upload', 'lbf_buffer_download')
 
 
 class Label(C.Structure):
@@ -144,6 +145,14 @@ def load_library(path=None):
     L.lbf_batch_crop_populations.restype = i32
     L.lbf_batch_time.argtypes = [vp]
     L.lbf_batch_time.restype = u64
+    L.lbf_buffer_alloc.argtypes = [u64, C.POINTER(vp)]
+    L.lbf_buffer_alloc.restype = i32
+    L.lbf_buffer_free.argtypes = [vp]
+    L.lbf_buffer_free.restype = None
+    L.lbf_buffer_upload.argtypes = [vp, vp, u64]
+    L.lbf_buffer_upload.restype = i32
+    L.lbf_buffer_download.argtypes = [vp, vp, u64]
+    L.lbf_buffer_download.restype = i32
     if path is None:
         _LIB = L
     return L
@@ -155,6 +164,48 @@ def _ptr(a):
 
 class EngineError(RuntimeError):
     pass
+
+
+class Buffer:
+    """Device memory of the engine's HIP runtime (lbf_buffer_*), for entry
+    points that take device pointers; shape and dtype describe its contents."""
+
+    def __init__(self, shape, dtype, data=None):
+        self.L = load_library()
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = C.c_void_p()
+        if self.L.lbf_buffer_alloc(self.nbytes, C.byref(h)):
+            raise EngineError(f'cannot allocate {self.nbytes} bytes of device memory')
+        self.h = h
+        if data is not None:
+            self.upload(data)
+
+    @property
+    def ptr(self):
+        return self.h
+
+    def upload(self, data):
+        a = np.ascontiguousarray(data, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f'{a.nbytes} bytes for a buffer of {self.nbytes}')
+        if self.L.lbf_buffer_upload(self.h, a.ctypes.data_as(C.c_void_p), a.nbytes):
+            raise EngineError('device upload failed')
+        return self
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.L.lbf_buffer_download(out.ctypes.data_as(C.c_void_p), self.h, self.nbytes):
+            raise EngineError('device download failed')
+        return out
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.lbf_buffer_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def copy_bandwidth(nbytes=1 << 30, iters=20):

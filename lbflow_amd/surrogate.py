@@ -12,9 +12,11 @@ Pipeline: generate sphere-pack crops -> solve them to steady state with the
 batched engine -> train the network on (mask, fields) -> evaluate cold vs
 warm starts on unseen crops.  A warm start (run_crops) runs entirely in
 the engine's library: masks go in from host memory and the prediction stays
-in the network's device buffers (lbf_batch_warm_start).  Training keeps its
-data set in torch tensors (device memory only); all arithmetic of the
-network runs in the engine's HIP kernels.
+in the network's device buffers (lbf_batch_warm_start).  Training runs there
+too: the solved crops' steady fields move from the batch into a device-side
+training set (lbf_dataset_*), which scales them into targets and gathers
+each minibatch, mapped by a random lattice symmetry, into the buffers the
+network trains on.  No other GPU runtime is involved.
 """
 import ctypes as C
 import json
@@ -24,7 +26,7 @@ from multiprocessing import get_context
 import numpy as np
 
 from . import geometry as G
-from .engine import Batch, EngineError, load_library
+from .engine import Batch, Buffer, EngineError, load_library
 
 PORE, SOLID = 75, 255
 
@@ -70,6 +72,26 @@ def _bind(L):
     L.lbf_batch_relax.restype = i32
     L.lbf_batch_warm_start.argtypes = [vp, vp, vp, u32, vp, vp, u32]
     L.lbf_batch_warm_start.restype = i32
+    L.lbf_net_side.argtypes = [vp]
+    L.lbf_net_side.restype = u32
+    L.lbf_dataset_create.argtypes = [u32, u32, C.POINTER(vp)]
+    L.lbf_dataset_create.restype = i32
+    L.lbf_dataset_destroy.argtypes = [vp]
+    L.lbf_dataset_error.argtypes = [vp]
+    L.lbf_dataset_error.restype = C.c_char_p
+    for name in ('lbf_dataset_set_masks', 'lbf_dataset_set_targets'):
+        getattr(L, name).argtypes = [vp, u32, u32, vp]
+        getattr(L, name).restype = i32
+    L.lbf_dataset_take_fields.argtypes = [vp, u32, vp, u32, u32]
+    L.lbf_dataset_take_fields.restype = i32
+    L.lbf_dataset_normalise.argtypes = [vp, C.c_double, C.c_double, i32, vp]
+    L.lbf_dataset_normalise.restype = i32
+    L.lbf_dataset_sample.argtypes = [vp, u32, u32, vp, vp]
+    L.lbf_dataset_sample.restype = i32
+    L.lbf_dataset_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+    L.lbf_dataset_device.restype = i32
+    L.lbf_net_train_samples.argtypes = [vp, vp, vp, u32, u32, C.POINTER(Adam), C.POINTER(C.c_double)]
+    L.lbf_net_train_samples.restype = i32
     L._net_bound = True
     return L
 
@@ -78,22 +100,27 @@ def lib():
     return _bind(load_library())
 
 
-def _ptr(t):
-    return C.c_void_p(t.data_ptr())
+class _View:
+    """Device memory owned elsewhere (a Dataset's samples): address and shape."""
+
+    def __init__(self, ptr, shape):
+        self.ptr, self.shape = ptr, tuple(shape)
 
 
-def _torch():
-    """torch with its GPU runtime started.  The torch wheel carries its own HIP
-    runtime next to the engine's; both work in one process only if torch's
-    starts first, so every entry point that mixes them calls this first."""
-    import torch
-    torch.cuda.init()
-    return torch
+def _dev(a, dtype, shape=None):
+    """Device memory (a Buffer or a _View) as is, or host data uploaded into a
+    new Buffer."""
+    if isinstance(a, (Buffer, _View)):
+        return a
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return Buffer(a.shape if shape is None else shape, dtype, a)
 
 
 class Net:
-    """The warm-start network on the GPU.  forward/gradients/train_step take
-    torch CUDA tensors; warm starts (run_crops) need no torch."""
+    """The warm-start network on the GPU.  forward, gradients and train_step
+    take host arrays (uploaded per call) or device Buffers (masks uint8
+    [n, S^3], targets float32 [n, S^3, 4]); training from a Dataset
+    (train_samples) and warm starts (run_crops) keep everything on the device."""
 
     def __init__(self, side=64, capacity=4, seed=1):
         self.L = lib()
@@ -126,33 +153,45 @@ class Net:
         assert flat.size == self.n_params
         self._check(self.L.lbf_net_set_params(self.h, flat.ctypes.data_as(C.c_void_p)))
 
-    def forward(self, mask, out):
-        """mask: uint8 CUDA tensor [n, S, S, S]; out: float32 CUDA tensor [n, S^3, 4]."""
-        import torch
-        torch.cuda.synchronize()
-        self._check(self.L.lbf_net_forward(self.h, _ptr(mask), mask.shape[0], _ptr(out)))
+    def forward(self, mask, out=None):
+        """Network output (u_z, u_y, u_x, rho channels, normalised) of masks
+        [n, S^3]; into the device Buffer `out` [n, S^3, 4] when given, else
+        returned as a host array."""
+        m = _dev(mask, np.uint8)
+        n = m.shape[0]
+        o = out if out is not None else Buffer((n, self.side ** 3, 4), np.float32)
+        self._check(self.L.lbf_net_forward(self.h, m.ptr, n, o.ptr))
+        return None if out is not None else o.numpy()
 
     def gradients(self, mask, target, weights=(1, 1, 1, 1)):
-        import torch
-        torch.cuda.synchronize()
+        m, t = _dev(mask, np.uint8), _dev(target, np.float32)
         w = np.ascontiguousarray(weights, dtype=np.float32)
         g = np.zeros(self.n_params, np.float32)
         loss = C.c_double()
-        self._check(self.L.lbf_net_gradients(self.h, _ptr(mask), _ptr(target), mask.shape[0],
-                                             w.ctypes.data_as(C.c_void_p), C.byref(loss),
-                                             g.ctypes.data_as(C.c_void_p)))
+        self._check(self.L.lbf_net_gradients(self.h, m.ptr, t.ptr, m.shape[0], w.ctypes.data_as(C.c_void_p),
+                                             C.byref(loss), g.ctypes.data_as(C.c_void_p)))
         return loss.value, g
 
     def train_step(self, mask, target, opt):
+        m, t = _dev(mask, np.uint8), _dev(target, np.float32)
         loss = C.c_double()
-        self._check(self.L.lbf_net_train_step(self.h, _ptr(mask), _ptr(target), mask.shape[0], C.byref(opt),
-                                              C.byref(loss)))
+        self._check(self.L.lbf_net_train_step(self.h, m.ptr, t.ptr, m.shape[0], C.byref(opt), C.byref(loss)))
+        return loss.value
+
+    def train_samples(self, data, idx, sym, opt):
+        """One Adam step on samples idx of a Dataset mapped by symmetry sym."""
+        i = np.ascontiguousarray(idx, dtype=np.uint32)
+        loss = C.c_double()
+        if self.L.lbf_net_train_samples(self.h, data.h, i.ctypes.data_as(C.c_void_p), i.size, int(sym),
+                                        C.byref(opt), C.byref(loss)):
+            raise EngineError(data.error())
         return loss.value
 
     def timed_train(self, mask, target, opt, steps):
+        m, t = _dev(mask, np.uint8), _dev(target, np.float32)
         ms, fl = C.c_double(), C.c_double()
-        self._check(self.L.lbf_net_timed_train(self.h, _ptr(mask), _ptr(target), mask.shape[0], C.byref(opt),
-                                               steps, C.byref(ms), C.byref(fl)))
+        self._check(self.L.lbf_net_timed_train(self.h, m.ptr, t.ptr, m.shape[0], C.byref(opt), steps,
+                                               C.byref(ms), C.byref(fl)))
         return ms.value, fl.value
 
 
@@ -188,17 +227,81 @@ def labels(force):
     return {PORE: ('GV', force, 1, 0), SOLID: ('SD', 90.0, 0, 0)}
 
 
+class Dataset:
+    """Training set on the device (lbf_dataset_*): per sample a pore mask and
+    (u_z, u_y, u_x, rho) per voxel, raw steady fields until normalise() turns
+    them into targets."""
+
+    def __init__(self, side, count):
+        self.L = lib()
+        h = C.c_void_p()
+        if self.L.lbf_dataset_create(side, count, C.byref(h)):
+            raise EngineError(self.L.lbf_dataset_error(None).decode())
+        self.h = h
+        self.side, self.count = side, count
+        self.rms = None
+
+    def error(self):
+        return self.L.lbf_dataset_error(self.h).decode()
+
+    def _check(self, rc):
+        if rc:
+            raise EngineError(self.error())
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.lbf_dataset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_masks(self, first, masks):
+        m = np.ascontiguousarray(masks, dtype=np.uint8)
+        self._check(self.L.lbf_dataset_set_masks(self.h, first, m.shape[0], m.ctypes.data_as(C.c_void_p)))
+
+    def set_targets(self, first, targets):
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        self._check(self.L.lbf_dataset_set_targets(self.h, first, t.shape[0], t.ctypes.data_as(C.c_void_p)))
+
+    def take_fields(self, first, batch, crop0, n):
+        self._check(self.L.lbf_dataset_take_fields(self.h, first, batch.h, crop0, n))
+
+    def normalise(self, tau, force, rms=None):
+        """Raw fields -> targets in place; fits the per-channel RMS unless given.
+        Returns the RMS."""
+        r = np.zeros(4) if rms is None else np.ascontiguousarray(rms, dtype=np.float64)
+        self._check(self.L.lbf_dataset_normalise(self.h, tau, force, int(rms is None), r.ctypes.data_as(C.c_void_p)))
+        self.rms = r
+        return r
+
+    def sample(self, i, sym=0):
+        """(mask [S^3] uint8, fields or targets [S^3, 4] float32) of sample i
+        mapped by symmetry sym."""
+        s3 = self.side ** 3
+        m = np.empty(s3, np.uint8)
+        t = np.empty((s3, 4), np.float32)
+        self._check(self.L.lbf_dataset_sample(self.h, i, sym, m.ctypes.data_as(C.c_void_p),
+                                              t.ctypes.data_as(C.c_void_p)))
+        return m, t
+
+    def device(self):
+        """Device addresses of the masks and targets of sample 0."""
+        m, t = C.c_void_p(), C.c_void_p()
+        self._check(self.L.lbf_dataset_device(self.h, C.byref(m), C.byref(t)))
+        return m, t
+
+
 def solve(crops, tau=0.8, force=1e-5, tol=1e-5, interval=100, max_steps=40000, log=None, chunk=4096):
     """Steady fields of every crop with the batched engine, `chunk` crops per
-    batch.  Returns (fields as a float32 CUDA tensor [n, S^3, 4] =
+    batch.  Returns (a Dataset holding the crops' pore masks and raw fields
     (u_z, u_y, u_x, rho), per-crop results)."""
-    torch = _torch()
-    s3 = crops[0].size
-    fields = torch.empty((len(crops), s3, 4), dtype=torch.float32, device='cuda')
-    L = lib()
+    side = crops[0].shape[0]
+    data = Dataset(side, len(crops))
     res = []
     for c0 in range(0, len(crops), chunk):
         part = crops[c0:c0 + chunk]
+        data.set_masks(c0, np.stack([(c == PORE) for c in part]).astype(np.uint8).reshape(len(part), -1))
         b = Batch(3, 19, 'MRT', tau, 1.0, labels(force), part)
         b.tune(steps=4)
         b.init()
@@ -207,12 +310,9 @@ def solve(crops, tau=0.8, force=1e-5, tol=1e-5, interval=100, max_steps=40000, l
         if log:
             log(f'solved crops {c0}-{c0 + len(part) - 1} in {sec:.1f} s, steps {min(x["steps"] for x in r)}-'
                 f'{max(x["steps"] for x in r)}, converged {sum(x["converged"] for x in r)}')
-        torch.cuda.synchronize()
-        for c in range(len(part)):
-            if L.lbf_batch_crop_fields(b.h, c, C.c_void_p(fields[c0 + c].data_ptr())):
-                raise EngineError(L.lbf_batch_error(b.h).decode())
+        data.take_fields(c0, b, 0, len(part))
         b.close()
-    return fields, res
+    return data, res
 
 
 class Scaling:
@@ -224,40 +324,20 @@ class Scaling:
         self.force = force
 
     @classmethod
-    def fit(cls, fields, masks, tau, force):
-        import torch
-        nu = (tau - 0.5) / 3.0
-        sq = torch.zeros(4, dtype=torch.float64, device=fields.device)
-        count = 0
-        for i in range(0, fields.shape[0], 256):   # chunks keep every tensor below 2^31 elements
-            m = masks[i:i + 256].reshape(-1).bool()
-            raw = fields[i:i + 256].reshape(-1, 4)[m].double()
-            raw[:, :3] *= nu / force
-            raw[:, 3] = (raw[:, 3] - 1.0) / force
-            sq += (raw ** 2).sum(dim=0)
-            count += raw.shape[0]
-        rms = [float(torch.sqrt(sq[c] / count)) for c in range(4)]
-        return cls(rms, tau, force)
-
-    def normalise(self, fields, masks):
-        """Training targets; solid sites (no flow, no density) are 0."""
-        out = fields.clone()
-        out[..., :3] *= self.nu / self.force
-        out[..., 3] = (out[..., 3] - 1.0) / self.force
-        for c in range(4):
-            out[..., c] /= self.rms[c]
-        return out * masks.reshape(masks.shape[0], -1, 1).to(out.dtype)
+    def fit(cls, data, tau, force):
+        """Fits the RMS on a Dataset of raw fields and normalises it in place."""
+        return cls(data.normalise(tau, force), tau, force)
 
     def fields(self, pred, tau, force, mask):
-        """Network output -> physical (u, rho) for a run with (tau, force)."""
+        """Network output [n, S^3, 4] -> physical (u, rho) for a run with
+        (tau, force); velocities 0 on solid voxels (mask [n, S^3])."""
         nu = (tau - 0.5) / 3.0
-        out = pred.clone()
+        out = np.array(pred, dtype=np.float32)
         for c in range(4):
-            out[..., c] *= self.rms[c]
-        out[..., :3] *= force / nu
-        out[..., 3] = 1.0 + out[..., 3] * force
-        m = mask.reshape(mask.shape[0], -1, 1).to(out.dtype)
-        out[..., :3] *= m
+            out[..., c] *= np.float32(self.rms[c])
+        out[..., :3] *= np.float32(force / nu)
+        out[..., 3] = np.float32(1.0) + out[..., 3] * np.float32(force)
+        out[..., :3] *= np.asarray(mask, dtype=np.float32).reshape(out.shape[0], -1, 1)
         return out
 
     def affine(self, tau, force):
@@ -276,52 +356,49 @@ class Scaling:
 
 
 def symmetry(masks, targets, k, rms):
-    """One of the 16 lattice symmetries that keep a +x body-force problem a
-    +x body-force problem, applied to masks [n, S^3] and normalised targets
-    [n, S^3, 4] = (u_z, u_y, u_x, rho): bit 0 flips z (u_z -> -u_z), bit 1
-    flips y (u_y -> -u_y), bit 2 swaps y and z, bit 3 flips x (the mirrored
-    field of the reversed force, negated: u_y, u_z and the density
-    deviation change sign).  Targets are 0 on solid sites.  The periodic
-    D3Q19 lattice with halfway bounce-back is invariant under bits 0-2, so
-    the field of the mapped mask is the mapped field exactly; bit 3 holds in
-    the Stokes (linear) limit, to relative size ~1e-4 at the training force."""
-    import torch
+    """Host statement of the lattice symmetry the training set applies on the
+    device (lbf_dataset_sample, lbf_net_train_samples), for masks [n, S^3]
+    and normalised targets [n, S^3, 4] = (u_z, u_y, u_x, rho): bit 0 flips z
+    (u_z -> -u_z), bit 1 flips y (u_y -> -u_y), bit 3 flips x (the mirrored
+    field of the reversed force, negated: u_y, u_z and the density deviation
+    change sign), then bit 2 swaps y and z (u_y and u_z swapped and carried
+    across their RMS).  Targets are 0 on solid sites.  The periodic D3Q19
+    lattice with halfway bounce-back is invariant under bits 0-2, so the field
+    of the mapped mask is the mapped field exactly; bit 3 holds in the Stokes
+    (linear) limit, to relative size ~1e-4 at the training force."""
     n = masks.shape[0]
-    shape = masks.shape
     s = round(targets.shape[1] ** (1.0 / 3.0))
-    m = masks.reshape(n, s, s, s)
-    t = targets.reshape(n, s, s, s, 4)
-    sign = torch.ones(4, device=t.device, dtype=t.dtype)
-    dims = []
+    m = np.asarray(masks).reshape(n, s, s, s)
+    t = np.asarray(targets).reshape(n, s, s, s, 4)
+    sign = np.ones(4, t.dtype)
+    axes = []
     if k & 1:
-        dims.append(1)
+        axes.append(1)
         sign[0] = -sign[0]
     if k & 2:
-        dims.append(2)
+        axes.append(2)
         sign[1] = -sign[1]
     if k & 8:
-        dims.append(3)
+        axes.append(3)
         sign[0], sign[1], sign[3] = -sign[0], -sign[1], -sign[3]
-    if dims:
-        m = torch.flip(m, dims)
-        t = torch.flip(t, dims)
+    if axes:
+        m = np.flip(m, axes)
+        t = np.flip(t, axes)
     t = t * sign
     if k & 4:
-        m = m.transpose(1, 2)
-        t = t.transpose(1, 2)[..., [1, 0, 2, 3]]
-        # channels are normalised per component: carry u_y <-> u_z across their RMS
-        ratio = torch.tensor([rms[1] / rms[0], rms[0] / rms[1], 1.0, 1.0], device=t.device, dtype=t.dtype)
+        m = m.transpose(0, 2, 1, 3)
+        t = t.transpose(0, 2, 1, 3, 4)[..., [1, 0, 2, 3]]
+        ratio = np.array([rms[1] / rms[0], rms[0] / rms[1], 1.0, 1.0], dtype=t.dtype)
         t = t * ratio
-    return m.reshape(shape).contiguous(), t.reshape(n, -1, 4).contiguous()
+    return np.ascontiguousarray(m).reshape(n, -1), np.ascontiguousarray(t).reshape(n, -1, 4)
 
 
-def train(net, masks, targets, epochs, lr=1e-3, batch=None, log=None, seed=0, lr_final=None, augment=None):
-    """Adam over shuffled minibatches; cosine learning-rate decay to lr_final.
-    With `augment` (the per-channel RMS of the target scaling) every
-    minibatch is mapped by a random one of the 16 lattice symmetries."""
-    import torch
+def train(net, data, epochs, lr=1e-3, batch=None, log=None, seed=0, lr_final=None, augment=False):
+    """Adam over shuffled minibatches of a normalised Dataset; cosine
+    learning-rate decay to lr_final.  With `augment` every minibatch is mapped
+    by a random one of the 16 lattice symmetries."""
     batch = batch or net.capacity
-    n = masks.shape[0]
+    n = data.count
     rng = np.random.default_rng(seed)
     steps = epochs * (n // batch)
     lr_final = lr * 0.05 if lr_final is None else lr_final
@@ -330,16 +407,12 @@ def train(net, masks, targets, epochs, lr=1e-3, batch=None, log=None, seed=0, lr
     history = []
     t0 = time.perf_counter()
     for ep in range(epochs):
-        order = torch.as_tensor(rng.permutation(n), device='cuda')
+        order = rng.permutation(n).astype(np.uint32)
         tot = 0.0
         for i in range(n // batch):
-            idx = order[i * batch:(i + 1) * batch]
-            mb, tb = masks[idx].contiguous(), targets[idx].contiguous()
-            if augment is not None:
-                mb, tb = symmetry(mb, tb, int(rng.integers(16)), augment)
-            torch.cuda.synchronize()
+            sym = int(rng.integers(16)) if augment else 0
             opt.lr = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * k / max(1, steps)))
-            tot += net.train_step(mb, tb, opt)
+            tot += net.train_samples(data, order[i * batch:(i + 1) * batch], sym, opt)
             k += 1
         history.append(tot / max(1, n // batch))
         if log:
@@ -403,22 +476,19 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
     eval_crops = make_crops(n_eval, 20000, side)
     rep['crop_generation_s'] = round(time.perf_counter() - t0, 2)
     log(f'crops generated in {rep["crop_generation_s"]} s')
-    torch = _torch()   # after the crop pool has forked
     t0 = time.perf_counter()
-    fields, res = solve(train_crops, tau=tau, force=train_force, tol=train_tol, interval=100, max_steps=max_steps,
-                        log=log)
+    data, res = solve(train_crops, tau=tau, force=train_force, tol=train_tol, interval=100, max_steps=max_steps,
+                      log=log)
     rep['training_data'] = dict(seconds=round(time.perf_counter() - t0, 2),
                                 converged=int(sum(r['converged'] for r in res)),
                                 steps_mean=float(np.mean([r['steps'] for r in res])),
                                 fluid_sites=int(sum(int((c == PORE).sum()) for c in train_crops)))
-    masks = torch.as_tensor(np.stack([(c == PORE) for c in train_crops]).astype(np.uint8), device='cuda')
     del train_crops
-    scaling = Scaling.fit(fields, masks, tau, train_force)
-    targets = scaling.normalise(fields, masks).contiguous()
-    del fields
-    torch.cuda.synchronize()
+    scaling = Scaling.fit(data, tau, train_force)
     net = Net(side=side, capacity=capacity, seed=1)
-    ms, flops = net.timed_train(masks[:capacity].contiguous(), targets[:capacity].contiguous(), adam(0.0), 3)
+    masks, targets = data.device()
+    ms, flops = net.timed_train(_View(masks, (capacity, side ** 3)), _View(targets, (capacity, side ** 3, 4)),
+                                adam(0.0), 3)
     rep['train_step'] = dict(ms=round(ms, 3), tflops=round(flops / ms / 1e9, 1), samples_per_step=capacity,
                              conv_flops_per_step=flops)
     log(f'train step {ms:.2f} ms, {flops / ms / 1e9:.1f} TFLOP/s (convolutions)')
@@ -428,12 +498,10 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
         net.set_params(np.load(init, allow_pickle=False)['params'].astype(np.float32))
         rep['init'] = str(init)
     t0 = time.perf_counter()
-    hist = train(net, masks, targets, epochs, lr=lr, batch=capacity, log=log,
-                 augment=scaling.rms if augment else None)
+    hist = train(net, data, epochs, lr=lr, batch=capacity, log=log, augment=augment)
     rep['augment'] = bool(augment)
     rep['training'] = dict(seconds=round(time.perf_counter() - t0, 2), loss=[round(h, 5) for h in hist])
-    del masks, targets
-    torch.cuda.empty_cache()
+    data.close()
     if out:
         save(out, net, scaling)
     rep['evaluation'] = evaluate(net, scaling, eval_crops, tau, force, eval_tol, interval, max_steps, relax_sweeps,
@@ -474,26 +542,21 @@ def diagnose(net, scaling, crops, tau=0.8, force=1e-6, log=print):
     fluid sites, the flux ratio sum(u_x pred) / sum(u_x true), and the error
     left after the best single scale factor (how much of the error is the
     global magnitude, the slowest mode of the warm-started run)."""
-    torch = _torch()
-    fields, _ = solve(crops, tau=tau, force=force, tol=1e-5, interval=100)
-    side = crops[0].shape[0]
-    masks = torch.as_tensor(np.stack([(c == PORE) for c in crops]).astype(np.uint8), device='cuda')
-    pred = torch.empty((net.capacity, side ** 3, 4), dtype=torch.float32, device='cuda')
+    data, _ = solve(crops, tau=tau, force=force, tol=1e-5, interval=100)
+    masks = np.stack([(c == PORE) for c in crops]).astype(np.uint8).reshape(len(crops), -1)
     rows = []
     for i in range(0, len(crops), net.capacity):
-        m = masks[i:i + net.capacity].contiguous()
-        p = pred[:m.shape[0]]
-        net.forward(m, p)
-        f = scaling.fields(p, tau, force, m)
-        torch.cuda.synchronize()
+        m = masks[i:i + net.capacity]
+        f = scaling.fields(net.forward(m), tau, force, m)
         for j in range(m.shape[0]):
-            up = f[j, :, :3].double()
-            ut = fields[i + j, :, :3].double()
-            err = float(torch.linalg.norm(up - ut) / torch.linalg.norm(ut))
+            up = f[j, :, :3].astype(np.float64)
+            ut = data.sample(i + j)[1][:, :3].astype(np.float64)
+            err = float(np.linalg.norm(up - ut) / np.linalg.norm(ut))
             alpha = float((up * ut).sum() / (up * up).sum())
-            err_s = float(torch.linalg.norm(alpha * up - ut) / torch.linalg.norm(ut))
+            err_s = float(np.linalg.norm(alpha * up - ut) / np.linalg.norm(ut))
             flux = float(up[:, 2].sum() / ut[:, 2].sum())
             rows.append((err, flux, alpha, err_s))
+    data.close()
     r = np.array(rows)
     rep = dict(n=len(rows), rel_l2_mean=float(r[:, 0].mean()), flux_ratio_mean=float(r[:, 1].mean()),
                flux_ratio_std=float(r[:, 1].std()), best_scale_mean=float(r[:, 2].mean()),
@@ -505,11 +568,12 @@ def diagnose(net, scaling, crops, tau=0.8, force=1e-6, log=print):
 def train_bench(side, capacity, steps, log=print):
     """Training-step throughput on synthetic masks and targets (no solver):
     `steps` Adam steps of `capacity` crops, timed with HIP events."""
-    torch = _torch()
     crops = make_crops(capacity, 30000, side, workers=1)
-    masks = torch.as_tensor(np.stack([(c == PORE) for c in crops]).astype(np.uint8), device='cuda')
+    masks = Buffer((capacity, side ** 3), np.uint8,
+                   np.stack([(c == PORE) for c in crops]).astype(np.uint8).reshape(capacity, -1))
     rng = np.random.default_rng(0)
-    targets = torch.as_tensor(rng.standard_normal((capacity, side ** 3, 4)).astype(np.float32), device='cuda')
+    targets = Buffer((capacity, side ** 3, 4), np.float32,
+                     rng.standard_normal((capacity, side ** 3, 4)).astype(np.float32))
     net = Net(side=side, capacity=capacity, seed=1)
     net.timed_train(masks, targets, adam(1e-4), 2)        # warm-up
     ms, flops = net.timed_train(masks, targets, adam(1e-4), steps)

@@ -9,15 +9,19 @@
   agree to BF16 accuracy;
 * training reduces the loss; the batch engine's field export / warm start
   round-trips a converged state.
+
+Device memory comes from the engine's own runtime (lbf_buffer_*); BF16
+operands are built on the host as the upper 16 bits of float32 values
+rounded to nearest even.
 """
 import sys
 
 import numpy as np
 import pytest
-import torch
 
 from lbflow_amd import geometry as G
 from lbflow_amd import surrogate as S
+from lbflow_amd.engine import Buffer
 from oracle import surrogate_ref as R
 
 pytestmark = pytest.mark.gpu
@@ -31,16 +35,30 @@ def kp(cin, taps):
     return (taps * cin + 31) // 32 * 32
 
 
+def to_bf16(x):
+    """float32 -> BF16 bits (uint16), round to nearest even."""
+    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+    return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16(h):
+    return (np.asarray(h, dtype=np.uint32) << 16).view(np.float32)
+
+
 def rand_bf16(rng, shape, scale=1.0):
-    return torch.from_numpy((rng.standard_normal(shape) * scale).astype(np.float32)).to(torch.bfloat16)
+    return to_bf16((rng.standard_normal(shape) * scale).astype(np.float32))
 
 
 def packed_weights(w):
-    """[cout][taps][cin] float -> device layout [cout][kp] bf16 (zero padded)."""
+    """[cout][taps][cin] BF16 bits -> device layout [cout][kp] BF16 bits (zero padded)."""
     cout, taps, cin = w.shape
-    out = torch.zeros((cout, kp(cin, taps)), dtype=torch.bfloat16)
+    out = np.zeros((cout, kp(cin, taps)), dtype=np.uint16)
     out[:, :taps * cin] = w.reshape(cout, taps * cin)
     return out
+
+
+def ptr_at(buf, offset_bytes):
+    return buf.ptr.value + offset_bytes
 
 
 @pytest.mark.parametrize('cin,cout,taps,side,n,in_extra,out_extra,relu,f32', [
@@ -62,30 +80,30 @@ def test_conv_forward(cin, cout, taps, side, n, in_extra, out_extra, relu, f32):
     nv = n * side ** 3
     x = rand_bf16(rng, (nv, cin + in_extra))
     w = rand_bf16(rng, (cout, taps, cin), 1.0 / np.sqrt(taps * cin))
-    b = torch.from_numpy(rng.standard_normal(cout).astype(np.float32))
+    b = rng.standard_normal(cout).astype(np.float32)
     say('conv forward', cin, cout, taps, side, n)
-    dx = x.cuda()
-    dw = packed_weights(w).cuda()
-    db = b.cuda()
+    dx = Buffer(x.shape, np.uint16, x)
+    dw = Buffer((cout, kp(cin, taps)), np.uint16, packed_weights(w))
+    db = Buffer(b.shape, np.float32, b)
     ostride = cout + out_extra
-    out = torch.zeros((nv, ostride), dtype=torch.float32 if f32 else torch.bfloat16, device='cuda')
+    out = Buffer((nv, ostride), np.float32 if f32 else np.uint16)
     # read channels [in_extra, in_extra + cin) of the wider input, write channels [out_extra, ...)
-    in_ptr = dx.data_ptr() + 2 * in_extra
-    out_ptr = out.data_ptr() + out.element_size() * out_extra
-    torch.cuda.synchronize()
-    rc = L.lbf_conv3d_forward(cin, cout, taps, side, n, in_ptr, cin + in_extra, dw.data_ptr(), db.data_ptr(),
+    in_ptr = ptr_at(dx, 2 * in_extra)
+    out_ptr = ptr_at(out, out.dtype.itemsize * out_extra)
+    rc = L.lbf_conv3d_forward(cin, cout, taps, side, n, in_ptr, cin + in_extra, dw.ptr, db.ptr,
                               out_ptr, ostride, int(relu), int(f32))
     assert rc == 0, L.lbf_net_error(None)
-    got = out.float().cpu().numpy()[:, out_extra:]
-    xs = x.float().numpy()[:, in_extra:].reshape(n, side, side, side, cin)
-    want = R.conv_bruteforce(xs, w.float().numpy().astype(np.float64), b.numpy()).reshape(nv, cout)
+    full = out.numpy() if f32 else from_bf16(out.numpy())
+    got = full[:, out_extra:]
+    xs = from_bf16(x)[:, in_extra:].reshape(n, side, side, side, cin)
+    want = R.conv_bruteforce(xs, from_bf16(w).astype(np.float64), b).reshape(nv, cout)
     if relu:
         want = np.maximum(want, 0)
     err = np.max(np.abs(got - want)) / np.max(np.abs(want))
     say('  max rel err', err)
     assert err < (1e-5 if f32 else 8e-3)
     if out_extra:
-        assert np.all(out.float().cpu().numpy()[:, :out_extra] == 0)   # channels outside the range untouched
+        assert np.all(full[:, :out_extra] == 0)   # channels outside the range untouched
 
 
 @pytest.mark.parametrize('cin,cout,taps,side,n,splits', [
@@ -99,15 +117,13 @@ def test_conv_weight_gradient(cin, cout, taps, side, n, splits):
     x = rand_bf16(rng, (nv, cin))
     dz = rand_bf16(rng, (nv, cout))
     say('conv wgrad', cin, cout, taps, side, n, splits)
-    dx, ddz = x.cuda(), dz.cuda()
-    dw = torch.zeros((cout, kp(cin, taps)), dtype=torch.float32, device='cuda')
-    torch.cuda.synchronize()
-    rc = L.lbf_conv3d_wgrad(cin, cout, taps, side, n, dx.data_ptr(), cin, ddz.data_ptr(), cout, splits,
-                            dw.data_ptr())
+    dx, ddz = Buffer(x.shape, np.uint16, x), Buffer(dz.shape, np.uint16, dz)
+    dw = Buffer((cout, kp(cin, taps)), np.float32)
+    rc = L.lbf_conv3d_wgrad(cin, cout, taps, side, n, dx.ptr, cin, ddz.ptr, cout, splits, dw.ptr)
     assert rc == 0, L.lbf_net_error(None)
-    got = dw.cpu().numpy()
-    xs = x.float().numpy().astype(np.float64).reshape(n, side, side, side, cin)
-    g = dz.float().numpy().astype(np.float64)
+    got = dw.numpy()
+    xs = from_bf16(x).astype(np.float64).reshape(n, side, side, side, cin)
+    g = from_bf16(dz).astype(np.float64)
     want = np.zeros((cout, taps, cin))
     for t in range(taps):
         dzz, dy, dxx = (t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1) if taps == 27 else (0, 0, 0)
@@ -136,10 +152,7 @@ def test_network_forward_matches_reference():
     p = p + rng.standard_normal(p.size).astype(np.float32) * 0.01   # non-zero biases and head rows
     net.set_params(p)
     mask, _ = masks_and_targets(rng, n, side)
-    dm = torch.as_tensor(mask, device='cuda')
-    out = torch.empty((n, side ** 3, 4), dtype=torch.float32, device='cuda')
-    net.forward(dm, out)
-    got = out.cpu().numpy().astype(np.float64)
+    got = net.forward(mask.reshape(n, -1)).astype(np.float64)
     p64 = p.astype(np.float64)
     want = R.forward(mask, R.unpack(p64), round_bf16=True).numpy()
     rel = np.linalg.norm(got - want) / np.linalg.norm(want)
@@ -155,7 +168,7 @@ def test_network_gradients_match_reference():
     net.set_params(p)
     mask, target = masks_and_targets(rng, n, side)
     w4 = (1.0, 0.5, 2.0, 1.0)
-    loss, g = net.gradients(torch.as_tensor(mask, device='cuda'), torch.as_tensor(target, device='cuda'), w4)
+    loss, g = net.gradients(mask.reshape(n, -1), target, w4)
     lref, gref = R.gradients(mask, target, p.astype(np.float64), w4, round_bf16=True)
     _, gexact = R.gradients(mask, target, p.astype(np.float64), w4, round_bf16=False)
     say('loss', loss, lref)
@@ -187,7 +200,7 @@ def test_training_reduces_loss():
     m = mask.astype(np.float32).reshape(n, side, side, side)
     blur = (m + np.roll(m, 1, 1) + np.roll(m, -1, 1) + np.roll(m, 1, 2) + np.roll(m, -1, 2)) / 5
     target = np.stack([blur, 0.5 * blur, m, 1 - blur], axis=-1).reshape(n, side ** 3, 4).astype(np.float32)
-    dm, dt = torch.as_tensor(mask, device='cuda'), torch.as_tensor(target, device='cuda')
+    dm, dt = Buffer((n, side ** 3), np.uint8, mask.reshape(n, -1)), Buffer(target.shape, np.float32, target)
     opt = S.adam(3e-3)
     first = net.train_step(dm, dt, opt)
     for _ in range(120):
@@ -203,16 +216,16 @@ def test_field_export_and_warm_start_roundtrip():
     the run stops after the minimum number of reports, with the same K."""
     crops = [G.sphere_pack((16, 16, 16), 0.4, 1.5, 3.0, 200 + i) for i in range(3)]
     force, tau, tol, interval = 1e-5, 0.8, 1e-5, 40
-    fields, res = S.solve(crops, tau=tau, force=force, tol=tol, interval=interval, max_steps=20000)
+    data, res = S.solve(crops, tau=tau, force=force, tol=tol, interval=interval, max_steps=20000)
     assert all(r['converged'] for r in res)
+    fields = [Buffer((16 ** 3, 4), np.float32, data.sample(c)[1]) for c in range(len(crops))]
     from lbflow_amd.engine import Batch
     L = S.lib()
     out = {}
     for relax in (0, 5, 20):
         b = Batch(3, 19, 'MRT', tau, 1.0, S.labels(force), crops).init()
-        torch.cuda.synchronize()
         for c in range(len(crops)):
-            assert L.lbf_batch_set_fields(b.h, c, fields[c].data_ptr()) == 0
+            assert L.lbf_batch_set_fields(b.h, c, fields[c].ptr) == 0
         if relax:
             assert L.lbf_batch_relax(b.h, relax) == 0
         out[relax], _ = b.run(axis=2, tol=tol, interval=interval, max_steps=20000)
@@ -239,18 +252,17 @@ def test_warm_start_leaves_stagnant_pockets_at_rest():
     g[9:12, 9:12, 3:8] = 75          # and a closed 3x3x5 pocket
     crops = [g]
     force, tau, tol, interval = 1e-5, 0.8, 1e-4, 40
-    fields, res = S.solve(crops, tau=tau, force=force, tol=tol, interval=interval, max_steps=20000)
+    data, res = S.solve(crops, tau=tau, force=force, tol=tol, interval=interval, max_steps=20000)
     assert res[0]['converged']
     rng = np.random.default_rng(1)
-    noisy = fields.clone()
-    scale = float(fields[..., :3].abs().max())
-    noisy[..., :3] += torch.as_tensor(rng.standard_normal(noisy[..., :3].shape).astype(np.float32) * 0.3 * scale,
-                                      device='cuda')
+    noisy = data.sample(0)[1]
+    scale = float(np.abs(noisy[:, :3]).max())
+    noisy[:, :3] += rng.standard_normal(noisy[:, :3].shape).astype(np.float32) * 0.3 * scale
     from lbflow_amd.engine import Batch
     L = S.lib()
     b = Batch(3, 19, 'MRT', tau, 1.0, S.labels(force), crops).init()
-    torch.cuda.synchronize()
-    assert L.lbf_batch_set_fields(b.h, 0, noisy[0].contiguous().data_ptr()) == 0
+    dn = Buffer(noisy.shape, np.float32, noisy)
+    assert L.lbf_batch_set_fields(b.h, 0, dn.ptr) == 0
     assert L.lbf_batch_relax(b.h, 5) == 0
     warm, _ = b.run(axis=2, tol=tol, interval=interval, max_steps=20000)
     say('cold', res[0]['steps'], res[0]['K'], 'noisy warm', warm[0]['steps'], warm[0]['K'])

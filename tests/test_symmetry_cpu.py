@@ -10,7 +10,6 @@ the quadratic equilibrium terms (1e-4 relative measured at F = 1e-5).
 """
 import numpy as np
 import pytest
-import torch
 
 from lbflow_amd import geometry as G
 from lbflow_amd import cases as CS
@@ -41,12 +40,12 @@ def base():
 def test_symmetry_maps_solutions_to_solutions(base, k, tol):
     labels, field = base
     # masks as the training pipeline holds them: [n, S, S, S]
-    mask = torch.as_tensor((labels == 75).reshape(1, N, N, N).astype(np.uint8))
-    tgt = torch.as_tensor(field.reshape(1, -1, 4))
+    mask = (labels == 75).reshape(1, -1).astype(np.uint8)
+    tgt = field.reshape(1, -1, 4)
     m2, t2 = symmetry(mask, tgt, k, np.ones(4))
-    lab2 = np.where(m2.numpy().reshape(N, N, N) == 1, 75, 255).astype(np.uint8)
+    lab2 = np.where(m2.reshape(N, N, N) == 1, 75, 255).astype(np.uint8)
     want = run(lab2)
-    got = t2.numpy().reshape(-1, 4)
+    got = t2.reshape(-1, 4)
     err = np.abs(got - want).max(axis=0) / np.abs(want).max(axis=0)
     assert np.all(err < tol), (k, err)
 
@@ -54,9 +53,9 @@ def test_symmetry_maps_solutions_to_solutions(base, k, tol):
 def test_density_parity_is_pinned(base):
     """The x flip must negate the density deviation: keeping its sign is caught."""
     labels, field = base
-    mask = torch.as_tensor((labels == 75).reshape(1, N, N, N).astype(np.uint8))
-    m2, t2 = symmetry(mask, torch.as_tensor(field.reshape(1, -1, 4)), 8, np.ones(4))
-    lab2 = np.where(m2.numpy().reshape(N, N, N) == 1, 75, 255).astype(np.uint8)
+    mask = (labels == 75).reshape(1, -1).astype(np.uint8)
+    m2, t2 = symmetry(mask, field.reshape(1, -1, 4), 8, np.ones(4))
+    lab2 = np.where(m2.reshape(N, N, N) == 1, 75, 255).astype(np.uint8)
     want = run(lab2)
-    wrong = t2.numpy().reshape(-1, 4)[:, 3] * -1.0
+    wrong = t2.reshape(-1, 4)[:, 3] * -1.0
     assert np.abs(wrong - want[:, 3]).max() / np.abs(want[:, 3]).max() > 1.0
