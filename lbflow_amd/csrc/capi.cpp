@@ -64,6 +64,8 @@ struct lbf_ctx {
     float *d_A32 = nullptr, *d_B32 = nullptr;
     int32_t* d_wall = nullptr;
     DevWall* d_walls = nullptr;
+    WallF32* d_walls32 = nullptr;   // FP32 colour: the wall entries in FP32
+    float* d_dens_slot = nullptr;   // FP32 colour, sparse: indicator per fluid slot
     uint64_t* d_film = nullptr;      // per-site film links, only with HF labels
     double *d_capgrad = nullptr, *d_capfield = nullptr;   // CSF gradient and curvature field
     double* d_tanhg = nullptr;       // tanh wall gradient per wall entry
@@ -400,6 +402,10 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         rc |= dalloc(c, &c->d_densN, N * 8);
         rc |= dalloc(c, &c->d_wall, N * 4);
         rc |= dalloc(c, &c->d_walls, c->links.nwall * sizeof(DevWall));
+        if (c->elem == 4) {
+            rc |= dalloc(c, &c->d_walls32, c->links.nwall * sizeof(WallF32));
+            if (c->layout == 2) rc |= dalloc(c, &c->d_dens_slot, c->nfluid * sizeof(float));
+        }
         if (!c->links.film.empty()) rc |= dalloc(c, &c->d_film, N * sizeof(uint64_t));
         if (d.wall_gradient == LBF_WALL_TANH) rc |= dalloc(c, &c->d_tanhg, c->links.nwall * 3 * 8);
         if (d.capillary == LBF_CAPILLARY_CSF) {
@@ -503,6 +509,20 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
             }
         }
         rc |= upload(c, c->d_walls, walls.data(), walls.size() * sizeof(DevWall));
+        if (c->d_walls32) {
+            std::vector<WallF32> w32(c->links.nwall);
+            for (uint64_t w = 0; w < c->links.nwall; ++w) {
+                std::memset(&w32[w], 0, sizeof(WallF32));
+                w32[w].sinca = float(walls[w].sinca);
+                w32[w].cosca = float(walls[w].cosca);
+                for (int k = 0; k < 9; ++k) {
+                    w32[w].tib[k] = float(walls[w].tib[k]);
+                    w32[w].tibinv[k] = float(walls[w].tibinv[k]);
+                    w32[w].ginv[k] = float(walls[w].ginv[k]);
+                }
+            }
+            rc |= upload(c, c->d_walls32, w32.data(), w32.size() * sizeof(WallF32));
+        }
         if (c->d_film) rc |= upload(c, c->d_film, c->links.film.data(), N * sizeof(uint64_t));
     }
     if (rc) {
@@ -706,6 +726,8 @@ static int run_step(lbf_ctx* c, bool init, uint32_t flags)
             a.fin32 = static_cast<const float*>(c->f[c->cur]);
             a.fout32 = static_cast<float*>(init ? c->f[c->cur] : c->f[c->cur ^ 1]);
             a.densN32 = reinterpret_cast<float*>(c->d_densN);
+            a.dens_slot = c->d_dens_slot;
+            a.walls32 = c->d_walls32;
             a.list = c->d_list;
             a.rank = c->d_rank;
             a.P = uint32_t(c->P);

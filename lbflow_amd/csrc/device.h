@@ -50,6 +50,17 @@ struct DevWall {
     double ginv[9];      // renormalized wall gradient: inverse fluid-link moment (colorfluid.cpp:917-957)
 };
 
+// A wall entry in FP32 for the FP32 colour kernels: the frames and the
+// contact angle as the kernels use them (the float of each FP64 value), and
+// the renormalizing inverse; one 128-byte line per wall site.
+struct __attribute__((aligned(128))) WallF32 {
+    float tib[9];
+    float tibinv[9];
+    float sinca, cosca;
+    float ginv[9];
+    float pad[3];
+};
+
 // Device failure record.
 struct DevStatus {
     uint32_t bits;
@@ -234,6 +245,8 @@ struct ColorArgs {
     double rest_err[19], mrest_err[19];   // s_q - float(s_q) and M of it
     // FP32 layouts: DENSE_FLUID (slot = site) or SPARSE (slot = fluid index)
     const uint32_t* list;          // SPARSE: fluid site of each slot
+    float* dens_slot;              // SPARSE: the FP32 indicator per fluid slot
+    const WallF32* walls32;        // the wall entries in FP32
     const uint2* rank;             // SPARSE: rank index (see stream.h)
     uint32_t P;                    // slots per plane
     // Half-permeable films (simulator.cpp:418-421, colorfluid.cpp:156-159):

@@ -25,6 +25,8 @@
 // equilibrium's rest part is written without cancelling order-one terms.
 // FP32 tolerance: the negative-component check uses 1e-5 rho (the FP64
 // kernel's 1e-10 rho is below the rounding of the stored FP32 populations).
+#include <type_traits>
+
 #include "collide_f32.h"
 #include "device.h"
 #include "stream.h"
@@ -58,10 +60,17 @@ __device__ __forceinline__ bool resolve32(const ColorArgs& a, uint32_t& site, ui
 
 // PB/VB sites apply the Zou-He boundary to the pulled populations and return
 // its density deviation and velocity in dzh / uzh.
-template <int Q, bool INIT, int LAYOUT, typename O, bool OWN = false>
+struct NoPre {
+    template <typename O> __device__ void operator()(const O*) const {}
+};
+
+// PRE (sparse): called with the pull offsets before the populations are
+// loaded (the step computes its colour gradient there, while only the
+// offsets are live).
+template <int Q, bool INIT, int LAYOUT, typename O, bool OWN = false, typename PRE = NoPre>
 __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, uint32_t slot, uint32_t desc,
                                              const DevLabel& lab, float* h, float* f0, double& r0, double& dr,
-                                             float& dzh, float* uzh)
+                                             float& dzh, float* uzh, PRE pre = PRE())
 {
     using L = Lattice<Q>;
     r0 = 0.0;
@@ -79,6 +88,13 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
         return;
     }
     if (INIT) {
+        if constexpr (LAYOUT == SPARSE && !std::is_same<PRE, NoPre>::value) {
+            // only the neighbours' offsets (the populations are set, not pulled)
+            O off[Q];
+            pull_offsets<Q, LAYOUT, O, true, 2 * Q>(a.rank, desc & DESC_LINK_MASK, slot, make_stencil(a.g, site),
+                                                    a.g.nx, a.P, off);
+            pre(off);
+        }
         // colorfluid.cpp:214-236: f[phase] = ai_phase rho_phase - 1.5 w (e.F), other phase 0
         const int ph = lab.phase;
 #pragma unroll
@@ -98,6 +114,7 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
     O off[Q];
     off[0] = pop_index<2 * Q, BLK, O>(a.P, 0, slot);
     pull_offsets<Q, LAYOUT, O, BLK, 2 * Q>(a.rank, desc & DESC_LINK_MASK, slot, stn, a.g.nx, uint32_t(P), off);
+    pre(off);
     // The boundary path is chosen before the loads, so that plain sites can
     // fold their populations into the sums as they arrive.
     const bool film_links = desc & DESC_FILM;
@@ -168,8 +185,11 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
 // colorGradient, colorfluid.cpp:435-570): bulk stencil, or at wall sites the
 // one-sided stencil, the renormalized or tanh reconstruction and the
 // contact-angle rotation.  Returns |g|.
-template <int Q>
-__device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t i, int x, int y, int z, float* g)
+//
+// dens(q) is the indicator at x + e_q (q >= 1; SOLID_N32 on solid sites)
+// and dens(0) the site's own.
+template <int Q, typename DN>
+__device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t i, DN dens, float* g)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
@@ -182,7 +202,7 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t
 #pragma unroll
             for (int k = 0; k < L::NREL; ++k) {
                 const int q = L::reld[d][k];
-                const float dnq = a.densN32[push_target<Q>(a.g, x, y, z, q)];
+                const float dnq = dens(q);
                 if (L::e[q][d] == 1) g[d] += float(L::w[q]) * dnq;
                 else g[d] -= float(L::w[q]) * dnq;
             }
@@ -193,13 +213,13 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t
         for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
         norm = sqrtf(n2);
     } else {
-        const float dn0 = a.densN32[i];
+        const float dn0 = dens(0);
 #pragma unroll
         for (int d = 0; d < D; ++d)
 #pragma unroll
             for (int k = 0; k < L::NREL; ++k) {
                 const int q = L::reld[d][k];
-                const float dnq = a.densN32[push_target<Q>(a.g, x, y, z, q)];
+                const float dnq = dens(q);
                 if (dnq == SOLID_N32) continue;
                 if (L::e[q][d] == 1) g[d] += float(L::w[q]) * (dnq - dn0);
                 else g[d] -= float(L::w[q]) * (dnq - dn0);
@@ -210,7 +230,7 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t
 #pragma unroll
         for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
         norm = sqrtf(n2);
-        const DevWall& wl = a.walls[wi];
+        const WallF32& wl = a.walls32[wi];
         if (a.renormalized) {
             // wallGradient renormalized (colorfluid.cpp:487-492): g = inverse g / 3
             float r[3];
@@ -218,7 +238,7 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t
             for (int c = 0; c < D; ++c) {
                 float t = 0.0f;
 #pragma unroll
-                for (int b = 0; b < D; ++b) t += g[b] * float(wl.ginv[c * D + b]);
+                for (int b = 0; b < D; ++b) t += g[b] * wl.ginv[c * D + b];
                 r[c] = t;
             }
 #pragma unroll
@@ -237,13 +257,13 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t
             for (int d = 0; d < D; ++d) n2 += g[d] * g[d];
             norm = sqrtf(n2);
         }
-        const float cA = float(wl.cosca), sA = float(wl.sinca);
+        const float cA = wl.cosca, sA = wl.sinca;
         if constexpr (D == 3) {
             float T[9], Ti[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                T[k] = float(wl.tib[k]);
-                Ti[k] = float(wl.tibinv[k]);
+                T[k] = wl.tib[k];
+                Ti[k] = wl.tibinv[k];
             }
             float gn[3];
 #pragma unroll
@@ -269,10 +289,10 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t
                 g[r] = s;
             }
         } else {
-            const float gn0 = float(wl.tib[0]) * g[0] + float(wl.tib[1]) * g[1];
+            const float gn0 = wl.tib[0] * g[0] + wl.tib[1] * g[1];
             const float hv0 = gn0 >= 0.0f ? norm * sA : -norm * sA, hv1 = norm * cA;
-            const float g0 = float(wl.tibinv[0]) * hv0 + float(wl.tibinv[1]) * hv1;
-            const float g1 = float(wl.tibinv[2]) * hv0 + float(wl.tibinv[3]) * hv1;
+            const float g0 = wl.tibinv[0] * hv0 + wl.tibinv[1] * hv1;
+            const float g1 = wl.tibinv[2] * hv0 + wl.tibinv[3] * hv1;
             g[0] = g0;
             g[1] = g1;
         }
@@ -368,7 +388,11 @@ __global__ void __launch_bounds__(256) color_density_f32(ColorArgs a)
     double r0, dr;
     load_color32<Q, INIT, LAYOUT, O, OWN>(a, site, slot, desc, lab, h, f0, r0, dr, dzh, uzh);
     const double rho = dr + a.rest32;
-    a.densN32[site] = float((2.0 * r0 - rho) / rho);   // (rho0 - rho1) / rho
+    const float chi = float((2.0 * r0 - rho) / rho);   // (rho0 - rho1) / rho
+    // the sparse step reads the indicator by slot; the CSF and tanh-wall
+    // passes read it per voxel
+    if (LAYOUT == SPARSE) a.dens_slot[slot] = chi;
+    if (LAYOUT != SPARSE || a.csf || a.tanh_wall) a.densN32[site] = chi;
 }
 
 template <int Q, bool MRT, bool INIT, int LAYOUT, typename O, bool CSF = false, bool OWN = false>
@@ -388,7 +412,28 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
 
     float h[Q], f0[Q], dzh = 0.0f, uzh[D];
     double r0d, drd;
-    load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0d, drd, dzh, uzh);
+    // sparse: the colour gradient from the indicator per slot, at the
+    // neighbours the pull has just located (x + e_q is the pull source of
+    // op(q); solid where link q bounces back)
+    constexpr bool SLOT_GRADIENT = LAYOUT == SPARSE && !CSF;
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    float norm = 0.0f;
+    if constexpr (SLOT_GRADIENT) {
+        // neighbour x + e_q is the pull source of op(q), its slot recovered
+        // from the blocked offset (block, lane) where link q does not bounce
+        const uint32_t links = desc & DESC_LINK_MASK;
+        auto gradient = [&](const O* off) __attribute__((always_inline)) {
+            norm = color_gradient_f32<Q>(a, i, [&](int q) __attribute__((always_inline)) {
+                if (q == 0) return a.dens_slot[slot];
+                const O o = off[L::op[q]];
+                const uint32_t m = (uint32_t(o / O(2 * Q * 64)) << 6) | uint32_t(o & O(63));
+                return ((links >> q) & 1u) ? SOLID_N32 : a.dens_slot[m];
+            }, g);
+        };
+        load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0d, drd, dzh, uzh, gradient);
+    } else {
+        load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0d, drd, dzh, uzh);
+    }
     int cx, cy, cz;
     site_coords(a.g, i, cx, cy, cz);
 
@@ -565,9 +610,8 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     }
 
     // colour gradient: the stored capillary gradient under CSF (no
-    // perturbation, colorfluid.cpp:389-411), else colorGradient
-    float g[3] = {0.0f, 0.0f, 0.0f};
-    float norm;
+    // perturbation, colorfluid.cpp:389-411), else colorGradient (sparse:
+    // computed above)
     if constexpr (CSF) {
         float n2 = 0.0f;
 #pragma unroll
@@ -576,8 +620,10 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
             n2 += g[d] * g[d];
         }
         norm = sqrtf(n2);
-    } else {
-        norm = color_gradient_f32<Q>(a, i, cx, cy, cz, g);
+    } else if constexpr (!SLOT_GRADIENT) {
+        norm = color_gradient_f32<Q>(a, i, [&](int q) {
+            return a.densN32[q == 0 ? i : push_target<Q>(a.g, cx, cy, cz, q)];
+        }, g);
     }
 
     // perturbation + recolouring (colorfluid.cpp:389-428).  Phase 0 takes the
@@ -690,7 +736,9 @@ __global__ void __launch_bounds__(256) color_capillary_f32(ColorArgs a)
     int x, y, z;
     site_coords(a.g, site, x, y, z);
     float g[3];
-    const float norm = color_gradient_f32<Q>(a, site, x, y, z, g);
+    const float norm = color_gradient_f32<Q>(a, site, [&](int q) {
+        return a.densN32[q == 0 ? site : push_target<Q>(a.g, x, y, z, q)];
+    }, g);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         a.capgrad[uint64_t(d) * N + site] = double(g[d]);
