@@ -585,6 +585,12 @@ static ScspParams scsp_params(lbf_ctx* c, uint32_t flags)
     for (int q = 0; q < c->Q; ++q) a.idist[q] = (c->Q == 9 ? Lattice<9>::w[q] : Lattice<19>::w[q]) * c->d.density[0];
     a.inv_rho0 = float(1.0 / c->d.density[0]);
     a.rho0_32 = float(c->d.density[0]);
+    // dense 3D lattices whose layers are whole 256-site chunks: columns of 32
+    // chunks through all layers
+    if (c->layout != 2 && c->D == 3 && c->g.nxy % 256 == 0 && (c->g.nxy / 256) % 32 == 0) {
+        a.tile_chunks = 32;
+        a.layer_chunks = c->g.nxy / 256;
+    }
     a.omega32 = float(a.omega);
     a.omh32 = float(a.omh);
     const double su = 1.0 / c->d.tau[0], sq = 8.0 * (2.0 - su) / (8.0 - su);
@@ -702,6 +708,25 @@ static ColorArgs color_args(lbf_ctx* c, uint32_t flags)
         a.mdai[k] = m;
         a.mrest_err[k] = e;
     }
+    for (int q = 0; q < 19; ++q) {
+        a.ai0f[q] = float(a.ai0[q]);
+        a.ai1f[q] = float(a.ai1[q]);
+        a.bif[q] = float(a.bi[q]);
+        a.dist0f[q] = float(a.dist0[q]);
+        a.dist0r[q] = double(float(a.dist0[q]));
+        a.daif[q] = float(a.dai[q]);
+        a.rest_errf[q] = float(a.rest_err[q]);
+        a.mai0f[q] = float(a.mai0[q]);
+        a.mdaif[q] = float(a.mdai[q]);
+        a.mrest_errf[q] = float(a.mrest_err[q]);
+    }
+    a.relax0f = float(a.relax0);
+    a.relax1f = float(a.relax1);
+    a.nu0f = float(a.nu0);
+    a.nu1f = float(a.nu1);
+    a.betaf = float(a.beta);
+    a.akf = float(a.ak);
+    a.rho_reff = float(a.rho_ref);
     return a;
 }
 

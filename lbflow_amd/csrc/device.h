@@ -178,6 +178,10 @@ struct ScspParams {
     double idist[19];        // w_q * rho0
     float inv_rho0, rho0_32, omega32, omh32;
     float c1_nu, c2_nu, c1_q, c2_q;   // FP32 MRT: s and 1 - s/2 of the two non-conserved rate classes
+    // dense layouts: workgroups walk columns of tile_chunks 256-site chunks
+    // through all z layers (layer_chunks chunks per layer), so the z +- 1
+    // rows a workgroup pulls were just read by its neighbours; 0: site order
+    uint32_t tile_chunks, layer_chunks;
 };
 
 // Device buffers of the single-phase step kernels.
@@ -246,6 +250,14 @@ struct ColorArgs {
     // FP32 layouts: DENSE_FLUID (slot = site) or SPARSE (slot = fluid index)
     const uint32_t* list;          // SPARSE: fluid site of each slot
     float* dens_slot;              // SPARSE: the FP32 indicator per fluid slot
+    // the FP32 kernels' constants as they use them (the float of each FP64
+    // value above, double(float(dist0)) for the FP64 recolouring), so that
+    // they enter the arithmetic as scalar operands instead of being
+    // converted per site
+    float ai0f[19], ai1f[19], bif[19], dist0f[19], daif[19], rest_errf[19];
+    float mai0f[19], mdaif[19], mrest_errf[19];
+    double dist0r[19];
+    float relax0f, relax1f, nu0f, nu1f, betaf, akf, rho_reff;
     const WallF32* walls32;        // the wall entries in FP32
     const uint2* rank;             // SPARSE: rank index (see stream.h)
     uint32_t P;                    // slots per plane

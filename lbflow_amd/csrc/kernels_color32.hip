@@ -100,7 +100,7 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const double f = (ph == 0 ? a.dist0[q] : a.dist1[q]) - 1.5 * L::w[q] * lab.ef[q];
-            h[q] = float(f - double(float(a.dist0[q])));
+            h[q] = float(f - a.dist0r[q]);
             f0[q] = ph == 0 ? float(f) : 0.0f;
             r0 += double(f0[q]);
             dr += double(h[q]);
@@ -162,7 +162,7 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 if (lab.phase == 0) {
-                    h[q] = f0[q] - float(a.dist0[q]);
+                    h[q] = f0[q] - a.dist0f[q];
                 } else {
                     h[q] = h[q] - f0[q];
                     f0[q] = 0.0f;
@@ -170,7 +170,7 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
             }
         }
         if (lab.kind & KIND_ZOUHE)
-            zouhe_color_f32<Q>(h, f0, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, float(a.rho_ref), a.dist0, dzh, uzh);
+            zouhe_color_f32<Q>(h, f0, a.zh[desc >> DESC_LABEL_SHIFT], lab.phase, a.rho_reff, a.dist0, dzh, uzh);
         r0 = 0.0;
         dr = 0.0;
 #pragma unroll
@@ -446,10 +446,10 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     else if (r0d < -NEG_TOL32 * rtd || r1d < -NEG_TOL32 * rtd) raise_status(a.status, 1, a.t);
     const float inv_rt = 1.0f / rt;
     const float dn = (r0 - r1) * inv_rt;
-    float tau = (1.0f + dn) * float(a.relax0) + (1.0f - dn) * float(a.relax1);
+    float tau = (1.0f + dn) * a.relax0f + (1.0f - dn) * a.relax1f;
     if (a.harmonic) {
         const float phi = r0 * inv_rt;
-        tau = 0.5f + 3.0f / (phi / float(a.nu0) + (1.0f - phi) / float(a.nu1));
+        tau = 0.5f + 3.0f / (phi / a.nu0f + (1.0f - phi) / a.nu1f);
     }
     if (!isfinite(tau) || tau <= 0.5f) raise_status(a.status, 2, a.t);
     // bodyForce (colorfluid.cpp:736-750): under CSF first (F_cap/2)/rho on
@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
                 else if (L::e[q][d] == -1) { eu -= u[d]; ef -= F[d]; }
             }
             const float w = float(L::w[q]);
-            const float heq = drest * float(a.ai0[q]) + r1 * float(a.dai[q]) + float(a.rest_err[q]) +
+            const float heq = drest * a.ai0f[q] + r1 * a.daif[q] + a.rest_errf[q] +
                               rt * w * (eu * (3.0f + 4.5f * eu) - 1.5f * usq);
             h[q] += (heq - h[q]) * inv_tau;
             if (forced) h[q] += cg * w * (3.0f * (ef - uF) + 9.0f * eu * ef);
@@ -583,7 +583,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
                     if (MT.T2[k][p][b] != 0.0) meq += float(4.5 * MT.T2[k][p][b]) * u[p] * u[b];
             }
             if (MT.T0[k] != 0.0) meq -= float(1.5 * MT.T0[k]) * usq;
-            meq = meq * rt + drest * float(a.mai0[k]) + r1 * float(a.mdai[k]) + float(a.mrest_err[k]);
+            meq = meq * rt + drest * a.mai0f[k] + r1 * a.mdaif[k] + a.mrest_errf[k];
             const float s = MT.rate[k] == 1 ? su : sq;
             y[k] = (s * (meq - m) + (1.0f - 0.5f * s) * mg) * float(1.0 / MT.n2[k]);
         }
@@ -636,8 +636,8 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     const float dens0T = float(dens0T64);
     bool finite = true;
     if (norm > 1e-9f) {
-        const float b01 = float(a.beta) * dens0T * r1 * inv_rt;
-        const float akn = float(a.ak) * norm / tau;
+        const float b01 = a.betaf * dens0T * r1 * inv_rt;
+        const float akn = a.akf * norm / tau;
         const float inv_norm = 1.0f / norm;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -648,10 +648,10 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
                 else if (L::e[q][d] == -1) angle -= g[d];
             }
             angle *= inv_norm;
-            if constexpr (!CSF) h[q] += akn * (float(L::w[q]) * angle * angle - float(a.bi[q]));
+            if constexpr (!CSF) h[q] += akn * (float(L::w[q]) * angle * angle - a.bif[q]);
             if (L::diagonal(q)) angle *= (1.0f / SQRT2_32);
-            const float feq0 = r0 * float(a.ai0[q]) + r1 * float(a.ai1[q]);
-            const float fo = float(fma(dens0T64, double(h[q]) + double(float(a.dist0[q])), double(b01 * angle * feq0)));
+            const float feq0 = r0 * a.ai0f[q] + r1 * a.ai1f[q];
+            const float fo = float(fma(dens0T64, double(h[q]) + a.dist0r[q], double(b01 * angle * feq0)));
             finite = finite && isfinite(h[q]) && isfinite(fo);
             st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
             st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
@@ -659,7 +659,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const float fo = float(dens0T64 * (double(h[q]) + double(float(a.dist0[q]))));
+            const float fo = float(dens0T64 * (double(h[q]) + a.dist0r[q]));
             finite = finite && isfinite(h[q]) && isfinite(fo);
             st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
             st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
@@ -784,7 +784,7 @@ __global__ void __launch_bounds__(256) color_csf_init_f32(ColorArgs a)
         }
         const double g0 = f0[q] - 1.5 * L::w[q] * ef * r0 / rt;
         const double g1 = f1[q] - 1.5 * L::w[q] * ef * r1 / rt;
-        st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), float(g0 + g1 - double(float(a.dist0[q]))));
+        st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), float(g0 + g1 - a.dist0r[q]));
         st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), float(g0));
     }
 }

@@ -16,8 +16,8 @@
 // Forward: a workgroup of 4 waves covers 256 voxels x BN output channels;
 // each wave owns 64 voxels (4 row tiles) x BN.  A is loaded straight from
 // memory into MFMA registers (neighbour gather, 16 B per lane); the weight
-// panel is staged through LDS (128 k per stage, rows padded by 16 B so the 16
-// lanes of a group read 16 distinct bank quads).  The input gradient is the
+// panel is staged through LDS (128 k per stage, rows padded by 32 B: 72
+// dwords, conflict-free, see LDS_NOTE).  The input gradient is the
 // same kernel on the transposed, tap-mirrored weights.
 //
 // Weight gradient: the reduction runs over voxels, so both operands need 8
@@ -27,6 +27,15 @@
 // own FP32 partial, summed in split order by a second kernel (deterministic).
 #include "conv_types.h"
 #include "device.h"
+
+// LDS_NOTE.  Every MFMA operand read from LDS is a 16-byte ds_read_b128 in
+// which the 16 lanes of a row group read 16 rows (voxels or channels) at a
+// fixed row stride and the 4 groups sit 16 bytes apart.  Measured on the
+// MI355X with SQ_LDS_BANK_CONFLICT: a row stride of 4 (mod 8) dwords (80 B,
+// 112 B, 528 B, ...) costs about 4 conflict cycles per read, 0 (mod 16)
+// dwords more; strides of 8 (mod 16) dwords (96 B, 288 B, 544 B, ...) and
+// 2 or 6 (mod 8) are conflict-free.  The tiles below pad their rows to 8
+// (mod 16) dwords, keeping 16-byte alignment.
 
 namespace lbf {
 
@@ -68,7 +77,7 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
     constexpr int KP = conv_kp(CIN, TAPS);
     constexpr int KREAL = TAPS * CIN;
     constexpr int KC = 128;
-    constexpr int ROW = KC + 8;
+    constexpr int ROW = KC + 16;                   // 72 dwords per LDS row (see LDS_NOTE)
     constexpr int NT = BN / 16;
     static_assert(CIN % 8 == 0 && BN % 16 == 0, "channel counts must be multiples of 8 / 16");
     __shared__ __attribute__((aligned(16))) __bf16 sw[BN * ROW];
@@ -156,13 +165,13 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
 // with its one-voxel halo staged in LDS once per 32-channel chunk and reused
 // by all 27 taps (the direct kernel above gathers every tap from L2).
 // A workgroup covers a BZ x BY x BX block of 256 voxels (BX = min(side, 32));
-// the halo tile is (BZ+2)(BY+2)(BX+2) voxels x 32 channels, 80 B per voxel so
-// that the 16 lanes of an MFMA row group read 16 distinct bank quads.  Each
+// the halo tile is (BZ+2)(BY+2)(BX+2) voxels x 32 channels, 96 B per voxel
+// (conflict-free reads, see LDS_NOTE).  Each
 // wave owns 64 voxels (4 row tiles) x BN output channels; the weight
 // fragments come from global memory (the [Cout][KP] panel of a layer is
 // shared by every workgroup and stays in the caches).
 // ---------------------------------------------------------------------------
-constexpr int HALO_VS = 40;          // bf16 per halo voxel in LDS
+constexpr int HALO_VS = 48;          // bf16 per halo voxel in LDS: 24 dwords (see LDS_NOTE)
 constexpr int HALO_MAX = 34 * 6 * 4; // largest halo tile (side >= 32: 32 x 4 x 2 block)
 
 struct HaloShape {
@@ -299,7 +308,7 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
     constexpr int KP = conv_kp(CIN, TAPS);
     constexpr int KREAL = TAPS * CIN;
     constexpr int VC = 128;                        // voxels per LDS stage (4 K steps)
-    constexpr int ROW = VC + 8;                    // bf16 per LDS row: 272 B, 16 lanes hit 16 bank quads
+    constexpr int ROW = VC + 16;                   // bf16 per LDS row: 72 dwords (see LDS_NOTE)
     constexpr int WM = BM >= 32 ? 2 : 1, WN = 4 / WM;
     constexpr int MT = BM / WM / 16, NT = BN / WN / 16;
     constexpr int NA = (BM / 8) * VC / 256, NB = (BN / 8) * VC / 256;   // 16-byte loads per thread per stage
@@ -409,8 +418,9 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
 // are written as dwords (voxel pairs), and the next block's operands are
 // fetched into registers while the current block runs on MFMA.
 // ---------------------------------------------------------------------------
-constexpr int WG_DZS = 256 + 8;         // bf16 per dz^T row (132 dwords: 16 rows hit 16 bank quads)
-constexpr int WG_XPLANE = 968;          // bf16 per channel plane of one copy (max rows * (BX + 8) + 8)
+constexpr int WG_DZS = 256 + 16;        // bf16 per dz^T row: 136 dwords (see LDS_NOTE)
+constexpr int WG_XPLANE = 976;          // bf16 per channel plane of one copy (>= max rows * (BX + 8) + 8;
+                                        // 488 dwords, see LDS_NOTE)
 
 __device__ __forceinline__ uint32_t pack2(__bf16 lo, __bf16 hi)
 {

@@ -58,11 +58,17 @@ template <int LAYOUT>
 __device__ __forceinline__ bool resolve(const ScspParams& p, const uint32_t* __restrict__ list, uint32_t& site,
                                         uint32_t& slot)
 {
-    const uint32_t k = xcd_block() * 256u + threadIdx.x;
+    uint32_t k = xcd_block() * 256u + threadIdx.x;
     if constexpr (LAYOUT == SPARSE) {
         if (k >= p.nfluid) return false;
         site = list[k];
     } else {
+        if (p.tile_chunks) {
+            // block b -> (column t, layer z, chunk j of the column)
+            const uint32_t b = k >> 8, T = p.tile_chunks;
+            const uint32_t j = b % T, r = b / T, z = r % p.g.nz, t = r / p.g.nz;
+            k = ((z * p.layer_chunks + t * T + j) << 8) | threadIdx.x;
+        }
         if (k >= p.g.N) return false;
         site = k;
     }
