@@ -25,6 +25,8 @@
 // input are fetched into registers one stage ahead, transposed into LDS
 // ([channel][voxel] rows, padded to 272 B), and every voxel split writes its
 // own FP32 partial, summed in split order by a second kernel (deterministic).
+#include <type_traits>
+
 #include "conv_types.h"
 #include "device.h"
 
@@ -224,6 +226,20 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
         hv[i] = ((lz + 1) * h.hy + (ly + 1)) * h.hx + (lx + 1);
     }
     const int oy = h.hx, oz = h.hx * h.hy;
+    // the lane's four A rows as LDS pointers at tap (0, 0, 0); a tap adds a
+    // uniform offset
+    const __bf16* arow[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) arow[i] = tile + hv[i] * HALO_VS + 8 * grp;
+    // staging: thread t copies 8-channel group q of halo voxels t/4, t/4 + 64,
+    // ...; its first voxel's halo coordinates once, then steps of 64 voxels
+    // as (+ d64x, + d64y) with carries, no division per voxel (d64y + 1 <= hy
+    // for every side 8 .. 256, so one carry into z suffices); the voxel's
+    // offset inside its sample fits 32 bits
+    const int sq = threadIdx.x & 3, sv0 = threadIdx.x >> 2;
+    const int sx0 = sv0 % h.hx, sy0 = (sv0 / h.hx) % h.hy, sz0 = sv0 / (h.hx * h.hy);
+    const int d64x = 64 % h.hx, d64y = 64 / h.hx;
+    const __bf16* in_s = in + sbase * g.in_stride + 8 * sq;
     f32x4 acc[4][NT];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -235,22 +251,44 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
 
     for (int c0 = 0; c0 < CIN; c0 += CC) {
         __syncthreads();   // the previous chunk's MFMAs are done with the tile
-        for (int t = threadIdx.x; t < hv_n * 4; t += 256) {
-            const int v = t >> 2, q = t & 3;
-            const int hx = v % h.hx, hy = (v / h.hx) % h.hy, hz = v / (h.hx * h.hy);
-            const uint32_t x = (x0 + uint32_t(hx) - 1u) & m, y = (y0 + uint32_t(hy) - 1u) & m,
-                           z = (z0 + uint32_t(hz) - 1u) & m;
-            const size_t gv = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls) | x);
-            *reinterpret_cast<bf16x8*>(tile + v * HALO_VS + 8 * q) =
-                *reinterpret_cast<const bf16x8*>(in + gv * g.in_stride + c0 + 8 * q);
+        {
+            // 4 voxels per trip: their 4 loads are in flight together
+            const __bf16* src = in_s + c0;
+            int hx = sx0, hy = sy0, hz = sz0;
+            for (int v = sv0; v < hv_n; v += 256) {
+                bf16x8 r[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (v + 64 * u < hv_n) {
+                        const uint32_t x = (x0 + uint32_t(hx) - 1u) & m, y = (y0 + uint32_t(hy) - 1u) & m,
+                                       z = (z0 + uint32_t(hz) - 1u) & m;
+                        const uint32_t lv = (z << (2 * ls)) | (y << ls) | x;
+                        r[u] = *reinterpret_cast<const bf16x8*>(src + uint64_t(lv) * g.in_stride);
+                    }
+                    hx += d64x;
+                    hy += d64y;
+                    if (hx >= h.hx) {
+                        hx -= h.hx;
+                        ++hy;
+                    }
+                    if (hy >= h.hy) {
+                        hy -= h.hy;
+                        ++hz;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (v + 64 * u < hv_n) *reinterpret_cast<bf16x8*>(tile + (v + 64 * u) * HALO_VS + 8 * sq) = r[u];
+            }
         }
         __syncthreads();
         bf16x8 bnext[NT];
 #pragma unroll
         for (int j = 0; j < NT; ++j) bnext[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + c0);
-        // 9 (dz, dy) rows of 3 taps; the next tap's weight fragments are
-        // loaded one tap ahead
-#pragma unroll 1
+        // 9 (dz, dy) rows of 3 taps, 3 rows per loop trip (a trip per row
+        // rotates the accumulators through AGPR moves at the back edge);
+        // the next tap's weight fragments are loaded one tap ahead
+#pragma unroll 3
         for (int t9 = 0; t9 < 9; ++t9) {
             const int row = (t9 / 3 - 1) * oz + (t9 % 3 - 1) * oy;
 #pragma unroll
@@ -265,9 +303,9 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
                         bnext[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + (tap + 1) * CIN + c0);
                 }
                 bf16x8 a[4];
+                const int toff = (row + dx - 1) * HALO_VS;   // uniform
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    a[i] = *reinterpret_cast<const bf16x8*>(tile + (hv[i] + row + dx - 1) * HALO_VS + 8 * grp);
+                for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(arow[i] + toff);
 #pragma unroll
                 for (int j = 0; j < NT; ++j)
 #pragma unroll
@@ -276,23 +314,31 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
             }
         }
     }
+    // output rows 4 grp .. 4 grp + 3 of a row tile are 4 consecutive voxels
+    // along x (the block extent along x is at least 8): one element offset
+    // per row tile, then + r out_stride
+    using OutT = typename std::conditional<OUTF32, float, __bf16>::type;
+    OutT* obase[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int lv = wave * 64 + 16 * i + 4 * grp;
+        const uint32_t lx = lv & ((1 << h.lbx) - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1),
+                       lz = lv >> (h.lbx + h.lby);
+        const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
+        obase[i] = static_cast<OutT*>(out) + v * g.out_stride + co0 + r16;
+    }
+    const uint32_t os = g.out_stride;
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-        const int co = co0 + 16 * j + r16;
-        if (co >= COUT) continue;
-        const float bb = BIAS ? bias[co] : 0.0f;
+        if (co0 + 16 * j + r16 >= COUT) continue;
+        const float bb = BIAS ? bias[co0 + 16 * j + r16] : 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int lv = wave * 64 + 16 * i + 4 * grp + r;
-                const uint32_t lx = lv & ((1 << h.lbx) - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1),
-                               lz = lv >> (h.lbx + h.lby);
-                const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
                 float x = acc[i][j][r] + bb;
                 if (ACT == ACT_RELU) x = x > 0.0f ? x : 0.0f;
-                if (OUTF32) static_cast<float*>(out)[v * g.out_stride + co] = x;
-                else static_cast<__bf16*>(out)[v * g.out_stride + co] = __bf16(x);
+                obase[i][r * os + 16 * j] = OutT(x);
             }
     }
 }
