@@ -95,7 +95,7 @@ __device__ __forceinline__ void batch_site(const BatchParams& p, const float* __
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         finite = finite && isfinite(h[q]);
-        st(fout, base + pop_index<Q, true, O>(0u, q, local), h[q]);
+        st_pop<true>(fout, base + pop_index<Q, true, O>(0u, q, local), h[q]);
     }
     if (!finite) raise_status(status, 4, p.t);
 }

@@ -653,16 +653,16 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
             const float feq0 = r0 * a.ai0f[q] + r1 * a.ai1f[q];
             const float fo = float(fma(dens0T64, double(h[q]) + a.dist0r[q], double(b01 * angle * feq0)));
             finite = finite && isfinite(h[q]) && isfinite(fo);
-            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
-            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
+            st_pop<LAYOUT == SPARSE>(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
+            st_pop<LAYOUT == SPARSE>(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
         }
     } else {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const float fo = float(dens0T64 * (double(h[q]) + a.dist0r[q]));
             finite = finite && isfinite(h[q]) && isfinite(fo);
-            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
-            st(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
+            st_pop<LAYOUT == SPARSE>(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, q, slot), h[q]);
+            st_pop<LAYOUT == SPARSE>(a.fout32, pop_index<2 * Q, LAYOUT == SPARSE, O>(a.P, Q + q, slot), fo);
         }
     }
     if (!finite) raise_status(a.status, 4, a.t);

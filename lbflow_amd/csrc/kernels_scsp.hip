@@ -344,7 +344,7 @@ scsp_step_f32(ScspParams p, const float* __restrict__ fin, float* __restrict__ f
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         finite = finite && isfinite(h[q]);
-        st(fout, pop_index<Q, BLK, O>(P, q, slot), h[q]);
+        st_pop<BLK>(fout, pop_index<Q, BLK, O>(P, q, slot), h[q]);
     }
     if (!finite) raise_status(status, 4, p.t);
 }
