@@ -65,8 +65,7 @@ struct NoPre {
 };
 
 // PRE (sparse): called with the pull offsets before the populations are
-// loaded (the step computes its colour gradient there, while only the
-// offsets are live).
+// loaded (the step issues its indicator gathers there).
 template <int Q, bool INIT, int LAYOUT, typename O, bool OWN = false, typename PRE = NoPre>
 __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, uint32_t slot, uint32_t desc,
                                              const DevLabel& lab, float* h, float* f0, double& r0, double& dr,
@@ -187,15 +186,14 @@ __device__ __forceinline__ void load_color32(const ColorArgs& a, uint32_t site, 
 // contact-angle rotation.  Returns |g|.
 //
 // dens(q) is the indicator at x + e_q (q >= 1; SOLID_N32 on solid sites)
-// and dens(0) the site's own.
+// and dens(0) the site's own; wi is the site's wall entry (0: bulk).
 template <int Q, typename DN>
-__device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, uint32_t i, DN dens, float* g)
+__device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, int32_t wi, DN dens, float* g)
 {
     using L = Lattice<Q>;
     constexpr int D = L::D;
     g[0] = g[1] = g[2] = 0.0f;
     float norm;
-    const int32_t wi = a.wall[i];
     if (!wi) {
 #pragma unroll
         for (int d = 0; d < D; ++d)
@@ -419,18 +417,27 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
     float g[3] = {0.0f, 0.0f, 0.0f};
     float norm = 0.0f;
     if constexpr (SLOT_GRADIENT) {
-        // neighbour x + e_q is the pull source of op(q), its slot recovered
-        // from the blocked offset (block, lane) where link q does not bounce
+        // The indicator at the 18 neighbours is gathered with the pull
+        // offsets, before the populations are loaded, and the gradient is
+        // formed once both have arrived: one memory round trip for the site
+        // (a gradient formed before the population loads waited on its own
+        // gathers and on the wall entry first).  Neighbour x + e_q is the
+        // pull source of op(q), its slot recovered from the blocked offset
+        // (block, lane); solid where link q bounces back.
         const uint32_t links = desc & DESC_LINK_MASK;
-        auto gradient = [&](const O* off) __attribute__((always_inline)) {
-            norm = color_gradient_f32<Q>(a, i, [&](int q) __attribute__((always_inline)) {
-                if (q == 0) return a.dens_slot[slot];
+        const int32_t wi = a.wall[i];
+        float dn[Q];
+        auto gather = [&](const O* off) __attribute__((always_inline)) {
+            dn[0] = a.dens_slot[slot];
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
                 const O o = off[L::op[q]];
                 const uint32_t m = (uint32_t(o / O(2 * Q * 64)) << 6) | uint32_t(o & O(63));
-                return ((links >> q) & 1u) ? SOLID_N32 : a.dens_slot[m];
-            }, g);
+                dn[q] = ((links >> q) & 1u) ? SOLID_N32 : a.dens_slot[m];
+            }
         };
-        load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0d, drd, dzh, uzh, gradient);
+        load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0d, drd, dzh, uzh, gather);
+        norm = color_gradient_f32<Q>(a, wi, [&](int q) __attribute__((always_inline)) { return dn[q]; }, g);
     } else {
         load_color32<Q, INIT, LAYOUT, O, OWN>(a, i, slot, desc, lab, h, f0, r0d, drd, dzh, uzh);
     }
@@ -621,7 +628,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
         }
         norm = sqrtf(n2);
     } else if constexpr (!SLOT_GRADIENT) {
-        norm = color_gradient_f32<Q>(a, i, [&](int q) {
+        norm = color_gradient_f32<Q>(a, a.wall[i], [&](int q) {
             return a.densN32[q == 0 ? i : push_target<Q>(a.g, cx, cy, cz, q)];
         }, g);
     }
@@ -736,7 +743,7 @@ __global__ void __launch_bounds__(256) color_capillary_f32(ColorArgs a)
     int x, y, z;
     site_coords(a.g, site, x, y, z);
     float g[3];
-    const float norm = color_gradient_f32<Q>(a, site, [&](int q) {
+    const float norm = color_gradient_f32<Q>(a, a.wall[site], [&](int q) {
         return a.densN32[q == 0 ? site : push_target<Q>(a.g, x, y, z, q)];
     }, g);
 #pragma unroll
