@@ -65,6 +65,8 @@ struct lbf_ctx {
     int32_t* d_wall = nullptr;
     DevWall* d_walls = nullptr;
     WallF32* d_walls32 = nullptr;   // FP32 colour: the wall entries in FP32
+    float* d_ginv32 = nullptr;      // FP32 colour, renormalized walls: ginv per wall entry
+    int32_t* d_wall_slot = nullptr; // FP32 colour, sparse: wall entry per fluid slot
     float* d_dens_slot = nullptr;   // FP32 colour, sparse: indicator per fluid slot
     uint64_t* d_film = nullptr;      // per-site film links, only with HF labels
     double *d_capgrad = nullptr, *d_capfield = nullptr;   // CSF gradient and curvature field
@@ -404,7 +406,11 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         rc |= dalloc(c, &c->d_walls, c->links.nwall * sizeof(DevWall));
         if (c->elem == 4) {
             rc |= dalloc(c, &c->d_walls32, c->links.nwall * sizeof(WallF32));
-            if (c->layout == 2) rc |= dalloc(c, &c->d_dens_slot, c->nfluid * sizeof(float));
+            if (!c->links.ginv.empty()) rc |= dalloc(c, &c->d_ginv32, c->links.nwall * 9 * sizeof(float));
+            if (c->layout == 2) {
+                rc |= dalloc(c, &c->d_dens_slot, c->nfluid * sizeof(float));
+                rc |= dalloc(c, &c->d_wall_slot, c->nfluid * sizeof(int32_t));
+            }
         }
         if (!c->links.film.empty()) rc |= dalloc(c, &c->d_film, N * sizeof(uint64_t));
         if (d.wall_gradient == LBF_WALL_TANH) rc |= dalloc(c, &c->d_tanhg, c->links.nwall * 3 * 8);
@@ -510,18 +516,37 @@ int lbf_create(const lbf_desc* desc, const lbf_label* labels, const uint8_t* geo
         }
         rc |= upload(c, c->d_walls, walls.data(), walls.size() * sizeof(DevWall));
         if (c->d_walls32) {
+            // The FP32 kernels take the inverse frame as the transpose of the
+            // frame: tibinv = txInv tzInv multiplies the transposes of tz and
+            // tx (setup.cpp, rotation(-o, a) = (c, -s) exactly), the same
+            // products summed in the same order, so the two are equal (as
+            // values: where a rotation degenerates to the identity, s = 0 and
+            // -s = -0 trade places, and a zero may differ in sign).
+            const int D = d.D;
+            bool transpose = true;
+            for (uint64_t w = 0; w < c->links.nwall && transpose; ++w)
+                for (int r = 0; r < D; ++r)
+                    for (int k = 0; k < D; ++k)
+                        transpose = transpose && walls[w].tibinv[r * D + k] == walls[w].tib[k * D + r];
+            if (!transpose) rc |= fail(c, "wall frames: the inverse frame is not the transpose of the frame");
             std::vector<WallF32> w32(c->links.nwall);
+            std::vector<float> g32(c->d_ginv32 ? c->links.nwall * 9 : 0);
             for (uint64_t w = 0; w < c->links.nwall; ++w) {
                 std::memset(&w32[w], 0, sizeof(WallF32));
                 w32[w].sinca = float(walls[w].sinca);
                 w32[w].cosca = float(walls[w].cosca);
                 for (int k = 0; k < 9; ++k) {
                     w32[w].tib[k] = float(walls[w].tib[k]);
-                    w32[w].tibinv[k] = float(walls[w].tibinv[k]);
-                    w32[w].ginv[k] = float(walls[w].ginv[k]);
+                    if (!g32.empty()) g32[w * 9 + k] = float(walls[w].ginv[k]);
                 }
             }
             rc |= upload(c, c->d_walls32, w32.data(), w32.size() * sizeof(WallF32));
+            if (c->d_ginv32) rc |= upload(c, c->d_ginv32, g32.data(), g32.size() * sizeof(float));
+            if (c->d_wall_slot) {
+                std::vector<int32_t> ws(c->nfluid);
+                for (uint64_t s = 0; s < c->nfluid; ++s) ws[s] = c->links.wall[c->list[s]];
+                rc |= upload(c, c->d_wall_slot, ws.data(), ws.size() * sizeof(int32_t));
+            }
         }
         if (c->d_film) rc |= upload(c, c->d_film, c->links.film.data(), N * sizeof(uint64_t));
     }
@@ -753,6 +778,8 @@ static int run_step(lbf_ctx* c, bool init, uint32_t flags)
             a.densN32 = reinterpret_cast<float*>(c->d_densN);
             a.dens_slot = c->d_dens_slot;
             a.walls32 = c->d_walls32;
+            a.ginv32 = c->d_ginv32;
+            a.wall_slot = c->d_wall_slot;
             a.list = c->d_list;
             a.rank = c->d_rank;
             a.P = uint32_t(c->P);

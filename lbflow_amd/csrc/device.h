@@ -50,15 +50,16 @@ struct DevWall {
     double ginv[9];      // renormalized wall gradient: inverse fluid-link moment (colorfluid.cpp:917-957)
 };
 
-// A wall entry in FP32 for the FP32 colour kernels: the frames and the
-// contact angle as the kernels use them (the float of each FP64 value), and
-// the renormalizing inverse; one 128-byte line per wall site.
-struct __attribute__((aligned(128))) WallF32 {
+// A wall entry in FP32 for the FP32 colour kernels: the frame and the
+// contact angle as the kernels use them (the float of each FP64 value); 48
+// bytes, three 16-byte loads.  The inverse frame is the transpose of the
+// frame, bit for bit (the host checks it when it packs the entries), and the
+// renormalizing inverse, read only with renormalized walls, is an array of
+// its own (ColorArgs::ginv32).
+struct __attribute__((aligned(16))) WallF32 {
     float tib[9];
-    float tibinv[9];
     float sinca, cosca;
-    float ginv[9];
-    float pad[3];
+    float pad;
 };
 
 // Device failure record.
@@ -259,6 +260,8 @@ struct ColorArgs {
     double dist0r[19];
     float relax0f, relax1f, nu0f, nu1f, betaf, akf, rho_reff;
     const WallF32* walls32;        // the wall entries in FP32
+    const float* ginv32;           // renormalized walls: ginv per wall entry in FP32 (9 floats)
+    const int32_t* wall_slot;      // SPARSE: the wall entry per fluid slot
     const uint2* rank;             // SPARSE: rank index (see stream.h)
     uint32_t P;                    // slots per plane
     // Half-permeable films (simulator.cpp:418-421, colorfluid.cpp:156-159):

@@ -231,12 +231,13 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, int32_t 
         const WallF32& wl = a.walls32[wi];
         if (a.renormalized) {
             // wallGradient renormalized (colorfluid.cpp:487-492): g = inverse g / 3
+            const float* gi = a.ginv32 + size_t(wi) * 9;
             float r[3];
 #pragma unroll
             for (int c = 0; c < D; ++c) {
                 float t = 0.0f;
 #pragma unroll
-                for (int b = 0; b < D; ++b) t += g[b] * wl.ginv[c * D + b];
+                for (int b = 0; b < D; ++b) t += g[b] * gi[c * D + b];
                 r[c] = t;
             }
 #pragma unroll
@@ -257,12 +258,10 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, int32_t 
         }
         const float cA = wl.cosca, sA = wl.sinca;
         if constexpr (D == 3) {
-            float T[9], Ti[9];
+            // the inverse frame is the transpose: Ti[r][c] = T[c][r]
+            float T[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                T[k] = wl.tib[k];
-                Ti[k] = wl.tibinv[k];
-            }
+            for (int k = 0; k < 9; ++k) T[k] = wl.tib[k];
             float gn[3];
 #pragma unroll
             for (int r = 0; r < 3; ++r) gn[r] = T[r * 3] * g[0] + T[r * 3 + 1] * g[1] + T[r * 3 + 2] * g[2];
@@ -281,7 +280,7 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, int32_t 
                 float s = 0.0f;
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc) {
-                    const float tic = Ti[r * 3] * tx[cc] + Ti[r * 3 + 1] * tx[3 + cc] + Ti[r * 3 + 2] * tx[6 + cc];
+                    const float tic = T[r] * tx[cc] + T[3 + r] * tx[3 + cc] + T[6 + r] * tx[6 + cc];
                     s += tic * hv[cc];
                 }
                 g[r] = s;
@@ -289,8 +288,8 @@ __device__ __forceinline__ float color_gradient_f32(const ColorArgs& a, int32_t 
         } else {
             const float gn0 = wl.tib[0] * g[0] + wl.tib[1] * g[1];
             const float hv0 = gn0 >= 0.0f ? norm * sA : -norm * sA, hv1 = norm * cA;
-            const float g0 = wl.tibinv[0] * hv0 + wl.tibinv[1] * hv1;
-            const float g1 = wl.tibinv[2] * hv0 + wl.tibinv[3] * hv1;
+            const float g0 = wl.tib[0] * hv0 + wl.tib[2] * hv1;   // the inverse frame is the transpose
+            const float g1 = wl.tib[1] * hv0 + wl.tib[3] * hv1;
             g[0] = g0;
             g[1] = g1;
         }
@@ -425,7 +424,7 @@ __global__ void __launch_bounds__(256) color_step_f32(ColorArgs a)
         // pull source of op(q), its slot recovered from the blocked offset
         // (block, lane); solid where link q bounces back.
         const uint32_t links = desc & DESC_LINK_MASK;
-        const int32_t wi = a.wall[i];
+        const int32_t wi = a.wall_slot[slot];
         float dn[Q];
         auto gather = [&](const O* off) __attribute__((always_inline)) {
             dn[0] = a.dens_slot[slot];
