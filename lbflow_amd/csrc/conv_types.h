@@ -23,9 +23,13 @@ struct ConvGeom {
     uint32_t nvox;           // voxels over all samples (samples * side^3)
     uint32_t in_stride;      // channels per voxel row of the input buffer
     uint32_t out_stride;     // channels per voxel row of the output buffer
+    const __bf16* gate;      // ACT_RELU_GATE: the tensor whose sign gates the output (same voxels, channels)
+    uint32_t gate_stride;    // channels per voxel row of gate
 };
 
-enum ConvAct { ACT_NONE = 0, ACT_RELU = 1 };
+// ACT_RELU_GATE: out = gate > 0 ? x : 0, the ReLU gradient of the layer
+// whose output is gate, applied to an input gradient as it is stored
+enum ConvAct { ACT_NONE = 0, ACT_RELU = 1, ACT_RELU_GATE = 2 };
 
 struct Chan {           // a channel range of a channels-last BF16 buffer
     __bf16* p;

@@ -156,6 +156,7 @@ conv_fwd(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ w
                 if (v >= g.nvox) continue;
                 float x = acc[i][j][r] + bb;
                 if (ACT == ACT_RELU) x = x > 0.0f ? x : 0.0f;
+                if (ACT == ACT_RELU_GATE) x = float(g.gate[size_t(v) * g.gate_stride + co]) > 0.0f ? x : 0.0f;
                 if (OUTF32) static_cast<float*>(out)[size_t(v) * g.out_stride + co] = x;
                 else static_cast<__bf16*>(out)[size_t(v) * g.out_stride + co] = __bf16(x);
             }
@@ -319,6 +320,7 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
     // per row tile, then + r out_stride
     using OutT = typename std::conditional<OUTF32, float, __bf16>::type;
     OutT* obase[4];
+    const __bf16* gbase[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int lv = wave * 64 + 16 * i + 4 * grp;
@@ -326,8 +328,9 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
                        lz = lv >> (h.lbx + h.lby);
         const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
         obase[i] = static_cast<OutT*>(out) + v * g.out_stride + co0 + r16;
+        if constexpr (ACT == ACT_RELU_GATE) gbase[i] = g.gate + v * g.gate_stride + co0 + r16;
     }
-    const uint32_t os = g.out_stride;
+    const uint32_t os = g.out_stride, gs = g.gate_stride;
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
         if (co0 + 16 * j + r16 >= COUT) continue;
@@ -338,6 +341,7 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
             for (int r = 0; r < 4; ++r) {
                 float x = acc[i][j][r] + bb;
                 if (ACT == ACT_RELU) x = x > 0.0f ? x : 0.0f;
+                if constexpr (ACT == ACT_RELU_GATE) x = float(gbase[i][r * gs + 16 * j]) > 0.0f ? x : 0.0f;
                 obase[i][r * os + 16 * j] = OutT(x);
             }
     }
@@ -813,6 +817,7 @@ struct ConvLaunch {
         if (out_f32 && act == ACT_RELU) fwd<ACT_RELU, true, true>(g, in, w, bias, out, s);
         else if (out_f32) fwd<ACT_NONE, true, true>(g, in, w, bias, out, s);
         else if (act == ACT_RELU) fwd<ACT_RELU, true, false>(g, in, w, bias, out, s);
+        else if (act == ACT_RELU_GATE) fwd<ACT_RELU_GATE, false, false>(g, in, w, bias, out, s);
         else if (bias) fwd<ACT_NONE, true, false>(g, in, w, bias, out, s);
         else fwd<ACT_NONE, false, false>(g, in, w, bias, out, s);
     }

@@ -79,19 +79,6 @@ __global__ void net_upsample(Chan in, Chan out, uint32_t C, uint32_t ls, uint32_
     *reinterpret_cast<bf16x8*>(out.p + size_t(v) * out.stride + out.off + c) = ld8(in, parent_of(v, ls), c);
 }
 
-// dz = dy * (y > 0)
-__global__ void net_relu_grad(Chan dy, Chan y, Chan dz, uint32_t C, uint32_t nvox)
-{
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t v = t / (C / 8), c = (t % (C / 8)) * 8;
-    if (v >= nvox) return;
-    const bf16x8 a = ld8(dy, v, c), b = ld8(y, v, c);
-    float o[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) o[k] = float(b[k]) > 0.0f ? float(a[k]) : 0.0f;
-    st8(dz, v, c, o);
-}
-
 // Encoder output feeding both a pooling and a skip connection:
 // dz[v][c] = (dpool[parent(v)][c] / 8 + dskip[v][c]) * (y[v][c] > 0).  ls = log2 side of v.
 __global__ void net_enc_grad(Chan dpool, Chan dskip, Chan y, Chan dz, uint32_t C, uint32_t ls, uint32_t nvox)
@@ -325,11 +312,6 @@ hipError_t launch_net_upsample(Chan in, Chan out, uint32_t C, uint32_t ls_out, u
 {
     hipLaunchKernelGGL(net_upsample, grid_for(uint64_t(nvox_out) * (C / 8)), dim3(256), 0, s, in, out, C, ls_out,
                        nvox_out);
-    return hipGetLastError();
-}
-hipError_t launch_net_relu_grad(Chan dy, Chan y, Chan dz, uint32_t C, uint32_t nvox, hipStream_t s)
-{
-    hipLaunchKernelGGL(net_relu_grad, grid_for(uint64_t(nvox) * (C / 8)), dim3(256), 0, s, dy, y, dz, C, nvox);
     return hipGetLastError();
 }
 hipError_t launch_net_enc_grad(Chan dpool, Chan dskip, Chan y, Chan dz, uint32_t C, uint32_t ls, uint32_t nvox,

@@ -35,7 +35,6 @@ bool conv_shape_supported(int cin, int cout, int taps);
 hipError_t launch_net_input(const uint8_t* mask, uint32_t nvox, __bf16* x0, hipStream_t s);
 hipError_t launch_net_pool(Chan in, Chan out, uint32_t C, uint32_t ls_out, uint32_t nvox_out, hipStream_t s);
 hipError_t launch_net_upsample(Chan in, Chan out, uint32_t C, uint32_t ls_out, uint32_t nvox_out, hipStream_t s);
-hipError_t launch_net_relu_grad(Chan dy, Chan y, Chan dz, uint32_t C, uint32_t nvox, hipStream_t s);
 hipError_t launch_net_enc_grad(Chan dpool, Chan dskip, Chan y, Chan dz, uint32_t C, uint32_t ls, uint32_t nvox,
                                hipStream_t s);
 hipError_t launch_net_up_grad(Chan dup, Chan y, Chan dz, uint32_t C, uint32_t ls, uint32_t nvox, hipStream_t s);
@@ -172,12 +171,18 @@ int wgrad(lbf_net* n, int i, uint32_t count, const __bf16* in, uint32_t in_strid
     return 0;
 }
 
-// input gradient of layer i: the forward kernel on the transposed-mirrored weights
-int dgrad(lbf_net* n, int i, uint32_t count, const __bf16* dz, uint32_t dz_stride, __bf16* dx, uint32_t dx_stride)
+// input gradient of layer i: the forward kernel on the transposed-mirrored
+// weights; with a gate, the ReLU gradient of the layer that produced the
+// input (dx = gate > 0 ? dx : 0) is applied as dx is stored
+int dgrad(lbf_net* n, int i, uint32_t count, const __bf16* dz, uint32_t dz_stride, __bf16* dx, uint32_t dx_stride,
+          const __bf16* gate = nullptr, uint32_t gate_stride = 0)
 {
     const Layer& l = n->L[i];
-    NHIP(n, launch_conv_forward(l.cout, l.cin, l.taps, geom(n, l.level, count, dz_stride, dx_stride), dz, l.wt,
-                                nullptr, dx, ACT_NONE, false, n->stream));
+    ConvGeom g = geom(n, l.level, count, dz_stride, dx_stride);
+    g.gate = gate;
+    g.gate_stride = gate_stride;
+    NHIP(n, launch_conv_forward(l.cout, l.cin, l.taps, g, dz, l.wt, nullptr, dx, gate ? ACT_RELU_GATE : ACT_NONE,
+                                false, n->stream));
     return 0;
 }
 
@@ -210,43 +215,36 @@ int backward(lbf_net* n, uint32_t c)
     hipStream_t s = n->stream;
     __bf16 **A = n->gA, **B = n->gB;
     // head and level-0 decoder
-    if (wgrad(n, 14, c, n->d1, 32, n->dout, HEAD_PAD) || dgrad(n, 14, c, n->dout, HEAD_PAD, A[0], 32)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(A[0], 32), ch(n->d1, 32), ch(A[0], 32), 32, v0, s));
-    if (wgrad(n, 13, c, n->d1a, 32, A[0], 32) || dgrad(n, 13, c, A[0], 32, B[0], 32)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(B[0], 32), ch(n->d1a, 32), ch(B[0], 32), 32, v0, s));
+    if (wgrad(n, 14, c, n->d1, 32, n->dout, HEAD_PAD) || dgrad(n, 14, c, n->dout, HEAD_PAD, A[0], 32, n->d1, 32))
+        return -1;
+    if (wgrad(n, 13, c, n->d1a, 32, A[0], 32) || dgrad(n, 13, c, A[0], 32, B[0], 32, n->d1a, 32)) return -1;
     if (wgrad(n, 12, c, n->cat1, 96, B[0], 32) || dgrad(n, 12, c, B[0], 32, n->gcat1, 96)) return -1;
     // level-1 decoder
     NHIP(n, launch_net_up_grad(ch(n->gcat1, 96, 0), ch(n->d2, 64), ch(A[1], 64), 64, l0 - 1, v1, s));
-    if (wgrad(n, 11, c, n->d2a, 64, A[1], 64) || dgrad(n, 11, c, A[1], 64, B[1], 64)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(B[1], 64), ch(n->d2a, 64), ch(B[1], 64), 64, v1, s));
+    if (wgrad(n, 11, c, n->d2a, 64, A[1], 64) || dgrad(n, 11, c, A[1], 64, B[1], 64, n->d2a, 64)) return -1;
     if (wgrad(n, 10, c, n->cat2, 192, B[1], 64) || dgrad(n, 10, c, B[1], 64, n->gcat2, 192)) return -1;
     // level-2 decoder
     NHIP(n, launch_net_up_grad(ch(n->gcat2, 192, 0), ch(n->d3, 128), ch(A[2], 128), 128, l0 - 2, v2, s));
-    if (wgrad(n, 9, c, n->d3a, 128, A[2], 128) || dgrad(n, 9, c, A[2], 128, B[2], 128)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(B[2], 128), ch(n->d3a, 128), ch(B[2], 128), 128, v2, s));
+    if (wgrad(n, 9, c, n->d3a, 128, A[2], 128) || dgrad(n, 9, c, A[2], 128, B[2], 128, n->d3a, 128)) return -1;
     if (wgrad(n, 8, c, n->cat3, 384, B[2], 128) || dgrad(n, 8, c, B[2], 128, n->gcat3, 384)) return -1;
     // bottom
     NHIP(n, launch_net_up_grad(ch(n->gcat3, 384, 0), ch(n->e4, 256), ch(A[3], 256), 256, l0 - 3, v3, s));
-    if (wgrad(n, 7, c, n->e4a, 256, A[3], 256) || dgrad(n, 7, c, A[3], 256, B[3], 256)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(B[3], 256), ch(n->e4a, 256), ch(B[3], 256), 256, v3, s));
+    if (wgrad(n, 7, c, n->e4a, 256, A[3], 256) || dgrad(n, 7, c, A[3], 256, B[3], 256, n->e4a, 256)) return -1;
     if (wgrad(n, 6, c, n->p3, 128, B[3], 256) || dgrad(n, 6, c, B[3], 256, n->gp3, 128)) return -1;
     // level-2 encoder
     NHIP(n, launch_net_enc_grad(ch(n->gp3, 128), ch(n->gcat3, 384, 256), ch(n->cat3, 384, 256), ch(A[2], 128), 128,
                                 l0 - 2, v2, s));
-    if (wgrad(n, 5, c, n->e3a, 128, A[2], 128) || dgrad(n, 5, c, A[2], 128, B[2], 128)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(B[2], 128), ch(n->e3a, 128), ch(B[2], 128), 128, v2, s));
+    if (wgrad(n, 5, c, n->e3a, 128, A[2], 128) || dgrad(n, 5, c, A[2], 128, B[2], 128, n->e3a, 128)) return -1;
     if (wgrad(n, 4, c, n->p2, 64, B[2], 128) || dgrad(n, 4, c, B[2], 128, n->gp2, 64)) return -1;
     // level-1 encoder
     NHIP(n, launch_net_enc_grad(ch(n->gp2, 64), ch(n->gcat2, 192, 128), ch(n->cat2, 192, 128), ch(A[1], 64), 64,
                                 l0 - 1, v1, s));
-    if (wgrad(n, 3, c, n->e2a, 64, A[1], 64) || dgrad(n, 3, c, A[1], 64, B[1], 64)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(B[1], 64), ch(n->e2a, 64), ch(B[1], 64), 64, v1, s));
+    if (wgrad(n, 3, c, n->e2a, 64, A[1], 64) || dgrad(n, 3, c, A[1], 64, B[1], 64, n->e2a, 64)) return -1;
     if (wgrad(n, 2, c, n->p1, 32, B[1], 64) || dgrad(n, 2, c, B[1], 64, n->gp1, 32)) return -1;
     // level-0 encoder
     NHIP(n, launch_net_enc_grad(ch(n->gp1, 32), ch(n->gcat1, 96, 64), ch(n->cat1, 96, 64), ch(A[0], 32), 32, l0, v0,
                                 s));
-    if (wgrad(n, 1, c, n->e1a, 32, A[0], 32) || dgrad(n, 1, c, A[0], 32, B[0], 32)) return -1;
-    NHIP(n, launch_net_relu_grad(ch(B[0], 32), ch(n->e1a, 32), ch(B[0], 32), 32, v0, s));
+    if (wgrad(n, 1, c, n->e1a, 32, A[0], 32) || dgrad(n, 1, c, A[0], 32, B[0], 32, n->e1a, 32)) return -1;
     return wgrad(n, 0, c, n->x0, 8, B[0], 32);
 }
 
