@@ -630,10 +630,32 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
 // dx.  Per output z-layer a wave keeps the dz^T fragments of its 4 rows in
 // registers and loads the input fragment of each of the 18 halo rows it
 // touches once; that fragment serves every tap dy whose output row lies in
-// the layer (up to 3 taps x 2 output-channel tiles), so LDS reads per MFMA
-// drop from ~0.55 KB to ~0.3 KB.
-// Tiles and copies are those of conv_wgrad_halo.
+// the layer (up to 3 taps x 2 output-channel tiles).
+// Both tiles are staged in their global layout, voxel rows of channels, by
+// 16-byte copies, and read as MFMA operands (K along the voxels) with the
+// transpose loads ds_read_b64_tr_b16 (tr_rows8); a shift of the taps along x
+// is a shift by whole rows, so one copy of the input halo serves all three.
 // ---------------------------------------------------------------------------
+typedef short short4v __attribute__((ext_vector_type(4)));
+
+// Lane li (0..15) of each 16-lane group receives column col + li of rows
+// v0 .. v0 + 7 (v0 per group) of a row-major LDS tile with a pitch of P
+// bf16: ds_read_b64_tr_b16 hands lane li element li % 4 of the 8 bytes read
+// by lane 4 k + li / 4 of its group, k = 0..3, so lane li reads row
+// v0 + li / 4 at column col + 4 (li % 4); a second read covers rows +4..+7.
+__device__ __forceinline__ bf16x8 tr_rows8(const __bf16* tile, int v0, int P, int col, int li)
+{
+    const __bf16* p = tile + (v0 + (li >> 2)) * P + col + 4 * (li & 3);
+    const short4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) short4v*)(p));
+    const short4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) short4v*)(p + 4 * P));
+    bf16x8 r;
+    __builtin_memcpy(&r, &lo, 8);
+    __builtin_memcpy(reinterpret_cast<char*>(&r) + 8, &hi, 8);
+    return r;
+}
+
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(2)))
 conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ dz,
@@ -642,13 +664,14 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
     constexpr int KP = conv_kp(CIN, 27);
     constexpr int CC = 16, BM = 32, NTH = 192;
     constexpr int NCH = CIN / CC, NCO = COUT / BM;
-    // halo of the 2 x 4 x 32 block: ROWS = 4 x HY rows of 34 voxels, loaded as
-    // HP = 17 voxel pairs and stored with a row pitch of RW = 40 elements
-    constexpr int HY = 6, ROWS = 24, HP = 17, RW = 40;
-    constexpr int NXI = (ROWS * HP * 2 + NTH - 1) / NTH, NDZ = (512 + NTH - 1) / NTH;
+    // halo of the 2 x 4 x 32 block: 4 z-layers x HY rows of HX voxels
+    constexpr int HY = 6, HX = 34, HV = 4 * HY * HX;
+    // row pitches (bf16): 16-byte aligned rows, padded off a multiple of 64 B
+    constexpr int PD = BM + 8, PX = CC + 8;
+    constexpr int NDZ = (256 * (BM / 8) + NTH - 1) / NTH, NXI = (HV * (CC / 8) + NTH - 1) / NTH;
     static_assert(CIN % 32 == 0 && COUT % BM == 0, "shape");
-    __shared__ __attribute__((aligned(16))) __bf16 sdz[BM * WG_DZS];
-    __shared__ __attribute__((aligned(16))) __bf16 sx[2 * CC * WG_XPLANE];
+    __shared__ __attribute__((aligned(16))) __bf16 sdz[256 * PD];
+    __shared__ __attribute__((aligned(16))) __bf16 sx[HV * PX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;   // wave = dx + 1
     const int r16 = lane & 15, grp = lane >> 4;
     const uint32_t wg = xcd_block();
@@ -669,67 +692,56 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
         const uint32_t smp = blk / bps, b = blk % bps;
         const uint32_t x0 = (b % nbx) << 5, y0 = ((b / nbx) % nby) << 2, z0 = (b / (nbx * nby)) << 1;
         const size_t sbase = size_t(smp) << (3 * ls);
+        // every load first, then every LDS store
+        bf16x8 rdz[NDZ], rx[NXI];
 #pragma unroll
         for (int n = 0; n < NDZ; ++n) {
             const int t = threadIdx.x + NTH * n;
-            if (t >= 512) break;
-            const int c8 = t >> 7, pv = t & 127;
-            bf16x8 rdz[2];
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int lv = 2 * pv + e;
+            if (t < 256 * (BM / 8)) {
+                const int lv = t >> 2, c8 = t & 3;
                 const uint32_t lx = lv & 31, ly = (lv >> 5) & 3, lz = lv >> 7;
                 const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
-                rdz[e] = *reinterpret_cast<const bf16x8*>(dz + v * g.out_stride + co0 + 8 * c8);
+                rdz[n] = *reinterpret_cast<const bf16x8*>(dz + v * g.out_stride + co0 + 8 * c8);
             }
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                *reinterpret_cast<uint32_t*>(sdz + (8 * c8 + j) * WG_DZS + 2 * pv) = pack2(rdz[0][j], rdz[1][j]);
         }
 #pragma unroll
         for (int n = 0; n < NXI; ++n) {
             const int t = threadIdx.x + NTH * n;
-            if (t >= ROWS * HP * 2) break;
-            const int p = t % HP, row = (t / HP) % ROWS, c8 = t / (HP * ROWS);
-            const int hy = row % HY, hz = row / HY;
-            const uint32_t y = (y0 + uint32_t(hy) - 1u) & m, z = (z0 + uint32_t(hz) - 1u) & m;
-            const size_t rbase = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls));
-            const bool shifted = 2 * p + 1 < 32;
-            bf16x8 rx[3];
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                const int hx = 2 * p + e;
-                if (e == 2 && !shifted) break;
-                const uint32_t x = (x0 + uint32_t(hx) - 1u) & m;
-                rx[e] = *reinterpret_cast<const bf16x8*>(in + (rbase + x) * g.in_stride + c0 + 8 * c8);
+            if (t < HV * (CC / 8)) {
+                const int hv = t >> 1, c8 = t & 1;
+                const int hx = hv % HX, row = hv / HX, hy = row % HY, hz = row / HY;
+                const uint32_t x = (x0 + uint32_t(hx) - 1u) & m, y = (y0 + uint32_t(hy) - 1u) & m,
+                               z = (z0 + uint32_t(hz) - 1u) & m;
+                const size_t v = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls) | x);
+                rx[n] = *reinterpret_cast<const bf16x8*>(in + v * g.in_stride + c0 + 8 * c8);
             }
+        }
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                __bf16* plane0 = sx + (8 * c8 + j) * WG_XPLANE + row * RW + 2 * p;
-                *reinterpret_cast<uint32_t*>(plane0) = pack2(rx[0][j], rx[1][j]);
-                if (shifted)
-                    *reinterpret_cast<uint32_t*>(plane0 + CC * WG_XPLANE) = pack2(rx[1][j], rx[2][j]);
-            }
+        for (int n = 0; n < NDZ; ++n) {
+            const int t = threadIdx.x + NTH * n;
+            if (t < 256 * (BM / 8)) *reinterpret_cast<bf16x8*>(sdz + (t >> 2) * PD + 8 * (t & 3)) = rdz[n];
+        }
+#pragma unroll
+        for (int n = 0; n < NXI; ++n) {
+            const int t = threadIdx.x + NTH * n;
+            if (t < HV * (CC / 8)) *reinterpret_cast<bf16x8*>(sx + (t >> 1) * PX + 8 * (t & 1)) = rx[n];
         }
     };
 
     for (uint32_t blk = b_begin; blk < b_end; ++blk) {
-        // no register prefetch here (the dz^T fragments of all rows take the
-        // registers); the two workgroups per CU overlap loads and MFMAs
         __syncthreads();   // the previous block's MFMAs are done with the tiles
         load_tiles(blk);
         __syncthreads();
-        // this wave's copy and x offset: dx = -1 copy 0 at x, dx = 0 copy 1 at x, dx = +1 copy 0 at x + 2
-        const __bf16* xc = sx + r16 * WG_XPLANE + (wave == 1 ? CC * WG_XPLANE : 0) + 8 * grp + (wave == 2 ? 2 : 0);
         // one output z-layer at a time: its 4 rows' dz^T fragments in
-        // registers, the 3 x 6 halo rows it touches loaded once each
+        // registers, the 3 x 6 halo rows it touches loaded once each; this
+        // wave's input fragment starts at halo x = 8 grp + dx + 1
 #pragma unroll
         for (int lz = 0; lz < 2; ++lz) {
             bf16x8 a[4][2];
 #pragma unroll
             for (int ly = 0; ly < 4; ++ly) {
-                a[ly][0] = *reinterpret_cast<const bf16x8*>(sdz + r16 * WG_DZS + 32 * (4 * lz + ly) + 8 * grp);
-                a[ly][1] = *reinterpret_cast<const bf16x8*>(sdz + (16 + r16) * WG_DZS + 32 * (4 * lz + ly) + 8 * grp);
+                a[ly][0] = tr_rows8(sdz, 32 * (4 * lz + ly) + 8 * grp, PD, 0, r16);
+                a[ly][1] = tr_rows8(sdz, 32 * (4 * lz + ly) + 8 * grp, PD, 16, r16);
             }
 #pragma unroll
             for (int dzt = -1; dzt <= 1; ++dzt) {
@@ -737,14 +749,7 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
 #pragma unroll
                 for (int hy = 0; hy < HY; ++hy) {
                     const int rho = hz * HY + hy;
-                    bf16x8 bv;
-                    if (wave == 2) {
-                        const uint32_t* q = reinterpret_cast<const uint32_t*>(xc + rho * RW);
-                        uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
-                        __builtin_memcpy(&bv, w4, 16);
-                    } else {
-                        bv = *reinterpret_cast<const bf16x8*>(xc + rho * RW);
-                    }
+                    const bf16x8 bv = tr_rows8(sx, rho * HX + 8 * grp + wave, PX, 0, r16);
 #pragma unroll
                     for (int dyt = -1; dyt <= 1; ++dyt) {
                         const int ly = hy - dyt - 1;
