@@ -21,10 +21,10 @@
 // same kernel on the transposed, tap-mirrored weights.
 //
 // Weight gradient: the reduction runs over voxels, so both operands need 8
-// consecutive voxels per lane; 128-voxel stages of dz and of the gathered
-// input are fetched into registers one stage ahead, transposed into LDS
-// ([channel][voxel] rows, padded to 272 B), and every voxel split writes its
-// own FP32 partial, summed in split order by a second kernel (deterministic).
+// consecutive voxels per lane; the dz and input tiles are staged in LDS in
+// their global layout (voxel rows of channels) and read transposed with
+// ds_read_b64_tr_b16, and every voxel split writes its own FP32 partial,
+// summed in split order by a second kernel (deterministic).
 #include <type_traits>
 
 #include "conv_types.h"
@@ -457,24 +457,35 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
 // weight gradient, 3x3x3 with Cin a multiple of 32: a workgroup owns 32
 // output channels x 16 input channels x all 27 taps and runs over the
 // 256-voxel blocks of its voxel split (the direct kernel above re-gathers
-// every tap of the input for every 64-wide k tile).  Per block it stages
-//   dz^T   [32 channels][256 voxels]                     (rows 264 bf16)
-//   input  [16 channels][(BZ+2)(BY+2) rows][x]  twice:   copy 0 holds halo
-//          x positions 0..BX+1, copy 1 the same row shifted by one
-// so that the 8 consecutive voxels of an MFMA lane group start 16-byte
-// aligned for tap dx = -1 (copy 0) and dx = 0 (copy 1) and 4-byte aligned for
-// dx = +1 (copy 0, read as 4 dwords).  One MFMA reduces 32 voxels; wave w
-// accumulates the taps w, w+4, ... for both 16-channel output tiles.  Tiles
-// are written as dwords (voxel pairs), and the next block's operands are
-// fetched into registers while the current block runs on MFMA.
+// every tap of the input for every 64-wide k tile).  Per block it stages, in
+// their global layout (voxel rows of channels) by 16-byte copies,
+//   dz     [256 voxels][32 channels]                 (rows of PD bf16)
+//   input  [halo voxels, x fastest][16 channels]     (rows of PX bf16)
+// and reads both as MFMA operands, K along the voxels, with the transpose
+// loads of tr_rows8: the 8 consecutive voxels of a lane group are 8
+// consecutive rows, and a tap's shift along x is a shift by whole rows.  One
+// MFMA reduces 32 voxels; wave w accumulates the taps w, w+4, ... for both
+// 16-channel output tiles.  The next block's operands are fetched into
+// registers while the current block runs on MFMA.
 // ---------------------------------------------------------------------------
-constexpr int WG_DZS = 256 + 16;        // bf16 per dz^T row: 136 dwords (see LDS_NOTE)
-constexpr int WG_XPLANE = 976;          // bf16 per channel plane of one copy (>= max rows * (BX + 8) + 8;
-                                        // 488 dwords, see LDS_NOTE)
+typedef short short4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint32_t pack2(__bf16 lo, __bf16 hi)
+// Lane li (0..15) of each 16-lane group receives column col + li of rows
+// v0 .. v0 + 7 (v0 per group) of a row-major LDS tile with a pitch of P
+// bf16: ds_read_b64_tr_b16 hands lane li element li % 4 of the 8 bytes read
+// by lane 4 k + li / 4 of its group, k = 0..3, so lane li reads row
+// v0 + li / 4 at column col + 4 (li % 4); a second read covers rows +4..+7.
+__device__ __forceinline__ bf16x8 tr_rows8(const __bf16* tile, int v0, int P, int col, int li)
 {
-    return uint32_t(__builtin_bit_cast(uint16_t, lo)) | (uint32_t(__builtin_bit_cast(uint16_t, hi)) << 16);
+    const __bf16* p = tile + (v0 + (li >> 2)) * P + col + 4 * (li & 3);
+    const short4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) short4v*)(p));
+    const short4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) short4v*)(p + 4 * P));
+    bf16x8 r;
+    __builtin_memcpy(&r, &lo, 8);
+    __builtin_memcpy(reinterpret_cast<char*>(&r) + 8, &hi, 8);
+    return r;
 }
 
 template <int CIN, int COUT>
@@ -486,10 +497,13 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
     constexpr int CC = 16, BM = 32;
     constexpr int TW = 7;                                  // taps per wave (27 over 4 waves)
     constexpr int NCH = CIN / CC, NCO = COUT / BM;
-    constexpr int NXI = 4;                                 // input-tile items per thread (<= 816 / 256)
+    // row pitches (bf16): 16-byte aligned rows, padded off a multiple of 64 B
+    constexpr int PD = BM + 8, PX = CC + 8;
+    constexpr int NDZ = 256 * (BM / 8) / 256;              // dz chunks (16 B) per thread
+    constexpr int NXI = (HALO_MAX * (CC / 8) + 255) / 256; // input chunks per thread
     static_assert(CIN % 32 == 0 && COUT % BM == 0, "shape");
-    __shared__ __attribute__((aligned(16))) __bf16 sdz[BM * WG_DZS];
-    __shared__ __attribute__((aligned(16))) __bf16 sx[2 * CC * WG_XPLANE];
+    __shared__ __attribute__((aligned(16))) __bf16 sdz[256 * PD];
+    __shared__ __attribute__((aligned(16))) __bf16 sx[HALO_MAX * PX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, grp = lane >> 4;
     // workgroups of one voxel split (all channel tiles) run on one XCD
@@ -499,9 +513,8 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
     const int ls = int(g.log_s);
     const uint32_t S = 1u << ls, m = S - 1u;
     const HaloShape h = halo_shape(ls);
-    const int bx = 1 << h.lbx, rw = bx + 8;                // copy row width (bf16), 16-byte multiple
-    const int rows = h.hy * h.hz, hp = h.hx / 2;           // halo rows, x pairs per row
-    const int nxi = rows * hp * 2;                         // input items: (row, pair, 8-channel group)
+    const int bx = 1 << h.lbx;
+    const int nxi = h.hx * h.hy * h.hz * (CC / 8);         // input chunks: (halo voxel, 8-channel group)
     const uint32_t nblocks = g.nvox >> 8;
     const uint32_t b_begin = split * blocks_per_split;
     const uint32_t b_end = b_begin + blocks_per_split < nblocks ? b_begin + blocks_per_split : nblocks;
@@ -511,65 +524,43 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
 #pragma unroll
     for (int t = 0; t < TW; ++t) acc[t][0] = acc[t][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    // one block's operands in registers: dz of 2 voxel pairs x 8 channels,
-    // input of up to NXI (row, x pair) items x 8 channels (3 voxels each)
-    bf16x8 rdz[2][2], rx[NXI][3];
+    // one block's operands in registers
+    bf16x8 rdz[NDZ], rx[NXI];
     auto fetch = [&](uint32_t blk) {
         const uint32_t smp = blk / bps, b = blk % bps;
         const uint32_t x0 = (b % nbx) << h.lbx, y0 = ((b / nbx) % nby) << h.lby, z0 = (b / (nbx * nby)) << h.lbz;
         const size_t sbase = size_t(smp) << (3 * ls);
 #pragma unroll
-        for (int n = 0; n < 2; ++n) {
+        for (int n = 0; n < NDZ; ++n) {
             const int t = threadIdx.x + 256 * n;
-            const int c8 = t >> 7, pv = t & 127;
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int lv = 2 * pv + e;
-                const uint32_t lx = lv & (bx - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1),
-                               lz = lv >> (h.lbx + h.lby);
-                const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
-                rdz[n][e] = *reinterpret_cast<const bf16x8*>(dz + v * g.out_stride + co0 + 8 * c8);
-            }
+            const int lv = t >> 2, c8 = t & 3;
+            const uint32_t lx = lv & (bx - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1), lz = lv >> (h.lbx + h.lby);
+            const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
+            rdz[n] = *reinterpret_cast<const bf16x8*>(dz + v * g.out_stride + co0 + 8 * c8);
         }
 #pragma unroll
         for (int n = 0; n < NXI; ++n) {
             const int t = threadIdx.x + 256 * n;
-            if (t >= nxi) break;
-            const int p = t % hp, row = (t / hp) % rows, c8 = t / (hp * rows);
-            const int hy = row % h.hy, hz = row / h.hy;
-            const uint32_t y = (y0 + uint32_t(hy) - 1u) & m, z = (z0 + uint32_t(hz) - 1u) & m;
-            const size_t rbase = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls));
-#pragma unroll
-            for (int e = 0; e < 3; ++e) {
-                const int hx = 2 * p + e;
-                if (e == 2 && 2 * p + 1 >= bx) break;      // the last pair feeds copy 0 only
-                const uint32_t x = (x0 + uint32_t(hx) - 1u) & m;
-                rx[n][e] = *reinterpret_cast<const bf16x8*>(in + (rbase + x) * g.in_stride + c0 + 8 * c8);
+            if (t < nxi) {
+                const int hv = t >> 1, c8 = t & 1;
+                const int hx = hv % h.hx, row = hv / h.hx, hy = row % h.hy, hz = row / h.hy;
+                const uint32_t x = (x0 + uint32_t(hx) - 1u) & m, y = (y0 + uint32_t(hy) - 1u) & m,
+                               z = (z0 + uint32_t(hz) - 1u) & m;
+                const size_t v = sbase + ((size_t(z) << (2 * ls)) | (size_t(y) << ls) | x);
+                rx[n] = *reinterpret_cast<const bf16x8*>(in + v * g.in_stride + c0 + 8 * c8);
             }
         }
     };
     auto store = [&]() {
 #pragma unroll
-        for (int n = 0; n < 2; ++n) {
+        for (int n = 0; n < NDZ; ++n) {
             const int t = threadIdx.x + 256 * n;
-            const int c8 = t >> 7, pv = t & 127;
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                *reinterpret_cast<uint32_t*>(sdz + (8 * c8 + j) * WG_DZS + 2 * pv) = pack2(rdz[n][0][j], rdz[n][1][j]);
+            *reinterpret_cast<bf16x8*>(sdz + (t >> 2) * PD + 8 * (t & 3)) = rdz[n];
         }
 #pragma unroll
         for (int n = 0; n < NXI; ++n) {
             const int t = threadIdx.x + 256 * n;
-            if (t >= nxi) break;
-            const int p = t % hp, row = (t / hp) % rows, c8 = t / (hp * rows);
-            const bool shifted = 2 * p + 1 < bx;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                __bf16* plane0 = sx + (8 * c8 + j) * WG_XPLANE + row * rw + 2 * p;
-                *reinterpret_cast<uint32_t*>(plane0) = pack2(rx[n][0][j], rx[n][1][j]);
-                if (shifted)
-                    *reinterpret_cast<uint32_t*>(plane0 + CC * WG_XPLANE) = pack2(rx[n][1][j], rx[n][2][j]);
-            }
+            if (t < nxi) *reinterpret_cast<bf16x8*>(sx + (t >> 1) * PX + 8 * (t & 1)) = rx[n];
         }
     };
 
@@ -581,28 +572,18 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
         if (blk + 1 < b_end) fetch(blk + 1);   // the next block's loads overlap this block's MFMAs
 #pragma unroll 1
         for (int r = 0; r < 8; ++r) {
-            // this lane's 8 voxels: block positions 32 r + 8 grp .. + 7 (one x run)
+            // this lane group's 8 voxels: block positions 32 r + 8 grp .. + 7 (one x run)
             const int lv = 32 * r + 8 * grp;
             const int lx = lv & (bx - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1), lz = lv >> (h.lbx + h.lby);
-            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(sdz + r16 * WG_DZS + lv);
-            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(sdz + (16 + r16) * WG_DZS + lv);
-            const __bf16* xc = sx + r16 * WG_XPLANE;
+            const bf16x8 a0 = tr_rows8(sdz, lv, PD, 0, r16);
+            const bf16x8 a1 = tr_rows8(sdz, lv, PD, 16, r16);
 #pragma unroll
             for (int tt = 0; tt < TW; ++tt) {
                 const int tap = wave + 4 * tt;
                 if (tap >= 27) break;
                 const int dzt = tap / 9 - 1, dyt = (tap / 3) % 3 - 1, dxt = tap % 3 - 1;
                 const int row = (lz + dzt + 1) * h.hy + (ly + dyt + 1);
-                bf16x8 bv;
-                if (dxt == -1) {
-                    bv = *reinterpret_cast<const bf16x8*>(xc + row * rw + lx);
-                } else if (dxt == 0) {
-                    bv = *reinterpret_cast<const bf16x8*>(xc + CC * WG_XPLANE + row * rw + lx);
-                } else {
-                    const uint32_t* q = reinterpret_cast<const uint32_t*>(xc + row * rw + lx + 2);
-                    uint32_t w4[4] = {q[0], q[1], q[2], q[3]};
-                    __builtin_memcpy(&bv, w4, 16);
-                }
+                const bf16x8 bv = tr_rows8(sx, row * h.hx + lx + dxt + 1, PX, 0, r16);
                 acc[tt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[tt][0], 0, 0, 0);
                 acc[tt][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[tt][1], 0, 0, 0);
             }
@@ -636,26 +617,6 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
 // transpose loads ds_read_b64_tr_b16 (tr_rows8); a shift of the taps along x
 // is a shift by whole rows, so one copy of the input halo serves all three.
 // ---------------------------------------------------------------------------
-typedef short short4v __attribute__((ext_vector_type(4)));
-
-// Lane li (0..15) of each 16-lane group receives column col + li of rows
-// v0 .. v0 + 7 (v0 per group) of a row-major LDS tile with a pitch of P
-// bf16: ds_read_b64_tr_b16 hands lane li element li % 4 of the 8 bytes read
-// by lane 4 k + li / 4 of its group, k = 0..3, so lane li reads row
-// v0 + li / 4 at column col + 4 (li % 4); a second read covers rows +4..+7.
-__device__ __forceinline__ bf16x8 tr_rows8(const __bf16* tile, int v0, int P, int col, int li)
-{
-    const __bf16* p = tile + (v0 + (li >> 2)) * P + col + 4 * (li & 3);
-    const short4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-        (__attribute__((address_space(3))) short4v*)(p));
-    const short4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(
-        (__attribute__((address_space(3))) short4v*)(p + 4 * P));
-    bf16x8 r;
-    __builtin_memcpy(&r, &lo, 8);
-    __builtin_memcpy(reinterpret_cast<char*>(&r) + 8, &hi, 8);
-    return r;
-}
-
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(2)))
 conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ dz,
