@@ -737,15 +737,44 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
     }
 }
 
-// dW[i] = sum over splits of partial[s][i], in split order.
+// dW[i] = sum over splits of partial[s][i] in a fixed order (deterministic).
+// Up to 64 splits: in split order.  More: the splits are summed in groups of
+// SPLIT_GROUP consecutive ones in parallel (each group's sum stored in the
+// group's first row, a row only that thread reads), then the group sums in
+// order.
+constexpr uint32_t SPLIT_GROUP = 32;
+
 __global__ void __launch_bounds__(256)
-sum_splits(const float* __restrict__ partial, uint32_t n, uint32_t splits, float* __restrict__ out)
+sum_split_groups(float* __restrict__ partial, uint32_t n, uint32_t splits)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k0 = blockIdx.y * SPLIT_GROUP, k1 = k0 + SPLIT_GROUP < splits ? k0 + SPLIT_GROUP : splits;
+    float s = 0.0f;
+    for (uint32_t k = k0; k < k1; ++k) s += partial[size_t(k) * n + i];
+    partial[size_t(k0) * n + i] = s;
+}
+
+__global__ void __launch_bounds__(256)
+sum_splits(const float* __restrict__ partial, uint32_t n, uint32_t splits, uint32_t step, float* __restrict__ out)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     float s = 0.0f;
-    for (uint32_t k = 0; k < splits; ++k) s += partial[size_t(k) * n + i];
+    for (uint32_t k = 0; k < splits; k += step) s += partial[size_t(k) * n + i];
     out[i] = s;
+}
+
+static void launch_sum_splits(float* partial, uint32_t n, uint32_t splits, float* dw, hipStream_t s)
+{
+    const uint32_t blocks = (n + 255) / 256;
+    if (splits > 64) {
+        hipLaunchKernelGGL(sum_split_groups, dim3(blocks, (splits + SPLIT_GROUP - 1) / SPLIT_GROUP), dim3(256), 0, s,
+                           partial, n, splits);
+        hipLaunchKernelGGL(sum_splits, dim3(blocks), dim3(256), 0, s, partial, n, splits, SPLIT_GROUP, dw);
+    } else {
+        hipLaunchKernelGGL(sum_splits, dim3(blocks), dim3(256), 0, s, partial, n, splits, 1u, dw);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -804,16 +833,14 @@ struct ConvLaunch {
                 else
                     hipLaunchKernelGGL((conv_wgrad_halo<CIN, COUT>), dim3((CIN / 16) * (COUT / 32) * splits),
                                        dim3(256), 0, s, g, in, dz, partial, per);
-                const uint32_t n = uint32_t(COUT) * KP;
-                hipLaunchKernelGGL(sum_splits, dim3((n + 255) / 256), dim3(256), 0, s, partial, n, splits, dw);
+                launch_sum_splits(partial, uint32_t(COUT) * KP, splits, dw, s);
                 return;
             }
         }
         const uint32_t per = ((g.nvox + splits - 1) / splits + 127u) / 128u * 128u;
         const dim3 grid((KP + BN_W - 1) / BN_W, (COUT + BM_W - 1) / BM_W, splits);
         hipLaunchKernelGGL((conv_wgrad<CIN, COUT, TAPS, BM_W, BN_W>), grid, dim3(256), 0, s, g, in, dz, partial, per);
-        const uint32_t n = uint32_t(COUT) * KP;
-        hipLaunchKernelGGL(sum_splits, dim3((n + 255) / 256), dim3(256), 0, s, partial, n, splits, dw);
+        launch_sum_splits(partial, uint32_t(COUT) * KP, splits, dw, s);
     }
 };
 
@@ -846,13 +873,17 @@ uint32_t conv_wgrad_splits(int cin, int cout, int taps, uint32_t nvox)
     const uint32_t max_splits = nvox / 256 > 0 ? nvox / 256 : 1;
     uint32_t splits;
     if (taps == 27 && cin % 32 == 0 && cout % 32 == 0 && nvox >= 512) {
+        // at most 1024 workgroups: two full rounds of two per CU (a count
+        // just above 1024 would run a third, nearly empty round)
         const uint32_t base = uint32_t(cin / 16) * uint32_t(cout / 32);
-        splits = (1024u + base - 1) / base;
+        splits = 1024u / base;
         if (splits > 256u) splits = 256u;
     } else {
+        // the direct kernel: about 2048 workgroups (the 1x1x1 head has a
+        // single 64 x 64 tile, so its voxels carry all the parallelism)
         const uint32_t kb = (uint32_t(conv_kp(cin, taps)) + 63) / 64, cb = (uint32_t(cout) + 63) / 64;
-        splits = (2048u + kb * cb - 1) / (kb * cb);
-        if (splits > 64u) splits = 64u;
+        splits = 2048u / (kb * cb);
+        if (splits > 1024u) splits = 1024u;
     }
     if (splits > max_splits) splits = max_splits;
     return splits < 1 ? 1 : splits;
