@@ -25,6 +25,9 @@ struct ConvGeom {
     uint32_t out_stride;     // channels per voxel row of the output buffer
     const __bf16* gate;      // ACT_RELU_GATE: the tensor whose sign gates the output (same voxels, channels)
     uint32_t gate_stride;    // channels per voxel row of gate
+    float* kpart;            // forward split over input channels: FP32 partials [split][voxel][Cout]
+                             // (null: no split)
+    uint32_t kpart_floats;   // floats available at kpart
 };
 
 // ACT_RELU_GATE: out = gate > 0 ? x : 0, the ReLU gradient of the layer

@@ -250,7 +250,10 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
 #pragma unroll
     for (int j = 0; j < NT; ++j) wrow[j] = w + size_t(co0 + 16 * j + r16) * KP + 8 * grp;
 
-    for (int c0 = 0; c0 < CIN; c0 += CC) {
+    // split over input channels (gridDim.z > 1): this workgroup's chunks
+    const int cps = (CIN / CC) / int(gridDim.z);
+    const int c_begin = int(blockIdx.z) * cps * CC, c_end = c_begin + cps * CC;
+    for (int c0 = c_begin; c0 < c_end; c0 += CC) {
         __syncthreads();   // the previous chunk's MFMAs are done with the tile
         {
             // 4 voxels per trip: their 4 loads are in flight together
@@ -318,6 +321,24 @@ conv_fwd_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restric
     // output rows 4 grp .. 4 grp + 3 of a row tile are 4 consecutive voxels
     // along x (the block extent along x is at least 8): one element offset
     // per row tile, then + r out_stride
+    if (gridDim.z > 1) {
+        // split: this split's FP32 partial, summed by conv_ksplit_sum
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int lv = wave * 64 + 16 * i + 4 * grp;
+            const uint32_t lx = lv & ((1 << h.lbx) - 1), ly = (lv >> h.lbx) & ((1 << h.lby) - 1),
+                           lz = lv >> (h.lbx + h.lby);
+            const size_t v = sbase + ((size_t(z0 + lz) << (2 * ls)) | (size_t(y0 + ly) << ls) | (x0 + lx));
+            float* kb = g.kpart + (size_t(blockIdx.z) * g.nvox + v) * COUT + co0 + r16;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                if (co0 + 16 * j + r16 >= COUT) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) kb[r * COUT + 16 * j] = acc[i][j][r];
+            }
+        }
+        return;
+    }
     using OutT = typename std::conditional<OUTF32, float, __bf16>::type;
     OutT* obase[4];
     const __bf16* gbase[4];
@@ -451,6 +472,35 @@ conv_wgrad(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__
                 if (co < COUT) partial[(size_t(blockIdx.z) * COUT + co) * KP + k] = acc[i][j][r];
             }
         }
+}
+
+// Forward split over input channels: out = act(sum of the splits' FP32
+// partials in split order + bias), one thread per voxel and 4 channels.
+template <int ACT, bool BIAS, bool OUTF32>
+__global__ void __launch_bounds__(256)
+conv_ksplit_sum(ConvGeom g, uint32_t cout, uint32_t ks, const float* __restrict__ bias, void* __restrict__ out)
+{
+    const uint32_t q = cout / 4;
+    const uint64_t t = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (t >= uint64_t(g.nvox) * q) return;
+    const uint32_t v = uint32_t(t / q), c = uint32_t(t % q) * 4;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < ks; ++k) {
+        const float4 p = *reinterpret_cast<const float4*>(g.kpart + (uint64_t(k) * g.nvox + v) * cout + c);
+        s[0] += p.x;
+        s[1] += p.y;
+        s[2] += p.z;
+        s[3] += p.w;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint32_t co = c + e;
+        float x = s[e] + (BIAS ? bias[co] : 0.0f);
+        if (ACT == ACT_RELU) x = x > 0.0f ? x : 0.0f;
+        if (ACT == ACT_RELU_GATE) x = float(g.gate[size_t(v) * g.gate_stride + co]) > 0.0f ? x : 0.0f;
+        if (OUTF32) static_cast<float*>(out)[size_t(v) * g.out_stride + co] = x;
+        else static_cast<__bf16*>(out)[size_t(v) * g.out_stride + co] = __bf16(x);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -793,9 +843,24 @@ struct ConvLaunch {
         if constexpr (HALO) {
             // a 256-voxel block must fit one sample: side >= 8
             if (g.log_s >= 3) {
-                const dim3 grid(g.nvox / 256u, (COUT + BN_HALO - 1) / BN_HALO);
+                // with a workspace, a grid that fills less than one round of
+                // two workgroups per CU (512) is split over input-channel
+                // chunks, doubled while the chunks divide evenly
+                const uint32_t wgs = (g.nvox / 256u) * ((COUT + BN_HALO - 1) / BN_HALO);
+                constexpr uint32_t NCHUNK = CIN / 32;
+                uint32_t ks = 1;
+                if (g.kpart)
+                    while (NCHUNK % (2 * ks) == 0 && wgs * 2 * ks <= 512u &&
+                           size_t(2 * ks) * g.nvox * COUT <= g.kpart_floats)
+                        ks *= 2;
+                const dim3 grid(g.nvox / 256u, (COUT + BN_HALO - 1) / BN_HALO, ks);
                 hipLaunchKernelGGL((conv_fwd_halo<CIN, COUT, BN_HALO, ACT, BIAS, OUTF32>), grid, dim3(256), 0, s, g,
                                    in, w, bias, out);
+                if (ks > 1) {
+                    const uint64_t n = uint64_t(g.nvox) * (COUT / 4);
+                    hipLaunchKernelGGL((conv_ksplit_sum<ACT, BIAS, OUTF32>), dim3(uint32_t((n + 255) / 256)), dim3(256),
+                                       0, s, g, uint32_t(COUT), ks, bias, out);
+                }
                 return;
             }
         }

@@ -59,6 +59,9 @@ namespace {
 constexpr int NLAYERS = 15;
 constexpr int HEAD = 14;
 constexpr int HEAD_PAD = 16;   // the head's 4 outputs padded to one MFMA column tile
+// forward split over input channels: at most 512 workgroups x 256 voxels x 64
+// output channels of FP32 partials
+constexpr uint32_t KPART_FLOATS = 512u * 256u * 64u;
 
 // cin, cout, taps, level
 constexpr int kLayers[NLAYERS][4] = {
@@ -90,6 +93,7 @@ struct lbf_net {
     // gradients
     __bf16 *dout, *gA[4], *gB[4], *gcat1, *gcat2, *gcat3, *gp1, *gp2, *gp3;
     float *wpart, *bpart;
+    float* kpart;                  // forward split over input channels: FP32 partials
     double *lpart, *loss;
     size_t wpart_floats = 0;
     // warm starts: capacity pore masks and their physical fields, allocated on first use
@@ -129,7 +133,10 @@ uint32_t nvox_at(const lbf_net* n, int level, uint32_t count) { return count * (
 
 ConvGeom geom(const lbf_net* n, int level, uint32_t count, uint32_t in_stride, uint32_t out_stride)
 {
-    return ConvGeom{n->ls - uint32_t(level), nvox_at(n, level, count), in_stride, out_stride};
+    ConvGeom g{n->ls - uint32_t(level), nvox_at(n, level, count), in_stride, out_stride};
+    g.kpart = n->kpart;
+    g.kpart_floats = KPART_FLOATS;
+    return g;
 }
 
 Chan ch(__bf16* p, uint32_t stride, uint32_t off = 0) { return Chan{p, stride, off}; }
@@ -371,6 +378,7 @@ int lbf_net_create(uint32_t side, uint32_t capacity, uint64_t seed, lbf_net** ou
         n->wpart_floats = wmax;
         rc |= nalloc(n, &n->wpart, wmax);
         rc |= nalloc(n, &n->bpart, size_t(4096) * 256);
+        rc |= nalloc(n, &n->kpart, KPART_FLOATS);
         rc |= nalloc(n, &n->lpart, size_t(v[0] / 256 + 1));
         rc |= nalloc(n, &n->loss, 1);
     }

@@ -203,7 +203,11 @@ def test_training_reduces_loss():
     dm, dt = Buffer((n, side ** 3), np.uint8, mask.reshape(n, -1)), Buffer(target.shape, np.float32, target)
     opt = S.adam(3e-3)
     first = net.train_step(dm, dt, opt)
-    for _ in range(120):
+    # Adam at this rate has a transient with loss spikes in the first ~150
+    # steps whose timing depends on FP32 summation order (with and without
+    # the forward's split over input channels: a spike at step 100 in one,
+    # none in the other); by step 300 both are near 0.1 x the first loss
+    for _ in range(300):
         last = net.train_step(dm, dt, opt)
     say('loss', first, last)
     assert last < 0.2 * first
