@@ -541,7 +541,7 @@ __device__ __forceinline__ bf16x8 tr_rows8(const __bf16* tile, int v0, int P, in
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(256)
 conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ dz,
-                float* __restrict__ partial, uint32_t blocks_per_split)
+                float* __restrict__ partial, uint32_t blocks_per_split, float* __restrict__ bpart)
 {
     constexpr int KP = conv_kp(CIN, 27);
     constexpr int CC = 16, BM = 32;
@@ -601,11 +601,18 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
             }
         }
     };
+    // bias gradient (bpart, workgroups of the first input-channel chunk):
+    // every dz chunk of this thread holds channels co0 + 8 (tid % 4) .. + 7
+    const bool bias_wg = bpart != nullptr && c0 == 0;
+    float bs[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     auto store = [&]() {
 #pragma unroll
         for (int n = 0; n < NDZ; ++n) {
             const int t = threadIdx.x + 256 * n;
             *reinterpret_cast<bf16x8*>(sdz + (t >> 2) * PD + 8 * (t & 3)) = rdz[n];
+            if (bias_wg)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) bs[j] += float(rdz[n][j]);
         }
 #pragma unroll
         for (int n = 0; n < NXI; ++n) {
@@ -639,6 +646,20 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
             }
         }
     }
+    if (bias_wg) {
+        // this split's bias partial: the threads of one 8-channel group in
+        // thread order, through the (now idle) dz tile
+        __syncthreads();
+        float* red = reinterpret_cast<float*>(sdz);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) red[threadIdx.x * 8 + j] = bs[j];
+        __syncthreads();
+        if (threadIdx.x < BM) {
+            float t = 0.0f;
+            for (int k = threadIdx.x >> 3; k < int(blockDim.x); k += 4) t += red[k * 8 + (threadIdx.x & 7)];
+            bpart[size_t(split) * COUT + co0 + threadIdx.x] = t;
+        }
+    }
     // C[co][ci]: lane holds column ci = c0 + r16, rows co = 4 grp .. + 3 of each 16-row tile
 #pragma unroll
     for (int tt = 0; tt < TW; ++tt) {
@@ -670,7 +691,7 @@ conv_wgrad_halo(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(2)))
 conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restrict__ dz,
-                float* __restrict__ partial, uint32_t blocks_per_split)
+                float* __restrict__ partial, uint32_t blocks_per_split, float* __restrict__ bpart)
 {
     constexpr int KP = conv_kp(CIN, 27);
     constexpr int CC = 16, BM = 32, NTH = 192;
@@ -698,6 +719,10 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
     f32x4 acc[9][2];
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[t][0] = acc[t][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // bias gradient (bpart, workgroups of the first input-channel chunk):
+    // every dz chunk of this thread holds channels co0 + 8 (tid % 4) .. + 7
+    const bool bias_wg = bpart != nullptr && c0 == 0;
+    float bs[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 
     auto load_tiles = [&](uint32_t blk) {
         const uint32_t smp = blk / bps, b = blk % bps;
@@ -730,7 +755,12 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
 #pragma unroll
         for (int n = 0; n < NDZ; ++n) {
             const int t = threadIdx.x + NTH * n;
-            if (t < 256 * (BM / 8)) *reinterpret_cast<bf16x8*>(sdz + (t >> 2) * PD + 8 * (t & 3)) = rdz[n];
+            if (t < 256 * (BM / 8)) {
+                *reinterpret_cast<bf16x8*>(sdz + (t >> 2) * PD + 8 * (t & 3)) = rdz[n];
+                if (bias_wg)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) bs[j] += float(rdz[n][j]);
+            }
         }
 #pragma unroll
         for (int n = 0; n < NXI; ++n) {
@@ -773,6 +803,20 @@ conv_wgrad_rows(ConvGeom g, const __bf16* __restrict__ in, const __bf16* __restr
             }
         }
     }
+    if (bias_wg) {
+        // this split's bias partial: the threads of one 8-channel group in
+        // thread order, through the (now idle) dz tile
+        __syncthreads();
+        float* red = reinterpret_cast<float*>(sdz);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) red[threadIdx.x * 8 + j] = bs[j];
+        __syncthreads();
+        if (threadIdx.x < BM) {
+            float t = 0.0f;
+            for (int k = threadIdx.x >> 3; k < int(blockDim.x); k += 4) t += red[k * 8 + (threadIdx.x & 7)];
+            bpart[size_t(split) * COUT + co0 + threadIdx.x] = t;
+        }
+    }
 #pragma unroll
     for (int t9 = 0; t9 < 9; ++t9) {
         const int tap = t9 * 3 + wave;    // (dz+1)*9 + (dy+1)*3 + (dx+1)
@@ -813,6 +857,17 @@ sum_splits(const float* __restrict__ partial, uint32_t n, uint32_t splits, uint3
     float s = 0.0f;
     for (uint32_t k = 0; k < splits; k += step) s += partial[size_t(k) * n + i];
     out[i] = s;
+}
+
+// db[c] = sum over splits of bpart[split][c], in split order.
+__global__ void __launch_bounds__(256)
+sum_bias_splits(const float* __restrict__ bpart, uint32_t cout, uint32_t splits, float* __restrict__ db)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= cout) return;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < splits; ++k) s += bpart[size_t(k) * cout + c];
+    db[c] = s;
 }
 
 static void launch_sum_splits(float* partial, uint32_t n, uint32_t splits, float* dw, hipStream_t s)
@@ -884,21 +939,28 @@ struct ConvLaunch {
 
     static constexpr bool HALO_W = HALO && COUT % 32 == 0;
 
+    // the tiled kernels also reduce the bias gradient into db when given one
+    static bool fuses_bias(uint32_t log_s) { return HALO_W && log_s >= 3; }
+
     static void wgrad(const ConvGeom& g, const __bf16* in, const __bf16* dz, float* partial, uint32_t splits,
-                      float* dw, hipStream_t s)
+                      float* dw, float* bpart, float* db, hipStream_t s)
     {
         constexpr int KP = conv_kp(CIN, TAPS);
         if constexpr (HALO_W) {
             if (g.log_s >= 3) {
                 const uint32_t nblocks = g.nvox >> 8;
                 const uint32_t per = (nblocks + splits - 1) / splits;
+                float* bp = db ? bpart : nullptr;
                 if (g.log_s >= 5)
                     hipLaunchKernelGGL((conv_wgrad_rows<CIN, COUT>), dim3((CIN / 16) * (COUT / 32) * splits),
-                                       dim3(192), 0, s, g, in, dz, partial, per);
+                                       dim3(192), 0, s, g, in, dz, partial, per, bp);
                 else
                     hipLaunchKernelGGL((conv_wgrad_halo<CIN, COUT>), dim3((CIN / 16) * (COUT / 32) * splits),
-                                       dim3(256), 0, s, g, in, dz, partial, per);
+                                       dim3(256), 0, s, g, in, dz, partial, per, bp);
                 launch_sum_splits(partial, uint32_t(COUT) * KP, splits, dw, s);
+                if (db)
+                    hipLaunchKernelGGL(sum_bias_splits, dim3((COUT + 255) / 256), dim3(256), 0, s, bpart,
+                                       uint32_t(COUT), splits, db);
                 return;
             }
         }
@@ -975,17 +1037,27 @@ hipError_t launch_conv_forward(int cin, int cout, int taps, const ConvGeom& g, c
     return hipErrorInvalidValue;
 }
 
+// db (optional, with bpart: splits x cout floats): the bias gradient, reduced
+// by the weight-gradient kernel where conv_wgrad_fuses_bias says so
 hipError_t launch_conv_wgrad(int cin, int cout, int taps, const ConvGeom& g, const __bf16* in, const __bf16* dz,
-                             float* partial, uint32_t splits, float* dw, hipStream_t s)
+                             float* partial, uint32_t splits, float* dw, float* bpart, float* db, hipStream_t s)
 {
 #define X(a, b, c)                                                                       \
     if (cin == a && cout == b && taps == c) {                                            \
-        ConvLaunch<a, b, c>::wgrad(g, in, dz, partial, splits, dw, s);                   \
+        ConvLaunch<a, b, c>::wgrad(g, in, dz, partial, splits, dw, bpart, db, s);        \
         return hipGetLastError();                                                        \
     }
     LBF_CONV_SHAPES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+bool conv_wgrad_fuses_bias(int cin, int cout, int taps, uint32_t log_s)
+{
+#define X(a, b, c) if (cin == a && cout == b && taps == c) return ConvLaunch<a, b, c>::fuses_bias(log_s);
+    LBF_CONV_SHAPES(X)
+#undef X
+    return false;
 }
 
 }  // namespace lbf

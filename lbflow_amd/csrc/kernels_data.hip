@@ -47,15 +47,15 @@ __global__ void __launch_bounds__(256) data_sumsq(const uint8_t* __restrict__ ma
 __global__ void __launch_bounds__(256) data_normalise(const uint8_t* __restrict__ mask, float* __restrict__ f,
                                                       uint64_t nvox, float nu_over_f, float force, float4 rms)
 {
-    const uint64_t v = uint64_t(blockIdx.x) * 256u + threadIdx.x;
-    if (v >= nvox) return;
-    float4 x = reinterpret_cast<float4*>(f)[v];
-    const float m = mask[v] ? 1.0f : 0.0f;
-    x.x = (x.x * nu_over_f) / rms.x * m;
-    x.y = (x.y * nu_over_f) / rms.y * m;
-    x.z = (x.z * nu_over_f) / rms.z * m;
-    x.w = ((x.w - 1.0f) / force) / rms.w * m;
-    reinterpret_cast<float4*>(f)[v] = x;
+    for (uint64_t v = uint64_t(blockIdx.x) * 256u + threadIdx.x; v < nvox; v += uint64_t(gridDim.x) * 256u) {
+        float4 x = reinterpret_cast<float4*>(f)[v];
+        const float m = mask[v] ? 1.0f : 0.0f;
+        x.x = (x.x * nu_over_f) / rms.x * m;
+        x.y = (x.y * nu_over_f) / rms.y * m;
+        x.z = (x.z * nu_over_f) / rms.z * m;
+        x.w = ((x.w - 1.0f) / force) / rms.w * m;
+        reinterpret_cast<float4*>(f)[v] = x;
+    }
 }
 
 // Minibatch gather: output sample j is stored sample idx[j] mapped by
@@ -65,13 +65,10 @@ __global__ void __launch_bounds__(256) data_normalise(const uint8_t* __restrict_
 // f. flipping its axis when its bit is set.  Output channel c is source
 // channel perm[c] times mul[c] (perm swaps u_z and u_y under bit 2; mul
 // carries the signs and the RMS ratio of the swapped channels).
-__global__ void __launch_bounds__(256) data_gather(const uint8_t* __restrict__ masks, const float* __restrict__ t,
-                                                   const uint32_t* __restrict__ idx, uint32_t ls, uint64_t nvox,
-                                                   uint32_t sym, float4 mul, uint8_t* __restrict__ mo,
-                                                   float* __restrict__ to)
+__device__ __forceinline__ void gather_voxel(const uint8_t* __restrict__ masks, const float* __restrict__ t,
+                                             const uint32_t* __restrict__ idx, uint32_t ls, uint64_t v, uint32_t sym,
+                                             float4 mul, uint8_t* __restrict__ mo, float* __restrict__ to)
 {
-    const uint64_t v = uint64_t(blockIdx.x) * 256u + threadIdx.x;
-    if (v >= nvox) return;
     const uint32_t s = 1u << ls, m = s - 1u;
     const uint64_t j = v >> (3 * ls);
     const uint32_t local = uint32_t(v & ((uint64_t(1) << (3 * ls)) - 1u));
@@ -96,7 +93,23 @@ __global__ void __launch_bounds__(256) data_gather(const uint8_t* __restrict__ m
     reinterpret_cast<float4*>(to)[v] = o;
 }
 
-static dim3 grid_of(uint64_t n) { return dim3(uint32_t((n + 255) / 256)); }
+__global__ void __launch_bounds__(256) data_gather(const uint8_t* __restrict__ masks, const float* __restrict__ t,
+                                                   const uint32_t* __restrict__ idx, uint32_t ls, uint64_t nvox,
+                                                   uint32_t sym, float4 mul, uint8_t* __restrict__ mo,
+                                                   float* __restrict__ to)
+{
+    for (uint64_t v = uint64_t(blockIdx.x) * 256u + threadIdx.x; v < nvox; v += uint64_t(gridDim.x) * 256u)
+        gather_voxel(masks, t, idx, ls, v, sym, mul, mo, to);
+}
+
+// A launch's threads along x must stay below 2^32 (16384 crops of 64^3 are
+// 2^32 voxels): grids are capped and the kernels loop over the rest.
+constexpr uint64_t MAX_BLOCKS = 1u << 20;
+static dim3 grid_of(uint64_t n)
+{
+    const uint64_t b = (n + 255) / 256;
+    return dim3(uint32_t(b < MAX_BLOCKS ? b : MAX_BLOCKS));
+}
 
 hipError_t launch_data_sumsq(const uint8_t* mask, const float* f, uint64_t nvox, double nu_over_f, double inv_f,
                              double* partial, hipStream_t s)

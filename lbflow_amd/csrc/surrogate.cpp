@@ -30,7 +30,8 @@ uint32_t conv_wgrad_splits(int cin, int cout, int taps, uint32_t nvox);
 hipError_t launch_conv_forward(int cin, int cout, int taps, const ConvGeom& g, const __bf16* in, const __bf16* w,
                                const float* bias, void* out, int act, bool out_f32, hipStream_t s);
 hipError_t launch_conv_wgrad(int cin, int cout, int taps, const ConvGeom& g, const __bf16* in, const __bf16* dz,
-                             float* partial, uint32_t splits, float* dw, hipStream_t s);
+                             float* partial, uint32_t splits, float* dw, float* bpart, float* db, hipStream_t s);
+bool conv_wgrad_fuses_bias(int cin, int cout, int taps, uint32_t log_s);
 bool conv_shape_supported(int cin, int cout, int taps);
 hipError_t launch_net_input(const uint8_t* mask, uint32_t nvox, __bf16* x0, hipStream_t s);
 hipError_t launch_net_pool(Chan in, Chan out, uint32_t C, uint32_t ls_out, uint32_t nvox_out, hipStream_t s);
@@ -169,7 +170,14 @@ int wgrad(lbf_net* n, int i, uint32_t count, const __bf16* in, uint32_t in_strid
     uint32_t splits = conv_wgrad_splits(l.cin, l.cout, l.taps, g.nvox);
     if (size_t(splits) * l.cout * l.kp > n->wpart_floats) splits = uint32_t(n->wpart_floats / (size_t(l.cout) * l.kp));
     if (splits < 1) splits = 1;
-    NHIP(n, launch_conv_wgrad(l.cin, l.cout, l.taps, g, in, dz, n->wpart, splits, l.gw, n->stream));
+    // the tiled weight-gradient kernels reduce the bias gradient on the way
+    if (conv_wgrad_fuses_bias(l.cin, l.cout, l.taps, g.log_s)) {
+        NHIP(n, launch_conv_wgrad(l.cin, l.cout, l.taps, g, in, dz, n->wpart, splits, l.gw, n->bpart, l.gb,
+                                  n->stream));
+        return 0;
+    }
+    NHIP(n, launch_conv_wgrad(l.cin, l.cout, l.taps, g, in, dz, n->wpart, splits, l.gw, nullptr, nullptr,
+                              n->stream));
     uint32_t blocks = g.nvox / 1024;
     if (blocks > 1024) blocks = 1024;
     if (blocks < 1) blocks = 1;
@@ -574,7 +582,8 @@ int lbf_conv3d_wgrad(int cin, int cout, int taps, uint32_t side, uint32_t sample
     const size_t nw = size_t(cout) * conv_kp(cin, taps);
     NHIP(nullptr, hipMalloc(&partial, nw * splits * 4));
     const hipError_t e = launch_conv_wgrad(cin, cout, taps, g, static_cast<const __bf16*>(in),
-                                           static_cast<const __bf16*>(dz), partial, splits, dw, nullptr);
+                                           static_cast<const __bf16*>(dz), partial, splits, dw, nullptr, nullptr,
+                                           nullptr);
     const hipError_t e2 = hipDeviceSynchronize();
     (void)hipFree(partial);
     NHIP(nullptr, e);
