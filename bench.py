@@ -240,7 +240,7 @@ def batch_tune_bench(crops, steps=20):
 
 
 def warm_start_bench(crops, interval=50, max_steps=60000):
-    """Learned warm start on unseen 64^3 crops (seeds 20000+i, porosity
+    """Learned warm start on unseen 64^3 crops (seeds 1000000+i, porosity
     0.15-0.35): the batched FP32 engine from rest and from the network's
     prediction (+5 consistent-initialisation sweeps) to the unchanged stop
     rule at tolerance 1e-3; wall clock from init to the last stop, the
@@ -353,7 +353,7 @@ def main():
         log(f'batch: {args.batch} crops generated in {time.perf_counter() - t0:.1f} s')
     if 'warm' in legs and args.warm_crops and WEIGHTS.exists():
         from lbflow_amd import surrogate as S
-        warm_crops = S.make_crops(args.warm_crops, 20000, 64)
+        warm_crops = S.make_crops(args.warm_crops, S.EVAL_SEED0, 64)
     spec, labels = workload(args.size)
     fluid = int((labels == 75).sum())
     porosity = fluid / labels.size

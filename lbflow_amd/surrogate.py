@@ -212,6 +212,13 @@ def _crop(args):
     return G.sphere_pack((side, side, side), p, rmin, rmax, seed)
 
 
+# Crop seeds: training crops are TRAIN_SEED0 + i, evaluation crops
+# EVAL_SEED0 + i; pipeline() refuses a training set that would reach the
+# evaluation seeds (crop i uses seed seed0 + i, so ranges must not overlap).
+TRAIN_SEED0 = 10000
+EVAL_SEED0 = 1000000
+
+
 def make_crops(n, seed0, side=64, phi=(0.15, 0.35), rmin=4.0, rmax=10.0, workers=16):
     """n cubic sphere-pack crops of the given side; crop i uses seed seed0 + i for
     its porosity (uniform in phi) and its spheres.  Call before the GPU is used
@@ -472,8 +479,10 @@ def pipeline(n_train=4096, n_eval=256, side=64, epochs=8, capacity=8, lr=1e-3, t
                force=force, train_force=train_force, train_tol=train_tol, eval_tol=eval_tol,
                report_interval=interval)
     t0 = time.perf_counter()
-    train_crops = make_crops(n_train, 10000, side)
-    eval_crops = make_crops(n_eval, 20000, side)
+    if TRAIN_SEED0 + n_train > EVAL_SEED0:
+        raise ValueError(f'{n_train} training crops would reach the evaluation seeds ({EVAL_SEED0}+)')
+    train_crops = make_crops(n_train, TRAIN_SEED0, side)
+    eval_crops = make_crops(n_eval, EVAL_SEED0, side)
     rep['crop_generation_s'] = round(time.perf_counter() - t0, 2)
     log(f'crops generated in {rep["crop_generation_s"]} s')
     t0 = time.perf_counter()
@@ -608,11 +617,11 @@ def main(argv=None):
     if a.train_bench:
         rep = train_bench(a.side, a.capacity, a.train_bench, log)
     elif a.diagnose:
-        crops = make_crops(a.eval, 20000, a.side)
+        crops = make_crops(a.eval, EVAL_SEED0, a.side)
         net, scaling = load(a.diagnose, capacity=a.capacity)
         rep = dict(weights=a.diagnose, diagnose=diagnose(net, scaling, crops, log=log))
     elif a.evaluate:
-        crops = make_crops(a.eval, 20000, a.side)
+        crops = make_crops(a.eval, EVAL_SEED0, a.side)
         net, scaling = load(a.evaluate, capacity=a.capacity)
         rep = dict(weights=a.evaluate, n_eval=a.eval, evaluation=evaluate(net, scaling, crops, log=log))
     else:
