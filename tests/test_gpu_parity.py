@@ -143,6 +143,29 @@ def test_restart_equivalence():
     assert np.array_equal(res.populations(), cont.populations())
 
 
+@pytest.mark.parametrize('precision,stream_solids', [(FP64, True), (FP32, False)], ids=['fp64-dense', 'fp32-sparse'])
+def test_set_populations_waits_for_queued_steps(precision, stream_solids):
+    """lbf_set_populations on an engine with steps still queued on its
+    stream (lbf_advance returns without waiting): the upload waits for them,
+    so restoring the step-10 state right after queueing 7 more steps goes on
+    exactly like the same restore with nothing queued."""
+    spec = sphere_case(model='scsp', eq='MRT', tau=(0.8,))
+    case, geo = CS.lattice(spec), CS.geometry(spec)
+    a = Engine(case, geo, precision, stream_solids).init()
+    a.advance(10)
+    a.set_populations(a.populations(), 10)
+    a.advance(10)
+    want = a.populations()
+    b = Engine(case, geo, precision, stream_solids).init()
+    b.advance(10)
+    p10 = b.populations()
+    b.advance(7)                 # queued, not waited for
+    b.set_populations(p10, 10)
+    b.advance(10)
+    assert b.t == 20
+    assert np.array_equal(b.populations().view(np.uint64), want.view(np.uint64))
+
+
 def test_solid_entries_are_upstream_copies_gpu():
     """The engine streams solid sites like the reference: every solid entry
     of plane q after a step is the previous entry of the upstream site, one
