@@ -316,8 +316,9 @@ int lbf_batch_relax(lbf_batch* batch, uint32_t steps);
  * a crop's fluid mask to its steady (u, rho); BF16 MFMA convolutions with
  * FP32 accumulation, FP32 master weights, Adam.  Masks, targets and outputs
  * are DEVICE memory: mask[sample][site] (1 = fluid), target/out
- * [sample][site][4] FP32 in the channel order the caller chose (the driver
- * uses u_z, u_y, u_x, rho, normalised).  Sites are z,y,x with side `side`. */
+ * [sample][site][4] FP32 in the channel order the caller chose
+ * (lbflow_amd/surrogate.py uses u_z, u_y, u_x, rho, normalised).  Sites are
+ * z,y,x with side `side`. */
 typedef struct lbf_net lbf_net;
 typedef struct lbf_adam {
     float lr, beta1, beta2, eps;
